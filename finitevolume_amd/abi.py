@@ -1,0 +1,194 @@
+"""ctypes mirror of the C ABI structs in include/fvk.h.
+
+Shared by the product wrapper (finitevolume_amd.solver) and by test
+infrastructure that drives other libraries taking the same structs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+# enums (include/fvk.h)
+EQ_EULER, EQ_NAVIER_STOKES, EQ_RANS_SA, EQ_RANS_SST, EQ_DIFFUSION = 0, 1, 2, 3, 4
+FLUX_RUSANOV, FLUX_HLLC, FLUX_EXACT = 0, 1, 2
+GRAD_GREEN_GAUSS, GRAD_LEAST_SQUARES = 0, 1
+BC_WALL, BC_FARFIELD, BC_OUTFLOW = 0, 1, 2
+BC_DIRICHLET, BC_NEUMANN = 0, 1
+SST_VORTICITY, SST_STRAIN_RATE = 0, 1
+IC_FREESTREAM, IC_TWO_REGION, IC_NONE = 0, 1, 2
+LIMITER_NONE, LIMITER_VENKATAKRISHNAN = 0, 1
+
+FLUX_NAMES = {"rusanov": FLUX_RUSANOV, "hllc": FLUX_HLLC, "exact": FLUX_EXACT}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class MeshDescC(C.Structure):
+    _fields_ = [
+        ("n_nodes", C.c_int64), ("n_cells", C.c_int64), ("n_faces", C.c_int64),
+        ("n_patches", C.c_int32), ("reserved0", C.c_int32),
+        ("node_xy", _dp), ("face_nodes", _ip), ("face_cells", _ip), ("face_patch", _ip),
+        ("cell_face_ptr", _ip), ("cell_faces", _ip), ("cell_volume", _dp), ("cell_xy", _dp),
+        ("face_shift", _dp),
+    ]
+
+
+class BCC(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("is_isothermal_wall", C.c_int32),
+        ("wall_u", C.c_double), ("wall_v", C.c_double), ("wall_temperature", C.c_double),
+        ("farfield_state", C.c_double * 4),
+        ("farfield_nut", C.c_double), ("farfield_k", C.c_double), ("farfield_omega", C.c_double),
+        ("value", C.c_double),
+    ]
+
+
+class SAConstantsC(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("cb1", "cb2", "sigma", "kappa", "cw2", "cw3", "cv1", "cv2", "cv3")]
+
+
+class SSTConstantsC(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("beta_star", "kappa", "a1", "sigma_k1", "sigma_k2", "sigma_omega1",
+                                          "sigma_omega2", "beta1", "beta2")]
+
+
+class ParamsC(C.Structure):
+    _fields_ = [
+        ("equation", C.c_int32), ("flux_scheme", C.c_int32), ("gradient_scheme", C.c_int32),
+        ("exact_riemann_max_iter", C.c_int32),
+        ("gamma", C.c_double), ("gas_constant", C.c_double), ("mu", C.c_double), ("prandtl", C.c_double),
+        ("prandtl_t", C.c_double), ("cfl", C.c_double), ("exact_riemann_tol", C.c_double),
+        ("sa", SAConstantsC), ("sst", SSTConstantsC),
+        ("sst_variant", C.c_int32), ("kato_launder", C.c_int32),
+        ("alpha", C.c_double), ("fixed_dt", C.c_double),
+        ("order", C.c_int32), ("limiter", C.c_int32), ("venkat_k", C.c_double),
+        ("rk_stages", C.c_int32), ("local_dt", C.c_int32),
+    ]
+
+
+class InitialConditionC(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("reserved0", C.c_int32)] + [
+        (n, C.c_double) for n in ("rho", "u", "v", "p", "rho_l", "u_l", "v_l", "p_l", "rho_r", "u_r", "v_r", "p_r",
+                                  "x0", "initial_nut", "initial_k", "initial_omega", "initial_value",
+                                  "initial_radius")]
+
+
+def default_params(**kw) -> ParamsC:
+    """Reference defaults (mirrors fvk_params_default in the library)."""
+    p = ParamsC()
+    p.equation = EQ_EULER
+    p.flux_scheme = FLUX_RUSANOV
+    p.gradient_scheme = GRAD_LEAST_SQUARES
+    p.exact_riemann_max_iter = 20
+    p.gamma = 1.4
+    p.gas_constant = 1.0
+    p.mu = 0.0
+    p.prandtl = 0.72
+    p.prandtl_t = 0.9
+    p.cfl = 0.5
+    p.exact_riemann_tol = 1e-6
+    p.sa = SAConstantsC(0.1355, 0.622, 2.0 / 3.0, 0.41, 0.3, 2.0, 7.1, 0.7, 0.9)
+    p.sst = SSTConstantsC(0.09, 0.41, 0.31, 0.85, 1.0, 0.5, 0.856, 0.075, 0.0828)
+    p.sst_variant = SST_VORTICITY
+    p.kato_launder = 0
+    p.alpha = 0.0
+    p.fixed_dt = 0.0
+    p.order = 1
+    p.limiter = LIMITER_NONE
+    p.venkat_k = 5.0
+    p.rk_stages = 1
+    p.local_dt = 0
+    for k, v in kw.items():
+        if k == "flux_scheme" and isinstance(v, str):
+            v = FLUX_NAMES[v]
+        setattr(p, k, v)
+    return p
+
+
+def initial_condition(mode: int = IC_NONE, **kw) -> InitialConditionC:
+    ic = InitialConditionC()
+    ic.mode = mode
+    ic.rho, ic.u, ic.v, ic.p = 1.0, 0.0, 0.0, 1.0
+    ic.rho_l, ic.u_l, ic.v_l, ic.p_l = 1.0, 0.0, 0.0, 1.0
+    ic.rho_r, ic.u_r, ic.v_r, ic.p_r = 0.125, 0.0, 0.0, 0.1
+    for k, v in kw.items():
+        setattr(ic, k, v)
+    return ic
+
+
+def make_bc(type: int, farfield_state: Optional[Sequence[float]] = None, **kw) -> BCC:
+    b = BCC()
+    b.type = type
+    if farfield_state is not None:
+        for i in range(4):
+            b.farfield_state[i] = float(farfield_state[i])
+    for k, v in kw.items():
+        setattr(b, k, v)
+    return b
+
+
+def bc_array(bcs: Sequence[BCC]):
+    arr = (BCC * max(1, len(bcs)))()
+    for i, b in enumerate(bcs):
+        arr[i] = b
+    return arr, len(bcs)
+
+
+def _ptr(a: np.ndarray, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+class MeshDesc:
+    """Keeps contiguous copies of a mesh's arrays alive and exposes the
+    fvk_mesh_desc struct pointing at them."""
+
+    def __init__(self, mesh):
+        self.node_xy = np.ascontiguousarray(mesh.node_xy, dtype=np.float64)
+        self.face_nodes = np.ascontiguousarray(mesh.face_nodes, dtype=np.int32)
+        self.face_cells = np.ascontiguousarray(mesh.face_cells, dtype=np.int32)
+        self.face_patch = np.ascontiguousarray(mesh.face_patch, dtype=np.int32)
+        self.cell_face_ptr = np.ascontiguousarray(mesh.cell_face_ptr, dtype=np.int32)
+        self.cell_faces = np.ascontiguousarray(mesh.cell_faces, dtype=np.int32)
+        self.cell_volume = np.ascontiguousarray(mesh.cell_volume, dtype=np.float64)
+        self.cell_xy = np.ascontiguousarray(mesh.cell_xy, dtype=np.float64)
+        self.face_shift = None if mesh.face_shift is None else np.ascontiguousarray(mesh.face_shift, dtype=np.float64)
+        d = MeshDescC()
+        d.n_nodes = self.node_xy.shape[0]
+        d.n_cells = self.cell_volume.shape[0]
+        d.n_faces = self.face_nodes.shape[0]
+        d.n_patches = max(len(mesh.patches), int(self.face_patch.max()) + 1 if self.face_patch.size else 0)
+        d.node_xy = _ptr(self.node_xy, C.c_double)
+        d.face_nodes = _ptr(self.face_nodes, C.c_int32)
+        d.face_cells = _ptr(self.face_cells, C.c_int32)
+        d.face_patch = _ptr(self.face_patch, C.c_int32)
+        d.cell_face_ptr = _ptr(self.cell_face_ptr, C.c_int32)
+        d.cell_faces = _ptr(self.cell_faces, C.c_int32)
+        d.cell_volume = _ptr(self.cell_volume, C.c_double)
+        d.cell_xy = _ptr(self.cell_xy, C.c_double)
+        d.face_shift = _ptr(self.face_shift, C.c_double) if self.face_shift is not None else _dp()
+        self.desc = d
+
+    @property
+    def ref(self):
+        return C.byref(self.desc)
+
+
+def nvars(equation: int) -> int:
+    return {EQ_EULER: 4, EQ_NAVIER_STOKES: 4, EQ_RANS_SA: 5, EQ_RANS_SST: 6, EQ_DIFFUSION: 1}[equation]
+
+
+def as_dptr(a: Optional[np.ndarray]):
+    if a is None:
+        return _dp()
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def as_iptr(a: Optional[np.ndarray]):
+    if a is None:
+        return _ip()
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)

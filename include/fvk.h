@@ -1,0 +1,259 @@
+/*
+ * fvk.h -- C ABI of the MI355X (gfx950) explicit finite-volume residual
+ * library.  Plain pointers and sizes only; no C++ or torch types cross it.
+ *
+ * The entry points replace the public API of the reference solver classes
+ * (happy12/FiniteVolume), one opaque fvk_solver per solver object:
+ *
+ *   fvk_create        <- EulerFVMSolver::EulerFVMSolver        include/EulerFVMSolver.h:107-112
+ *                        NavierStokesFVMSolver::...            include/NavierStokesFVMSolver.h:118-121
+ *                        RANSTurbulenceSASolver::...           include/RANSTurbulenceSASolver.h:107-111
+ *                        RANSTurbulenceSSTSolver::...          src/RANSTurbulenceSSTSolver.cpp:40-46
+ *                        UnstructuredFVMSolver::...            include/UnstructuredFVMSolver.h:34-35
+ *   fvk_step          <- *::step() / *::run(int)               src/EulerFVMSolver.cpp:58-121
+ *   fvk_residual      <- *::residual(), nut_residual(),
+ *                        k_residual(), omega_residual(),
+ *                        UnstructuredFVMSolver::residual_norm() include/EulerFVMSolver.h:141
+ *   fvk_get_field     <- *::field(), nut_field(), k_field(),
+ *                        omega_field()                         include/EulerFVMSolver.h:129
+ *   fvk_set_field     <- *::set_field(...)                     include/EulerFVMSolver.h:151-155
+ *   fvk_set_cfl       <- *::set_cfl(double)                    include/EulerFVMSolver.h:160
+ *   fvk_gas_gamma     <- *::gas_gamma()                        include/EulerFVMSolver.h:134
+ *   fvk_gradient      <- GradientCalculator::compute           src/GradientReconstruction.cpp:83-150
+ *   fvk_wall_distance <- compute_wall_distance                 src/WallDistance.cpp:26-44
+ *
+ * Entry points with no counterpart in the reference (extensions that
+ * reduce to the reference path when switched off, see fvk_params):
+ *   fvk_residual_eval, fvk_residual_vjp, fvk_wall_forces, fvk_batch_*.
+ *
+ * Conventions (same as the reference):
+ *  - cell and face indices are the caller's (reference) order; the library
+ *    renumbers internally and maps back on every copy in or out;
+ *  - a NaN/Inf residual is NOT an error: it is returned in the residual
+ *    norms so the caller's divergence check stays in charge
+ *    (src/main.cpp:65-69, 3496);
+ *  - every function returns FVK_OK (0) or a nonzero status; fvk_last_error()
+ *    gives a message for the last nonzero status on the calling thread.
+ */
+#ifndef FVK_H_INCLUDED
+#define FVK_H_INCLUDED
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVK_ABI_VERSION 1
+
+enum fvk_status {
+    FVK_OK = 0,
+    FVK_ERR_INVALID = 1,     /* bad argument, size mismatch, unknown enum value */
+    FVK_ERR_HIP = 2,         /* a HIP runtime call failed */
+    FVK_ERR_NOMEM = 3,       /* device or host allocation failed */
+    FVK_ERR_UNSUPPORTED = 4  /* option combination not built */
+};
+
+/* Which solver class the handle stands in for. */
+enum fvk_equation {
+    FVK_EQ_EULER = 0,          /* EulerFVMSolver */
+    FVK_EQ_NAVIER_STOKES = 1,  /* NavierStokesFVMSolver */
+    FVK_EQ_RANS_SA = 2,        /* RANSTurbulenceSASolver */
+    FVK_EQ_RANS_SST = 3,       /* RANSTurbulenceSSTSolver */
+    FVK_EQ_DIFFUSION = 4       /* UnstructuredFVMSolver (scalar diffusion) */
+};
+
+/* NumericalFluxScheme, include/EulerFVMSolver.h:40-44 (same order). */
+enum fvk_flux_scheme { FVK_FLUX_RUSANOV = 0, FVK_FLUX_HLLC = 1, FVK_FLUX_EXACT = 2 };
+
+/* GradientScheme, include/GradientReconstruction.h (same order). */
+enum fvk_gradient_scheme { FVK_GRAD_GREEN_GAUSS = 0, FVK_GRAD_LEAST_SQUARES = 1 };
+
+/* EulerBoundaryType (include/EulerFVMSolver.h:17-21) and NSBoundaryType
+ * (include/NavierStokesFVMSolver.h) share codes: 0 = wall (slip for Euler,
+ * no-slip for NS/RANS), 1 = farfield, 2 = outflow.  For FVK_EQ_DIFFUSION the
+ * codes are BoundaryType (include/UnstructuredMesh.h): 0 = Dirichlet,
+ * 1 = Neumann, and fvk_bc.value is the prescribed value/flux. */
+enum fvk_bc_type { FVK_BC_WALL = 0, FVK_BC_FARFIELD = 1, FVK_BC_OUTFLOW = 2 };
+enum fvk_diffusion_bc_type { FVK_BC_DIRICHLET = 0, FVK_BC_NEUMANN = 1 };
+
+/* SSTLimiterVariant, include/SSTKOmega.h (same order). */
+enum fvk_sst_variant { FVK_SST_VORTICITY = 0, FVK_SST_STRAIN_RATE = 1 };
+
+/* EulerICMode, include/EulerFVMSolver.h:32-35 (same order). */
+enum fvk_ic_mode { FVK_IC_FREESTREAM = 0, FVK_IC_TWO_REGION = 1, FVK_IC_NONE = 2 };
+
+/* Extension switches (no reference code; defaults reproduce the reference). */
+enum fvk_limiter { FVK_LIMITER_NONE = 0, FVK_LIMITER_VENKATAKRISHNAN = 1 };
+
+/*
+ * Mesh, in the reference's UnstructuredMesh layout (include/UnstructuredMesh.h:
+ * Node :246-248, Face :254-262, Cell :265-271) flattened to arrays.  Face
+ * geometry (normal, length, midpoint) is computed by the library from the
+ * node coordinates, as the reference solver constructors do
+ * (UnstructuredMesh::compute_geometry, include/UnstructuredMesh.h:89-105);
+ * cell volumes and centroids are taken as given (the reference's mesh
+ * reader fills them, src/MeshReader.cpp:56-76).
+ */
+typedef struct fvk_mesh_desc {
+    int64_t n_nodes, n_cells, n_faces;
+    int32_t n_patches;             /* number of boundary patches (bcs[] length) */
+    int32_t reserved0;
+    const double* node_xy;         /* [2*n_nodes]  x,y */
+    const int32_t* face_nodes;     /* [2*n_faces]  node1,node2 (cell_left's CCW walk) */
+    const int32_t* face_cells;     /* [2*n_faces]  cell_left, cell_right (-1 = boundary) */
+    const int32_t* face_patch;     /* [n_faces]    patch id (-1 = interior) */
+    const int32_t* cell_face_ptr;  /* [n_cells+1]  CSR offsets into cell_faces */
+    const int32_t* cell_faces;     /* Cell::faces, in the reference order */
+    const double* cell_volume;     /* [n_cells] */
+    const double* cell_xy;         /* [2*n_cells]  centroid */
+    /* Optional (NULL = none): translation added to cell_right's centroid to
+     * place it across the face as seen from cell_left (periodic face pairs).
+     * Only gradient/reconstruction geometry uses it. [2*n_faces] */
+    const double* face_shift;
+} fvk_mesh_desc;
+
+/* Per-patch boundary condition (EulerBoundaryCondition, NSBoundaryCondition,
+ * RANSBoundaryConditionSA/SST, BoundaryPatch), indexed by patch id. */
+typedef struct fvk_bc {
+    int32_t type;                  /* fvk_bc_type, or fvk_diffusion_bc_type */
+    int32_t is_isothermal_wall;    /* NS/RANS no-slip wall: 0 adiabatic, 1 isothermal */
+    double wall_u, wall_v;         /* NS/RANS no-slip wall velocity */
+    double wall_temperature;       /* NS/RANS isothermal wall */
+    double farfield_state[4];      /* conserved rho, rho_u, rho_v, E */
+    double farfield_nut;           /* SA farfield nut */
+    double farfield_k, farfield_omega; /* SST farfield k, omega */
+    double value;                  /* diffusion: Dirichlet value or Neumann flux */
+} fvk_bc;
+
+/* SAModelConstants, include/SpalartAllmaras.h:13-23. */
+typedef struct fvk_sa_constants {
+    double cb1, cb2, sigma, kappa, cw2, cw3, cv1, cv2, cv3;
+} fvk_sa_constants;
+
+/* SSTModelConstants, include/SSTKOmega.h. */
+typedef struct fvk_sst_constants {
+    double beta_star, kappa, a1, sigma_k1, sigma_k2, sigma_omega1, sigma_omega2, beta1, beta2;
+} fvk_sst_constants;
+
+typedef struct fvk_params {
+    int32_t equation;              /* fvk_equation */
+    int32_t flux_scheme;           /* fvk_flux_scheme */
+    int32_t gradient_scheme;       /* fvk_gradient_scheme (NS/RANS) */
+    int32_t exact_riemann_max_iter;
+    double gamma, gas_constant, mu, prandtl, prandtl_t, cfl;
+    double exact_riemann_tol;
+    fvk_sa_constants sa;
+    fvk_sst_constants sst;
+    int32_t sst_variant;           /* fvk_sst_variant */
+    int32_t kato_launder;
+    double alpha, fixed_dt;        /* FVK_EQ_DIFFUSION: diffusivity, time step */
+    /* ---- extensions; the defaults (1, NONE, 1, 0) are the reference method ---- */
+    int32_t order;                 /* 1: first order (reference); 2: MUSCL with LSQ primitive gradients (Euler) */
+    int32_t limiter;               /* fvk_limiter (order 2 only) */
+    double venkat_k;               /* Venkatakrishnan K; eps^2 = (K*sqrt(V))^3 */
+    int32_t rk_stages;             /* 1: forward Euler (reference); 2: SSP-RK2 (Heun); 3: SSP-RK3 (Shu-Osher) */
+    int32_t local_dt;              /* 0: one global CFL dt (reference); 1: per-cell local dt */
+} fvk_params;
+
+/* Initial condition (EulerInitialCondition, include/EulerFVMSolver.h:59-67,
+ * plus the turbulence initial values and the diffusion initial blob). */
+typedef struct fvk_initial_condition {
+    int32_t mode;                  /* fvk_ic_mode */
+    int32_t reserved0;
+    double rho, u, v, p;
+    double rho_l, u_l, v_l, p_l;
+    double rho_r, u_r, v_r, p_r;
+    double x0;
+    double initial_nut, initial_k, initial_omega;
+    double initial_value, initial_radius; /* diffusion */
+} fvk_initial_condition;
+
+typedef struct fvk_solver fvk_solver;
+
+/* Reference defaults: gamma 1.4, R 287-free (1.0), mu 0, Pr 0.72, Pr_t 0.9,
+ * cfl 0.5, Rusanov, least squares, exact tol 1e-6 / 20 iterations, SA and SST
+ * constants of include/SpalartAllmaras.h:13-23 and include/SSTKOmega.h,
+ * first order, forward Euler, global dt. */
+void fvk_params_default(fvk_params* p);
+void fvk_initial_condition_default(fvk_initial_condition* ic);
+
+/* Number of conserved variables per cell for an equation: 4 (Euler, NS),
+ * 5 (SA: + nut), 6 (SST: + k, omega), 1 (diffusion). */
+int fvk_num_vars(int32_t equation);
+
+int fvk_create(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, const fvk_params* params,
+               const fvk_initial_condition* ic, fvk_solver** out);
+void fvk_destroy(fvk_solver* s);
+
+/* Advance nsteps steps (one reference step() each).  residual_hist, if not
+ * NULL, receives nsteps * fvk_num_vars rows of L2 residual norms; dt_hist,
+ * if not NULL, receives the nsteps time steps (for local_dt: the minimum
+ * local dt).  One device->host copy at the end of the call. */
+int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hist);
+
+/* Last step's residual norms [fvk_num_vars]: rho, rho_u, rho_v, E (, nut | k, omega);
+ * all zero before the first step and after fvk_set_field. */
+int fvk_residual(const fvk_solver* s, double* norms);
+
+/* Conserved field in reference cell order.  U4 is AoS [4*n_cells]
+ * (EulerState rho, rho_u, rho_v, E; diffusion: phi [n_cells]); turb is
+ * NULL or AoS [n_turb*n_cells]: nut (SA) or k, omega (SST).  set_field with
+ * turb == NULL keeps the turbulence field, like the reference's
+ * set_field(U) overloads. */
+int fvk_set_field(fvk_solver* s, const double* U4, const double* turb, int64_t n_cells);
+int fvk_get_field(const fvk_solver* s, double* U4, double* turb, int64_t n_cells);
+
+int fvk_set_cfl(fvk_solver* s, double cfl);
+double fvk_gas_gamma(const fvk_solver* s);
+
+/* GradientCalculator::compute for this solver's mesh and scheme: phi
+ * [n_cells], boundary_phi [n_faces] (reference face order; only boundary
+ * entries read) -> grad [2*n_cells] (dphidx, dphidy). */
+int fvk_gradient(fvk_solver* s, int32_t scheme, const double* phi, const double* boundary_phi, double* grad);
+
+/* Wall distance of every cell to the solver's no-slip wall faces
+ * (RANS constructors, src/RANSTurbulenceSASolver.cpp:67), [n_cells]. */
+int fvk_wall_distance(fvk_solver* s, double* dist);
+/* Standalone: distance of every cell centroid to the given faces. */
+int fvk_wall_distance_faces(const fvk_mesh_desc* mesh, const int32_t* wall_faces, int64_t n_wall, double* dist);
+
+/* Extension: one residual evaluation R(U) (net flux in, per cell, the
+ * quantity whose L2 norm residual() reports; for order 2 with the limiter
+ * and gradients recomputed) without updating the state.  R is AoS
+ * [nvars*n_cells]. */
+int fvk_residual_eval(fvk_solver* s, double* R);
+
+/* Extension: discrete adjoint (vector-Jacobian product) of R(U, X) for the
+ * Euler path: given lambda [4*n_cells], returns U_bar = (dR/dU)^T lambda
+ * [4*n_cells] and, if X_bar != NULL, (dR/dX)^T lambda [2*n_nodes] with X the
+ * node coordinates.  Gradients and limiters are recomputed. */
+int fvk_residual_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
+
+/* Extension: pressure lift and drag coefficients on the wall patches
+ * (src/WallTraction.cpp:163-216, pressure part: p of the wall cell times
+ * A n, drag along (dir_x, dir_y), lift along it rotated +90 degrees,
+ * normalised by 0.5 rho_ref |V|^2 L_ref). */
+int fvk_wall_forces(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref,
+                    double* cl, double* cd);
+
+/* Per-kernel device time accumulated while timing is on, measured with HIP
+ * events on the solver's stream around each launch.  names is a
+ * '\n'-separated list written into a caller buffer. */
+int fvk_set_kernel_timing(fvk_solver* s, int32_t enable);
+int fvk_kernel_times(const fvk_solver* s, char* names, int64_t names_cap, double* total_ms, int64_t* count,
+                     int32_t cap, int32_t* n_kernels);
+
+/* Device memory the solver holds, bytes. */
+int64_t fvk_device_bytes(const fvk_solver* s);
+
+/* A plain device copy of n doubles (bandwidth calibration for counters). */
+int fvk_copy_probe(int64_t n_doubles, int32_t reps, double* ms_per_copy);
+
+const char* fvk_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FVK_H_INCLUDED */

@@ -1,0 +1,190 @@
+"""Test infrastructure only (the oracle): Python driver for the C
+restatement (oracle/fvoracle.c, built to oracle/_build/libfvoracle.so).
+Used by tests/ as the parity checker, by the smoke check, and by bench.py's
+cpu_baseline leg as a timed CPU port; never by the product."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from finitevolume_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libfvoracle.so")
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} not built (make -C oracle restatement)")
+    L = C.CDLL(LIB_PATH)
+    dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_void_p
+    L.fvo_create.argtypes = [C.POINTER(abi.MeshDescC), C.POINTER(abi.BCC), C.c_int32, C.POINTER(abi.ParamsC),
+                             C.POINTER(abi.InitialConditionC), C.POINTER(vp)]
+    L.fvo_destroy.argtypes = [vp]
+    L.fvo_step.argtypes = [vp, C.c_int32, dp, dp]
+    L.fvo_residual.argtypes = [vp, dp]
+    L.fvo_set_field.argtypes = [vp, dp, dp]
+    L.fvo_get_field.argtypes = [vp, dp, dp]
+    L.fvo_set_cfl.argtypes = [vp, C.c_double]
+    L.fvo_residual_eval.argtypes = [vp, dp]
+    L.fvo_gradient.argtypes = [vp, C.c_int32, dp, dp, dp]
+    L.fvo_wall_distance.argtypes = [vp, dp]
+    L.fvo_wall_distance_faces.argtypes = [C.POINTER(abi.MeshDescC), ip, C.c_int64, dp]
+    L.fvo_wall_forces.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
+    L.fvo_last_gradients.argtypes = [vp, dp, dp]
+    L.fvo_flux.argtypes = [C.c_int32, C.c_int64, dp, dp, dp, C.c_double, C.c_double, C.c_int32, dp]
+    L.fvo_sa_source.argtypes = [C.c_int64, dp, dp, dp, dp, dp, C.POINTER(abi.SAConstantsC), dp, dp]
+    L.fvo_sst_source.argtypes = [C.c_int64, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_int32,
+                                 C.POINTER(abi.SSTConstantsC), C.c_int32, dp]
+    L.fvo_next_cfl.argtypes = [C.c_double, C.c_double, dp, C.c_int32, C.c_double, C.c_double, C.c_double,
+                               C.c_int32, C.c_double]
+    L.fvo_next_cfl.restype = C.c_double
+    L.fvo_venkatakrishnan.argtypes = [C.c_double, C.c_double, C.c_double]
+    L.fvo_venkatakrishnan.restype = C.c_double
+    _lib = L
+    return L
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class OracleSolver:
+    """The restatement's solver object, same construction arguments as the
+    product's fvk_create."""
+
+    def __init__(self, mesh, bcs, params: abi.ParamsC, ic: Optional[abi.InitialConditionC] = None):
+        self.L = lib()
+        self.md = abi.MeshDesc(mesh)
+        self.mesh = mesh
+        self.n = mesh.n_cells
+        self.eq = params.equation
+        self.nv = abi.nvars(self.eq)
+        arr, nb = abi.bc_array(bcs)
+        self._bcs = arr
+        self.h = C.c_void_p()
+        rc = self.L.fvo_create(self.md.ref, arr, nb, C.byref(params), C.byref(ic) if ic is not None else None,
+                               C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError(f"fvo_create failed: {rc}")
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.fvo_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def set_field(self, U, turb=None):
+        U = _f64(U)
+        t = None if turb is None else _f64(turb)
+        self.L.fvo_set_field(self.h, abi.as_dptr(U), abi.as_dptr(t))
+
+    def field(self):
+        if self.eq == abi.EQ_DIFFUSION:
+            U = np.empty(self.n)
+            self.L.fvo_get_field(self.h, abi.as_dptr(U), abi.as_dptr(None))
+            return U
+        U = np.empty((self.n, 4))
+        nt = self.nv - 4
+        t = np.empty((self.n, nt)) if nt > 0 else None
+        self.L.fvo_get_field(self.h, abi.as_dptr(U), abi.as_dptr(t))
+        return U if t is None else (U, t)
+
+    def step(self, nsteps: int = 1, with_dt: bool = False):
+        hist = np.empty((nsteps, self.nv))
+        dts = np.empty(nsteps)
+        self.L.fvo_step(self.h, nsteps, abi.as_dptr(hist), abi.as_dptr(dts))
+        return (hist, dts) if with_dt else hist
+
+    def residual(self):
+        r = np.empty(self.nv)
+        self.L.fvo_residual(self.h, abi.as_dptr(r))
+        return r
+
+    def residual_eval(self):
+        R = np.empty((self.n, self.nv))
+        self.L.fvo_residual_eval(self.h, abi.as_dptr(R))
+        return R
+
+    def set_cfl(self, cfl):
+        self.L.fvo_set_cfl(self.h, cfl)
+
+    def gradient(self, scheme, phi, bphi):
+        g = np.empty((self.n, 2))
+        self.L.fvo_gradient(self.h, scheme, abi.as_dptr(_f64(phi)), abi.as_dptr(_f64(bphi)), abi.as_dptr(g))
+        return g
+
+    def wall_distance(self):
+        d = np.empty(self.n)
+        self.L.fvo_wall_distance(self.h, abi.as_dptr(d))
+        return d
+
+    def wall_forces(self, rho_ref, vx, vy, length=1.0):
+        cl, cd = C.c_double(), C.c_double()
+        self.L.fvo_wall_forces(self.h, rho_ref, vx, vy, length, C.byref(cl), C.byref(cd))
+        return cl.value, cd.value
+
+    def last_gradients(self):
+        g = np.empty((self.n, 8))
+        ph = np.empty((self.n, 4))
+        self.L.fvo_last_gradients(self.h, abi.as_dptr(g), abi.as_dptr(ph))
+        return g, ph
+
+
+def flux(scheme, UL, UR, nxy, gamma=1.4, tol=1e-6, max_iter=20):
+    L = lib()
+    UL, UR, nxy = _f64(UL), _f64(UR), _f64(nxy)
+    out = np.empty((UL.shape[0], 4))
+    L.fvo_flux(scheme, UL.shape[0], abi.as_dptr(UL), abi.as_dptr(UR), abi.as_dptr(nxy), gamma, tol, max_iter,
+               abi.as_dptr(out))
+    return out
+
+
+def wall_distance_faces(mesh, wall_faces):
+    L = lib()
+    md = abi.MeshDesc(mesh)
+    wf = np.ascontiguousarray(wall_faces, dtype=np.int32)
+    out = np.empty(mesh.n_cells)
+    L.fvo_wall_distance_faces(md.ref, abi.as_iptr(wf), wf.shape[0], abi.as_dptr(out))
+    return out
+
+
+def sa_source(nut, nu, omega, dist, grad_nut, consts=None):
+    L = lib()
+    c = consts if consts is not None else abi.default_params().sa
+    nut, nu, omega, dist, grad_nut = map(_f64, (nut, nu, omega, dist, grad_nut))
+    n = nut.shape[0]
+    out, eddy = np.empty((n, 3)), np.empty(n)
+    L.fvo_sa_source(n, abi.as_dptr(nut), abi.as_dptr(nu), abi.as_dptr(omega), abi.as_dptr(dist),
+                    abi.as_dptr(grad_nut), C.byref(c), abi.as_dptr(out), abi.as_dptr(eddy))
+    return out, eddy
+
+
+def sst_source(rho, k, omega, nu, dist, gu, gv, gk, gw, variant=0, consts=None, kato=0):
+    L = lib()
+    c = consts if consts is not None else abi.default_params().sst
+    arrs = list(map(_f64, (rho, k, omega, nu, dist, gu, gv, gk, gw)))
+    out = np.empty((arrs[0].shape[0], 10))
+    L.fvo_sst_source(arrs[0].shape[0], *[abi.as_dptr(a) for a in arrs], variant, C.byref(c), kato, abi.as_dptr(out))
+    return out
+
+
+def next_cfl(current, cfl_max, hist, cfl_min=0.05, growth=1.5, shrink=0.5, window=15, divergence_threshold=2.0):
+    L = lib()
+    h = _f64(hist)
+    return L.fvo_next_cfl(current, cfl_max, abi.as_dptr(h), h.shape[0], cfl_min, growth, shrink, window,
+                          divergence_threshold)
+
+
+def venkatakrishnan(d1, d2, eps2):
+    return lib().fvo_venkatakrishnan(d1, d2, eps2)

@@ -1,0 +1,185 @@
+"""Test infrastructure only (the oracle): Python driver for the reference
+solvers built from the reference's own sources (oracle/_ref, see
+oracle/Makefile and ref_harness.cpp).  Only tests/, the smoke check and
+bench.py's cpu_baseline leg use it, as a checker or a timed baseline."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from finitevolume_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_libs = {}
+
+
+def available(flavour: str = "strict") -> bool:
+    return os.path.exists(os.path.join(HERE, "_ref", f"libfvref_{flavour}.so"))
+
+
+def lib(flavour: str = "strict"):
+    if flavour in _libs:
+        return _libs[flavour]
+    path = os.path.join(HERE, "_ref", f"libfvref_{flavour}.so")
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built (make -C oracle ref, needs the reference sources)")
+    L = C.CDLL(path)
+    dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_void_p
+    L.fvr_create.argtypes = [C.POINTER(abi.MeshDescC), C.POINTER(abi.BCC), C.c_int32, C.POINTER(abi.ParamsC),
+                             C.POINTER(abi.InitialConditionC), C.POINTER(vp)]
+    L.fvr_destroy.argtypes = [vp]
+    L.fvr_set_field.argtypes = [vp, dp, dp]
+    L.fvr_get_field.argtypes = [vp, dp, dp]
+    L.fvr_step.argtypes = [vp, C.c_int32, dp]
+    L.fvr_set_cfl.argtypes = [vp, C.c_double]
+    L.fvr_flux.argtypes = [C.c_int32, C.c_int64, dp, dp, dp, C.c_double, C.c_double, C.c_int32, dp]
+    L.fvr_star_pressure.argtypes = [C.c_int64, dp, C.c_double, C.c_double, C.c_int32, dp, dp]
+    L.fvr_geometry.argtypes = [C.POINTER(abi.MeshDescC), dp, dp, dp]
+    L.fvr_gradient.argtypes = [C.POINTER(abi.MeshDescC), C.c_int32, dp, dp, dp]
+    L.fvr_face_gradient.argtypes = [C.POINTER(abi.MeshDescC), dp, dp, dp, dp, dp, dp]
+    L.fvr_wall_distance.argtypes = [C.POINTER(abi.MeshDescC), ip, C.c_int64, dp]
+    L.fvr_sa_source.argtypes = [C.c_int64, dp, dp, dp, dp, dp, C.POINTER(abi.SAConstantsC), dp, dp]
+    L.fvr_sst_source.argtypes = [C.c_int64, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_int32,
+                                 C.POINTER(abi.SSTConstantsC), C.c_int32, dp]
+    L.fvr_next_cfl.argtypes = [C.c_double, C.c_double, dp, C.c_int32, C.c_double, C.c_double, C.c_double,
+                               C.c_int32, C.c_double]
+    L.fvr_next_cfl.restype = C.c_double
+    _libs[flavour] = L
+    return L
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class RefSolver:
+    """One reference solver object (EulerFVMSolver, NavierStokesFVMSolver,
+    RANSTurbulenceSASolver, RANSTurbulenceSSTSolver or UnstructuredFVMSolver)."""
+
+    def __init__(self, mesh, bcs, params: abi.ParamsC, ic: Optional[abi.InitialConditionC] = None,
+                 flavour: str = "strict"):
+        self.L = lib(flavour)
+        self.md = abi.MeshDesc(mesh)
+        self.n = mesh.n_cells
+        self.eq = params.equation
+        self.nv = abi.nvars(self.eq)
+        arr, nb = abi.bc_array(bcs)
+        self._bcs = arr
+        self.h = C.c_void_p()
+        rc = self.L.fvr_create(self.md.ref, arr, nb, C.byref(params), C.byref(ic) if ic is not None else None,
+                               C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError(f"fvr_create failed: {rc}")
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.fvr_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def set_field(self, U, turb=None):
+        U = _f64(U)
+        t = None if turb is None else _f64(turb)
+        self.L.fvr_set_field(self.h, abi.as_dptr(U), abi.as_dptr(t))
+
+    def field(self):
+        if self.eq == abi.EQ_DIFFUSION:
+            U = np.empty(self.n)
+            self.L.fvr_get_field(self.h, abi.as_dptr(U), abi.as_dptr(None))
+            return U
+        U = np.empty((self.n, 4))
+        nt = self.nv - 4
+        t = np.empty((self.n, nt)) if nt > 0 else None
+        self.L.fvr_get_field(self.h, abi.as_dptr(U), abi.as_dptr(t))
+        return U if t is None else (U, t)
+
+    def step(self, nsteps: int = 1) -> np.ndarray:
+        hist = np.empty((nsteps, self.nv))
+        self.L.fvr_step(self.h, nsteps, abi.as_dptr(hist))
+        return hist
+
+    def set_cfl(self, cfl: float):
+        self.L.fvr_set_cfl(self.h, cfl)
+
+
+def flux(scheme: int, UL, UR, nxy, gamma=1.4, tol=1e-6, max_iter=20, flavour="strict"):
+    L = lib(flavour)
+    UL, UR, nxy = _f64(UL), _f64(UR), _f64(nxy)
+    n = UL.shape[0]
+    out = np.empty((n, 4))
+    L.fvr_flux(scheme, n, abi.as_dptr(UL), abi.as_dptr(UR), abi.as_dptr(nxy), gamma, tol, max_iter, abi.as_dptr(out))
+    return out
+
+
+def star_pressure(prim, gamma=1.4, tol=1e-6, max_iter=20):
+    L = lib()
+    prim = _f64(prim)
+    n = prim.shape[0]
+    ps, us = np.empty(n), np.empty(n)
+    L.fvr_star_pressure(n, abi.as_dptr(prim), gamma, tol, max_iter, abi.as_dptr(ps), abi.as_dptr(us))
+    return ps, us
+
+
+def geometry(mesh):
+    L = lib()
+    md = abi.MeshDesc(mesh)
+    mid, area, nxy = np.empty((mesh.n_faces, 2)), np.empty(mesh.n_faces), np.empty((mesh.n_faces, 2))
+    L.fvr_geometry(md.ref, abi.as_dptr(mid), abi.as_dptr(area), abi.as_dptr(nxy))
+    return mid, area, nxy
+
+
+def gradient(mesh, scheme, phi, bphi):
+    L = lib()
+    md = abi.MeshDesc(mesh)
+    g = np.empty((mesh.n_cells, 2))
+    L.fvr_gradient(md.ref, scheme, abi.as_dptr(_f64(phi)), abi.as_dptr(_f64(bphi)), abi.as_dptr(g))
+    return g
+
+
+def face_gradient(mesh, phi, bphi, grad):
+    L = lib()
+    md = abi.MeshDesc(mesh)
+    nf = mesh.n_faces
+    dn, gv, nd = np.empty(nf), np.empty((nf, 2)), np.empty(nf)
+    L.fvr_face_gradient(md.ref, abi.as_dptr(_f64(phi)), abi.as_dptr(_f64(bphi)), abi.as_dptr(_f64(grad)),
+                        abi.as_dptr(dn), abi.as_dptr(gv), abi.as_dptr(nd))
+    return dn, gv, nd
+
+
+def wall_distance(mesh, wall_faces):
+    L = lib()
+    md = abi.MeshDesc(mesh)
+    wf = np.ascontiguousarray(wall_faces, dtype=np.int32)
+    out = np.empty(mesh.n_cells)
+    L.fvr_wall_distance(md.ref, abi.as_iptr(wf), wf.shape[0], abi.as_dptr(out))
+    return out
+
+
+def sa_source(nut, nu, omega, dist, grad_nut, consts=None):
+    L = lib()
+    c = consts if consts is not None else abi.default_params().sa
+    nut, nu, omega, dist, grad_nut = map(_f64, (nut, nu, omega, dist, grad_nut))
+    n = nut.shape[0]
+    out, eddy = np.empty((n, 3)), np.empty(n)
+    L.fvr_sa_source(n, abi.as_dptr(nut), abi.as_dptr(nu), abi.as_dptr(omega), abi.as_dptr(dist),
+                    abi.as_dptr(grad_nut), C.byref(c), abi.as_dptr(out), abi.as_dptr(eddy))
+    return out, eddy
+
+
+def sst_source(rho, k, omega, nu, dist, gu, gv, gk, gw, variant=0, consts=None, kato=0):
+    L = lib()
+    c = consts if consts is not None else abi.default_params().sst
+    arrs = list(map(_f64, (rho, k, omega, nu, dist, gu, gv, gk, gw)))
+    n = arrs[0].shape[0]
+    out = np.empty((n, 10))
+    L.fvr_sst_source(n, *[abi.as_dptr(a) for a in arrs], variant, C.byref(c), kato, abi.as_dptr(out))
+    return out
+
+
+def next_cfl(current, cfl_max, hist, cfl_min=0.05, growth=1.5, shrink=0.5, window=15, divergence_threshold=2.0):
+    L = lib()
+    h = _f64(hist)
+    return L.fvr_next_cfl(current, cfl_max, abi.as_dptr(h), h.shape[0], cfl_min, growth, shrink, window,
+                          divergence_threshold)
