@@ -1,0 +1,626 @@
+// C ABI of the fvk library (include/fvk.h): solver lifetime, state
+// transfer in reference cell order, stepping, and the standalone entry
+// points.  One HIP stream per solver; kernels are in kernels_*.hip.
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "solver.h"
+
+namespace fvk {
+
+int launch_euler_residual_eval(fvk_solver* s, double* R_dev);
+int launch_visc_residual_eval(fvk_solver* s, double* R_dev);
+int launch_diffusion_residual_eval(fvk_solver* s, double* R_dev);
+int launch_wall_distance_segments(const double* cx, const double* cy, int nc, const double* seg, int nw, double* out,
+                                  hipStream_t st);
+int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
+
+namespace {
+thread_local std::string g_err;
+}
+
+int set_error(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) return set_error(FVK_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+    return FVK_OK;
+}
+
+void* dev_alloc(fvk_solver* s, size_t bytes) {
+    void* p = nullptr;
+    size_t b = (bytes + 255) & ~size_t(255);
+    if (hipMalloc(&p, b) != hipSuccess) return nullptr;
+    s->allocations.push_back(p);
+    s->device_bytes += (int64_t)b;
+    return p;
+}
+
+void timer_begin(fvk_solver* s, const char* name, hipEvent_t* ev_stop) {
+    *ev_stop = nullptr;
+    if (!s->timing) return;
+    KernelTimer* t = nullptr;
+    for (auto& k : s->timers)
+        if (k.name == name) { t = &k; break; }
+    if (!t) {
+        s->timers.push_back(KernelTimer{});
+        t = &s->timers.back();
+        t->name = name;
+    }
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return;
+    if (hipEventCreate(&b) != hipSuccess) { hipEventDestroy(a); return; }
+    hipEventRecord(a, s->stream);
+    t->start.push_back(a);
+    t->stop.push_back(b);
+    *ev_stop = b;
+}
+
+void timer_end(fvk_solver* s, hipEvent_t ev_stop) {
+    if (ev_stop) hipEventRecord(ev_stop, s->stream);
+}
+
+static void timers_collect(fvk_solver* s) {
+    for (auto& t : s->timers) {
+        for (size_t i = 0; i < t.start.size(); ++i) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, t.start[i], t.stop[i]) == hipSuccess) {
+                t.total_ms += ms;
+                t.count += 1;
+            }
+            hipEventDestroy(t.start[i]);
+            hipEventDestroy(t.stop[i]);
+        }
+        t.start.clear();
+        t.stop.clear();
+    }
+}
+
+}  // namespace fvk
+
+using namespace fvk;
+
+static int nvars_eq(int eq) {
+    return eq == FVK_EQ_RANS_SA ? 5 : (eq == FVK_EQ_RANS_SST ? 6 : (eq == FVK_EQ_DIFFUSION ? 1 : 4));
+}
+
+extern "C" {
+
+void fvk_params_default(fvk_params* p) {
+    std::memset(p, 0, sizeof *p);
+    p->equation = FVK_EQ_EULER;
+    p->flux_scheme = FVK_FLUX_RUSANOV;
+    p->gradient_scheme = FVK_GRAD_LEAST_SQUARES;
+    p->exact_riemann_max_iter = 20;
+    p->gamma = 1.4;
+    p->gas_constant = 1.0;
+    p->mu = 0.0;
+    p->prandtl = 0.72;
+    p->prandtl_t = 0.9;
+    p->cfl = 0.5;
+    p->exact_riemann_tol = 1e-6;
+    p->sa = fvk_sa_constants{0.1355, 0.622, 2.0 / 3.0, 0.41, 0.3, 2.0, 7.1, 0.7, 0.9};
+    p->sst = fvk_sst_constants{0.09, 0.41, 0.31, 0.85, 1.0, 0.5, 0.856, 0.075, 0.0828};
+    p->sst_variant = FVK_SST_VORTICITY;
+    p->order = 1;
+    p->limiter = FVK_LIMITER_NONE;
+    p->venkat_k = 5.0;
+    p->rk_stages = 1;
+    p->local_dt = 0;
+}
+
+void fvk_initial_condition_default(fvk_initial_condition* ic) {
+    std::memset(ic, 0, sizeof *ic);
+    ic->mode = FVK_IC_NONE;
+    ic->rho = 1.0; ic->p = 1.0;
+    ic->rho_l = 1.0; ic->p_l = 1.0;
+    ic->rho_r = 0.125; ic->p_r = 0.1;
+}
+
+int fvk_num_vars(int32_t equation) { return nvars_eq(equation); }
+
+const char* fvk_last_error(void) { return fvk::g_err.c_str(); }
+
+void fvk_destroy(fvk_solver* s) {
+    if (!s) return;
+    if (s->stream) hipStreamSynchronize(s->stream);
+    timers_collect(s);
+    for (void* p : s->allocations) hipFree(p);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+}
+
+static int upload_state(fvk_solver* s, const double* U4, const double* turb, bool keep_turb) {
+    const int64_t nc = s->nc, ld = s->ld;
+    const int nv = s->nv;
+    std::vector<double> h((size_t)nv * ld, 0.0);
+    if (!U4) return set_error(FVK_ERR_INVALID, "state pointer is NULL");
+    if (s->p.equation == FVK_EQ_DIFFUSION) {
+        for (int64_t i = 0; i < nc; ++i) h[i] = U4[s->cell_new2old[i]];
+    } else {
+        for (int64_t i = 0; i < nc; ++i) {
+            const double* u = U4 + 4 * (int64_t)s->cell_new2old[i];
+            for (int q = 0; q < 4; ++q) h[q * ld + i] = u[q];
+        }
+    }
+    int nt = nv > 4 ? nv - 4 : 0;
+    if (nt && (!turb || keep_turb)) {
+        // keep the device turbulence rows
+        hipError_t e = hipMemcpyAsync(h.data() + 4 * ld, s->U[s->cur] + 4 * ld, sizeof(double) * nt * ld,
+                                      hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) return hip_check(e, "read turbulence field");
+    } else if (nt) {
+        for (int64_t i = 0; i < nc; ++i) {
+            const double* t = turb + nt * (int64_t)s->cell_new2old[i];
+            for (int q = 0; q < nt; ++q) h[(4 + q) * ld + i] = t[q];
+        }
+    }
+    hipError_t e = hipMemcpyAsync(s->U[s->cur], h.data(), sizeof(double) * nv * ld, hipMemcpyHostToDevice, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    return hip_check(e, "upload state");
+}
+
+int fvk_create(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, const fvk_params* params,
+               const fvk_initial_condition* ic, fvk_solver** out) {
+    if (!mesh || !params || !out) return set_error(FVK_ERR_INVALID, "NULL argument to fvk_create");
+    *out = nullptr;
+    if (params->equation < FVK_EQ_EULER || params->equation > FVK_EQ_DIFFUSION)
+        return set_error(FVK_ERR_INVALID, "unknown equation %d", params->equation);
+    if (params->flux_scheme < 0 || params->flux_scheme > 2)
+        return set_error(FVK_ERR_INVALID, "unknown flux scheme %d", params->flux_scheme);
+    if (params->gradient_scheme < 0 || params->gradient_scheme > 1)
+        return set_error(FVK_ERR_INVALID, "unknown gradient scheme %d", params->gradient_scheme);
+    if (params->order != 1 && params->order != 2)
+        return set_error(FVK_ERR_INVALID, "order must be 1 or 2 (got %d)", params->order);
+    if (params->order == 2 && params->equation != FVK_EQ_EULER)
+        return set_error(FVK_ERR_UNSUPPORTED, "second-order reconstruction is built for the Euler equations only");
+    if (params->rk_stages < 1 || params->rk_stages > 3)
+        return set_error(FVK_ERR_INVALID, "rk_stages must be 1, 2 or 3 (got %d)", params->rk_stages);
+    if (params->equation == FVK_EQ_DIFFUSION && (params->rk_stages != 1 || params->local_dt))
+        return set_error(FVK_ERR_UNSUPPORTED, "the diffusion solver is forward Euler with a fixed dt");
+    if (n_bcs < 0 || n_bcs > kMaxPatches) return set_error(FVK_ERR_INVALID, "n_bcs out of range (%d)", n_bcs);
+    if (n_bcs > 0 && !bcs) return set_error(FVK_ERR_INVALID, "bcs is NULL");
+
+    fvk_solver* s = new (std::nothrow) fvk_solver;
+    if (!s) return set_error(FVK_ERR_NOMEM, "host allocation failed");
+    s->p = *params;
+    s->nv = nvars_eq(params->equation);
+    s->n_bcs = n_bcs;
+    s->bcs.assign(bcs, bcs + n_bcs);
+    for (int i = 0; i < n_bcs; ++i) {
+        int t = bcs[i].type;
+        if (t < 0 || t > 2) {
+            delete s;
+            return set_error(FVK_ERR_INVALID, "boundary condition %d has unknown type %d", i, t);
+        }
+    }
+    int rc = hip_check(hipGetDevice(&s->device), "hipGetDevice");
+    if (rc == FVK_OK) rc = hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (rc == FVK_OK) rc = build_layout(s, mesh);
+    if (rc != FVK_OK) {
+        std::string msg = fvk::g_err;
+        fvk_destroy(s);
+        fvk::g_err = msg;
+        return rc;
+    }
+    const int64_t pad = 256;
+    s->ld = (s->nc + pad - 1) / pad * pad;
+    s->ldf = (s->nf + pad - 1) / pad * pad;
+    const int nv = s->nv;
+    int nbuf = params->rk_stages > 1 ? 3 : 1;
+    for (int b = 0; b < 3; ++b) s->U[b] = nullptr;
+    for (int b = 0; b < nbuf; ++b) s->U[b] = static_cast<double*>(dev_alloc(s, sizeof(double) * nv * s->ld));
+    for (int b = nbuf; b < 3; ++b) s->U[b] = s->U[0];
+    s->flux = static_cast<double*>(dev_alloc(s, sizeof(double) * nv * s->ldf));
+    s->dtf = static_cast<double*>(dev_alloc(s, sizeof(double) * s->ldf));
+    s->dtc = static_cast<double*>(dev_alloc(s, sizeof(double) * s->ld));
+    int ng = params->equation == FVK_EQ_EULER ? (params->order == 2 ? 8 : 0) : (params->equation == FVK_EQ_DIFFUSION ? 0 : 10);
+    if (ng) s->grad = static_cast<double*>(dev_alloc(s, sizeof(double) * ng * s->ld));
+    if (params->equation == FVK_EQ_RANS_SA || params->equation == FVK_EQ_RANS_SST)
+        s->aux = static_cast<double*>(dev_alloc(s, sizeof(double) * 4 * s->ld));
+    int64_t max_blocks = (s->nc + 255) / 256;
+    s->partials = static_cast<double*>(dev_alloc(s, sizeof(double) * 6 * max_blocks));
+    s->ticket = static_cast<unsigned int*>(dev_alloc(s, 256));
+    s->dtmin = static_cast<unsigned long long*>(dev_alloc(s, 256));
+    s->hist_cap = 1024;
+    s->res_hist = static_cast<double*>(dev_alloc(s, sizeof(double) * 6 * s->hist_cap));
+    s->dt_hist = static_cast<double*>(dev_alloc(s, sizeof(double) * s->hist_cap));
+    double* wd = static_cast<double*>(dev_alloc(s, sizeof(double) * s->ld));
+    if (!s->U[0] || !s->flux || !s->dtf || !s->dtc || (ng && !s->grad) || !s->partials || !s->ticket || !s->dtmin ||
+        !s->res_hist || !s->dt_hist || !wd || (nbuf == 3 && (!s->U[1] || !s->U[2]))) {
+        fvk_destroy(s);
+        return set_error(FVK_ERR_NOMEM, "device allocation failed");
+    }
+    s->mesh.wdist = wd;
+    unsigned long long init[2];
+    double dm = DBL_MAX;
+    std::memcpy(&init[0], &dm, 8);
+    init[1] = init[0];
+    rc = hip_check(hipMemcpy(s->dtmin, init, sizeof init, hipMemcpyHostToDevice), "init dt slots");
+    if (rc == FVK_OK) rc = hip_check(hipMemset(s->ticket, 0, 256), "init ticket");
+    if (rc == FVK_OK) rc = hip_check(hipMemset(s->U[0], 0, sizeof(double) * nv * s->ld), "init state");
+    if (rc == FVK_OK && s->grad) rc = hip_check(hipMemset(s->grad, 0, sizeof(double) * ng * s->ld), "init gradients");
+    // wall distance to the no-slip walls (RANS constructors, src/RANSTurbulenceSASolver.cpp:67)
+    if (rc == FVK_OK) {
+        std::vector<double> inf((size_t)s->ld, INFINITY);
+        rc = hip_check(hipMemcpy(wd, inf.data(), sizeof(double) * s->ld, hipMemcpyHostToDevice), "init wall distance");
+        if (rc == FVK_OK && (params->equation == FVK_EQ_RANS_SA || params->equation == FVK_EQ_RANS_SST)) {
+            std::vector<int> wf;
+            for (int64_t f = 0; f < s->nf; ++f)
+                if (mesh->face_cells[2 * f + 1] == -1 && s->bcs[mesh->face_patch[f]].type == FVK_BC_WALL) wf.push_back((int)f);
+            if (!wf.empty()) {
+                int nw = (int)wf.size();
+                std::vector<double> seg(4 * (size_t)nw);
+                for (int k = 0; k < nw; ++k) {
+                    const double* a = mesh->node_xy + 2 * mesh->face_nodes[2 * wf[k]];
+                    const double* b = mesh->node_xy + 2 * mesh->face_nodes[2 * wf[k] + 1];
+                    seg[k] = a[0]; seg[nw + k] = a[1]; seg[2 * nw + k] = b[0]; seg[3 * nw + k] = b[1];
+                }
+                double* dseg = nullptr;
+                rc = hip_check(hipMalloc(&dseg, sizeof(double) * seg.size()), "wall segments");
+                if (rc == FVK_OK) rc = hip_check(hipMemcpy(dseg, seg.data(), sizeof(double) * seg.size(), hipMemcpyHostToDevice), "wall segments");
+                if (rc == FVK_OK) rc = launch_wall_distance_segments(s->mesh.cx, s->mesh.cy, (int)s->nc, dseg, nw, wd, s->stream);
+                if (rc == FVK_OK) rc = hip_check(hipStreamSynchronize(s->stream), "wall distance");
+                if (dseg) hipFree(dseg);
+            }
+        }
+    }
+    if (rc != FVK_OK) {
+        std::string msg = fvk::g_err;
+        fvk_destroy(s);
+        fvk::g_err = msg;
+        return rc;
+    }
+    // initial condition (solver constructors, e.g. src/EulerFVMSolver.cpp:24-37)
+    std::vector<double> U4((size_t)(s->p.equation == FVK_EQ_DIFFUSION ? 1 : 4) * s->nc, 0.0);
+    std::vector<double> T((size_t)(nv > 4 ? nv - 4 : 1) * s->nc, 0.0);
+    if (ic) {
+        double g = params->gamma;
+        auto prim = [g](double* o, double rho, double u, double v, double p) {
+            o[0] = rho; o[1] = rho * u; o[2] = rho * v; o[3] = p / (g - 1.0) + 0.5 * rho * (u * u + v * v);
+        };
+        for (int64_t c = 0; c < s->nc; ++c) {
+            double x = mesh->cell_xy[2 * c], y = mesh->cell_xy[2 * c + 1];
+            if (params->equation == FVK_EQ_DIFFUSION) {
+                // src/UnstructuredFVMSolver.cpp:13-21
+                if (ic->mode != FVK_IC_NONE && std::sqrt(x * x + y * y) < ic->initial_radius) U4[c] = ic->initial_value;
+                continue;
+            }
+            if (ic->mode == FVK_IC_FREESTREAM) prim(&U4[4 * c], ic->rho, ic->u, ic->v, ic->p);
+            else if (ic->mode == FVK_IC_TWO_REGION) {
+                if (x < ic->x0) prim(&U4[4 * c], ic->rho_l, ic->u_l, ic->v_l, ic->p_l);
+                else prim(&U4[4 * c], ic->rho_r, ic->u_r, ic->v_r, ic->p_r);
+            }
+            if (nv == 5) T[c] = ic->initial_nut;
+            if (nv == 6) { T[2 * c] = ic->initial_k; T[2 * c + 1] = ic->initial_omega; }
+        }
+    }
+    // the set-up above used the null stream; the solver's stream is non-blocking
+    rc = hip_check(hipDeviceSynchronize(), "solver set-up");
+    if (rc == FVK_OK) rc = upload_state(s, U4.data(), nv > 4 ? T.data() : nullptr, false);
+    if (rc != FVK_OK) {
+        std::string msg = fvk::g_err;
+        fvk_destroy(s);
+        fvk::g_err = msg;
+        return rc;
+    }
+    *out = s;
+    return FVK_OK;
+}
+
+int fvk_set_field(fvk_solver* s, const double* U4, const double* turb, int64_t n_cells) {
+    if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
+    if (n_cells != s->nc)
+        return set_error(FVK_ERR_INVALID, "set_field: %lld cells given, mesh has %lld", (long long)n_cells, (long long)s->nc);
+    int rc = upload_state(s, U4, turb, turb == nullptr);
+    if (rc == FVK_OK) std::memset(s->last_res, 0, sizeof s->last_res);
+    return rc;
+}
+
+int fvk_get_field(const fvk_solver* sc, double* U4, double* turb, int64_t n_cells) {
+    fvk_solver* s = const_cast<fvk_solver*>(sc);
+    if (!s || !U4) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (n_cells != s->nc)
+        return set_error(FVK_ERR_INVALID, "get_field: %lld cells given, mesh has %lld", (long long)n_cells, (long long)s->nc);
+    const int64_t nc = s->nc, ld = s->ld;
+    const int nv = s->nv;
+    std::vector<double> h((size_t)nv * ld);
+    hipError_t e = hipMemcpyAsync(h.data(), s->U[s->cur], sizeof(double) * nv * ld, hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return hip_check(e, "download state");
+    if (s->p.equation == FVK_EQ_DIFFUSION) {
+        for (int64_t i = 0; i < nc; ++i) U4[s->cell_new2old[i]] = h[i];
+        return FVK_OK;
+    }
+    for (int64_t i = 0; i < nc; ++i) {
+        double* u = U4 + 4 * (int64_t)s->cell_new2old[i];
+        for (int q = 0; q < 4; ++q) u[q] = h[q * ld + i];
+    }
+    int nt = nv > 4 ? nv - 4 : 0;
+    if (turb && nt) {
+        for (int64_t i = 0; i < nc; ++i) {
+            double* t = turb + nt * (int64_t)s->cell_new2old[i];
+            for (int q = 0; q < nt; ++q) t[q] = h[(4 + q) * ld + i];
+        }
+    }
+    return FVK_OK;
+}
+
+int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hist) {
+    if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
+    if (nsteps < 0) return set_error(FVK_ERR_INVALID, "nsteps < 0");
+    const int nv = s->nv;
+    const int stages = s->p.equation == FVK_EQ_DIFFUSION ? 1 : s->p.rk_stages;
+    std::vector<double> hr((size_t)s->hist_cap * nv), hd((size_t)s->hist_cap);
+    int done = 0;
+    while (done < nsteps) {
+        int chunk = std::min(s->hist_cap, nsteps - done);
+        for (int i = 0; i < chunk; ++i) {
+            int rc = FVK_OK;
+            if (s->p.equation == FVK_EQ_DIFFUSION) {
+                rc = launch_diffusion_step(s, 1, i);
+            } else {
+                for (int st = 0; st < stages && rc == FVK_OK; ++st) {
+                    // the history index is the step within this chunk; the dt slot follows the global count
+                    int64_t step = s->step_count;
+                    if (s->p.equation == FVK_EQ_EULER) rc = launch_euler_stage(s, st, stages, 0, 1, step);
+                    else rc = launch_visc_stage(s, st, stages, 1, step);
+                }
+            }
+            if (rc != FVK_OK) return rc;
+            s->step_count++;
+            (void)i;
+        }
+        hipError_t e = hipMemcpyAsync(hr.data(), s->res_hist, sizeof(double) * nv * s->hist_cap, hipMemcpyDeviceToHost,
+                                      s->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(hd.data(), s->dt_hist, sizeof(double) * s->hist_cap, hipMemcpyDeviceToHost, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) return hip_check(e, "step");
+        if (s->timing) timers_collect(s);
+        // rows were written at index (global step % hist_cap)
+        for (int i = 0; i < chunk; ++i) {
+            int64_t gstep = s->step_count - chunk + i;
+            int64_t idx = s->p.equation == FVK_EQ_DIFFUSION ? i : gstep % s->hist_cap;
+            if (residual_hist) std::memcpy(residual_hist + (int64_t)(done + i) * nv, &hr[idx * nv], sizeof(double) * nv);
+            if (dt_hist) dt_hist[done + i] = hd[idx];
+            if (i == chunk - 1) std::memcpy(s->last_res, &hr[idx * nv], sizeof(double) * nv);
+        }
+        done += chunk;
+    }
+    return FVK_OK;
+}
+
+int fvk_residual(const fvk_solver* s, double* norms) {
+    if (!s || !norms) return set_error(FVK_ERR_INVALID, "NULL argument");
+    std::memcpy(norms, s->last_res, sizeof(double) * s->nv);
+    return FVK_OK;
+}
+
+int fvk_set_cfl(fvk_solver* s, double cfl) {
+    if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
+    s->p.cfl = cfl;
+    return FVK_OK;
+}
+
+double fvk_gas_gamma(const fvk_solver* s) { return s ? s->p.gamma : NAN; }
+
+int64_t fvk_device_bytes(const fvk_solver* s) { return s ? s->device_bytes : 0; }
+
+int fvk_residual_eval(fvk_solver* s, double* R) {
+    if (!s || !R) return set_error(FVK_ERR_INVALID, "NULL argument");
+    const int nv = s->nv;
+    double* d = nullptr;
+    int rc = hip_check(hipMalloc((void**)&d, sizeof(double) * nv * s->ld), "residual buffer");
+    if (rc != FVK_OK) return rc;
+    if (s->p.equation == FVK_EQ_EULER) rc = launch_euler_residual_eval(s, d);
+    else if (s->p.equation == FVK_EQ_DIFFUSION) rc = launch_diffusion_residual_eval(s, d);
+    else rc = launch_visc_residual_eval(s, d);
+    std::vector<double> h((size_t)nv * s->ld);
+    if (rc == FVK_OK) rc = hip_check(hipMemcpyAsync(h.data(), d, sizeof(double) * nv * s->ld, hipMemcpyDeviceToHost, s->stream), "residual copy");
+    int rs = hip_check(hipStreamSynchronize(s->stream), "residual eval");
+    hipFree(d);
+    if (rc == FVK_OK) rc = rs;
+    if (rc != FVK_OK) return rc;
+    for (int64_t i = 0; i < s->nc; ++i) {
+        int64_t c = s->cell_new2old[i];
+        for (int q = 0; q < nv; ++q) R[c * nv + q] = h[q * s->ld + i];
+    }
+    return FVK_OK;
+}
+
+int fvk_gradient(fvk_solver* s, int32_t scheme, const double* phi, const double* bphi, double* grad) {
+    if (!s || !phi || !bphi || !grad) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (scheme < 0 || scheme > 1) return set_error(FVK_ERR_INVALID, "unknown gradient scheme %d", scheme);
+    const int64_t nc = s->nc, nf = s->nf;
+    std::vector<double> hp(nc), hb(nf), hg(2 * nc);
+    for (int64_t i = 0; i < nc; ++i) hp[i] = phi[s->cell_new2old[i]];
+    for (int64_t i = 0; i < nf; ++i) hb[i] = bphi[s->face_new2old[i]];
+    double *dp = nullptr, *db = nullptr, *dg = nullptr;
+    int rc = hip_check(hipMalloc(&dp, sizeof(double) * nc), "gradient buffers");
+    if (rc == FVK_OK) rc = hip_check(hipMalloc(&db, sizeof(double) * nf), "gradient buffers");
+    if (rc == FVK_OK) rc = hip_check(hipMalloc(&dg, sizeof(double) * 2 * nc), "gradient buffers");
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(dp, hp.data(), sizeof(double) * nc, hipMemcpyHostToDevice), "gradient upload");
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(db, hb.data(), sizeof(double) * nf, hipMemcpyHostToDevice), "gradient upload");
+    if (rc == FVK_OK) rc = launch_gradient(s, scheme, dp, db, dg);
+    if (rc == FVK_OK) rc = hip_check(hipStreamSynchronize(s->stream), "gradient");
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(hg.data(), dg, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost), "gradient download");
+    if (dp) hipFree(dp);
+    if (db) hipFree(db);
+    if (dg) hipFree(dg);
+    if (rc != FVK_OK) return rc;
+    for (int64_t i = 0; i < nc; ++i) {
+        int64_t c = s->cell_new2old[i];
+        grad[2 * c] = hg[2 * i];
+        grad[2 * c + 1] = hg[2 * i + 1];
+    }
+    return FVK_OK;
+}
+
+int fvk_wall_distance(fvk_solver* s, double* dist) {
+    if (!s || !dist) return set_error(FVK_ERR_INVALID, "NULL argument");
+    std::vector<double> h(s->nc);
+    int rc = hip_check(hipMemcpy(h.data(), s->mesh.wdist, sizeof(double) * s->nc, hipMemcpyDeviceToHost), "wall distance");
+    if (rc != FVK_OK) return rc;
+    for (int64_t i = 0; i < s->nc; ++i) dist[s->cell_new2old[i]] = h[i];
+    return FVK_OK;
+}
+
+int fvk_wall_distance_faces(const fvk_mesh_desc* mesh, const int32_t* wall_faces, int64_t n_wall, double* dist) {
+    if (!mesh || !dist || (n_wall > 0 && !wall_faces)) return set_error(FVK_ERR_INVALID, "NULL argument");
+    const int64_t nc = mesh->n_cells;
+    if (n_wall == 0) {
+        for (int64_t c = 0; c < nc; ++c) dist[c] = INFINITY;
+        return FVK_OK;
+    }
+    int nw = (int)n_wall;
+    std::vector<double> seg(4 * (size_t)nw), cx(nc), cy(nc);
+    for (int k = 0; k < nw; ++k) {
+        if (wall_faces[k] < 0 || wall_faces[k] >= mesh->n_faces) return set_error(FVK_ERR_INVALID, "wall face out of range");
+        const double* a = mesh->node_xy + 2 * mesh->face_nodes[2 * wall_faces[k]];
+        const double* b = mesh->node_xy + 2 * mesh->face_nodes[2 * wall_faces[k] + 1];
+        seg[k] = a[0]; seg[nw + k] = a[1]; seg[2 * nw + k] = b[0]; seg[3 * nw + k] = b[1];
+    }
+    for (int64_t c = 0; c < nc; ++c) { cx[c] = mesh->cell_xy[2 * c]; cy[c] = mesh->cell_xy[2 * c + 1]; }
+    double *dseg = nullptr, *dcx = nullptr, *dcy = nullptr, *dout = nullptr;
+    int rc = hip_check(hipMalloc(&dseg, sizeof(double) * seg.size()), "wall distance buffers");
+    if (rc == FVK_OK) rc = hip_check(hipMalloc(&dcx, sizeof(double) * nc), "wall distance buffers");
+    if (rc == FVK_OK) rc = hip_check(hipMalloc(&dcy, sizeof(double) * nc), "wall distance buffers");
+    if (rc == FVK_OK) rc = hip_check(hipMalloc(&dout, sizeof(double) * nc), "wall distance buffers");
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(dseg, seg.data(), sizeof(double) * seg.size(), hipMemcpyHostToDevice), "upload");
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(dcx, cx.data(), sizeof(double) * nc, hipMemcpyHostToDevice), "upload");
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(dcy, cy.data(), sizeof(double) * nc, hipMemcpyHostToDevice), "upload");
+    if (rc == FVK_OK) rc = launch_wall_distance_segments(dcx, dcy, (int)nc, dseg, nw, dout, nullptr);
+    if (rc == FVK_OK) rc = hip_check(hipDeviceSynchronize(), "wall distance");
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(dist, dout, sizeof(double) * nc, hipMemcpyDeviceToHost), "download");
+    hipFree(dseg); hipFree(dcx); hipFree(dcy); hipFree(dout);
+    return rc;
+}
+
+int fvk_residual_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar) {
+    if (!s || !lambda || !U_bar) return set_error(FVK_ERR_INVALID, "NULL argument");
+    return adjoint_vjp(s, lambda, U_bar, X_bar);
+}
+
+int fvk_wall_forces(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref, double* cl,
+                    double* cd) {
+    if (!s || !cl || !cd) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (s->p.equation == FVK_EQ_DIFFUSION) return set_error(FVK_ERR_UNSUPPORTED, "no pressure for diffusion");
+    // src/WallTraction.cpp:163-216, pressure part: A p_cell n per wall face
+    std::vector<double> U(4 * (size_t)s->nc);
+    int rc = fvk_get_field(s, U.data(), nullptr, s->nc);
+    if (rc != FVK_OK) return rc;
+    double vm = std::hypot(vx_ref, vy_ref);
+    double dx = vx_ref / vm, dy = vy_ref / vm, lx = -dy, ly = dx;
+    double q = 0.5 * rho_ref * vm * vm * length_ref;
+    double drag = 0.0, lift = 0.0;
+    const double g = s->p.gamma;
+    const std::vector<int>& f2o = s->face_new2old;
+    (void)f2o;
+    // walk the wall faces in reference face order for a deterministic sum
+    std::vector<double> nx(s->nf), ny(s->nf), ar(s->nf);
+    const double* X = s->node_xy.data();
+    for (int64_t i = 0; i < s->nf; ++i) {
+        int f = s->face_new2old[i];
+        const double* n1 = X + 2 * s->face_nodes_dev[2 * i];
+        const double* n2 = X + 2 * s->face_nodes_dev[2 * i + 1];
+        double ddx = n2[0] - n1[0], ddy = n2[1] - n1[1];
+        ar[f] = std::sqrt(ddx * ddx + ddy * ddy);
+        nx[f] = ddy / ar[f];
+        ny[f] = -ddx / ar[f];
+    }
+    std::vector<int> fl(s->nf), fr(s->nf);
+    {
+        std::vector<int> dl(s->nf), dr(s->nf);
+        hipMemcpy(dl.data(), s->mesh.fl, sizeof(int) * s->nf, hipMemcpyDeviceToHost);
+        hipMemcpy(dr.data(), s->mesh.fr, sizeof(int) * s->nf, hipMemcpyDeviceToHost);
+        for (int64_t i = 0; i < s->nf; ++i) { fl[s->face_new2old[i]] = dl[i]; fr[s->face_new2old[i]] = dr[i]; }
+    }
+    for (int64_t f = 0; f < s->nf; ++f) {
+        if (fr[f] >= 0 || s->bcs[-1 - fr[f]].type != FVK_BC_WALL) continue;
+        const double* u = &U[4 * (size_t)s->cell_new2old[fl[f]]];
+        double uu = u[1] / u[0], vv = u[2] / u[0];
+        double p = (g - 1.0) * (u[3] - 0.5 * u[0] * (uu * uu + vv * vv));
+        double px = ar[f] * p * nx[f], py = ar[f] * p * ny[f];
+        drag += px * dx + py * dy;
+        lift += px * lx + py * ly;
+    }
+    *cd = drag / q;
+    *cl = lift / q;
+    return FVK_OK;
+}
+
+int fvk_set_kernel_timing(fvk_solver* s, int32_t enable) {
+    if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
+    hipStreamSynchronize(s->stream);
+    timers_collect(s);
+    s->timing = enable != 0;
+    if (enable) {
+        for (auto& t : s->timers) { t.total_ms = 0.0; t.count = 0; }
+    }
+    return FVK_OK;
+}
+
+int fvk_kernel_times(const fvk_solver* sc, char* names, int64_t names_cap, double* total_ms, int64_t* count,
+                     int32_t cap, int32_t* n_kernels) {
+    fvk_solver* s = const_cast<fvk_solver*>(sc);
+    if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
+    hipStreamSynchronize(s->stream);
+    timers_collect(s);
+    std::string all;
+    int n = 0;
+    for (auto& t : s->timers) {
+        if (n < cap) {
+            if (total_ms) total_ms[n] = t.total_ms;
+            if (count) count[n] = t.count;
+        }
+        all += t.name;
+        all += '\n';
+        ++n;
+    }
+    if (n_kernels) *n_kernels = n;
+    if (names && names_cap > 0) {
+        std::strncpy(names, all.c_str(), (size_t)names_cap - 1);
+        names[names_cap - 1] = '\0';
+    }
+    return FVK_OK;
+}
+
+int fvk_copy_probe(int64_t n_doubles, int32_t reps, double* ms_per_copy) {
+    if (n_doubles <= 0 || reps <= 0 || !ms_per_copy) return set_error(FVK_ERR_INVALID, "bad copy probe arguments");
+    double *a = nullptr, *b = nullptr;
+    int rc = hip_check(hipMalloc(&a, sizeof(double) * n_doubles), "probe alloc");
+    if (rc == FVK_OK) rc = hip_check(hipMalloc(&b, sizeof(double) * n_doubles), "probe alloc");
+    if (rc == FVK_OK) rc = hip_check(hipMemset(a, 0, sizeof(double) * n_doubles), "probe init");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == FVK_OK) rc = hip_check(hipEventCreate(&e0), "event");
+    if (rc == FVK_OK) rc = hip_check(hipEventCreate(&e1), "event");
+    if (rc == FVK_OK) rc = launch_copy_probe(b, a, n_doubles, nullptr);
+    if (rc == FVK_OK) rc = hip_check(hipEventRecord(e0, nullptr), "event");
+    for (int r = 0; r < reps && rc == FVK_OK; ++r) rc = launch_copy_probe(b, a, n_doubles, nullptr);
+    if (rc == FVK_OK) rc = hip_check(hipEventRecord(e1, nullptr), "event");
+    if (rc == FVK_OK) rc = hip_check(hipEventSynchronize(e1), "event");
+    float ms = 0.0f;
+    if (rc == FVK_OK) rc = hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+    if (rc == FVK_OK) *ms_per_copy = ms / reps;
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,894 @@
+// Navier-Stokes, RANS (Spalart-Allmaras, SST k-omega) and scalar diffusion
+// residual kernels for gfx950 (FP64).  Same three-pass structure as the
+// Euler path:
+//   k_visc_grad  per cell: u, v, T (+ nut | k, omega) gradients (LSQ or
+//                Green-Gauss, GradientCalculator::compute), then the
+//                turbulence source terms of the cell
+//   k_visc_face  per face: inviscid flux minus viscous flux (Jasak-corrected
+//                face gradients), turbulence advection/diffusion, dt candidate
+//   k_visc_cell  per cell: gather in Cell::faces order + volumetric sources +
+//                stage update + residual partials
+#include <cfloat>
+
+#include "physics.h"
+#include "solver.h"
+
+namespace fvk {
+
+constexpr int kVB = 256;
+
+struct ViscArgs {
+    MeshView m;
+    const double* U;   // [nv][ld] stage state: rho, rho_u, rho_v, E (, nut | k, omega)
+    double* Unew;
+    const double* U0;
+    double* G;         // [10][ld] gradients: u(0,1) v(2,3) T(4,5) nut|k(6,7) omega(8,9)
+    double* aux;       // [4][ld]  SA: src; SST: k src, omega src, F1, nu_t
+    double* flux;      // [nv][ldf]
+    double* dtf;
+    double* dtc;
+    double* R;
+    unsigned long long* dtmin;
+    double* partials;
+    unsigned int* ticket;
+    double* res_hist;
+    double* dt_hist;
+    int64_t ld, ldf;
+    int gscheme, maxit, sst_variant, kato;
+    double gamma, Rgas, mu, pr, prt, cfl, tol, alpha, fixed_dt;
+    fvk_sa_constants sa;
+    fvk_sst_constants sst;
+    double a, b;
+    int face_dt, dt_src, first, want_res, slot;
+    int64_t hist_idx;
+};
+
+enum : int { EQ_NS = 1, EQ_SA = 2, EQ_SST = 3 };
+
+__device__ __forceinline__ State ldS(const double* U, int64_t ld, int c) {
+    return State{U[c], U[ld + c], U[2 * ld + c], U[3 * ld + c]};
+}
+__device__ __forceinline__ unsigned long long dbits_v(double x) { return (unsigned long long)__double_as_longlong(x); }
+
+// ---- Spalart-Allmaras, src/SpalartAllmaras.cpp:8-69 ----
+__device__ __forceinline__ double sa_fv1(double nut, double nu, const fvk_sa_constants& c) {
+    if (nut <= 0.0) return 0.0;
+    double chi = nut / nu, chi3 = chi * chi * chi;
+    return chi3 / (chi3 + c.cv1 * c.cv1 * c.cv1);
+}
+__device__ __forceinline__ double sa_eddy(double nut, double nu, const fvk_sa_constants& c) {
+    return nut * sa_fv1(nut, nu, c);
+}
+__device__ double sa_source(double nut, double nu, double omega, double d, double gx, double gy,
+                            const fvk_sa_constants& c) {
+    double chi = (nut > 0.0) ? nut / nu : 0.0;
+    double fv1 = sa_fv1(nut, nu, c);
+    double fv2 = 1.0 - chi / (1.0 + chi * fv1);
+    double d2 = d * d;
+    double sbar = (nut / (c.kappa * c.kappa * d2)) * fv2;
+    double st;
+    if (sbar >= -c.cv2 * omega) st = omega + sbar;
+    else st = omega + omega * (c.cv2 * c.cv2 * omega + c.cv3 * sbar) / ((c.cv3 - 2.0 * c.cv2) * omega - sbar);
+    double P = c.cb1 * st * nut;
+    double D = 0.0;
+    if (nut > 0.0) {
+        double sts = smax(st, 1e-10);
+        double r = smin(nut / (sts * c.kappa * c.kappa * d2), 10.0);
+        double gg = r + c.cw2 * (pow(r, 6.0) - r);
+        double fw = gg * pow((1.0 + pow(c.cw3, 6.0)) / (pow(gg, 6.0) + pow(c.cw3, 6.0)), 1.0 / 6.0);
+        double nod = nut / d;
+        double cw1 = c.cb1 / (c.kappa * c.kappa) + (1.0 + c.cb2) / c.sigma;
+        D = cw1 * fw * nod * nod;
+    }
+    double CD = (c.cb2 / c.sigma) * (gx * gx + gy * gy);
+    return P - D + CD;
+}
+
+// ---- SST k-omega, src/SSTKOmega.cpp:8-101; out: k src, omega src, F1, nu_t ----
+__device__ void sst_source(double rho, double k, double w, double nu, double d, const double* gu, const double* gv,
+                           const double* gk, const double* gw, int variant, int kato, const fvk_sst_constants& c,
+                           double* out) {
+    double s11 = gu[0], s22 = gv[1], s12 = 0.5 * (gu[1] + gv[0]);
+    double S = sqrt(2.0 * (s11 * s11 + s22 * s22 + 2.0 * s12 * s12));
+    double Om = fabs(gv[0] - gu[1]);
+    if (k <= 0.0 || w <= 0.0) {
+        // all source terms, F1 and nu_t stay zero (compute_sst_source_terms early return)
+        out[0] = 0.0 - 0.0;
+        out[1] = 0.0 - 0.0 + 0.0;
+        out[2] = 0.0;
+        out[3] = 0.0;
+        return;
+    }
+    double d2 = d * d;
+    double gdot = gk[0] * gw[0] + gk[1] * gw[1];
+    double CD = smax(2.0 * rho * c.sigma_omega2 * gdot / w, 1e-20);
+    double t1 = smax(sqrt(k) / (c.beta_star * w * d), 500.0 * nu / (d2 * w));
+    double t2 = 4.0 * rho * c.sigma_omega2 * k / (CD * d2);
+    double a1 = smin(t1, t2);
+    double a14 = a1 * a1 * a1 * a1;
+    double F1 = tanh(a14);
+    double a2 = smax(2.0 * sqrt(k) / (c.beta_star * w * d), 500.0 * nu / (d2 * w));
+    double F2 = tanh(a2 * a2);
+    double lim = (variant == FVK_SST_VORTICITY) ? Om : S;
+    double nut = c.a1 * k / smax(c.a1 * w, lim * F2);
+    double prs = kato ? (S * Om) : (S * S);
+    double mut = rho * nut;
+    double bsrwk = c.beta_star * rho * w * k;
+    double clip = (variant == FVK_SST_VORTICITY) ? 20.0 : 10.0;
+    double kp = smin(mut * prs, clip * bsrwk);
+    double kd = bsrwk;
+    double g1 = c.beta1 / c.beta_star - c.sigma_omega1 * c.kappa * c.kappa / sqrt(c.beta_star);
+    double g2 = c.beta2 / c.beta_star - c.sigma_omega2 * c.kappa * c.kappa / sqrt(c.beta_star);
+    double gam = F1 * g1 + (1.0 - F1) * g2;
+    double beta = F1 * c.beta1 + (1.0 - F1) * c.beta2;
+    double wp = gam * rho * prs;
+    double wd = beta * rho * w * w;
+    double cdw = 2.0 * (1.0 - F1) * rho * c.sigma_omega2 * gdot / w;
+    out[0] = kp - kd;
+    out[1] = wp - wd + cdw;
+    out[2] = F1;
+    out[3] = nut;
+}
+
+// boundary field values of a boundary face (build_boundary_fields,
+// src/NavierStokesFVMSolver.cpp:55-84; build_boundary_nut,
+// src/RANSTurbulenceSASolver.cpp:98-115; build_boundary_k_omega,
+// src/RANSTurbulenceSSTSolver.cpp:101-139).  b[0..2] = u, v, T; b[3] =
+// nut | k; b[4] = omega; b[5], b[6] = rho k, rho omega.
+template <int EQ>
+__device__ __forceinline__ void boundary_values(const ViscArgs& a, const DevBC& bc, int f, const State& L, double uL,
+                                                double vL, double TL, double t0, double t1, double* b) {
+    if (bc.type == BC_WALL) {
+        b[0] = bc.wall_u;
+        b[1] = bc.wall_v;
+        b[2] = bc.isothermal ? bc.wall_T : TL;
+    } else {
+        b[0] = uL; b[1] = vL; b[2] = TL;
+    }
+    if (EQ == EQ_SA) b[3] = bc.type == BC_WALL ? 0.0 : (bc.type == BC_FARFIELD ? bc.ff_nut : t0);
+    if (EQ == EQ_SST) {
+        if (bc.type == BC_WALL) {
+            b[3] = 0.0;
+            double nu1 = a.mu / L.r;
+            double d1 = a.m.fdc[f];
+            b[4] = 60.0 * nu1 / (a.sst.beta1 * d1 * d1);
+            b[5] = 0.0;
+            b[6] = L.r * b[4];
+        } else if (bc.type == BC_FARFIELD) {
+            b[3] = bc.ff_k; b[4] = bc.ff_w;
+            b[5] = bc.ff[0] * bc.ff_k; b[6] = bc.ff[0] * bc.ff_w;
+        } else {
+            b[3] = t0; b[4] = t1;
+            b[5] = L.r * t0; b[6] = L.r * t1;
+        }
+    }
+}
+
+// NS / RANS ghost state, src/NavierStokesFVMSolver.cpp:265-284
+__device__ __forceinline__ State visc_ghost(const DevBC& bc, const State& L, double g) {
+    if (bc.type == BC_FARFIELD) return State{bc.ff[0], bc.ff[1], bc.ff[2], bc.ff[3]};
+    if (bc.type == BC_OUTFLOW) return L;
+    return ghost_noslip_wall(L, bc.wall_u, bc.wall_v, g);
+}
+
+// ---------------------------------------------------------------------------
+// gradients (GradientCalculator::compute, src/GradientReconstruction.cpp:83-150)
+// of NF fields at once, plus per-cell turbulence sources
+// ---------------------------------------------------------------------------
+template <int EQ, int GS, bool PERIODIC>
+__global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
+    constexpr int NF = EQ == EQ_SST ? 5 : (EQ == EQ_SA ? 4 : 3);
+    const MeshView& m = a.m;
+    int c = blockIdx.x * kVB + threadIdx.x;
+    if (c >= m.nc) return;
+    const double g = a.gamma;
+    State Uc = ldS(a.U, a.ld, c);
+    double pc[NF];
+    pc[0] = Uc.ru / Uc.r;
+    pc[1] = Uc.rv / Uc.r;
+    pc[2] = temperature(Uc, g, a.Rgas);
+    double t0c = 0.0, t1c = 0.0;
+    if (EQ == EQ_SA) { t0c = a.U[4 * a.ld + c]; pc[3] = t0c; }
+    if (EQ == EQ_SST) { t0c = a.U[4 * a.ld + c]; t1c = a.U[5 * a.ld + c]; pc[3] = t0c; pc[NF - 1] = t1c; }
+    const double cx = m.cx[c], cy = m.cy[c];
+    double sx[NF], sy[NF];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) { sx[q] = 0.0; sy[q] = 0.0; }
+    const int k0 = m.cptr[c], k1 = m.cptr[c + 1];
+    for (int k = k0; k < k1; ++k) {
+        int e = m.cent[k];
+        int f = e >> 1;
+        bool owner = e & 1;
+        int cr = m.fr[f];
+        double pn[NF];
+        double qx, qy;
+        if (cr < 0) {
+            double bv[7];
+            boundary_values<EQ>(a, m.bc[-1 - cr], f, Uc, pc[0], pc[1], pc[2], t0c, t1c, bv);
+#pragma unroll
+            for (int q = 0; q < NF; ++q) pn[q] = bv[q];
+            qx = m.fmx[f];
+            qy = m.fmy[f];
+        } else {
+            int o = owner ? cr : m.fl[f];
+            State Uo = ldS(a.U, a.ld, o);
+            pn[0] = Uo.ru / Uo.r;
+            pn[1] = Uo.rv / Uo.r;
+            pn[2] = temperature(Uo, g, a.Rgas);
+            if (EQ == EQ_SA) pn[3] = a.U[4 * a.ld + o];
+            if (EQ == EQ_SST) { pn[3] = a.U[4 * a.ld + o]; pn[NF - 1] = a.U[5 * a.ld + o]; }
+            qx = m.cx[o];
+            qy = m.cy[o];
+            if (PERIODIC) {
+                if (owner) { qx = qx + m.fsx[f]; qy = qy + m.fsy[f]; }
+                else { qx = qx - m.fsx[f]; qy = qy - m.fsy[f]; }
+            }
+        }
+        if (GS == FVK_GRAD_LEAST_SQUARES) {
+            double dx = qx - cx, dy = qy - cy;
+            double w = 1.0 / (dx * dx + dy * dy);
+#pragma unroll
+            for (int q = 0; q < NF; ++q) {
+                double dphi = pn[q] - pc[q];
+                sx[q] += w * dx * dphi;
+                sy[q] += w * dy * dphi;
+            }
+        } else {
+            double wc = owner ? m.fwl[f] : m.fwr[f];
+            double nx = owner ? m.fnx[f] : -m.fnx[f];
+            double ny = owner ? m.fny[f] : -m.fny[f];
+            double A = m.fa[f];
+#pragma unroll
+            for (int q = 0; q < NF; ++q) {
+                double pf = (cr < 0) ? pn[q] : wc * pc[q] + (1.0 - wc) * pn[q];
+                sx[q] += pf * nx * A;
+                sy[q] += pf * ny * A;
+            }
+        }
+    }
+    double gr[2 * NF];
+    if (GS == FVK_GRAD_LEAST_SQUARES) {
+        const double i0 = m.lsq[c], i1 = m.lsq[m.nc + c], i2 = m.lsq[2 * m.nc + c];
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            gr[2 * q] = i0 * sx[q] + i1 * sy[q];
+            gr[2 * q + 1] = i1 * sx[q] + i2 * sy[q];
+        }
+    } else {
+        const double V = m.vol[c];
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            gr[2 * q] = sx[q] / V;
+            gr[2 * q + 1] = sy[q] / V;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 2 * NF; ++q) a.G[q * a.ld + c] = gr[q];
+    if (EQ == EQ_SA) {
+        double nul = a.mu / Uc.r;
+        double om = fabs(gr[2] - gr[1]);
+        a.aux[c] = sa_source(t0c, nul, om, m.wdist[c], gr[6], gr[7], a.sa);
+    }
+    if (EQ == EQ_SST) {
+        double o4[4];
+        double nul = a.mu / Uc.r;
+        sst_source(Uc.r, t0c, t1c, nul, m.wdist[c], gr, gr + 2, gr + 6, gr + 8, a.sst_variant, a.kato, a.sst, o4);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a.aux[q * a.ld + c] = o4[q];
+    }
+}
+
+// Jasak corrected face gradient, src/GradientReconstruction.cpp:153-185, 195-200
+struct FG {
+    double dn, gx, gy;
+};
+__device__ __forceinline__ FG face_grad(const MeshView& m, int f, bool b, double phiL, double phiRv, double gLx,
+                                        double gLy, double gRx, double gRy) {
+    double direct = (phiRv - phiL) / m.fdc[f];
+    FG o;
+    if (b) { o.gx = gLx; o.gy = gLy; }
+    else {
+        double wL = m.fwl[f];
+        o.gx = wL * gLx + (1.0 - wL) * gRx;
+        o.gy = wL * gLy + (1.0 - wL) * gRy;
+    }
+    o.dn = direct + o.gx * m.fkx[f] + o.gy * m.fky[f];
+    return o;
+}
+__device__ __forceinline__ void corrected(const FG& fg, double nx, double ny, double& gx, double& gy) {
+    double avg = fg.gx * nx + fg.gy * ny;
+    double delta = fg.dn - avg;
+    gx = fg.gx + delta * nx;
+    gy = fg.gy + delta * ny;
+}
+
+// ---------------------------------------------------------------------------
+// face pass: NavierStokesFVMSolver::step pass 1 (src/NavierStokesFVMSolver.cpp:119-189),
+// RANSTurbulenceSASolver (src/RANSTurbulenceSASolver.cpp:157-259),
+// RANSTurbulenceSSTSolver (src/RANSTurbulenceSSTSolver.cpp:183-307) and the
+// compute_dt candidates (NS :231-258, SA :313-346, SST :380-429)
+// ---------------------------------------------------------------------------
+template <int EQ, int SCHEME>
+__global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
+    const MeshView& m = a.m;
+    int f = blockIdx.x * kVB + threadIdx.x;
+    double cand = DBL_MAX;
+    if (f < m.nf) {
+        const double g = a.gamma;
+        const int64_t ld = a.ld;
+        int cl = m.fl[f], cr = m.fr[f];
+        bool b = cr < 0;
+        const DevBC* bc = b ? &m.bc[-1 - cr] : nullptr;
+        double nx = m.fnx[f], ny = m.fny[f], A = m.fa[f];
+        State L = ldS(a.U, ld, cl);
+        State R = b ? L : ldS(a.U, ld, cr);
+        State Rs = b ? visc_ghost(*bc, L, g) : R;
+        State inv = num_flux<SCHEME>(L, Rs, nx, ny, g, a.tol, a.maxit);
+        double uL = L.ru / L.r, vL = L.rv / L.r, TL = temperature(L, g, a.Rgas);
+        double uR = 0.0, vR = 0.0, TR = 0.0;
+        if (!b) { uR = R.ru / R.r; vR = R.rv / R.r; TR = temperature(R, g, a.Rgas); }
+        double t0L = 0.0, t1L = 0.0, t0R = 0.0, t1R = 0.0;
+        if (EQ == EQ_SA || EQ == EQ_SST) { t0L = a.U[4 * ld + cl]; if (!b) t0R = a.U[4 * ld + cr]; }
+        if (EQ == EQ_SST) { t1L = a.U[5 * ld + cl]; if (!b) t1R = a.U[5 * ld + cr]; }
+        double bv[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (b) boundary_values<EQ>(a, *bc, f, L, uL, vL, TL, t0L, t1L, bv);
+        double vx = 0.0, vy = 0.0, vE = 0.0, adv0 = 0.0, adv1 = 0.0, dif0 = 0.0, dif1 = 0.0;
+        double Vn = 0.0;
+        if (EQ != EQ_NS) {
+            double uRg = Rs.ru / Rs.r, vRg = Rs.rv / Rs.r;
+            Vn = 0.5 * ((uL + uRg) * nx + (vL + vRg) * ny);
+        }
+        if (EQ == EQ_SA) {
+            double nR = b ? bv[3] : t0R;
+            double up = (Vn >= 0.0) ? t0L : nR;
+            adv0 = Vn * up * A;
+        }
+        if (EQ == EQ_SST) {
+            double rkL = L.r * t0L, rkR = b ? bv[5] : R.r * t0R;
+            adv0 = Vn * ((Vn >= 0.0) ? rkL : rkR) * A;
+            double rwL = L.r * t1L, rwR = b ? bv[6] : R.r * t1R;
+            adv1 = Vn * ((Vn >= 0.0) ? rwL : rwR) * A;
+        }
+        if (!b || bc->type == BC_WALL) {
+            const double* G = a.G;
+            double gLu0 = G[cl], gLu1 = G[ld + cl], gLv0 = G[2 * ld + cl], gLv1 = G[3 * ld + cl];
+            double gLT0 = G[4 * ld + cl], gLT1 = G[5 * ld + cl];
+            double gRu0 = 0, gRu1 = 0, gRv0 = 0, gRv1 = 0, gRT0 = 0, gRT1 = 0;
+            if (!b) {
+                gRu0 = G[cr]; gRu1 = G[ld + cr]; gRv0 = G[2 * ld + cr]; gRv1 = G[3 * ld + cr];
+                gRT0 = G[4 * ld + cr]; gRT1 = G[5 * ld + cr];
+            }
+            FG fu = face_grad(m, f, b, uL, b ? bv[0] : uR, gLu0, gLu1, gRu0, gRu1);
+            FG fv = face_grad(m, f, b, vL, b ? bv[1] : vR, gLv0, gLv1, gRv0, gRv1);
+            FG fT = face_grad(m, f, b, TL, b ? bv[2] : TR, gLT0, gLT1, gRT0, gRT1);
+            double gux, guy, gvx, gvy, gTx, gTy;
+            corrected(fu, nx, ny, gux, guy);
+            corrected(fv, nx, ny, gvx, gvy);
+            corrected(fT, nx, ny, gTx, gTy);
+            double cp = g * a.Rgas / (g - 1.0);
+            double mu_eff = a.mu, k_eff = a.mu * cp / a.pr;
+            double tdvf = 0.0, nulL = 0.0, nulR = 0.0, ntL = 0.0, ntR = 0.0;
+            if (EQ == EQ_SA) {
+                nulL = a.mu / L.r;
+                ntL = sa_eddy(t0L, nulL, a.sa);
+                nulR = b ? nulL : a.mu / R.r;
+                ntR = b ? sa_eddy(bv[3], nulR, a.sa) : sa_eddy(t0R, nulR, a.sa);
+                double rR = b ? L.r : R.r;
+                tdvf = 0.5 * (L.r * ntL + rR * ntR);
+                mu_eff = a.mu + tdvf;
+                k_eff = cp * a.mu / a.pr + cp * tdvf / a.prt;
+            }
+            if (EQ == EQ_SST) {
+                ntL = a.aux[3 * ld + cl];
+                ntR = b ? 0.0 : a.aux[3 * ld + cr];
+                double rR = b ? L.r : R.r;
+                tdvf = 0.5 * (L.r * ntL + rR * ntR);
+                mu_eff = a.mu + tdvf;
+                k_eff = cp * a.mu / a.pr + cp * tdvf / a.prt;
+            }
+            double uf = b ? bv[0] : 0.5 * (uL + uR);
+            double vf = b ? bv[1] : 0.5 * (vL + vR);
+            double div = gux + gvy;
+            double txx = mu_eff * (2.0 * gux - (2.0 / 3.0) * div);
+            double tyy = mu_eff * (2.0 * gvy - (2.0 / 3.0) * div);
+            double txy = mu_eff * (guy + gvx);
+            double qx = -k_eff * gTx, qy = -k_eff * gTy;
+            if (b && !bc->isothermal) { qx = 0.0; qy = 0.0; }
+            vx = txx * nx + txy * ny;
+            vy = txy * nx + tyy * ny;
+            vE = (uf * txx + vf * txy - qx) * nx + (uf * txy + vf * tyy - qy) * ny;
+            if (EQ == EQ_SA) {
+                double nlf = 0.5 * (nulL + nulR);
+                double ntf = 0.5 * (ntL + ntR);
+                FG fn = face_grad(m, f, b, t0L, b ? bv[3] : t0R, G[6 * ld + cl], G[7 * ld + cl],
+                                  b ? 0.0 : G[6 * ld + cr], b ? 0.0 : G[7 * ld + cr]);
+                double dif = (nlf + ntf) / a.sa.sigma;
+                dif0 = -dif * fn.dn * A;
+            }
+            if (EQ == EQ_SST) {
+                double F1L = a.aux[2 * ld + cl];
+                double F1R = b ? F1L : a.aux[2 * ld + cr];
+                double F1f = b ? F1L : 0.5 * (F1L + F1R);
+                double sk = F1f * a.sst.sigma_k1 + (1.0 - F1f) * a.sst.sigma_k2;
+                double sw = F1f * a.sst.sigma_omega1 + (1.0 - F1f) * a.sst.sigma_omega2;
+                double mek = a.mu + sk * tdvf, mew = a.mu + sw * tdvf;
+                FG fk = face_grad(m, f, b, t0L, b ? bv[3] : t0R, G[6 * ld + cl], G[7 * ld + cl],
+                                  b ? 0.0 : G[6 * ld + cr], b ? 0.0 : G[7 * ld + cr]);
+                FG fw = face_grad(m, f, b, t1L, b ? bv[4] : t1R, G[8 * ld + cl], G[9 * ld + cl],
+                                  b ? 0.0 : G[8 * ld + cr], b ? 0.0 : G[9 * ld + cr]);
+                dif0 = -mek * fk.dn * A;
+                dif1 = -mew * fw.dn * A;
+            }
+        }
+        const int64_t ldf = a.ldf;
+        a.flux[f] = A * (inv.r - 0.0);
+        a.flux[ldf + f] = A * (inv.ru - vx);
+        a.flux[2 * ldf + f] = A * (inv.rv - vy);
+        a.flux[3 * ldf + f] = A * (inv.E - vE);
+        if (EQ == EQ_SA) a.flux[4 * ldf + f] = adv0 + dif0;
+        if (EQ == EQ_SST) { a.flux[4 * ldf + f] = adv0 + dif0; a.flux[5 * ldf + f] = adv1 + dif1; }
+        if (a.face_dt) {
+            double VnL = uL * nx + vL * ny;
+            double Smax = fabs(VnL) + sound_speed(L, g);
+            double rmin = L.r;
+            double ntmax = 0.0, wmax = 0.0;
+            if (EQ == EQ_SA) ntmax = sa_eddy(t0L, a.mu / L.r, a.sa);
+            if (EQ == EQ_SST) { ntmax = (t0L > 0.0 && t1L > 0.0) ? t0L / t1L : 0.0; wmax = t1L; }
+            if (!b) {
+                double VnR = uR * nx + vR * ny;
+                Smax = smax(Smax, fabs(VnR) + sound_speed(R, g));
+                rmin = smin(rmin, R.r);
+                if (EQ == EQ_SA) ntmax = smax(ntmax, sa_eddy(t0R, a.mu / R.r, a.sa));
+                if (EQ == EQ_SST) {
+                    ntmax = smax(ntmax, (t0R > 0.0 && t1R > 0.0) ? t0R / t1R : 0.0);
+                    wmax = smax(wmax, t1R);
+                }
+            }
+            double len = m.fdc[f];
+            double nu = (a.mu > 0.0) ? a.mu / rmin : 0.0;
+            if (EQ == EQ_NS) {
+                double St = Smax + 2.0 * nu / len;
+                cand = a.cfl * len / St;
+            } else {
+                double nue = nu + ntmax;
+                double St = Smax + 2.0 * nue / len;
+                cand = a.cfl * len / St;
+                if (EQ == EQ_SST) {
+                    double coef = smax(a.sst.beta_star, smax(a.sst.beta1, a.sst.beta2));
+                    double dts = a.cfl / (coef * smax(wmax, 1e-30));
+                    cand = smin(cand, dts);
+                }
+            }
+            if (a.dt_src == 2) a.dtf[f] = cand;
+        }
+    }
+    if (a.face_dt) {
+        __shared__ double sm[kVB];
+        sm[threadIdx.x] = cand;
+        __syncthreads();
+        for (int s = kVB / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) sm[threadIdx.x] = smin(sm[threadIdx.x], sm[threadIdx.x + s]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) atomicMin(&a.dtmin[a.slot], dbits_v(sm[0]));
+    }
+}
+
+template <int NV>
+__device__ __forceinline__ bool vblock_partials(const double (&v)[NV], double* partials, unsigned int* ticket,
+                                                double (&tot)[NV]) {
+    __shared__ double sm[NV][kVB];
+    __shared__ bool last;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int s = kVB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) partials[q * gridDim.x + blockIdx.x] = sm[q][0];
+        __threadfence();
+        unsigned int t = atomicAdd(ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
+    for (unsigned int i = threadIdx.x; i < gridDim.x; i += kVB) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q)
+            acc[q] += __hip_atomic_load(&partials[q * gridDim.x + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int s = kVB / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) tot[q] = sm[q][0];
+    if (threadIdx.x == 0) *ticket = 0u;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// cell pass: gather (Cell::faces order), sources (SA :279, SST :323-325),
+// update (SST: rho k, rho omega divided by the new density, :335-353)
+// ---------------------------------------------------------------------------
+template <int EQ>
+__global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
+    constexpr int NV = EQ == EQ_SST ? 6 : (EQ == EQ_SA ? 5 : 4);
+    const MeshView& m = a.m;
+    int c = blockIdx.x * kVB + threadIdx.x;
+    double r[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) r[q] = 0.0;
+    const int64_t ld = a.ld, ldf = a.ldf;
+    if (c < m.nc) {
+        double dmin = DBL_MAX;
+        for (int k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+            int e = m.cent[k];
+            int f = e >> 1;
+            if (e & 1) {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) r[q] = r[q] - a.flux[q * ldf + f];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) r[q] += a.flux[q * ldf + f];
+            }
+            if (a.dt_src == 2) dmin = smin(dmin, a.dtf[f]);
+        }
+        const double V = m.vol[c];
+        if (EQ == EQ_SA) r[4] += V * a.aux[c];
+        if (EQ == EQ_SST) { r[4] += V * a.aux[c]; r[5] += V * a.aux[ld + c]; }
+        if (a.R) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) a.R[q * ld + c] = r[q];
+        } else {
+            double dt;
+            if (a.dt_src == 1) dt = __longlong_as_double((long long)a.dtmin[a.slot]);
+            else if (a.dt_src == 2) { dt = dmin; a.dtc[c] = dmin; }
+            else dt = a.dtc[c];
+            double h = dt / V;
+            double rho_old = a.U[c];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                double u = a.U[q * ld + c];
+                a.Unew[q * ld + c] = a.first ? u + h * r[q] : a.a * a.U0[q * ld + c] + a.b * (u + h * r[q]);
+            }
+#pragma unroll
+            for (int q = 4; q < NV; ++q) {
+                double t = a.U[q * ld + c];
+                if (EQ == EQ_SST) {
+                    double cur = rho_old * t;
+                    double nw = a.first ? cur + h * r[q]
+                                        : a.a * (a.U0[c] * a.U0[q * ld + c]) + a.b * (cur + h * r[q]);
+                    a.Unew[q * ld + c] = nw / a.Unew[c];
+                } else {
+                    a.Unew[q * ld + c] = a.first ? t + h * r[q] : a.a * a.U0[q * ld + c] + a.b * (t + h * r[q]);
+                }
+            }
+        }
+    }
+    if (!a.want_res) return;
+    double sq[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) sq[q] = r[q] * r[q];
+    double tot[NV];
+    if (vblock_partials<NV>(sq, a.partials, a.ticket, tot) && threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) a.res_hist[a.hist_idx * NV + q] = sqrt(tot[q]);
+        a.dt_hist[a.hist_idx] = __longlong_as_double((long long)a.dtmin[a.slot]);
+        a.dtmin[a.slot ^ 1] = dbits_v(DBL_MAX);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// scalar diffusion, UnstructuredFVMSolver::step (src/UnstructuredFVMSolver.cpp:44-92)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kVB) void k_diff_face(ViscArgs a) {
+    const MeshView& m = a.m;
+    int f = blockIdx.x * kVB + threadIdx.x;
+    if (f >= m.nf) return;
+    int cl = m.fl[f], cr = m.fr[f];
+    double phiL = a.U[cl];
+    double F;
+    if (cr >= 0) {
+        double dx = m.cx[cr] - m.cx[cl], dy = m.cy[cr] - m.cy[cl];
+        double d = sqrt(dx * dx + dy * dy);
+        double gn = (a.U[cr] - phiL) / d;
+        F = -a.alpha * gn * m.fa[f];
+    } else {
+        const DevBC& bc = m.bc[-1 - cr];
+        if (bc.type == 1) {
+            F = bc.value * m.fa[f];
+        } else {
+            double dx = m.fmx[f] - m.cx[cl], dy = m.fmy[f] - m.cy[cl];
+            double d = sqrt(dx * dx + dy * dy);
+            double gn = (bc.value - phiL) / d;
+            F = -a.alpha * gn * m.fa[f];
+        }
+    }
+    a.flux[f] = F;
+}
+
+__global__ __launch_bounds__(kVB) void k_diff_cell(ViscArgs a) {
+    const MeshView& m = a.m;
+    int c = blockIdx.x * kVB + threadIdx.x;
+    double r[1] = {0.0};
+    if (c < m.nc) {
+        for (int k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+            int e = m.cent[k];
+            if (e & 1) r[0] = r[0] - a.flux[e >> 1];
+            else r[0] += a.flux[e >> 1];
+        }
+        if (a.R) a.R[c] = r[0];
+        else a.Unew[c] = a.U[c] + (a.fixed_dt / m.vol[c]) * r[0];
+    }
+    if (!a.want_res) return;
+    double sq[1] = {r[0] * r[0]};
+    double tot[1];
+    if (vblock_partials<1>(sq, a.partials, a.ticket, tot) && threadIdx.x == 0) {
+        a.res_hist[a.hist_idx] = sqrt(tot[0]);
+        a.dt_hist[a.hist_idx] = a.fixed_dt;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// standalone gradient (fvk_gradient): one scalar field with caller-given
+// boundary values (device face order)
+// ---------------------------------------------------------------------------
+template <int GS>
+__global__ __launch_bounds__(kVB) void k_scalar_grad(MeshView m, const double* phi, const double* bphi, double* grad,
+                                                     int periodic) {
+    int c = blockIdx.x * kVB + threadIdx.x;
+    if (c >= m.nc) return;
+    double pc = phi[c], cx = m.cx[c], cy = m.cy[c];
+    double sx = 0.0, sy = 0.0;
+    for (int k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+        int e = m.cent[k];
+        int f = e >> 1;
+        bool owner = e & 1;
+        int cr = m.fr[f];
+        double pn, qx, qy;
+        if (cr < 0) { pn = bphi[f]; qx = m.fmx[f]; qy = m.fmy[f]; }
+        else {
+            int o = owner ? cr : m.fl[f];
+            pn = phi[o];
+            qx = m.cx[o];
+            qy = m.cy[o];
+            if (periodic) {
+                if (owner) { qx = qx + m.fsx[f]; qy = qy + m.fsy[f]; }
+                else { qx = qx - m.fsx[f]; qy = qy - m.fsy[f]; }
+            }
+        }
+        if (GS == FVK_GRAD_LEAST_SQUARES) {
+            double dx = qx - cx, dy = qy - cy;
+            double w = 1.0 / (dx * dx + dy * dy);
+            double dphi = pn - pc;
+            sx += w * dx * dphi;
+            sy += w * dy * dphi;
+        } else {
+            double wc = owner ? m.fwl[f] : m.fwr[f];
+            double nx = owner ? m.fnx[f] : -m.fnx[f];
+            double ny = owner ? m.fny[f] : -m.fny[f];
+            double pf = (cr < 0) ? pn : wc * pc + (1.0 - wc) * pn;
+            sx += pf * nx * m.fa[f];
+            sy += pf * ny * m.fa[f];
+        }
+    }
+    if (GS == FVK_GRAD_LEAST_SQUARES) {
+        double i0 = m.lsq[c], i1 = m.lsq[m.nc + c], i2 = m.lsq[2 * m.nc + c];
+        grad[2 * c] = i0 * sx + i1 * sy;
+        grad[2 * c + 1] = i1 * sx + i2 * sy;
+    } else {
+        grad[2 * c] = sx / m.vol[c];
+        grad[2 * c + 1] = sy / m.vol[c];
+    }
+}
+
+// compute_wall_distance (src/WallDistance.cpp:13-43): brute force over the
+// wall segments, staged through LDS in tiles shared by the block's cells
+__global__ __launch_bounds__(kVB) void k_wall_distance(const double* cx, const double* cy, int nc,
+                                                       const double* seg /* [4][nw] x1 y1 x2 y2 */, int nw,
+                                                       double* out) {
+    __shared__ double s[4][kVB];
+    int c = blockIdx.x * kVB + threadIdx.x;
+    double px = c < nc ? cx[c] : 0.0, py = c < nc ? cy[c] : 0.0;
+    double best = INFINITY;
+    for (int base = 0; base < nw; base += kVB) {
+        int j = base + threadIdx.x;
+        if (j < nw) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s[q][threadIdx.x] = seg[q * nw + j];
+        }
+        __syncthreads();
+        int n = min(kVB, nw - base);
+        for (int t = 0; t < n; ++t) {
+            double x1 = s[0][t], y1 = s[1][t], x2 = s[2][t], y2 = s[3][t];
+            double dx = x2 - x1, dy = y2 - y1;
+            double l2 = dx * dx + dy * dy;
+            double tt = (l2 > 0.0) ? ((px - x1) * dx + (py - y1) * dy) / l2 : 0.0;
+            tt = (tt < 0.0) ? 0.0 : ((1.0 < tt) ? 1.0 : tt);
+            double ex = px - (x1 + tt * dx), ey = py - (y1 + tt * dy);
+            best = smin(best, ex * ex + ey * ey);
+        }
+        __syncthreads();
+    }
+    if (c < nc) out[c] = sqrt(best);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+namespace {
+
+ViscArgs make_vargs(fvk_solver* s) {
+    ViscArgs a{};
+    a.m = s->mesh;
+    a.G = s->grad;
+    a.aux = s->aux;
+    a.flux = s->flux;
+    a.dtf = s->dtf;
+    a.dtc = s->dtc;
+    a.dtmin = s->dtmin;
+    a.partials = s->partials;
+    a.ticket = s->ticket;
+    a.res_hist = s->res_hist;
+    a.dt_hist = s->dt_hist;
+    a.ld = s->ld;
+    a.ldf = s->ldf;
+    a.gscheme = s->p.gradient_scheme;
+    a.maxit = s->p.exact_riemann_max_iter;
+    a.sst_variant = s->p.sst_variant;
+    a.kato = s->p.kato_launder;
+    a.gamma = s->p.gamma;
+    a.Rgas = s->p.gas_constant;
+    a.mu = s->p.mu;
+    a.pr = s->p.prandtl;
+    a.prt = s->p.prandtl_t;
+    a.cfl = s->p.cfl;
+    a.tol = s->p.exact_riemann_tol;
+    a.alpha = s->p.alpha;
+    a.fixed_dt = s->p.fixed_dt;
+    a.sa = s->p.sa;
+    a.sst = s->p.sst;
+    return a;
+}
+
+template <int EQ>
+void launch_grad_eq(fvk_solver* s, int grid, const ViscArgs& a) {
+    bool ls = s->p.gradient_scheme == FVK_GRAD_LEAST_SQUARES;
+    if (s->periodic) {
+        if (ls) k_visc_grad<EQ, FVK_GRAD_LEAST_SQUARES, true><<<grid, kVB, 0, s->stream>>>(a);
+        else k_visc_grad<EQ, FVK_GRAD_GREEN_GAUSS, true><<<grid, kVB, 0, s->stream>>>(a);
+    } else {
+        if (ls) k_visc_grad<EQ, FVK_GRAD_LEAST_SQUARES, false><<<grid, kVB, 0, s->stream>>>(a);
+        else k_visc_grad<EQ, FVK_GRAD_GREEN_GAUSS, false><<<grid, kVB, 0, s->stream>>>(a);
+    }
+}
+
+template <int EQ>
+void launch_face_eq(fvk_solver* s, int grid, const ViscArgs& a) {
+    if (s->p.flux_scheme == FLUX_HLLC) k_visc_face<EQ, FLUX_HLLC><<<grid, kVB, 0, s->stream>>>(a);
+    else if (s->p.flux_scheme == FLUX_EXACT) k_visc_face<EQ, FLUX_EXACT><<<grid, kVB, 0, s->stream>>>(a);
+    else k_visc_face<EQ, FLUX_RUSANOV><<<grid, kVB, 0, s->stream>>>(a);
+}
+
+void run_visc(fvk_solver* s, ViscArgs& a) {
+    const int gc = (int)((s->nc + kVB - 1) / kVB);
+    const int gf = (int)((s->nf + kVB - 1) / kVB);
+    hipEvent_t ev = nullptr;
+    timer_begin(s, "visc_grad", &ev);
+    switch (s->p.equation) {
+        case FVK_EQ_NAVIER_STOKES: launch_grad_eq<EQ_NS>(s, gc, a); break;
+        case FVK_EQ_RANS_SA: launch_grad_eq<EQ_SA>(s, gc, a); break;
+        default: launch_grad_eq<EQ_SST>(s, gc, a); break;
+    }
+    timer_end(s, ev);
+    timer_begin(s, "visc_face", &ev);
+    switch (s->p.equation) {
+        case FVK_EQ_NAVIER_STOKES: launch_face_eq<EQ_NS>(s, gf, a); break;
+        case FVK_EQ_RANS_SA: launch_face_eq<EQ_SA>(s, gf, a); break;
+        default: launch_face_eq<EQ_SST>(s, gf, a); break;
+    }
+    timer_end(s, ev);
+    timer_begin(s, "visc_cell", &ev);
+    switch (s->p.equation) {
+        case FVK_EQ_NAVIER_STOKES: k_visc_cell<EQ_NS><<<gc, kVB, 0, s->stream>>>(a); break;
+        case FVK_EQ_RANS_SA: k_visc_cell<EQ_SA><<<gc, kVB, 0, s->stream>>>(a); break;
+        default: k_visc_cell<EQ_SST><<<gc, kVB, 0, s->stream>>>(a); break;
+    }
+    timer_end(s, ev);
+}
+
+}  // namespace
+
+int launch_visc_stage(fvk_solver* s, int stage, int stages, int want_res, int64_t step) {
+    static const double RA[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0.5, 0}, {0, 0.75, 1.0 / 3.0}};
+    static const double RB[4][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0.5, 0}, {1, 0.25, 2.0 / 3.0}};
+    ViscArgs a = make_vargs(s);
+    const int A = s->cur;
+    int in = stage == 0 ? A : (A + stage) % 3;
+    int out = (stage == stages - 1) ? A : (A + stage + 1) % 3;
+    a.U = s->U[in];
+    a.Unew = s->U[out];
+    a.U0 = s->U[A];
+    a.face_dt = stage == 0;
+    a.dt_src = s->p.local_dt ? (stage == 0 ? 2 : 0) : 1;
+    a.first = stage == 0;
+    a.a = RA[stages][stage];
+    a.b = RB[stages][stage];
+    a.want_res = want_res && stage == 0;
+    a.slot = (int)(step & 1);
+    a.hist_idx = step % s->hist_cap;
+    run_visc(s, a);
+    return hip_check(hipGetLastError(), "viscous stage launch");
+}
+
+int launch_visc_residual_eval(fvk_solver* s, double* R_dev) {
+    ViscArgs a = make_vargs(s);
+    a.U = s->U[s->cur];
+    a.R = R_dev;
+    run_visc(s, a);
+    return hip_check(hipGetLastError(), "viscous residual launch");
+}
+
+int launch_diffusion_step(fvk_solver* s, int want_res, int64_t step) {
+    ViscArgs a = make_vargs(s);
+    a.U = s->U[s->cur];
+    a.Unew = s->U[s->cur];
+    a.want_res = want_res;
+    a.hist_idx = step % s->hist_cap;
+    const int gc = (int)((s->nc + kVB - 1) / kVB);
+    const int gf = (int)((s->nf + kVB - 1) / kVB);
+    hipEvent_t ev = nullptr;
+    timer_begin(s, "diffusion_face", &ev);
+    k_diff_face<<<gf, kVB, 0, s->stream>>>(a);
+    timer_end(s, ev);
+    timer_begin(s, "diffusion_cell", &ev);
+    k_diff_cell<<<gc, kVB, 0, s->stream>>>(a);
+    timer_end(s, ev);
+    return hip_check(hipGetLastError(), "diffusion launch");
+}
+
+int launch_diffusion_residual_eval(fvk_solver* s, double* R_dev) {
+    ViscArgs a = make_vargs(s);
+    a.U = s->U[s->cur];
+    a.R = R_dev;
+    const int gc = (int)((s->nc + kVB - 1) / kVB);
+    const int gf = (int)((s->nf + kVB - 1) / kVB);
+    k_diff_face<<<gf, kVB, 0, s->stream>>>(a);
+    k_diff_cell<<<gc, kVB, 0, s->stream>>>(a);
+    return hip_check(hipGetLastError(), "diffusion residual launch");
+}
+
+int launch_gradient(fvk_solver* s, int scheme, const double* phi, const double* bphi_dev, double* grad) {
+    const int gc = (int)((s->nc + kVB - 1) / kVB);
+    if (scheme == FVK_GRAD_LEAST_SQUARES)
+        k_scalar_grad<FVK_GRAD_LEAST_SQUARES><<<gc, kVB, 0, s->stream>>>(s->mesh, phi, bphi_dev, grad, s->periodic);
+    else
+        k_scalar_grad<FVK_GRAD_GREEN_GAUSS><<<gc, kVB, 0, s->stream>>>(s->mesh, phi, bphi_dev, grad, s->periodic);
+    return hip_check(hipGetLastError(), "gradient launch");
+}
+
+int launch_wall_distance_segments(const double* cx, const double* cy, int nc, const double* seg, int nw, double* out,
+                                  hipStream_t st) {
+    int grid = (nc + kVB - 1) / kVB;
+    k_wall_distance<<<grid, kVB, 0, st>>>(cx, cy, nc, seg, nw, out);
+    return hip_check(hipGetLastError(), "wall distance launch");
+}
+
+}  // namespace fvk

@@ -1,0 +1,341 @@
+// Device re-layout of a reference-layout mesh (host side).
+//
+//  * cells are renumbered by reverse Cuthill-McKee over the face graph, so
+//    that neighbours sit close in memory (the reordering the reference
+//    suggests but does not implement, include/UnstructuredMesh.h:10-15);
+//  * faces keep their orientation (cell_left, node order, normal) and are
+//    reordered: interior faces first, sorted by their lower new cell index,
+//    then boundary faces grouped by boundary type;
+//  * each cell's face list (CSR) keeps the reference's Cell::faces order, so
+//    the per-cell sum of face fluxes adds in the same order as the
+//    reference's cell pass (src/EulerFVMSolver.cpp:89-96);
+//  * every state-independent quantity is computed here once, with the
+//    reference's formulas evaluated in the same order: face geometry
+//    (include/UnstructuredMesh.h:89-105), the least-squares inverse
+//    (src/GradientReconstruction.cpp:45-80), the Jasak face geometry and
+//    distance weights (:22-40, :103-106, :174-176) and the Euler dt length
+//    (src/EulerFVMSolver.cpp:153-155).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "solver.h"
+
+namespace fvk {
+
+void rcm_order(int n, const std::vector<int>& ptr, const std::vector<int>& adj, std::vector<int>& order) {
+    order.clear();
+    order.reserve(n);
+    std::vector<int> deg(n);
+    for (int i = 0; i < n; ++i) deg[i] = ptr[i + 1] - ptr[i];
+    std::vector<char> seen(n, 0);
+    std::vector<int> level(n, -1);
+    std::vector<int> nb;
+    // components in order of their lowest-degree node
+    std::vector<int> by_deg(n);
+    std::iota(by_deg.begin(), by_deg.end(), 0);
+    std::stable_sort(by_deg.begin(), by_deg.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+    std::vector<int> queue;
+    queue.reserve(n);
+    auto bfs_levels = [&](int root, int& last_level_min_deg) {
+        // BFS from root over the unvisited component; returns the farthest
+        // node with the lowest degree (pseudo-peripheral node search step)
+        std::vector<int> q{root};
+        std::vector<int> comp;
+        level[root] = 0;
+        size_t h = 0;
+        int maxl = 0;
+        while (h < q.size()) {
+            int v = q[h++];
+            comp.push_back(v);
+            for (int k = ptr[v]; k < ptr[v + 1]; ++k) {
+                int w = adj[k];
+                if (level[w] < 0 && !seen[w]) {
+                    level[w] = level[v] + 1;
+                    maxl = std::max(maxl, level[w]);
+                    q.push_back(w);
+                }
+            }
+        }
+        int best = root, bd = 1 << 30;
+        for (int v : comp)
+            if (level[v] == maxl && deg[v] < bd) { bd = deg[v]; best = v; }
+        for (int v : comp) level[v] = -1;
+        last_level_min_deg = maxl;
+        return best;
+    };
+    for (int s0 : by_deg) {
+        if (seen[s0]) continue;
+        int root = s0, ecc = 0;
+        for (int it = 0; it < 4; ++it) {
+            int e2 = 0;
+            int cand = bfs_levels(root, e2);
+            if (it > 0 && e2 <= ecc) break;
+            ecc = e2;
+            root = cand;
+        }
+        size_t head = order.size();
+        order.push_back(root);
+        seen[root] = 1;
+        while (head < order.size()) {
+            int v = order[head++];
+            nb.clear();
+            for (int k = ptr[v]; k < ptr[v + 1]; ++k)
+                if (!seen[adj[k]]) { nb.push_back(adj[k]); seen[adj[k]] = 1; }
+            std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+            for (int w : nb) order.push_back(w);
+        }
+    }
+    std::reverse(order.begin(), order.end());
+}
+
+namespace {
+
+template <typename T>
+T* upload(fvk_solver* s, const std::vector<T>& h) {
+    if (h.empty()) return nullptr;
+    T* d = static_cast<T*>(dev_alloc(s, sizeof(T) * h.size()));
+    if (!d) return nullptr;
+    if (hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+    return d;
+}
+
+}  // namespace
+
+int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
+    const int64_t nc = d->n_cells, nf = d->n_faces, nn = d->n_nodes;
+    if (nc <= 0 || nf <= 0 || nn <= 0 || nc > (1LL << 30) || nf > (1LL << 30))
+        return set_error(FVK_ERR_INVALID, "mesh sizes out of range (cells %lld, faces %lld, nodes %lld)",
+                         (long long)nc, (long long)nf, (long long)nn);
+    if (!d->node_xy || !d->face_nodes || !d->face_cells || !d->face_patch || !d->cell_face_ptr || !d->cell_faces ||
+        !d->cell_volume || !d->cell_xy)
+        return set_error(FVK_ERR_INVALID, "mesh descriptor has a NULL array");
+    s->nc = nc;
+    s->nf = nf;
+    s->nn = nn;
+    const int32_t* fc = d->face_cells;
+    // ---- validation (the reference's reader rejects the same defects, src/MeshReader.cpp:407-443) ----
+    if (d->cell_face_ptr[0] != 0) return set_error(FVK_ERR_INVALID, "cell_face_ptr[0] must be 0");
+    for (int64_t c = 0; c < nc; ++c)
+        if (d->cell_face_ptr[c + 1] < d->cell_face_ptr[c]) return set_error(FVK_ERR_INVALID, "cell_face_ptr not monotonic");
+    const int64_t nnz = d->cell_face_ptr[nc];
+    for (int64_t k = 0; k < nnz; ++k)
+        if (d->cell_faces[k] < 0 || d->cell_faces[k] >= nf) return set_error(FVK_ERR_INVALID, "cell_faces entry out of range");
+    for (int64_t f = 0; f < nf; ++f) {
+        if (fc[2 * f] < 0 || fc[2 * f] >= nc || fc[2 * f + 1] >= nc || fc[2 * f + 1] < -1)
+            return set_error(FVK_ERR_INVALID, "face %lld has an out-of-range cell", (long long)f);
+        if (d->face_nodes[2 * f] < 0 || d->face_nodes[2 * f] >= nn || d->face_nodes[2 * f + 1] < 0 ||
+            d->face_nodes[2 * f + 1] >= nn)
+            return set_error(FVK_ERR_INVALID, "face %lld has an out-of-range node", (long long)f);
+        if (fc[2 * f + 1] == -1 && (d->face_patch[f] < 0 || d->face_patch[f] >= s->n_bcs))
+            return set_error(FVK_ERR_INVALID,
+                             "boundary face %lld has patch %d but %d boundary conditions were given "
+                             "(every boundary face must belong to a patch with a boundary condition)",
+                             (long long)f, d->face_patch[f], s->n_bcs);
+    }
+    s->periodic = d->face_shift != nullptr;
+    s->node_xy.assign(d->node_xy, d->node_xy + 2 * nn);
+    s->cell_ptr_old.assign(d->cell_face_ptr, d->cell_face_ptr + nc + 1);
+    s->cell_faces_old.assign(d->cell_faces, d->cell_faces + nnz);
+
+    // ---- face geometry, reference order (UnstructuredMesh::compute_geometry) ----
+    std::vector<double> mx(nf), my(nf), ar(nf), nx(nf), ny(nf);
+    for (int64_t f = 0; f < nf; ++f) {
+        const double* n1 = d->node_xy + 2 * d->face_nodes[2 * f];
+        const double* n2 = d->node_xy + 2 * d->face_nodes[2 * f + 1];
+        mx[f] = 0.5 * (n1[0] + n2[0]);
+        my[f] = 0.5 * (n1[1] + n2[1]);
+        double dx = n2[0] - n1[0], dy = n2[1] - n1[1];
+        ar[f] = std::sqrt(dx * dx + dy * dy);
+        nx[f] = dy / ar[f];
+        ny[f] = -dx / ar[f];
+    }
+    const double* sh = d->face_shift;
+    auto shx = [&](int64_t f) { return sh ? sh[2 * f] : 0.0; };
+    auto shy = [&](int64_t f) { return sh ? sh[2 * f + 1] : 0.0; };
+
+    // ---- RCM over the interior-face graph ----
+    std::vector<int> aptr(nc + 1, 0), adj;
+    for (int64_t f = 0; f < nf; ++f)
+        if (fc[2 * f + 1] >= 0 && fc[2 * f + 1] != fc[2 * f]) { aptr[fc[2 * f] + 1]++; aptr[fc[2 * f + 1] + 1]++; }
+    for (int64_t c = 0; c < nc; ++c) aptr[c + 1] += aptr[c];
+    adj.resize(aptr[nc]);
+    {
+        std::vector<int> fill(aptr.begin(), aptr.end() - 1);
+        for (int64_t f = 0; f < nf; ++f) {
+            int a = fc[2 * f], b = fc[2 * f + 1];
+            if (b >= 0 && a != b) { adj[fill[a]++] = b; adj[fill[b]++] = a; }
+        }
+    }
+    rcm_order((int)nc, aptr, adj, s->cell_new2old);
+    s->cell_old2new.assign(nc, 0);
+    for (int64_t i = 0; i < nc; ++i) s->cell_old2new[s->cell_new2old[i]] = (int)i;
+    const std::vector<int>& o2n = s->cell_old2new;
+
+    // ---- face order ----
+    std::vector<int64_t> key(nf);
+    std::vector<int> forder(nf);
+    std::iota(forder.begin(), forder.end(), 0);
+    for (int64_t f = 0; f < nf; ++f) {
+        int a = o2n[fc[2 * f]];
+        if (fc[2 * f + 1] >= 0) {
+            int b = o2n[fc[2 * f + 1]];
+            key[f] = (int64_t)std::min(a, b) * (int64_t)(nc + 1) + std::max(a, b);
+        } else {
+            int t = s->bcs[d->face_patch[f]].type;
+            key[f] = ((int64_t)(nc + 1) * (nc + 1)) + ((int64_t)t * (nc + 1) + a);
+        }
+    }
+    std::stable_sort(forder.begin(), forder.end(), [&](int a, int b) { return key[a] < key[b]; });
+    s->face_new2old = forder;
+    s->face_old2new.assign(nf, 0);
+    for (int64_t i = 0; i < nf; ++i) s->face_old2new[forder[i]] = (int)i;
+    int nfi = 0;
+    for (int64_t i = 0; i < nf; ++i)
+        if (fc[2 * forder[i] + 1] >= 0) nfi = (int)i + 1;
+    s->nfi = nfi;
+    s->nnz = (int)nnz;
+
+    // ---- CSR in new cell order, reference face order within a cell ----
+    std::vector<int> cptr(nc + 1, 0), cent(nnz);
+    for (int64_t i = 0; i < nc; ++i) {
+        int c = s->cell_new2old[i];
+        cptr[i + 1] = cptr[i] + (d->cell_face_ptr[c + 1] - d->cell_face_ptr[c]);
+    }
+    for (int64_t i = 0; i < nc; ++i) {
+        int c = s->cell_new2old[i];
+        int k0 = cptr[i];
+        for (int k = d->cell_face_ptr[c]; k < d->cell_face_ptr[c + 1]; ++k) {
+            int f = d->cell_faces[k];
+            int owner = (fc[2 * f] == c) ? 1 : 0;
+            cent[k0++] = (s->face_old2new[f] << 1) | owner;
+        }
+    }
+
+    // ---- per-cell data ----
+    std::vector<double> vol(nc), cx(nc), cy(nc), lsq(3 * nc);
+    for (int64_t i = 0; i < nc; ++i) {
+        int c = s->cell_new2old[i];
+        vol[i] = d->cell_volume[c];
+        cx[i] = d->cell_xy[2 * c];
+        cy[i] = d->cell_xy[2 * c + 1];
+    }
+    // least-squares inverse, src/GradientReconstruction.cpp:56-79 (neighbour:
+    // opposite centroid, periodic-shifted; boundary: face midpoint)
+    for (int64_t i = 0; i < nc; ++i) {
+        int c = s->cell_new2old[i];
+        double m00 = 0.0, m01 = 0.0, m11 = 0.0;
+        for (int k = d->cell_face_ptr[c]; k < d->cell_face_ptr[c + 1]; ++k) {
+            int f = d->cell_faces[k];
+            int cl = fc[2 * f], cr = fc[2 * f + 1];
+            double qx, qy;
+            if (cr == -1) { qx = mx[f]; qy = my[f]; }
+            else if (cl == c) {
+                qx = d->cell_xy[2 * cr]; qy = d->cell_xy[2 * cr + 1];
+                if (sh) { qx = qx + shx(f); qy = qy + shy(f); }
+            } else {
+                qx = d->cell_xy[2 * cl]; qy = d->cell_xy[2 * cl + 1];
+                if (sh) { qx = qx - shx(f); qy = qy - shy(f); }
+            }
+            double dx = qx - d->cell_xy[2 * c], dy = qy - d->cell_xy[2 * c + 1];
+            double w = 1.0 / (dx * dx + dy * dy);
+            m00 += w * dx * dx;
+            m01 += w * dx * dy;
+            m11 += w * dy * dy;
+        }
+        double det = m00 * m11 - m01 * m01;
+        lsq[i] = m11 / det;
+        lsq[nc + i] = -m01 / det;
+        lsq[2 * nc + i] = m00 / det;
+    }
+
+    // ---- per-face data in device order ----
+    std::vector<int> fl(nf), fr(nf), fnodes(2 * nf);
+    std::vector<double> fnx(nf), fny(nf), fa(nf), fmx(nf), fmy(nf), fdl(nf), fdc(nf), fkx(nf), fky(nf), fwl(nf),
+        fwr(nf), fsx, fsy;
+    if (sh) { fsx.resize(nf); fsy.resize(nf); }
+    for (int64_t i = 0; i < nf; ++i) {
+        int f = forder[i];
+        int cl = fc[2 * f], cr = fc[2 * f + 1];
+        fl[i] = o2n[cl];
+        fr[i] = cr >= 0 ? o2n[cr] : -1 - d->face_patch[f];
+        fnodes[2 * i] = d->face_nodes[2 * f];
+        fnodes[2 * i + 1] = d->face_nodes[2 * f + 1];
+        fnx[i] = nx[f]; fny[i] = ny[f]; fa[i] = ar[f]; fmx[i] = mx[f]; fmy[i] = my[f];
+        if (sh) { fsx[i] = shx(f); fsy[i] = shy(f); }
+        // Euler compute_dt length, src/EulerFVMSolver.cpp:153-155
+        fdl[i] = cr >= 0 ? std::min(d->cell_volume[cl], d->cell_volume[cr]) / ar[f] : d->cell_volume[cl] / ar[f];
+        // face_normal_geometry, src/GradientReconstruction.cpp:28-40
+        double xR, yR;
+        if (cr == -1) { xR = mx[f]; yR = my[f]; }
+        else {
+            xR = d->cell_xy[2 * cr]; yR = d->cell_xy[2 * cr + 1];
+            if (sh) { xR = xR + shx(f); yR = yR + shy(f); }
+        }
+        double xL = d->cell_xy[2 * cl], yL = d->cell_xy[2 * cl + 1];
+        double ddx = xR - xL, ddy = yR - yL;
+        double dist = std::hypot(ddx, ddy);
+        double dhx = ddx / dist, dhy = ddy / dist;
+        double cth = nx[f] * dhx + ny[f] * dhy;
+        fdc[i] = dist * cth;
+        fkx[i] = nx[f] - dhx / cth;
+        fky[i] = ny[f] - dhy / cth;
+        // distance weights, src/GradientReconstruction.cpp:103-106 and :174-176
+        if (cr >= 0) {
+            double dL = std::hypot(xL - mx[f], yL - my[f]);
+            double dR = std::hypot(xR - mx[f], yR - my[f]);
+            fwl[i] = dR / (dL + dR);
+            fwr[i] = dL / (dR + dL);
+        } else {
+            fwl[i] = 1.0;
+            fwr[i] = 0.0;
+        }
+    }
+    s->face_nodes_dev = fnodes;
+
+    // ---- upload ----
+    MeshView& m = s->mesh;
+    m.nc = (int)nc;
+    m.nf = (int)nf;
+    m.nfi = nfi;
+    m.nnz = (int)nnz;
+    m.vol = upload(s, vol);
+    m.cx = upload(s, cx);
+    m.cy = upload(s, cy);
+    m.lsq = upload(s, lsq);
+    m.cptr = upload(s, cptr);
+    m.cent = upload(s, cent);
+    m.fl = upload(s, fl);
+    m.fr = upload(s, fr);
+    m.fnx = upload(s, fnx);
+    m.fny = upload(s, fny);
+    m.fa = upload(s, fa);
+    m.fmx = upload(s, fmx);
+    m.fmy = upload(s, fmy);
+    m.fsx = sh ? upload(s, fsx) : nullptr;
+    m.fsy = sh ? upload(s, fsy) : nullptr;
+    m.fdl = upload(s, fdl);
+    m.fdc = upload(s, fdc);
+    m.fkx = upload(s, fkx);
+    m.fky = upload(s, fky);
+    m.fwl = upload(s, fwl);
+    m.fwr = upload(s, fwr);
+    std::vector<DevBC> hb(std::max(1, s->n_bcs));
+    for (int i = 0; i < s->n_bcs; ++i) {
+        const fvk_bc& b = s->bcs[i];
+        DevBC& o = hb[i];
+        o.type = b.type;
+        o.isothermal = b.is_isothermal_wall;
+        o.wall_u = b.wall_u; o.wall_v = b.wall_v; o.wall_T = b.wall_temperature;
+        for (int q = 0; q < 4; ++q) o.ff[q] = b.farfield_state[q];
+        o.ff_nut = b.farfield_nut; o.ff_k = b.farfield_k; o.ff_w = b.farfield_omega;
+        o.value = b.value;
+    }
+    m.bc = upload(s, hb);
+    if (!m.vol || !m.cptr || !m.cent || !m.fl || !m.bc)
+        return set_error(FVK_ERR_NOMEM, "device allocation for the mesh failed");
+    return FVK_OK;
+}
+
+}  // namespace fvk

@@ -1,0 +1,138 @@
+// Internal definitions of the fvk library: the device mesh layout, the
+// solver object behind the opaque fvk_solver handle, and the kernel
+// launchers.  Host code only includes this; kernels see the View structs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "fvk.h"
+
+namespace fvk {
+
+constexpr int kMaxPatches = 64;
+
+// Per-patch boundary data on the device (fvk_bc reduced to what kernels read).
+struct DevBC {
+    int type;          // BC_WALL / BC_FARFIELD / BC_OUTFLOW (diffusion: 0 Dirichlet, 1 Neumann)
+    int isothermal;
+    double wall_u, wall_v, wall_T;
+    double ff[4];      // farfield conserved state
+    double ff_nut, ff_k, ff_w;
+    double value;
+};
+
+// Read-only mesh arrays in device order (cells RCM-renumbered; faces
+// interior-first, sorted by their lower cell; boundary faces grouped by
+// boundary type).  Face orientation (cell_left, normal) is the caller's.
+struct MeshView {
+    int nc, nf, nfi;        // cells, faces, interior faces (faces [0, nfi) are interior)
+    int nnz;                // CSR entries
+    const double* vol;      // [nc]
+    const double* cx;       // [nc] centroid x
+    const double* cy;       // [nc] centroid y
+    const double* lsq;      // [3][nc] LSQ inverse {m11, -m01, m00}/det (GradientCalculator ctor)
+    const int* cptr;        // [nc+1]
+    const int* cent;        // [nnz] (device face << 1) | (cell is the face's owner)
+    const int* fl;          // [nf] cell_left
+    const int* fr;          // [nf] cell_right, or -1 - patch on boundary faces
+    const double* fnx;      // [nf]
+    const double* fny;      // [nf]
+    const double* fa;       // [nf] face length
+    const double* fmx;      // [nf] face midpoint
+    const double* fmy;      // [nf]
+    const double* fsx;      // [nf] periodic shift of cell_right (nullptr: none)
+    const double* fsy;      // [nf]
+    const double* fdl;      // [nf] Euler dt length: min(V_L, V_R)/A (boundary V_L/A)
+    const double* fdc;      // [nf] dist * cos(theta) = face_normal_distance (Jasak)
+    const double* fkx;      // [nf] n - dhat / cos(theta)
+    const double* fky;      // [nf]
+    const double* fwl;      // [nf] d_R / (d_L + d_R): distance weight of cell_left at the face
+    const double* fwr;      // [nf] d_L / (d_L + d_R)
+    const double* wdist;    // [nc] wall distance (RANS)
+    const DevBC* bc;        // [n_patches]
+};
+
+struct KernelTimer {
+    std::string name;
+    std::vector<hipEvent_t> start, stop;  // pending pairs
+    double total_ms = 0.0;
+    int64_t count = 0;
+};
+
+}  // namespace fvk
+
+struct fvk_solver {
+    fvk_params p;
+    int nv = 4;                 // variables per cell
+    int n_bcs = 0;
+    int64_t nc = 0, nf = 0, nn = 0;
+    int nfi = 0, nnz = 0;
+    bool periodic = false;
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // host copies
+    std::vector<int> cell_new2old, cell_old2new;
+    std::vector<int> face_new2old, face_old2new;
+    std::vector<fvk_bc> bcs;
+    std::vector<double> node_xy;                // for adjoint geometry
+    std::vector<int> face_nodes_dev;            // [2*nf] device face order
+    std::vector<int> cell_ptr_old, cell_faces_old;
+
+    // device arrays (owned)
+    std::vector<void*> allocations;
+    fvk::MeshView mesh{};
+    double* U[3] = {nullptr, nullptr, nullptr}; // state buffers [nv][nc_pad] (RK ping-pong: A, B, C)
+    int cur = 0;                                 // which buffer holds the current state
+    int64_t ld = 0;                              // leading dimension of SoA state arrays (padded nc)
+    int64_t ldf = 0;                             // padded nf
+    double* flux = nullptr;                      // [nv][ldf] face fluxes
+    double* dtf = nullptr;                       // [ldf] per-face dt candidates (local dt)
+    double* dtc = nullptr;                       // [ld] per-cell dt
+    double* grad = nullptr;                      // [ng][ld] cell gradients
+    double* aux = nullptr;                       // [na][ld] per-cell auxiliaries (sources, nu_t, F1 ...)
+    double* partials = nullptr;                  // [nv][max_blocks]
+    unsigned int* ticket = nullptr;
+    unsigned long long* dtmin = nullptr;         // [2] global dt (bits) per step parity
+    double* res_hist = nullptr;                  // [hist_cap][nv]
+    double* dt_hist = nullptr;                   // [hist_cap]
+    int hist_cap = 0;
+    int64_t step_count = 0;
+    double last_res[6] = {0, 0, 0, 0, 0, 0};
+    int64_t device_bytes = 0;
+
+    // timing
+    bool timing = false;
+    std::vector<fvk::KernelTimer> timers;
+};
+
+namespace fvk {
+
+// ---- layout.cpp ----
+int build_layout(fvk_solver* s, const fvk_mesh_desc* d);
+void rcm_order(int n, const std::vector<int>& ptr, const std::vector<int>& adj, std::vector<int>& order);
+
+// ---- api helpers ----
+int set_error(int code, const char* fmt, ...);
+int hip_check(hipError_t e, const char* what);
+void* dev_alloc(fvk_solver* s, size_t bytes);
+
+// ---- kernel launchers (kernels_*.hip) ----
+int launch_euler_stage(fvk_solver* s, int stage, int stages, int dt_mode_first, int want_res, int64_t step);
+int launch_visc_stage(fvk_solver* s, int stage, int stages, int want_res, int64_t step);
+int launch_diffusion_step(fvk_solver* s, int want_res, int64_t step);
+int launch_gradient(fvk_solver* s, int scheme, const double* phi, const double* bphi_dev, double* grad);
+int launch_wall_distance(fvk_solver* s, const int* wall_faces, int nw, const double* node_xy_dev,
+                         const int* fnodes_dev, double* out);
+int launch_residual_eval(fvk_solver* s, double* R_dev);
+int launch_copy_probe(double* dst, const double* src, int64_t n, hipStream_t st);
+
+// timers
+void timer_begin(fvk_solver* s, const char* name, hipEvent_t* ev_stop);
+void timer_end(fvk_solver* s, hipEvent_t ev_stop);
+
+}  // namespace fvk

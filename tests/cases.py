@@ -1,0 +1,119 @@
+"""Standard test cases shared by fixture generation (tests/golden), the CPU
+tests of the oracle and the GPU parity tests: each returns the mesh, the
+boundary conditions, the parameters and the initial condition, with the
+values the reference's own verification gates use where there is one
+(src/main.cpp gates cited per case)."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from finitevolume_amd import abi
+from finitevolume_amd import mesh as M
+
+GAMMA = 1.4
+
+
+def freestream(mach, aoa_deg, gamma=GAMMA, rho=1.0, p=None):
+    """rho_inf = 1, p_inf = 1/gamma (c_inf = 1) unless p is given."""
+    p = 1.0 / gamma if p is None else p
+    c = math.sqrt(gamma * p / rho)
+    a = math.radians(aoa_deg)
+    u, v = mach * c * math.cos(a), mach * c * math.sin(a)
+    return rho, u, v, p
+
+
+def case_1r(n=64, flux=abi.FLUX_RUSANOV, **ext):
+    """Config 1 (reference mode 1R): 64x64 periodic unit square, isentropic
+    vortex (strength 5, radius 0.1) on a Mach 0.5 stream at 45 degrees,
+    first order, global CFL 0.5."""
+    m = M.periodic_box(n, n)
+    U0 = M.isentropic_vortex(m.cell_xy)
+    p = abi.default_params(equation=abi.EQ_EULER, flux_scheme=flux, cfl=0.5, **ext)
+    return m, [], p, None, U0
+
+
+def case_sod(flux=abi.FLUX_HLLC, nx=64, ny=8):
+    """Sod tube between slip walls (euler_init tworegion), x0 = 0.5."""
+    m = M.cartesian(nx, ny)
+    bcs = [abi.make_bc(abi.BC_WALL) for _ in range(4)]
+    p = abi.default_params(equation=abi.EQ_EULER, flux_scheme=flux, cfl=0.5)
+    ic = abi.initial_condition(abi.IC_TWO_REGION, x0=0.5)
+    return m, bcs, p, ic, None
+
+
+def case_naca(ni=128, nj=64, flux=abi.FLUX_HLLC, mach=0.8, aoa=1.25, cfl=0.8, code="0012", first=None, **ext):
+    """Config 2 geometry at reduced size: NACA O-grid (slip wall + farfield),
+    freestream Mach 0.8, AoA 1.25 deg.  Reference mode 2R is the default
+    (first order, forward Euler, global dt)."""
+    m = M.naca_ogrid(ni, nj, code=code, first=first)
+    rho, u, v, pinf = freestream(mach, aoa)
+    ff = M.from_primitive(rho, u, v, pinf, GAMMA)
+    bcs = [abi.make_bc(abi.BC_WALL), abi.make_bc(abi.BC_FARFIELD, ff)]
+    p = abi.default_params(equation=abi.EQ_EULER, flux_scheme=flux, cfl=cfl, **ext)
+    ic = abi.initial_condition(abi.IC_FREESTREAM, rho=rho, u=u, v=v, p=pinf)
+    return m, bcs, p, ic, None
+
+
+def case_ns_wavy(gradient=abi.GRAD_LEAST_SQUARES, flux=abi.FLUX_HLLC, n=16, **ext):
+    """Navier-Stokes on the wavy mesh of the gradient gate (src/main.cpp:472-478):
+    isothermal no-slip bottom, farfield left/top, outflow right."""
+    m = M.skewed(n, 0.15)
+    ff = M.from_primitive(1.0, 0.3, 0.1, 1.0, GAMMA)
+    bcs = [abi.make_bc(abi.BC_WALL, is_isothermal_wall=1, wall_temperature=1.2), abi.make_bc(abi.BC_FARFIELD, ff),
+           abi.make_bc(abi.BC_FARFIELD, ff), abi.make_bc(abi.BC_OUTFLOW)]
+    for b in bcs:
+        b.farfield_nut, b.farfield_k, b.farfield_omega = 0.05, 2e-3, 3.0
+    p = abi.default_params(equation=abi.EQ_NAVIER_STOKES, flux_scheme=flux, gradient_scheme=gradient, mu=0.01,
+                           cfl=0.4, **ext)
+    ic = abi.initial_condition(abi.IC_FREESTREAM, rho=1.0, u=0.3, v=0.1, p=1.0, initial_nut=0.06, initial_k=1e-3,
+                               initial_omega=5.0)
+    return m, bcs, p, ic, None
+
+
+def case_rans_wavy(equation, gradient=abi.GRAD_LEAST_SQUARES, variant=0, kato=0, n=16, **ext):
+    m, bcs, p, ic, _ = case_ns_wavy(gradient=gradient, n=n, **ext)
+    p.equation = equation
+    p.sst_variant = variant
+    p.kato_launder = kato
+    return m, bcs, p, ic, None
+
+
+def case_couette(equation=abi.EQ_NAVIER_STOKES, n_x=1, n_y=16):
+    """--verify-couette / --verify-rans-stability setup (src/main.cpp:1701-1730,
+    2802-2830): 1x16 sheared mesh (shear 0.3), stationary bottom wall, top wall
+    at U = 0.1, outflow sides, mu 0.02, Rusanov, LSQ, cfl 0.3."""
+    m = M.sheared(n_x, n_y, 0.3)
+    bcs = [abi.make_bc(abi.BC_WALL), abi.make_bc(abi.BC_OUTFLOW), abi.make_bc(abi.BC_WALL, wall_u=0.1),
+           abi.make_bc(abi.BC_OUTFLOW)]
+    p = abi.default_params(equation=equation, flux_scheme=abi.FLUX_RUSANOV, gradient_scheme=abi.GRAD_LEAST_SQUARES,
+                           mu=0.02, prandtl=0.72, prandtl_t=0.9, cfl=0.3)
+    nu = 0.02
+    ic = abi.initial_condition(abi.IC_FREESTREAM, rho=1.0, u=0.0, v=0.0, p=1.0, initial_nut=3.0 * nu,
+                               initial_k=1e-4, initial_omega=10.0)
+    return m, bcs, p, ic, None
+
+
+def case_diffusion(n=16):
+    """UnstructuredFVMSolver on the wavy mesh: Dirichlet bottom/top, Neumann sides."""
+    m = M.skewed(n, 0.15)
+    bcs = [abi.make_bc(abi.BC_DIRICHLET, value=1.0), abi.make_bc(abi.BC_NEUMANN, value=0.3),
+           abi.make_bc(abi.BC_DIRICHLET, value=0.0), abi.make_bc(abi.BC_NEUMANN, value=0.0)]
+    p = abi.default_params(equation=abi.EQ_DIFFUSION, alpha=0.1, fixed_dt=1e-3)
+    ic = abi.initial_condition(abi.IC_FREESTREAM, initial_value=2.0, initial_radius=0.5)
+    return m, bcs, p, ic, None
+
+
+def rel_state_diff(a, b, floor=1e-300):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+
+
+def residual_close(r_gpu, r_ref, rtol=1e-10):
+    """|r - r_ref| <= rtol |r_ref| + rtol r_ref(step 1), per component."""
+    r_gpu = np.asarray(r_gpu)
+    r_ref = np.asarray(r_ref)
+    floor = rtol * np.abs(r_ref[0])[None, :]
+    return bool(np.all(np.abs(r_gpu - r_ref) <= rtol * np.abs(r_ref) + floor))
