@@ -1,0 +1,240 @@
+"""Benchmark of the explicit FP64 residual path on one MI355X.
+
+Workload: second-order Euler, HLLC, weighted least-squares
+primitive gradients with the Venkatakrishnan limiter (K = 5), SSP-RK3 with
+local time stepping at CFL 0.8, on a 1024 x 512 NACA 0012 O-grid split into
+1,048,576 triangles (far field 50 chords, wall-normal ratio 1.05), slip wall
+and far field, M 0.8 at 1.25 degrees.  Synthetic input: the mesh is
+generated, the flow starts from the free stream.
+
+One step = one call of the solver's step() followed by the residual read
+back, as the reference's driver loop does (src/main.cpp:3491-3515); a
+cell-update is one cell through one RK stage.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+from finitevolume_amd import abi  # noqa: E402
+from finitevolume_amd import mesh as M  # noqa: E402
+
+METRIC = "FP64 residual Gcell-updates/s (1 GPU, % HBM roofline); adjoint/primal cost"
+HBM_PEAK_GBS = 8000.0
+GAMMA = 1.4
+
+# Algorithmic bytes per cell-update (per cell per RK stage): every state array
+# read and written once, the RK base state on stages 2-3, per-cell static data
+# (volume 8, centroid 16, LSQ inverse 24), per-face static data (owner and
+# neighbour 8, normal and length 24, midpoint 16) times faces per cell, and the
+# local dt written on stage 1 and read on stages 2-3.
+def b_alg_config2(faces_per_cell: float) -> float:
+    stage = 32 + 32 + 8 + 16 + 24 + faces_per_cell * 48 + 8
+    return stage + (32 * 2 + 8) / 3.0
+
+
+def b_alg_2r(faces_per_cell: float) -> float:
+    return 32 + 32 + 8 + faces_per_cell * 32
+
+
+def naca_case(ni, nj, order2=True):
+    m = M.naca_ogrid(ni, nj, code="0012", r_far=50.0, ratio=1.05)
+    c = 1.0
+    a = math.radians(1.25)
+    rho, p = 1.0, 1.0 / GAMMA
+    u, v = 0.8 * c * math.cos(a), 0.8 * c * math.sin(a)
+    ff = M.from_primitive(rho, u, v, p, GAMMA)
+    bcs = [abi.make_bc(abi.BC_WALL), abi.make_bc(abi.BC_FARFIELD, ff)]
+    if order2:
+        params = abi.default_params(equation=abi.EQ_EULER, flux_scheme=abi.FLUX_HLLC, cfl=0.8, order=2,
+                                    limiter=abi.LIMITER_VENKATAKRISHNAN, venkat_k=5.0, rk_stages=3, local_dt=1)
+    else:
+        params = abi.default_params(equation=abi.EQ_EULER, flux_scheme=abi.FLUX_HLLC, cfl=0.8)
+    ic = abi.initial_condition(abi.IC_FREESTREAM, rho=rho, u=u, v=v, p=p)
+    return m, bcs, params, ic
+
+
+def time_gpu(solver, steps, warmup, per_step=True):
+    solver.step(warmup)
+    t0 = time.perf_counter()
+    if per_step:
+        for _ in range(steps):
+            solver.step(1)
+            r = solver.residual()
+            if not np.all(np.isfinite(r)):
+                raise RuntimeError(f"residual not finite: {r}")
+    else:
+        solver.step(steps)
+    return time.perf_counter() - t0
+
+
+def host_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    return int(omp) if omp and omp.isdigit() else n
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--ni", type=int, default=1024)
+    ap.add_argument("--nj", type=int, default=512)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-2r", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is a design, not a build, in this repository)", file=sys.stderr)
+        sys.exit(2)
+
+    from finitevolume_amd.solver import FVKSolver
+
+    m, bcs, params, ic = naca_case(args.ni, args.nj, order2=True)
+    nc, nf = m.n_cells, m.n_faces
+    fpc = nf / nc
+    stages = params.rk_stages
+    print(f"[bench] mesh {nc} cells {nf} faces; building solver", file=sys.stderr, flush=True)
+    s = FVKSolver(m, bcs, params, ic)
+    wall = time_gpu(s, args.steps, args.warmup, per_step=True)
+    ms_step = wall / args.steps * 1e3
+    cups = nc * stages * args.steps / wall
+    value = cups / 1e9
+    print(f"[bench] {args.steps} steps {wall:.3f}s {ms_step:.3f} ms/step {value:.3f} Gcell-updates/s",
+          file=sys.stderr, flush=True)
+
+    # device time per kernel with HIP events on the solver's stream, over a
+    # second timed region of the same length
+    s.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step(1)
+        s.residual()
+    wall_ev = time.perf_counter() - t0
+    times = s.kernel_times()
+    s.set_kernel_timing(False)
+    per_launch = {k: tot / cnt for k, (tot, cnt) in times.items() if cnt}
+    stage_ms = sum(per_launch.values())  # one launch of each kernel per stage
+    b_alg = b_alg_config2(fpc)
+    achieved = b_alg * nc / (stage_ms * 1e-3) / 1e9
+    dominant = max(per_launch, key=per_launch.get)
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": "one RK stage = " + " + ".join(sorted(per_launch)),
+        "b_alg_per_cell_update": round(b_alg, 1),
+        "stage_device_ms": round(stage_ms, 4),
+        "kernel_avg_ms": {k: round(v, 5) for k, v in sorted(per_launch.items())},
+        "dominant_kernel": dominant,
+        "wall_with_events_s": round(wall_ev, 3),
+    }
+    prof = os.path.join(REPO, "profiles", "traffic_config2.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            tr = json.load(f)
+        roofline["traffic"] = tr.get("stage_bytes")
+        roofline["traffic_per_cell_update"] = tr.get("bytes_per_cell_update")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated NACA 0012 O-grid, free-stream start)",
+        "config": {"workload": "config2: 2nd-order Euler HLLC + LSQ/Venkatakrishnan K=5, SSP-RK3, local dt CFL 0.8, "
+                               "NACA 0012 O-grid 1024x512 split to triangles",
+                   "cells": nc, "faces": nf, "rk_stages": stages, "ni": args.ni, "nj": args.nj},
+        "roofline": roofline,
+    }
+    del s
+
+    # reference mode 2R on the GPU: first-order HLLC, forward Euler, global dt
+    if not args.no_2r:
+        m2, bcs2, p2, ic2 = naca_case(args.ni, args.nj, order2=False)
+        s2 = FVKSolver(m2, bcs2, p2, ic2)
+        w2 = time_gpu(s2, args.steps, args.warmup, per_step=True)
+        out["gpu_2R"] = {"value": round(nc * args.steps / w2 / 1e9, 4), "unit": "Gcell-updates/s",
+                         "ms_per_step": round(w2 / args.steps * 1e3, 4),
+                         "roofline_frac": round(b_alg_2r(fpc) * nc * args.steps / w2 / 1e9 / HBM_PEAK_GBS, 4)}
+        del s2
+
+    if not args.no_cpu:
+        cores = host_cores()
+        try:
+            from oracle import pyoracle
+            o = pyoracle.OracleSolver(m, bcs, params, ic)
+            o.step(1)
+            t0 = time.perf_counter()
+            o.step(args.cpu_steps)
+            tc = time.perf_counter() - t0
+            cpu_cups = nc * stages * args.cpu_steps / tc
+            out["cpu_baseline"] = {"value": round(cpu_cups / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores,
+                                   "kind": "port",
+                                   "sample": f"oracle C restatement (OpenMP, {cores} threads, {cpu_model()}) on the "
+                                             f"same config-2 mesh and numerics, {args.cpu_steps} RK3 steps after "
+                                             f"1 warm-up step ({tc:.2f} s)"}
+            out["gpu_over_cpu"] = round(cups / cpu_cups, 1)
+            del o
+        except Exception as e:  # the baseline is reported, not required
+            out["cpu_baseline"] = {"value": None, "unit": "Gcell-updates/s", "cores": cores, "kind": "port",
+                                   "sample": f"unavailable: {e}"}
+        try:
+            from oracle import pyref
+            if pyref.available("fast") and "gpu_2R" in out:
+                m2, bcs2, p2, ic2 = naca_case(args.ni, args.nj, order2=False)
+                r = pyref.RefSolver(m2, bcs2, p2, ic2, flavour="fast")
+                r.step(2)
+                n2 = 10
+                t0 = time.perf_counter()
+                r.step(n2)
+                tr2 = time.perf_counter() - t0
+                ref_cups = nc * n2 / tr2
+                out["cpu_baseline_reference_2R"] = {
+                    "value": round(ref_cups / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores, "kind": "reference",
+                    "sample": f"reference solver built from its own sources with its Release flags "
+                              f"(-O2 -ffast-math -flto -fopenmp, {cores} threads), reference mode 2R "
+                              f"(first-order HLLC, forward Euler, global dt) on the same mesh, {n2} steps ({tr2:.2f} s)"}
+                out["gpu_2R_over_reference_2R"] = round(out["gpu_2R"]["value"] * 1e9 / ref_cups, 1)
+                del r
+        except Exception as e:
+            out["cpu_baseline_reference_2R"] = {"value": None, "sample": f"unavailable: {e}"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

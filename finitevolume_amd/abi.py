@@ -18,7 +18,7 @@ BC_WALL, BC_FARFIELD, BC_OUTFLOW = 0, 1, 2
 BC_DIRICHLET, BC_NEUMANN = 0, 1
 SST_VORTICITY, SST_STRAIN_RATE = 0, 1
 IC_FREESTREAM, IC_TWO_REGION, IC_NONE = 0, 1, 2
-LIMITER_NONE, LIMITER_VENKATAKRISHNAN = 0, 1
+LIMITER_NONE, LIMITER_VENKATAKRISHNAN, LIMITER_FIRST_ORDER = 0, 1, 2
 
 FLUX_NAMES = {"rusanov": FLUX_RUSANOV, "hllc": FLUX_HLLC, "exact": FLUX_EXACT}
 

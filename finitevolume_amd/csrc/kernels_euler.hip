@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kBlock) void k_euler_grad(EulerArgs a) {
         gy[q] = i1 * bx[q] + i2 * by[q];
         lim[q] = 1.0;
     }
-    if (a.limiter) {
+    if (a.limiter == FVK_LIMITER_VENKATAKRISHNAN) {
         double kh = a.venk_k * sqrt(m.vol[c]);
         double eps2 = kh * kh * kh;
         for (int k = k0; k < k1; ++k) {
@@ -144,6 +144,10 @@ __global__ __launch_bounds__(kBlock) void k_euler_grad(EulerArgs a) {
                 lim[q] = smin(lim[q], ph);
             }
         }
+    }
+    if (a.limiter == FVK_LIMITER_FIRST_ORDER) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lim[q] = 0.0;
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -372,7 +376,7 @@ EulerArgs make_args(fvk_solver* s) {
     a.tol = s->p.exact_riemann_tol;
     a.maxit = s->p.exact_riemann_max_iter;
     a.venk_k = s->p.venkat_k;
-    a.limiter = s->p.limiter == FVK_LIMITER_VENKATAKRISHNAN;
+    a.limiter = s->p.limiter;
     return a;
 }
 

@@ -758,6 +758,8 @@ static void limited_gradients(fvo_solver* s, const double* U) {
                 }
             }
         }
+        if (s->p.limiter == FVK_LIMITER_FIRST_ORDER)
+            for (int q = 0; q < 4; ++q) lim[q] = 0.0;
         for (int q = 0; q < 4; ++q) {
             s->grad8[8 * c + 2 * q] = lim[q] * gx[q];
             s->grad8[8 * c + 2 * q + 1] = lim[q] * gy[q];

@@ -106,14 +106,24 @@ def case_diffusion(n=16):
 
 
 def rel_state_diff(a, b, floor=1e-300):
-    a = np.asarray(a)
-    b = np.asarray(b)
-    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), floor)))
+    """Per-cell norm-wise relative difference max_k |a_k - b_k| / max_k |b_k|,
+    maximised over cells (a component that is zero by symmetry is judged
+    against the cell's state scale, not against its own round-off)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.ndim == 1:
+        a, b = a[:, None], b[:, None]
+    num = np.max(np.abs(a - b), axis=1)
+    den = np.maximum(np.max(np.abs(b), axis=1), floor)
+    return float(np.max(num / den))
 
 
 def residual_close(r_gpu, r_ref, rtol=1e-10):
-    """|r - r_ref| <= rtol |r_ref| + rtol r_ref(step 1), per component."""
+    """|r - r_ref| <= rtol |r_ref| + rtol max_k r_ref(step 1)_k.  The absolute
+    floor uses the largest first-step component: a component that is zero by
+    symmetry (rho_v in a 1D problem) carries only round-off, which differs at
+    O(1) relatively even between two builds of the reference."""
     r_gpu = np.asarray(r_gpu)
     r_ref = np.asarray(r_ref)
-    floor = rtol * np.abs(r_ref[0])[None, :]
+    floor = rtol * np.max(np.abs(r_ref[0]))
     return bool(np.all(np.abs(r_gpu - r_ref) <= rtol * np.abs(r_ref) + floor))
