@@ -77,6 +77,18 @@ def time_gpu(solver, steps, warmup, per_step=True):
     return time.perf_counter() - t0
 
 
+def timed_steps(solver, seconds):
+    """Whole CPU steps until `seconds` of wall time have passed (at least one)."""
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        solver.step(1)
+        n += 1
+        t = time.perf_counter() - t0
+        if t >= seconds:
+            return n, t
+
+
 def host_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -104,19 +116,46 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--ni", type=int, default=1024)
     ap.add_argument("--nj", type=int, default=512)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps timed for the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="wall time of each bounded CPU-baseline sample (whole steps, at least one)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-2r", action="store_true")
+    ap.add_argument("--variant", type=int, default=0, help="Euler kernel variant (0 = library default)")
+    ap.add_argument("--block", type=int, default=0, help="Euler workgroup size (0 = library default)")
+    ap.add_argument("--sweep", action="store_true", help="time every kernel variant and block size first")
+    ap.add_argument("--probe", action="store_true", help="run a plain 256 MiB device copy first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding is a design, not a build, in this repository)", file=sys.stderr)
         sys.exit(2)
 
-    from finitevolume_amd.solver import FVKSolver
+    from finitevolume_amd.solver import FVKSolver, copy_probe
+
+    probe = None
+    if args.probe:
+        n = 32 * 1024 * 1024  # doubles: 256 MiB read + 256 MiB written per copy
+        ms = copy_probe(n, 5)
+        probe = {"bytes_per_copy": 2 * 8 * n, "ms_per_copy": round(ms, 4), "GBps": round(2 * 8 * n / ms / 1e6, 1)}
+        print(f"[bench] copy probe {probe}", file=sys.stderr, flush=True)
 
     m, bcs, params, ic = naca_case(args.ni, args.nj, order2=True)
+    params.kernel_variant = args.variant
+    params.block_size = args.block
     nc, nf = m.n_cells, m.n_faces
+    sweep = {}
+    if args.sweep:
+        for variant in (1, 2):
+            for bs in (64, 128, 256, 512):
+                params.kernel_variant, params.block_size = variant, bs
+                sv = FVKSolver(m, bcs, params, ic)
+                w = time_gpu(sv, 50, 5, per_step=False)
+                sweep[f"v{variant}_b{bs}"] = round(w / 50 * 1e3, 4)
+                del sv
+                print(f"[bench] sweep variant {variant} block {bs}: {sweep[f'v{variant}_b{bs}']} ms/step",
+                      file=sys.stderr, flush=True)
+        best = min(sweep, key=sweep.get)
+        params.kernel_variant, params.block_size = int(best[1]), int(best.split("_b")[1])
     fpc = nf / nc
     stages = params.rk_stages
     print(f"[bench] mesh {nc} cells {nf} faces; building solver", file=sys.stderr, flush=True)
@@ -139,7 +178,8 @@ def main():
     times = s.kernel_times()
     s.set_kernel_timing(False)
     per_launch = {k: tot / cnt for k, (tot, cnt) in times.items() if cnt}
-    stage_ms = sum(per_launch.values())  # one launch of each kernel per stage
+    # device time of one RK stage: all kernels of the timed steps / stages
+    stage_ms = sum(tot for tot, _ in times.values()) / (args.steps * stages)
     b_alg = b_alg_config2(fpc)
     achieved = b_alg * nc / (stage_ms * 1e-3) / 1e9
     dominant = max(per_launch, key=per_launch.get)
@@ -178,9 +218,14 @@ def main():
         "data": "synthetic (generated NACA 0012 O-grid, free-stream start)",
         "config": {"workload": "config2: 2nd-order Euler HLLC + LSQ/Venkatakrishnan K=5, SSP-RK3, local dt CFL 0.8, "
                                "NACA 0012 O-grid 1024x512 split to triangles",
-                   "cells": nc, "faces": nf, "rk_stages": stages, "ni": args.ni, "nj": args.nj},
+                   "cells": nc, "faces": nf, "rk_stages": stages, "ni": args.ni, "nj": args.nj,
+                   "kernel_variant": params.kernel_variant, "block_size": params.block_size},
         "roofline": roofline,
     }
+    if sweep:
+        out["sweep_ms_per_step"] = sweep
+    if probe:
+        out["copy_probe"] = probe
     del s
 
     # reference mode 2R on the GPU: first-order HLLC, forward Euler, global dt
@@ -199,14 +244,12 @@ def main():
             from oracle import pyoracle
             o = pyoracle.OracleSolver(m, bcs, params, ic)
             o.step(1)
-            t0 = time.perf_counter()
-            o.step(args.cpu_steps)
-            tc = time.perf_counter() - t0
-            cpu_cups = nc * stages * args.cpu_steps / tc
+            n_cpu, tc = timed_steps(o, args.cpu_seconds)
+            cpu_cups = nc * stages * n_cpu / tc
             out["cpu_baseline"] = {"value": round(cpu_cups / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores,
                                    "kind": "port",
                                    "sample": f"oracle C restatement (OpenMP, {cores} threads, {cpu_model()}) on the "
-                                             f"same config-2 mesh and numerics, {args.cpu_steps} RK3 steps after "
+                                             f"same config-2 mesh and numerics, {n_cpu} RK3 steps after "
                                              f"1 warm-up step ({tc:.2f} s)"}
             out["gpu_over_cpu"] = round(cups / cpu_cups, 1)
             del o
@@ -219,10 +262,7 @@ def main():
                 m2, bcs2, p2, ic2 = naca_case(args.ni, args.nj, order2=False)
                 r = pyref.RefSolver(m2, bcs2, p2, ic2, flavour="fast")
                 r.step(2)
-                n2 = 10
-                t0 = time.perf_counter()
-                r.step(n2)
-                tr2 = time.perf_counter() - t0
+                n2, tr2 = timed_steps(r, args.cpu_seconds)
                 ref_cups = nc * n2 / tr2
                 out["cpu_baseline_reference_2R"] = {
                     "value": round(ref_cups / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores, "kind": "reference",

@@ -1,6 +1,7 @@
 // C ABI of the fvk library (include/fvk.h): solver lifetime, state
 // transfer in reference cell order, stepping, and the standalone entry
 // points.  One HIP stream per solver; kernels are in kernels_*.hip.
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
@@ -149,6 +150,12 @@ static int upload_state(fvk_solver* s, const double* U4, const double* turb, boo
     if (!U4) return set_error(FVK_ERR_INVALID, "state pointer is NULL");
     if (s->p.equation == FVK_EQ_DIFFUSION) {
         for (int64_t i = 0; i < nc; ++i) h[i] = U4[s->cell_new2old[i]];
+    } else if (s->p.equation == FVK_EQ_EULER) {
+        // Euler: one double4 per cell
+        for (int64_t i = 0; i < nc; ++i) {
+            const double* u = U4 + 4 * (int64_t)s->cell_new2old[i];
+            for (int q = 0; q < 4; ++q) h[4 * i + q] = u[q];
+        }
     } else {
         for (int64_t i = 0; i < nc; ++i) {
             const double* u = U4 + 4 * (int64_t)s->cell_new2old[i];
@@ -194,9 +201,17 @@ int fvk_create(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, cons
     if (n_bcs < 0 || n_bcs > kMaxPatches) return set_error(FVK_ERR_INVALID, "n_bcs out of range (%d)", n_bcs);
     if (n_bcs > 0 && !bcs) return set_error(FVK_ERR_INVALID, "bcs is NULL");
 
+    if (params->kernel_variant < 0 || params->kernel_variant > 2)
+        return set_error(FVK_ERR_INVALID, "kernel_variant must be 0, 1 or 2 (got %d)", params->kernel_variant);
+    if (params->block_size != 0 && params->block_size != 64 && params->block_size != 128 &&
+        params->block_size != 256 && params->block_size != 512)
+        return set_error(FVK_ERR_INVALID, "block_size must be 0, 64, 128, 256 or 512 (got %d)", params->block_size);
+
     fvk_solver* s = new (std::nothrow) fvk_solver;
     if (!s) return set_error(FVK_ERR_NOMEM, "host allocation failed");
     s->p = *params;
+    s->variant = params->kernel_variant == 0 ? 1 : params->kernel_variant;
+    s->block = params->block_size == 0 ? 512 : params->block_size;
     s->nv = nvars_eq(params->equation);
     s->n_bcs = n_bcs;
     s->bcs.assign(bcs, bcs + n_bcs);
@@ -216,22 +231,34 @@ int fvk_create(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, cons
         fvk::g_err = msg;
         return rc;
     }
+    if (params->equation == FVK_EQ_EULER && s->mesh.deg > kMaxEulerDegree) {
+        int deg = s->mesh.deg;
+        fvk_destroy(s);
+        return set_error(FVK_ERR_UNSUPPORTED, "a cell has %d faces; the Euler kernels take at most %d", deg,
+                         kMaxEulerDegree);
+    }
     const int64_t pad = 256;
     s->ld = (s->nc + pad - 1) / pad * pad;
     s->ldf = (s->nf + pad - 1) / pad * pad;
     const int nv = s->nv;
-    int nbuf = params->rk_stages > 1 ? 3 : 1;
+    // Euler reads neighbour states while writing its own: never in place
+    int nbuf = params->rk_stages > 1 ? 3 : (params->equation == FVK_EQ_EULER ? 2 : 1);
     for (int b = 0; b < 3; ++b) s->U[b] = nullptr;
     for (int b = 0; b < nbuf; ++b) s->U[b] = static_cast<double*>(dev_alloc(s, sizeof(double) * nv * s->ld));
     for (int b = nbuf; b < 3; ++b) s->U[b] = s->U[0];
-    s->flux = static_cast<double*>(dev_alloc(s, sizeof(double) * nv * s->ldf));
-    s->dtf = static_cast<double*>(dev_alloc(s, sizeof(double) * s->ldf));
+    const bool euler = params->equation == FVK_EQ_EULER;
+    // the fused Euler variant has no face-flux array: each cell evaluates its faces' fluxes
+    if (!euler || s->variant == 1) {
+        s->flux = static_cast<double*>(dev_alloc(s, sizeof(double) * nv * s->ldf));
+        s->dtf = static_cast<double*>(dev_alloc(s, sizeof(double) * s->ldf));
+    }
     s->dtc = static_cast<double*>(dev_alloc(s, sizeof(double) * s->ld));
-    int ng = params->equation == FVK_EQ_EULER ? (params->order == 2 ? 8 : 0) : (params->equation == FVK_EQ_DIFFUSION ? 0 : 10);
+    int ng = euler ? (params->order == 2 ? 8 : 0) : (params->equation == FVK_EQ_DIFFUSION ? 0 : 10);
     if (ng) s->grad = static_cast<double*>(dev_alloc(s, sizeof(double) * ng * s->ld));
     if (params->equation == FVK_EQ_RANS_SA || params->equation == FVK_EQ_RANS_SST)
         s->aux = static_cast<double*>(dev_alloc(s, sizeof(double) * 4 * s->ld));
-    int64_t max_blocks = (s->nc + 255) / 256;
+    // launch grids (workgroups of 64 threads or more) are rounded up to a multiple of the 8 XCDs
+    int64_t max_blocks = (std::max(s->nc, s->nf) + 63) / 64 + 8;
     s->partials = static_cast<double*>(dev_alloc(s, sizeof(double) * 6 * max_blocks));
     s->ticket = static_cast<unsigned int*>(dev_alloc(s, 256));
     s->dtmin = static_cast<unsigned long long*>(dev_alloc(s, 256));
@@ -239,7 +266,7 @@ int fvk_create(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, cons
     s->res_hist = static_cast<double*>(dev_alloc(s, sizeof(double) * 6 * s->hist_cap));
     s->dt_hist = static_cast<double*>(dev_alloc(s, sizeof(double) * s->hist_cap));
     double* wd = static_cast<double*>(dev_alloc(s, sizeof(double) * s->ld));
-    if (!s->U[0] || !s->flux || !s->dtf || !s->dtc || (ng && !s->grad) || !s->partials || !s->ticket || !s->dtmin ||
+    if (!s->U[0] || ((!euler || s->variant == 1) && (!s->flux || !s->dtf)) || !s->dtc || (ng && !s->grad) || !s->partials || !s->ticket || !s->dtmin ||
         !s->res_hist || !s->dt_hist || !wd || (nbuf == 3 && (!s->U[1] || !s->U[2]))) {
         fvk_destroy(s);
         return set_error(FVK_ERR_NOMEM, "device allocation failed");
@@ -345,9 +372,10 @@ int fvk_get_field(const fvk_solver* sc, double* U4, double* turb, int64_t n_cell
         for (int64_t i = 0; i < nc; ++i) U4[s->cell_new2old[i]] = h[i];
         return FVK_OK;
     }
+    const bool aos = s->p.equation == FVK_EQ_EULER;
     for (int64_t i = 0; i < nc; ++i) {
         double* u = U4 + 4 * (int64_t)s->cell_new2old[i];
-        for (int q = 0; q < 4; ++q) u[q] = h[q * ld + i];
+        for (int q = 0; q < 4; ++q) u[q] = aos ? h[4 * i + q] : h[q * ld + i];
     }
     int nt = nv > 4 ? nv - 4 : 0;
     if (turb && nt) {
@@ -437,7 +465,7 @@ int fvk_residual_eval(fvk_solver* s, double* R) {
     if (rc != FVK_OK) return rc;
     for (int64_t i = 0; i < s->nc; ++i) {
         int64_t c = s->cell_new2old[i];
-        for (int q = 0; q < nv; ++q) R[c * nv + q] = h[q * s->ld + i];
+        for (int q = 0; q < nv; ++q) R[c * nv + q] = s->p.equation == FVK_EQ_EULER ? h[nv * i + q] : h[q * s->ld + i];
     }
     return FVK_OK;
 }

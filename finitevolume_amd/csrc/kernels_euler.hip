@@ -1,13 +1,29 @@
 // Euler residual kernels for gfx950 (FP64, HBM-bound; no MFMA).
 //
-// One residual evaluation + stage update is three launches:
-//   k_euler_grad  (order 2 only)  per cell: primitive LSQ gradients + Venkatakrishnan limiter
-//   k_euler_face                  per face: (reconstruct) + numerical flux x length,
-//                                 plus the step's dt candidates on the first stage
-//   k_euler_cell                  per cell: gather of its faces' fluxes in the reference's
-//                                 Cell::faces order, RK stage update, L2 residual partials
-// The face pass then cell pass split is the reference's race-free structure
-// (src/EulerFVMSolver.cpp:41-57): no atomics on the state, deterministic sums.
+// One residual evaluation + RK stage update, two execution variants with
+// bit-identical results:
+//
+//  variant 1 (face sweep + gather)
+//   k_e_grad    (order 2 only)  per cell: primitive LSQ gradients + Venkatakrishnan limiter
+//   k_e_face    per face: numerical flux x length (one double4 per face) and, on the
+//               first stage, the step's dt candidates
+//   k_e_gather  per cell: the cell's face fluxes in the reference's Cell::faces
+//               order, owner minus (src/EulerFVMSolver.cpp:89-104), stage update,
+//               L2 residual partials
+//  variant 2 (fused)
+//   k_e_dt      (global dt, first stage only)  per face: compute_dt candidate
+//   k_e_grad    as above
+//   k_e_cell    per cell: the flux of every face of the cell, evaluated in the
+//               face's own orientation (owner on the left, stored normal) so both
+//               cells of a face add the same bits, then the update
+//
+// Per-cell kernels are instantiated for the mesh's largest face count D (3,
+// 4, 9 or 16) and fully unrolled: the D face entries are loaded together, then
+// the D neighbours, so a cell waits for two memory round trips rather than
+// one pair per face; faces are still summed in Cell::faces order.  State and
+// gradients are arrays of per-cell vectors (double4): a gathered neighbour
+// costs one or two cache lines.  Blocks are remapped so that each XCD (own
+// L2) sweeps one contiguous range of the RCM-ordered cells or faces.
 #include <cfloat>
 
 #include "physics.h"
@@ -15,40 +31,45 @@
 
 namespace fvk {
 
-constexpr int kBlock = 256;
+constexpr int kXcds = 8;
+constexpr int kMaxBlock = 512;
 
 struct EulerArgs {
     MeshView m;
-    const double* U;      // stage state [4][ld]
-    double* Unew;         // updated state [4][ld] (may alias U for forward Euler)
-    const double* U0;     // RK base state [4][ld]
-    double* G;            // limited primitive gradients [8][ld]
-    double* flux;         // [4][ldf]
-    double* dtf;          // [ldf] per-face dt candidate
+    const double4* U;     // stage state [ld]
+    double4* Unew;        // updated state [ld], never the buffer read as U
+    const double4* U0;    // RK base state [ld]
+    double4* G;           // limited primitive gradients [2*ld]: (rho_x, rho_y, u_x, u_y), (v_x, v_y, p_x, p_y)
+    double4* flux;        // [nf] face flux x length (variant 1)
+    double* dtf;          // [nf] per-face dt candidate (variant 1, local dt)
     double* dtc;          // [ld] per-cell dt
-    double* R;            // optional residual output [4][ld] (residual_eval)
+    double4* R;           // optional residual output [ld] (residual_eval)
     unsigned long long* dtmin;
     double* partials;
-    unsigned int* ticket;
     double* res_hist;
     double* dt_hist;
-    int64_t ld, ldf;
-    double gamma, cfl, tol, venk_k;
+    double gamma, cfl, tol;
     int maxit, limiter;
     double a, b;          // RK combination U' = a U0 + b (U + dt/V R) (stage > 0)
-    int face_dt;          // face pass computes dt candidates (first stage)
-    int dt_src;           // cell pass dt: 0 per-cell dtc, 1 global slot, 2 local min over faces (and store)
+    int face_dt;          // this pass evaluates the step's dt candidates
+    int dt_src;           // 0 per-cell dtc, 1 global slot, 2 local min over the cell's faces (and store)
     int first;            // first stage: U' = U + dt/V R
     int want_res;         // accumulate residual norms
     int slot;             // dt slot of this step
     int64_t hist_idx;
 };
 
-__device__ __forceinline__ State ldU(const double* U, int64_t ld, int c) {
-    return State{U[c], U[ld + c], U[2 * ld + c], U[3 * ld + c]};
+__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+
+// Workgroups are dispatched round-robin over the XCDs; remap so that XCD x
+// gets the contiguous range [x * per, (x + 1) * per) of logical blocks.
+// The grid is a multiple of kXcds.
+__device__ __forceinline__ int xcd_index() {
+    const int per = gridDim.x / kXcds;
+    return ((blockIdx.x % kXcds) * per + blockIdx.x / kXcds) * blockDim.x + threadIdx.x;
 }
 
-__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ __forceinline__ State st(const double4& u) { return State{u.x, u.y, u.z, u.w}; }
 
 // ghost state for the Euler equations, src/EulerFVMSolver.cpp:174-193
 __device__ __forceinline__ State euler_ghost(const DevBC& bc, const State& L, double nx, double ny, double g) {
@@ -57,402 +78,565 @@ __device__ __forceinline__ State euler_ghost(const DevBC& bc, const State& L, do
     return ghost_slip_wall(L, nx, ny, g);
 }
 
+// compute_dt candidate of one face (src/EulerFVMSolver.cpp:139-158), states in
+// the face's orientation; R == nullptr on a boundary face.
+__device__ __forceinline__ double dt_candidate(const State& L, const State* R, double nx, double ny, double len,
+                                               double g, double cfl) {
+    double uL = L.ru / L.r, vL = L.rv / L.r;
+    double VnL = uL * nx + vL * ny;
+    double Smax = fabs(VnL) + sound_speed(L, g);
+    if (R) {
+        double uR = R->ru / R->r, vR = R->rv / R->r;
+        double VnR = uR * nx + vR * ny;
+        Smax = smax(Smax, fabs(VnR) + sound_speed(*R, g));
+    }
+    return cfl * len / Smax;
+}
+
+// block minimum, then one atomic per block.  Positive doubles order like
+// their bit patterns and NaN bits exceed +inf and DBL_MAX, so a NaN
+// candidate never wins (std::min semantics of the reference's reduction).
+__device__ __forceinline__ void block_min_to(double v, unsigned long long* slot) {
+    __shared__ double sm[kMaxBlock];
+    sm[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] = smin(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMin(slot, dbits(sm[0]));
+}
+
+// deterministic block sum of the four squared residual components into
+// partials[q * gridDim.x + blockIdx.x] (fixed tree order); k_e_finalize
+// reduces them after the kernel ends, so no block needs a device-scope fence
+// (a release at agent scope writes the XCD's L2 back)
+__device__ __forceinline__ void block_partials4(const double (&v)[4], double* partials) {
+    __shared__ double sm[4][kMaxBlock];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sm[q][threadIdx.x] = v[q];
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) partials[q * gridDim.x + blockIdx.x] = sm[q][0];
+    }
+}
+
+// one block: residual norms = sqrt of the sums of the partials (fixed order),
+// the step's dt into the history, and the next step's dt slot re-armed
+constexpr int kFinBlock = 256;
+__global__ __launch_bounds__(kFinBlock) void k_e_finalize(const double* partials, int nblocks, int nv,
+                                                          double* res_hist, double* dt_hist, int64_t hist_idx,
+                                                          unsigned long long* dtmin, int slot) {
+    __shared__ double sm[kFinBlock];
+    for (int q = 0; q < nv; ++q) {
+        double acc = 0.0;
+        for (int i = threadIdx.x; i < nblocks; i += kFinBlock) acc += partials[q * nblocks + i];
+        sm[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = kFinBlock / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) res_hist[hist_idx * nv + q] = sqrt(sm[0]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        dt_hist[hist_idx] = __longlong_as_double((long long)dtmin[slot]);
+        dtmin[slot ^ 1] = dbits(DBL_MAX);
+    }
+}
+
+// stage update of one cell: U' = U + dt/V R (src/EulerFVMSolver.cpp:103) on
+// the first stage, a U0 + b (U + dt/V R) on later SSP-RK stages; or the
+// residual itself for residual_eval
+__device__ __forceinline__ void update_cell(const EulerArgs& a, int c, const double4& u, double r0, double r1,
+                                            double r2, double r3, double dmin) {
+    if (a.R) {
+        a.R[c] = double4{r0, r1, r2, r3};
+        return;
+    }
+    double dt;
+    if (a.dt_src == 1) dt = __longlong_as_double((long long)a.dtmin[a.slot]);
+    else if (a.dt_src == 2) { dt = dmin; a.dtc[c] = dmin; }
+    else dt = a.dtc[c];
+    const double h = dt / a.m.vol[c];
+    if (a.first) {
+        a.Unew[c] = double4{u.x + h * r0, u.y + h * r1, u.z + h * r2, u.w + h * r3};
+    } else {
+        const double4 u0 = a.U0[c];
+        a.Unew[c] = double4{a.a * u0.x + a.b * (u.x + h * r0), a.a * u0.y + a.b * (u.y + h * r1),
+                            a.a * u0.z + a.b * (u.z + h * r2), a.a * u0.w + a.b * (u.w + h * r3)};
+    }
+}
+
+// per-block partial sums of the squared residual (L2 norms of the stage)
+__device__ __forceinline__ void finish_residual(const EulerArgs& a, double r0, double r1, double r2, double r3) {
+    if (!a.want_res) return;
+    double sq[4] = {r0 * r0, r1 * r1, r2 * r2, r3 * r3};
+    block_partials4(sq, a.partials);
+}
+
+// ---------------------------------------------------------------------------
+// global dt: compute_dt over all faces (reference mode, variant 2)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kMaxBlock) void k_e_dt(EulerArgs a) {
+    const MeshView& m = a.m;
+    const int f = xcd_index();
+    double cand = DBL_MAX;
+    if (f < m.nf) {
+        const int cl = m.fl[f], cr = m.fr[f];
+        const double4 fg = m.fgeo[f];
+        const State L = st(a.U[cl]);
+        if (cr >= 0) {
+            const State R = st(a.U[cr]);
+            cand = dt_candidate(L, &R, fg.x, fg.y, fg.w, a.gamma, a.cfl);
+        } else {
+            cand = dt_candidate(L, nullptr, fg.x, fg.y, fg.w, a.gamma, a.cfl);
+        }
+    }
+    block_min_to(cand, &a.dtmin[a.slot]);
+}
+
 // ---------------------------------------------------------------------------
 // gradients + limiter (extension): LSQ of rho, u, v, p with the reference's
-// weights 1/d^2 and precomputed inverse; boundary faces contribute
-// 0.5 (W_c + W_ghost) at the face midpoint; Venkatakrishnan with
-// eps^2 = (K sqrt(V))^3, phi = min(1, min over faces).
+// weights 1/d^2 and precomputed inverse; a boundary face contributes
+// 0.5 (W_c + W_ghost) at its midpoint; Venkatakrishnan with
+// eps^2 = (K sqrt(V))^3, phi = min(1, min over faces); the first-order
+// limiter sets phi = 0.
 // ---------------------------------------------------------------------------
-template <bool PERIODIC>
-__global__ __launch_bounds__(kBlock) void k_euler_grad(EulerArgs a) {
+template <int D>
+__global__ __launch_bounds__(kMaxBlock) void k_e_grad(EulerArgs a) {
     const MeshView& m = a.m;
-    int c = blockIdx.x * kBlock + threadIdx.x;
+    const int c = xcd_index();
     if (c >= m.nc) return;
     const double g = a.gamma;
-    State Uc = ldU(a.U, a.ld, c);
-    double wc[4] = {Uc.r, Uc.ru / Uc.r, Uc.rv / Uc.r, pressure(Uc, g)};
-    double cx = m.cx[c], cy = m.cy[c];
+    const int64_t ld = m.ld;
+    int2 en[D];
+    double4 eg[D], un[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) en[k] = m.ell[k * ld + c];
+#pragma unroll
+    for (int k = 0; k < D; ++k) eg[k] = m.egeo[k * ld + c];  // w dx, w dy, face offset (precomputed, same bits)
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+        if (en[k].x >= 0 && en[k].y >= 0) un[k] = a.U[en[k].y];
+    const State Uc = st(a.U[c]);
+    const double wc[4] = {Uc.r, Uc.ru / Uc.r, Uc.rv / Uc.r, pressure(Uc, g)};
     double bx[4] = {0.0, 0.0, 0.0, 0.0}, by[4] = {0.0, 0.0, 0.0, 0.0};
     double wmin[4], wmax[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) wmin[q] = wmax[q] = wc[q];
-    const int k0 = m.cptr[c], k1 = m.cptr[c + 1];
-    for (int k = k0; k < k1; ++k) {
-        int e = m.cent[k];
-        int f = e >> 1;
-        bool owner = e & 1;
-        int cr = m.fr[f];
-        double qx, qy, wn[4];
-        if (cr < 0) {
-            const DevBC& bc = m.bc[-1 - cr];
-            State G = euler_ghost(bc, Uc, m.fnx[f], m.fny[f], g);
-            double wg0 = G.r, wg1 = G.ru / G.r, wg2 = G.rv / G.r, wg3 = pressure(G, g);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        if (en[k].x < 0) continue;  // ELL padding (always after the cell's faces)
+        double wn[4];
+        if (en[k].y < 0) {
+            const double4 fg = m.fgeo[en[k].x >> 1];
+            const State Gs = euler_ghost(m.bc[-1 - en[k].y], Uc, fg.x, fg.y, g);
+            const double wg0 = Gs.r, wg1 = Gs.ru / Gs.r, wg2 = Gs.rv / Gs.r, wg3 = pressure(Gs, g);
             wn[0] = 0.5 * (wc[0] + wg0);
             wn[1] = 0.5 * (wc[1] + wg1);
             wn[2] = 0.5 * (wc[2] + wg2);
             wn[3] = 0.5 * (wc[3] + wg3);
-            qx = m.fmx[f];
-            qy = m.fmy[f];
         } else {
-            int o = owner ? cr : m.fl[f];
-            State Uo = ldU(a.U, a.ld, o);
+            const State Uo = st(un[k]);
             wn[0] = Uo.r;
             wn[1] = Uo.ru / Uo.r;
             wn[2] = Uo.rv / Uo.r;
             wn[3] = pressure(Uo, g);
-            qx = m.cx[o];
-            qy = m.cy[o];
-            if (PERIODIC) {
-                if (owner) { qx = qx + m.fsx[f]; qy = qy + m.fsy[f]; }
-                else { qx = qx - m.fsx[f]; qy = qy - m.fsy[f]; }
-            }
         }
-        double dx = qx - cx, dy = qy - cy;
-        double w = 1.0 / (dx * dx + dy * dy);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            double dphi = wn[q] - wc[q];
-            bx[q] += w * dx * dphi;
-            by[q] += w * dy * dphi;
+            const double dphi = wn[q] - wc[q];
+            bx[q] += eg[k].x * dphi;
+            by[q] += eg[k].y * dphi;
             wmin[q] = smin(wmin[q], wn[q]);
             wmax[q] = smax(wmax[q], wn[q]);
         }
     }
-    const double i0 = m.lsq[c], i1 = m.lsq[m.nc + c], i2 = m.lsq[2 * m.nc + c];
+    const double4 iv = m.clsq[c];  // LSQ inverse and eps^2
     double gx[4], gy[4], lim[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        gx[q] = i0 * bx[q] + i1 * by[q];
-        gy[q] = i1 * bx[q] + i2 * by[q];
+        gx[q] = iv.x * bx[q] + iv.y * by[q];
+        gy[q] = iv.y * bx[q] + iv.z * by[q];
         lim[q] = 1.0;
     }
     if (a.limiter == FVK_LIMITER_VENKATAKRISHNAN) {
-        double kh = a.venk_k * sqrt(m.vol[c]);
-        double eps2 = kh * kh * kh;
-        for (int k = k0; k < k1; ++k) {
-            int e = m.cent[k];
-            int f = e >> 1;
-            double mx = m.fmx[f], my = m.fmy[f];
-            if (PERIODIC && !(e & 1)) { mx = mx - m.fsx[f]; my = my - m.fsy[f]; }
-            double rx = mx - cx, ry = my - cy;
+        const double eps2 = iv.w;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            if (en[k].x < 0) continue;
+            const double rx = eg[k].z, ry = eg[k].w;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                double d2 = gx[q] * rx + gy[q] * ry;
+                const double d2 = gx[q] * rx + gy[q] * ry;
                 double ph = 1.0;
                 if (d2 > 0.0) ph = venkatakrishnan(wmax[q] - wc[q], d2, eps2);
                 else if (d2 < 0.0) ph = venkatakrishnan(wmin[q] - wc[q], d2, eps2);
                 lim[q] = smin(lim[q], ph);
             }
         }
-    }
-    if (a.limiter == FVK_LIMITER_FIRST_ORDER) {
+    } else if (a.limiter == FVK_LIMITER_FIRST_ORDER) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) lim[q] = 0.0;
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        a.G[(2 * q) * a.ld + c] = lim[q] * gx[q];
-        a.G[(2 * q + 1) * a.ld + c] = lim[q] * gy[q];
-    }
+    a.G[2 * c] = double4{lim[0] * gx[0], lim[0] * gy[0], lim[1] * gx[1], lim[1] * gy[1]};
+    a.G[2 * c + 1] = double4{lim[2] * gx[2], lim[2] * gy[2], lim[3] * gx[3], lim[3] * gy[3]};
 }
 
 // MUSCL face state (extension): the cell's own state when the increment is
 // exactly zero or the reconstructed density/pressure is not positive.
-__device__ __forceinline__ State reconstruct(const EulerArgs& a, const State& Uc, int c, double rx, double ry) {
-    const double* G = a.G;
-    const int64_t ld = a.ld;
-    double d0 = G[c] * rx + G[ld + c] * ry;
-    double d1 = G[2 * ld + c] * rx + G[3 * ld + c] * ry;
-    double d2 = G[4 * ld + c] * rx + G[5 * ld + c] * ry;
-    double d3 = G[6 * ld + c] * rx + G[7 * ld + c] * ry;
+__device__ __forceinline__ State reconstruct(const State& Uc, const double4& g0, const double4& g1, double rx,
+                                             double ry, double gam) {
+    const double d0 = g0.x * rx + g0.y * ry;
+    const double d1 = g0.z * rx + g0.w * ry;
+    const double d2 = g1.x * rx + g1.y * ry;
+    const double d3 = g1.z * rx + g1.w * ry;
     if (d0 == 0.0 && d1 == 0.0 && d2 == 0.0 && d3 == 0.0) return Uc;
-    double u = Uc.ru / Uc.r, v = Uc.rv / Uc.r, p = pressure(Uc, a.gamma);
-    double rho = Uc.r + d0, pp = p + d3;
+    const double u = Uc.ru / Uc.r, v = Uc.rv / Uc.r, p = pressure(Uc, gam);
+    const double rho = Uc.r + d0, pp = p + d3;
     if (!(rho > 0.0) || !(pp > 0.0)) return Uc;
-    return from_primitive(rho, u + d1, v + d2, pp, a.gamma);
+    return from_primitive(rho, u + d1, v + d2, pp, gam);
 }
 
 // ---------------------------------------------------------------------------
-// face pass: EulerFVMSolver::step pass 1 (src/EulerFVMSolver.cpp:64-82) and,
-// on the first stage, the compute_dt candidate of every face (:138-158)
+// fused cell pass (variant 2): the cell's face fluxes, face by face in the
+// face's orientation (EulerFVMSolver::step pass 1), gathered in Cell::faces
+// order with the owner's minus sign (pass 2), then the update
 // ---------------------------------------------------------------------------
-template <int SCHEME, int ORDER, bool PERIODIC>
-__global__ __launch_bounds__(kBlock) void k_euler_face(EulerArgs a) {
+template <int D, int SCHEME, int ORDER>
+__global__ __launch_bounds__(kMaxBlock) void k_e_cell(EulerArgs a) {
     const MeshView& m = a.m;
-    int f = blockIdx.x * kBlock + threadIdx.x;
+    const int c = xcd_index();
+    const double g = a.gamma;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    double dmin = DBL_MAX;
+    if (c < m.nc) {
+        const int64_t ld = m.ld;
+        int2 en[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) en[k] = m.ell[k * ld + c];
+        const double4 uc4 = a.U[c];
+        const State Uc = st(uc4);
+        double4 gc0{}, gc1{};
+        if (ORDER == 2) {
+            gc0 = a.G[2 * c];
+            gc1 = a.G[2 * c + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            if (en[k].x < 0) continue;
+            const int f = en[k].x >> 1;
+            const bool owner = en[k].x & 1;
+            const double4 fg = m.fgeo[f];
+            State F;
+            double cand = DBL_MAX;
+            if (en[k].y < 0) {
+                // boundary face: this cell is the owner
+                State Lf = Uc;
+                if (ORDER == 2) {
+                    const double4 ro = m.frec[f];
+                    Lf = reconstruct(Uc, gc0, gc1, ro.x, ro.y, g);
+                }
+                const State Rf = euler_ghost(m.bc[-1 - en[k].y], Lf, fg.x, fg.y, g);
+                F = num_flux<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
+                if (a.face_dt) cand = dt_candidate(Uc, nullptr, fg.x, fg.y, fg.w, g, a.cfl);
+            } else {
+                const State Uo = st(a.U[en[k].y]);
+                const State& L = owner ? Uc : Uo;
+                const State& R = owner ? Uo : Uc;
+                State Lf = L, Rf = R;
+                if (ORDER == 2) {
+                    const double4 go0 = a.G[2 * en[k].y], go1 = a.G[2 * en[k].y + 1];
+                    const double4 ro = m.frec[f];
+                    Lf = reconstruct(L, owner ? gc0 : go0, owner ? gc1 : go1, ro.x, ro.y, g);
+                    Rf = reconstruct(R, owner ? go0 : gc0, owner ? go1 : gc1, ro.z, ro.w, g);
+                }
+                F = num_flux<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
+                if (a.face_dt) cand = dt_candidate(L, &R, fg.x, fg.y, fg.w, g, a.cfl);
+            }
+            const double A = fg.z;
+            const double F0 = A * F.r, F1 = A * F.ru, F2 = A * F.rv, F3 = A * F.E;
+            if (owner) {
+                r0 = r0 - F0; r1 = r1 - F1; r2 = r2 - F2; r3 = r3 - F3;
+            } else {
+                r0 += F0; r1 += F1; r2 += F2; r3 += F3;
+            }
+            if (a.face_dt) dmin = smin(dmin, cand);
+        }
+        update_cell(a, c, uc4, r0, r1, r2, r3, dmin);
+    }
+    // local dt: the global minimum is still reported (dt history)
+    if (a.face_dt) block_min_to(dmin, &a.dtmin[a.slot]);
+    finish_residual(a, r0, r1, r2, r3);
+}
+
+// ---------------------------------------------------------------------------
+// face sweep (variant 1): EulerFVMSolver::step pass 1 (src/EulerFVMSolver.cpp:
+// 64-82) with MUSCL states for order 2, one thread per face, writing the
+// face flux x length as one double4; the first stage also evaluates the
+// step's dt candidates (:139-158)
+// ---------------------------------------------------------------------------
+template <int SCHEME, int ORDER>
+__global__ __launch_bounds__(kMaxBlock) void k_e_face(EulerArgs a) {
+    const MeshView& m = a.m;
+    const int f = xcd_index();
+    const double g = a.gamma;
     double cand = DBL_MAX;
     if (f < m.nf) {
-        const double g = a.gamma;
-        int cl = m.fl[f], cr = m.fr[f];
-        double nx = m.fnx[f], ny = m.fny[f], A = m.fa[f];
-        State L = ldU(a.U, a.ld, cl);
-        State R, Lf, Rf;
+        const int cl = m.fl[f], cr = m.fr[f];
+        const double4 fg = m.fgeo[f];
+        const State L = st(a.U[cl]);
+        State R = L, Lf = L, Rf;
         if (ORDER == 2) {
-            double mx = m.fmx[f], my = m.fmy[f];
-            Lf = reconstruct(a, L, cl, mx - m.cx[cl], my - m.cy[cl]);
+            const double4 ro = m.frec[f];  // midpoint - centroid, left and right (precomputed, same bits)
+            Lf = reconstruct(L, a.G[2 * cl], a.G[2 * cl + 1], ro.x, ro.y, g);
             if (cr >= 0) {
-                R = ldU(a.U, a.ld, cr);
-                double rcx = m.cx[cr], rcy = m.cy[cr];
-                if (PERIODIC) { rcx = rcx + m.fsx[f]; rcy = rcy + m.fsy[f]; }
-                Rf = reconstruct(a, R, cr, mx - rcx, my - rcy);
+                R = st(a.U[cr]);
+                Rf = reconstruct(R, a.G[2 * cr], a.G[2 * cr + 1], ro.z, ro.w, g);
             } else {
-                Rf = euler_ghost(m.bc[-1 - cr], Lf, nx, ny, g);
+                Rf = euler_ghost(m.bc[-1 - cr], Lf, fg.x, fg.y, g);
             }
         } else {
-            Lf = L;
-            if (cr >= 0) { R = ldU(a.U, a.ld, cr); Rf = R; }
-            else Rf = euler_ghost(m.bc[-1 - cr], L, nx, ny, g);
+            if (cr >= 0) { R = st(a.U[cr]); Rf = R; }
+            else Rf = euler_ghost(m.bc[-1 - cr], L, fg.x, fg.y, g);
         }
-        State F = num_flux<SCHEME>(Lf, Rf, nx, ny, g, a.tol, a.maxit);
-        a.flux[f] = A * F.r;
-        a.flux[a.ldf + f] = A * F.ru;
-        a.flux[2 * a.ldf + f] = A * F.rv;
-        a.flux[3 * a.ldf + f] = A * F.E;
+        const State F = num_flux<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
+        const double A = fg.z;
+        a.flux[f] = double4{A * F.r, A * F.ru, A * F.rv, A * F.E};
         if (a.face_dt) {
-            double uL = L.ru / L.r, vL = L.rv / L.r;
-            double VnL = uL * nx + vL * ny;
-            double Smax = fabs(VnL) + sound_speed(L, g);
-            if (cr >= 0) {
-                double uR = R.ru / R.r, vR = R.rv / R.r;
-                double VnR = uR * nx + vR * ny;
-                Smax = smax(Smax, fabs(VnR) + sound_speed(R, g));
-            }
-            cand = a.cfl * m.fdl[f] / Smax;
+            cand = dt_candidate(L, cr >= 0 ? &R : nullptr, fg.x, fg.y, fg.w, g, a.cfl);
             if (a.dt_src == 2) a.dtf[f] = cand;
         }
     }
-    if (a.face_dt) {
-        // block minimum (std::min semantics: NaN candidates never win), one atomic per block
-        __shared__ double sm[kBlock];
-        sm[threadIdx.x] = cand;
-        __syncthreads();
-        for (int s = kBlock / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) sm[threadIdx.x] = smin(sm[threadIdx.x], sm[threadIdx.x + s]);
-            __syncthreads();
-        }
-        // positive doubles order like their bit patterns; NaN bits exceed +inf and DBL_MAX
-        if (threadIdx.x == 0) atomicMin(&a.dtmin[a.slot], dbits(sm[0]));
-    }
-}
-
-// deterministic block sum of NV values per thread into partials[q * gridDim.x + block];
-// the last block to finish reduces the partials in a fixed order.  Returns
-// true in the last block (all threads), with the totals in tot[].
-template <int NV>
-__device__ __forceinline__ bool block_partials(const double (&v)[NV], double* partials, unsigned int* ticket,
-                                               double (&tot)[NV]) {
-    __shared__ double sm[NV][kBlock];
-    __shared__ bool last;
-#pragma unroll
-    for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] = v[q];
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-#pragma unroll
-            for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) partials[q * gridDim.x + blockIdx.x] = sm[q][0];
-        __threadfence();
-        unsigned int t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-    for (unsigned int i = threadIdx.x; i < gridDim.x; i += kBlock) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q)
-            acc[q] += __hip_atomic_load(&partials[q * gridDim.x + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-#pragma unroll
-    for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] = acc[q];
-    __syncthreads();
-    for (int s = kBlock / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-#pragma unroll
-            for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < NV; ++q) tot[q] = sm[q][0];
-    if (threadIdx.x == 0) *ticket = 0u;
-    return true;
+    if (a.face_dt) block_min_to(cand, &a.dtmin[a.slot]);
 }
 
 // ---------------------------------------------------------------------------
-// cell pass: EulerFVMSolver::step pass 2 (src/EulerFVMSolver.cpp:89-111),
-// generalised to an SSP-RK stage and a per-cell dt
+// gather (variant 1): EulerFVMSolver::step pass 2 (src/EulerFVMSolver.cpp:
+// 89-104) over the face fluxes, Cell::faces order, owner minus; stage update
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_euler_cell(EulerArgs a) {
+template <int D>
+__global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
     const MeshView& m = a.m;
-    int c = blockIdx.x * kBlock + threadIdx.x;
-    double r[4] = {0.0, 0.0, 0.0, 0.0};
+    const int c = xcd_index();
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
     if (c < m.nc) {
+        const int64_t ld = m.ld;
+        int2 en[D];
+        double4 F[D];
+        double df[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) en[k] = m.ell[k * ld + c];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            if (en[k].x >= 0) {
+                F[k] = a.flux[en[k].x >> 1];
+                if (a.dt_src == 2) df[k] = a.dtf[en[k].x >> 1];
+            }
+        }
+        const double4 u = a.U[c];
         double dmin = DBL_MAX;
-        const int k0 = m.cptr[c], k1 = m.cptr[c + 1];
-        for (int k = k0; k < k1; ++k) {
-            int e = m.cent[k];
-            int f = e >> 1;
-            double F0 = a.flux[f], F1 = a.flux[a.ldf + f], F2 = a.flux[2 * a.ldf + f], F3 = a.flux[3 * a.ldf + f];
-            if (e & 1) {
-                r[0] = r[0] - F0; r[1] = r[1] - F1; r[2] = r[2] - F2; r[3] = r[3] - F3;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            if (en[k].x < 0) continue;
+            if (en[k].x & 1) {
+                r0 = r0 - F[k].x; r1 = r1 - F[k].y; r2 = r2 - F[k].z; r3 = r3 - F[k].w;
             } else {
-                r[0] += F0; r[1] += F1; r[2] += F2; r[3] += F3;
+                r0 += F[k].x; r1 += F[k].y; r2 += F[k].z; r3 += F[k].w;
             }
-            if (a.dt_src == 2) dmin = smin(dmin, a.dtf[f]);
+            if (a.dt_src == 2) dmin = smin(dmin, df[k]);
         }
-        if (a.R) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) a.R[q * a.ld + c] = r[q];
-        } else {
-            double dt;
-            if (a.dt_src == 1) dt = __longlong_as_double((long long)a.dtmin[a.slot]);
-            else if (a.dt_src == 2) { dt = dmin; a.dtc[c] = dmin; }
-            else dt = a.dtc[c];
-            double h = dt / m.vol[c];
-            if (a.first) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) a.Unew[q * a.ld + c] = a.U[q * a.ld + c] + h * r[q];
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    a.Unew[q * a.ld + c] = a.a * a.U0[q * a.ld + c] + a.b * (a.U[q * a.ld + c] + h * r[q]);
-            }
-        }
+        update_cell(a, c, u, r0, r1, r2, r3, dmin);
     }
-    if (!a.want_res) return;
-    double sq[4] = {r[0] * r[0], r[1] * r[1], r[2] * r[2], r[3] * r[3]};
-    double tot[4];
-    if (block_partials<4>(sq, a.partials, a.ticket, tot) && threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) a.res_hist[a.hist_idx * 4 + q] = sqrt(tot[q]);
-        a.dt_hist[a.hist_idx] = __longlong_as_double((long long)a.dtmin[a.slot]);
-        a.dtmin[a.slot ^ 1] = dbits(DBL_MAX);
-    }
+    finish_residual(a, r0, r1, r2, r3);
 }
 
 // ---------------------------------------------------------------------------
-// host launcher: one residual evaluation + stage update
+// host launchers
 // ---------------------------------------------------------------------------
 namespace {
 
-template <int SCHEME, int ORDER>
-void launch_face(bool periodic, int grid, hipStream_t st, const EulerArgs& a) {
-    if (periodic) k_euler_face<SCHEME, ORDER, true><<<grid, kBlock, 0, st>>>(a);
-    else k_euler_face<SCHEME, ORDER, false><<<grid, kBlock, 0, st>>>(a);
-}
-
-template <int ORDER>
-void launch_face_scheme(int scheme, bool periodic, int grid, hipStream_t st, const EulerArgs& a) {
-    if (scheme == FLUX_HLLC) launch_face<FLUX_HLLC, ORDER>(periodic, grid, st, a);
-    else if (scheme == FLUX_EXACT) launch_face<FLUX_EXACT, ORDER>(periodic, grid, st, a);
-    else launch_face<FLUX_RUSANOV, ORDER>(periodic, grid, st, a);
+int grid_for(int64_t n, int bs) {
+    int64_t g = (n + bs - 1) / bs;
+    return (int)((g + kXcds - 1) / kXcds * kXcds);
 }
 
 EulerArgs make_args(fvk_solver* s) {
     EulerArgs a{};
     a.m = s->mesh;
-    a.G = s->grad;
-    a.flux = s->flux;
+    a.G = reinterpret_cast<double4*>(s->grad);
+    a.flux = reinterpret_cast<double4*>(s->flux);
     a.dtf = s->dtf;
     a.dtc = s->dtc;
     a.dtmin = s->dtmin;
     a.partials = s->partials;
-    a.ticket = s->ticket;
     a.res_hist = s->res_hist;
     a.dt_hist = s->dt_hist;
-    a.ld = s->ld;
-    a.ldf = s->ldf;
     a.gamma = s->p.gamma;
     a.cfl = s->p.cfl;
     a.tol = s->p.exact_riemann_tol;
     a.maxit = s->p.exact_riemann_max_iter;
-    a.venk_k = s->p.venkat_k;
     a.limiter = s->p.limiter;
     return a;
 }
 
-void run_face_grad(fvk_solver* s, EulerArgs& a) {
-    const int gc = (int)((s->nc + kBlock - 1) / kBlock);
-    const int gf = (int)((s->nf + kBlock - 1) / kBlock);
+template <int D>
+struct Launch {
+    static void grad(fvk_solver* s, const EulerArgs& a) {
+        k_e_grad<D><<<grid_for(s->nc, s->block), s->block, 0, s->stream>>>(a);
+    }
+    template <int ORDER>
+    static void cell_o(fvk_solver* s, const EulerArgs& a) {
+        int g = grid_for(s->nc, s->block);
+        switch (s->p.flux_scheme) {
+            case FLUX_HLLC: k_e_cell<D, FLUX_HLLC, ORDER><<<g, s->block, 0, s->stream>>>(a); break;
+            case FLUX_EXACT: k_e_cell<D, FLUX_EXACT, ORDER><<<g, s->block, 0, s->stream>>>(a); break;
+            default: k_e_cell<D, FLUX_RUSANOV, ORDER><<<g, s->block, 0, s->stream>>>(a); break;
+        }
+    }
+    static void cell(fvk_solver* s, const EulerArgs& a) {
+        if (s->p.order == 2) cell_o<2>(s, a);
+        else cell_o<1>(s, a);
+    }
+    static void gather(fvk_solver* s, const EulerArgs& a) {
+        k_e_gather<D><<<grid_for(s->nc, s->block), s->block, 0, s->stream>>>(a);
+    }
+};
+
+template <int ORDER>
+void face_o(fvk_solver* s, const EulerArgs& a) {
+    int g = grid_for(s->nf, s->block);
+    switch (s->p.flux_scheme) {
+        case FLUX_HLLC: k_e_face<FLUX_HLLC, ORDER><<<g, s->block, 0, s->stream>>>(a); break;
+        case FLUX_EXACT: k_e_face<FLUX_EXACT, ORDER><<<g, s->block, 0, s->stream>>>(a); break;
+        default: k_e_face<FLUX_RUSANOV, ORDER><<<g, s->block, 0, s->stream>>>(a); break;
+    }
+}
+
+template <int D>
+void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
     hipEvent_t ev = nullptr;
-    if (s->p.order == 2) {
-        timer_begin(s, "euler_grad", &ev);
-        if (s->periodic) k_euler_grad<true><<<gc, kBlock, 0, s->stream>>>(a);
-        else k_euler_grad<false><<<gc, kBlock, 0, s->stream>>>(a);
+    const bool split = s->variant == 1;
+    if (dt_pass && !split) {
+        timer_begin(s, "euler_dt", &ev);
+        k_e_dt<<<grid_for(s->nf, s->block), s->block, 0, s->stream>>>(a);
         timer_end(s, ev);
     }
-    timer_begin(s, "euler_face", &ev);
-    if (s->p.order == 2) launch_face_scheme<2>(s->p.flux_scheme, s->periodic, gf, s->stream, a);
-    else launch_face_scheme<1>(s->p.flux_scheme, s->periodic, gf, s->stream, a);
-    timer_end(s, ev);
+    if (s->p.order == 2) {
+        timer_begin(s, "euler_grad", &ev);
+        Launch<D>::grad(s, a);
+        timer_end(s, ev);
+    }
+    if (split) {
+        EulerArgs af = a;
+        af.face_dt = dt_pass || a.face_dt;
+        timer_begin(s, "euler_face", &ev);
+        if (s->p.order == 2) face_o<2>(s, af);
+        else face_o<1>(s, af);
+        timer_end(s, ev);
+        EulerArgs ag = a;
+        ag.face_dt = 0;
+        timer_begin(s, "euler_gather", &ev);
+        Launch<D>::gather(s, ag);
+        timer_end(s, ev);
+    } else {
+        timer_begin(s, "euler_cell", &ev);
+        Launch<D>::cell(s, a);
+        timer_end(s, ev);
+    }
+    if (a.want_res) {
+        timer_begin(s, "euler_finalize", &ev);
+        k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(a.partials, grid_for(s->nc, s->block), 4, a.res_hist,
+                                                     a.dt_hist, a.hist_idx, a.dtmin, a.slot);
+        timer_end(s, ev);
+    }
+}
+
+void run_stage(fvk_solver* s, EulerArgs& a, bool dt_pass) {
+    const int deg = s->mesh.deg;
+    if (deg <= 3) run_stage_d<3>(s, a, dt_pass);
+    else if (deg <= 4) run_stage_d<4>(s, a, dt_pass);
+    else if (deg <= 9) run_stage_d<9>(s, a, dt_pass);
+    else run_stage_d<kMaxEulerDegree>(s, a, dt_pass);
 }
 
 }  // namespace
 
-// stage in [0, stages); the state buffers rotate A -> B -> C -> A so that no
-// copy of the step's base state is made: stage 0 reads A (= U^n) and writes
-// B (or A for one stage); later stages read the previous stage's buffer and
-// A as the RK base, the last writes A.
-int launch_euler_stage(fvk_solver* s, int stage, int stages, int /*dt_mode_first*/, int want_res, int64_t step) {
+// stage in [0, stages).  The per-cell kernels read their neighbours' stage
+// state while they write their own update, so the output buffer is never the
+// one read for neighbours: forward Euler alternates two buffers (A -> B, then
+// B is current); RK2/RK3 rotate A -> B -> C -> A, where the last stage writes
+// A, which it reads only as the cell's own RK base state.
+int launch_euler_stage(fvk_solver* s, int stage, int stages, int /*unused*/, int want_res, int64_t step) {
     static const double RA[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0.5, 0}, {0, 0.75, 1.0 / 3.0}};
     static const double RB[4][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0.5, 0}, {1, 0.25, 2.0 / 3.0}};
     EulerArgs a = make_args(s);
     const int A = s->cur;
-    int in = stage == 0 ? A : (A + stage) % 3;
-    int out = (stage == stages - 1) ? A : (A + stage + 1) % 3;
-    a.U = s->U[in];
-    a.Unew = s->U[out];
-    a.U0 = s->U[A];
-    a.face_dt = stage == 0;
-    a.dt_src = s->p.local_dt ? (stage == 0 ? 2 : 0) : 1;
+    int in, out;
+    if (stages == 1) {
+        in = A;
+        out = A ^ 1;
+    } else {
+        in = stage == 0 ? A : (A + stage) % 3;
+        out = (stage == stages - 1) ? A : (A + stage + 1) % 3;
+    }
+    a.U = reinterpret_cast<const double4*>(s->U[in]);
+    a.Unew = reinterpret_cast<double4*>(s->U[out]);
+    a.U0 = reinterpret_cast<const double4*>(s->U[A]);
     a.first = stage == 0;
     a.a = RA[stages][stage];
     a.b = RB[stages][stage];
     a.want_res = want_res && stage == 0;
     a.slot = (int)(step & 1);
     a.hist_idx = step % s->hist_cap;
-    run_face_grad(s, a);
-    const int gc = (int)((s->nc + kBlock - 1) / kBlock);
-    hipEvent_t ev = nullptr;
-    timer_begin(s, "euler_cell", &ev);
-    k_euler_cell<<<gc, kBlock, 0, s->stream>>>(a);
-    timer_end(s, ev);
+    bool dt_pass = false;
+    if (s->p.local_dt) {
+        a.face_dt = stage == 0;
+        a.dt_src = stage == 0 ? 2 : 0;
+    } else {
+        a.face_dt = 0;
+        a.dt_src = 1;
+        dt_pass = stage == 0;
+    }
+    run_stage(s, a, dt_pass);
+    if (stages == 1) s->cur = out;
     return hip_check(hipGetLastError(), "euler stage launch");
 }
 
 int launch_euler_residual_eval(fvk_solver* s, double* R_dev) {
     EulerArgs a = make_args(s);
-    a.U = s->U[s->cur];
+    a.U = reinterpret_cast<const double4*>(s->U[s->cur]);
     a.face_dt = 0;
     a.dt_src = 0;
     a.want_res = 0;
-    a.R = R_dev;
-    run_face_grad(s, a);
-    const int gc = (int)((s->nc + kBlock - 1) / kBlock);
-    k_euler_cell<<<gc, kBlock, 0, s->stream>>>(a);
+    a.R = reinterpret_cast<double4*>(R_dev);
+    run_stage(s, a, false);
     return hip_check(hipGetLastError(), "euler residual launch");
 }
 
 // plain streaming copy (bandwidth calibration of the HBM byte counters)
-__global__ __launch_bounds__(kBlock) void k_copy(double2* __restrict__ dst, const double2* __restrict__ src, int64_t n2) {
-    int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    int64_t stride = (int64_t)gridDim.x * kBlock;
+constexpr int kCopyBlock = 256;
+__global__ __launch_bounds__(kCopyBlock) void k_copy(double2* __restrict__ dst, const double2* __restrict__ src,
+                                                     int64_t n2) {
+    int64_t i = (int64_t)blockIdx.x * kCopyBlock + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * kCopyBlock;
     for (; i < n2; i += stride) dst[i] = src[i];
 }
 
 int launch_copy_probe(double* dst, const double* src, int64_t n, hipStream_t st) {
     int64_t n2 = n / 2;
     int grid = 256 * 16;
-    k_copy<<<grid, kBlock, 0, st>>>(reinterpret_cast<double2*>(dst), reinterpret_cast<const double2*>(src), n2);
+    k_copy<<<grid, kCopyBlock, 0, st>>>(reinterpret_cast<double2*>(dst), reinterpret_cast<const double2*>(src), n2);
     return hip_check(hipGetLastError(), "copy probe launch");
 }
 

@@ -335,6 +335,93 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
     m.bc = upload(s, hb);
     if (!m.vol || !m.cptr || !m.cent || !m.fl || !m.bc)
         return set_error(FVK_ERR_NOMEM, "device allocation for the mesh failed");
+
+    // ---- gather-friendly copies (cell-centric Euler kernels) ----
+    const int64_t ld = (nc + 255) / 256 * 256;
+    int deg = 0;
+    for (int64_t i = 0; i < nc; ++i) deg = std::max(deg, cptr[i + 1] - cptr[i]);
+    std::vector<int2> ell((size_t)deg * ld, int2{-1, -1});
+    for (int64_t i = 0; i < nc; ++i) {
+        for (int k = cptr[i]; k < cptr[i + 1]; ++k) {
+            int e = cent[k];
+            int f = e >> 1;
+            int nb;
+            if (fr[f] < 0) nb = fr[f];
+            else nb = (e & 1) ? fr[f] : fl[f];
+            ell[(size_t)(k - cptr[i]) * ld + i] = int2{e, nb};
+        }
+    }
+    // eps^2 = (K sqrt(V))^3 of the Venkatakrishnan limiter (same operation order as the kernels had)
+    std::vector<double4> cgeo(ld, double4{0.0, 0.0, 1.0, 0.0}), clsq(ld, double4{0.0, 0.0, 0.0, 0.0});
+    for (int64_t i = 0; i < nc; ++i) {
+        double kh = s->p.venkat_k * std::sqrt(vol[i]);
+        cgeo[i] = double4{cx[i], cy[i], vol[i], kh * kh * kh};
+        clsq[i] = double4{lsq[i], lsq[nc + i], lsq[2 * nc + i], kh * kh * kh};
+    }
+    // per ELL entry: LSQ weight times offset (w = 1/(dx^2+dy^2), GradientCalculator,
+    // src/GradientReconstruction.cpp:126-143) and the face midpoint offset, both
+    // in the cell's own frame (periodic shift applied), evaluated in the kernels' order
+    std::vector<double4> egeo((size_t)deg * ld, double4{0.0, 0.0, 0.0, 0.0});
+    for (int64_t i = 0; i < nc; ++i) {
+        for (int k = cptr[i]; k < cptr[i + 1]; ++k) {
+            int e = cent[k];
+            int f = e >> 1;
+            bool owner = e & 1;
+            double qx, qy;
+            if (fr[f] < 0) {
+                qx = fmx[f];
+                qy = fmy[f];
+            } else {
+                int o = owner ? fr[f] : fl[f];
+                qx = cx[o];
+                qy = cy[o];
+                if (sh) {
+                    if (owner) { qx = qx + fsx[f]; qy = qy + fsy[f]; }
+                    else { qx = qx - fsx[f]; qy = qy - fsy[f]; }
+                }
+            }
+            double dx = qx - cx[i], dy = qy - cy[i];
+            double w = 1.0 / (dx * dx + dy * dy);
+            double mx = fmx[f], my = fmy[f];
+            if (sh && !owner) { mx = mx - fsx[f]; my = my - fsy[f]; }
+            egeo[(size_t)(k - cptr[i]) * ld + i] = double4{w * dx, w * dy, mx - cx[i], my - cy[i]};
+        }
+    }
+    // per face: MUSCL offsets of both sides (midpoint - centroid; cell_right periodic-shifted)
+    std::vector<double4> frec(nf);
+    for (int64_t i = 0; i < nf; ++i) {
+        int l = fl[i], r = fr[i];
+        double rx = 0.0, ry = 0.0;
+        if (r >= 0) {
+            double ox = cx[r], oy = cy[r];
+            if (sh) { ox = ox + fsx[i]; oy = oy + fsy[i]; }
+            rx = fmx[i] - ox;
+            ry = fmy[i] - oy;
+        }
+        frec[i] = double4{fmx[i] - cx[l], fmy[i] - cy[l], rx, ry};
+    }
+    std::vector<double4> fgeo(nf);
+    std::vector<double2> fmid2(nf), fsh2;
+    for (int64_t i = 0; i < nf; ++i) {
+        fgeo[i] = double4{fnx[i], fny[i], fa[i], fdl[i]};
+        fmid2[i] = double2{fmx[i], fmy[i]};
+    }
+    if (sh) {
+        fsh2.resize(nf);
+        for (int64_t i = 0; i < nf; ++i) fsh2[i] = double2{fsx[i], fsy[i]};
+    }
+    m.deg = deg;
+    m.ld = ld;
+    m.ell = upload(s, ell);
+    m.cgeo = upload(s, cgeo);
+    m.clsq = upload(s, clsq);
+    m.fgeo = upload(s, fgeo);
+    m.fmid2 = upload(s, fmid2);
+    m.fsh2 = sh ? upload(s, fsh2) : nullptr;
+    m.egeo = upload(s, egeo);
+    m.frec = upload(s, frec);
+    if (!m.ell || !m.cgeo || !m.clsq || !m.fgeo || !m.fmid2 || (sh && !m.fsh2) || !m.egeo || !m.frec)
+        return set_error(FVK_ERR_NOMEM, "device allocation for the mesh failed");
     return FVK_OK;
 }
 

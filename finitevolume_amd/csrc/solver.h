@@ -14,6 +14,7 @@
 namespace fvk {
 
 constexpr int kMaxPatches = 64;
+constexpr int kMaxEulerDegree = 16;  // most faces of one cell the per-cell Euler kernels take
 
 // Per-patch boundary data on the device (fvk_bc reduced to what kernels read).
 struct DevBC {
@@ -54,6 +55,19 @@ struct MeshView {
     const double* fwr;      // [nf] d_L / (d_L + d_R)
     const double* wdist;    // [nc] wall distance (RANS)
     const DevBC* bc;        // [n_patches]
+    // ---- gather-friendly copies for the cell-centric Euler kernels ----
+    int deg;                // ELL width: the largest number of faces of a cell
+    int64_t ld;             // padded cell count (ELL and per-cell array stride)
+    const int2* ell;        // [deg][ld] per face of the cell, in Cell::faces order:
+                            //   x = (device face << 1) | owner, y = neighbour cell or -1 - patch; x = -1 pads
+    const double4* cgeo;    // [ld] centroid x, centroid y, volume, Venkatakrishnan eps^2
+    const double4* clsq;    // [ld] LSQ inverse {m11, -m01, m00}/det, 0
+    const double4* fgeo;    // [nf] nx, ny, length, Euler dt length
+    const double2* fmid2;   // [nf] face midpoint
+    const double2* fsh2;    // [nf] periodic shift of cell_right (nullptr: none)
+    const double4* egeo;    // [deg][ld] per ELL entry: w dx, w dy (LSQ weight 1/d^2 times the offset to the
+                            //   neighbour, or to the face midpoint on a boundary), face midpoint - centroid
+    const double4* frec;    // [nf] MUSCL offsets: midpoint - centroid of cell_left, of cell_right (periodic-shifted)
 };
 
 struct KernelTimer {
@@ -69,6 +83,8 @@ struct fvk_solver {
     fvk_params p;
     int nv = 4;                 // variables per cell
     int n_bcs = 0;
+    int variant = 1;            // Euler kernels: 1 face sweep + gather, 2 fused per cell
+    int block = 512;            // Euler workgroup size
     int64_t nc = 0, nf = 0, nn = 0;
     int nfi = 0, nnz = 0;
     bool periodic = false;

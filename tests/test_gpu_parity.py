@@ -131,3 +131,37 @@ def test_diffusion_bitwise():
     ro = o.step(200)
     assert cases.residual_close(rg, ro, 1e-12)
     assert_bitwise(g, o)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_kernel_variants_bit_identical(order):
+    """The two Euler kernel variants (face sweep + gather, fused per cell) and
+    every workgroup size give the same state bits; the residual norms are
+    block sums whose grouping follows the workgroup size, so they agree to
+    round-off (and bit for bit between the variants at one size)."""
+    ext = dict(order=order, limiter=abi.LIMITER_VENKATAKRISHNAN, rk_stages=3, local_dt=1) if order == 2 else {}
+    ref = None
+    by_bs = {}
+    for variant in (1, 2):
+        for bs in (64, 128, 256, 512):
+            m, bcs, p, ic, _ = cases.case_naca(64, 32, kernel_variant=variant, block_size=bs, **ext)
+            g = FVKSolver(m, bcs, p, ic)
+            r = g.step(10)
+            out = (g.field(), r)
+            if ref is None:
+                ref = out
+            else:
+                assert np.array_equal(out[0], ref[0]), (variant, bs)
+                assert np.allclose(out[1], ref[1], rtol=1e-13, atol=0.0), (variant, bs)
+            by_bs.setdefault(bs, []).append(r)
+    for bs, rs in by_bs.items():
+        assert np.array_equal(rs[0], rs[1]), bs
+
+
+def test_global_dt_variants_match_oracle():
+    for variant in (1, 2):
+        g, o = pair(cases.case_naca(64, 32, kernel_variant=variant))
+        rg, dg = g.step(20, with_dt=True)
+        ro, do = o.step(20, with_dt=True)
+        assert np.array_equal(dg, do)
+        assert_bitwise(g, o)
