@@ -89,6 +89,134 @@ def timed_steps(solver, seconds):
             return n, t
 
 
+def adjoint_leg(s, m, bcs, params, ic, steps_done, reps=5, n_check=32, h=1e-7):
+    """Discrete adjoint of the config-2 residual at its 1000-step state:
+    device time of one VJP (state and node coordinates, lambda ~ N(0, 1) from
+    default_rng(0)) against one residual evaluation with the same gradient
+    and limiter recompute; and central differences (step 1e-7) of
+    lambda . R at 32 cells and 32 nodes drawn by default_rng(1), each
+    evaluated on the six-ring sub-mesh around the perturbed entity (the
+    residual there is the whole mesh's up to an additive constant that the
+    difference cancels)."""
+    from finitevolume_amd import abi
+    from finitevolume_amd.solver import FVKSolver
+    if steps_done < 1000:
+        s.step(1000 - steps_done)
+    U = s.field()
+    nc = m.n_cells
+    lam = np.random.default_rng(0).standard_normal((nc, 4))
+    s.residual_eval()
+    s.residual_vjp(lam, want_x=True)  # builds the adjoint's index maps
+    s.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s.residual_eval()
+    wall_p = (time.perf_counter() - t0) / reps
+    tp = s.kernel_times()
+    s.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Ub, Xb = s.residual_vjp(lam, want_x=True)
+    wall_a = (time.perf_counter() - t0) / reps
+    ta = s.kernel_times()
+    s.set_kernel_timing(True)
+    for _ in range(reps):
+        s.residual_vjp(lam, want_x=False)
+    tu = s.kernel_times()
+    s.set_kernel_timing(False)
+    dev_p = sum(tot for tot, _ in tp.values()) / reps
+    dev_a = sum(tot for tot, _ in ta.values()) / reps
+    dev_u = sum(tot for tot, _ in tu.values()) / reps
+
+    cut_bc = next((b for b in bcs if b.type == abi.BC_FARFIELD), abi.make_bc(abi.BC_OUTFLOW))
+    sub_bcs = list(bcs) + [cut_bc]
+    cut = len(bcs)
+
+    def J(mesh, Us, ls):
+        g = FVKSolver(mesh, sub_bcs, params, ic)
+        g.set_field(Us)
+        return float(np.sum(ls * g.residual_eval()))
+
+    rng = np.random.default_rng(1)
+    cells = rng.choice(nc, n_check, replace=False)
+    nodes = rng.choice(m.n_nodes, n_check, replace=False)
+    scale_u, scale_x = float(np.abs(Ub).max()), float(np.abs(Xb).max())
+    # [max error where the difference is smooth, number refined to a smaller step, unresolved]
+    stats = {"state": [0.0, 0, 0], "nodes": [0.0, 0, 0]}
+
+    def check(kind, evalj, j0, h0, ad, scale):
+        """The limiter's min/max make lambda . R piecewise smooth, and in the
+        thin wall layers a step of 1e-7 can exceed the local variation of the
+        state.  Where the one-sided differences disagree the step is refined
+        (down to 1e-3 of it); points that never become smooth are counted."""
+        st = stats[kind]
+        den = max(abs(ad), 1e-6 * scale)
+        for r in range(4):
+            hh = h0 * 10.0 ** (-r)
+            jp, jm = evalj(hh)
+            f_fd, b_fd = (jp - j0) / hh, (j0 - jm) / hh
+            if abs(f_fd - b_fd) <= 1e-4 * max(abs(f_fd), abs(b_fd), 1e-6 * scale):
+                st[0] = max(st[0], abs(ad - (jp - jm) / (2 * hh)) / den)
+                st[1] += r > 0
+                return
+        st[2] += 1
+
+    for c in cells:
+        q = int(rng.integers(0, 4))
+        sel = m.cell_rings([c], 6)
+        sub, _ = m.submesh(sel, cut)
+        loc = int(np.searchsorted(sel, c))
+        ls, Us = lam[sel], U[sel]
+
+        def ev(hh):
+            Up, Um = Us.copy(), Us.copy()
+            Up[loc, q] += hh
+            Um[loc, q] -= hh
+            return J(sub, Up, ls), J(sub, Um, ls)
+        check("state", ev, J(sub, Us, ls), h, Ub[c, q], scale_u)
+    owner = np.repeat(np.arange(nc), np.diff(m.cell_node_ptr))
+    e = m.node_xy[m.face_nodes[:, 1]] - m.node_xy[m.face_nodes[:, 0]]
+    edge = np.hypot(e[:, 0], e[:, 1])
+    h_nodes = []
+    for n in nodes:
+        # the wall layers are far thinner than 1e-7: the step is capped at
+        # 1e-3 of the node's shortest edge so that no cell folds
+        hn = min(h, 1e-3 * float(edge[(m.face_nodes == n).any(axis=1)].min()))
+        h_nodes.append(hn)
+        sel = m.cell_rings(owner[m.cell_nodes == n], 6)
+        sub, used = m.submesh(sel, cut)
+        loc = int(np.searchsorted(used, n))
+        ls, Us = lam[sel], U[sel]
+        j0 = J(sub, Us, ls)
+        for d in range(2):
+            def ev(hh):
+                Xp, Xm = sub.node_xy.copy(), sub.node_xy.copy()
+                Xp[loc, d] += hh
+                Xm[loc, d] -= hh
+                return J(sub.with_nodes(Xp), Us, ls), J(sub.with_nodes(Xm), Us, ls)
+            check("nodes", ev, j0, hn, Xb[n, d], scale_x)
+    return {
+        "workload": "config4: VJP of the config-2 residual (state + node coordinates) at the 1000-step state",
+        "vjp_device_ms": round(dev_a, 4),
+        "primal_residual_device_ms": round(dev_p, 4),
+        "adjoint_over_primal": round(dev_a / dev_p, 3),
+        "vjp_state_only_device_ms": round(dev_u, 4),
+        "adjoint_state_only_over_primal": round(dev_u / dev_p, 3),
+        "vjp_kernel_avg_ms": {k: round(tot / cnt, 5) for k, (tot, cnt) in sorted(ta.items()) if cnt},
+        "vjp_wall_ms": round(wall_a * 1e3, 3),
+        "primal_residual_wall_ms": round(wall_p * 1e3, 3),
+        "wall_note": "wall times include the host copies and reordering of lambda, U_bar, X_bar and R",
+        "fd_check": {"cells": n_check, "nodes": n_check, "step": h,
+                     "node_steps_below_1e-7": int(sum(x < h for x in h_nodes)),
+                     "min_node_step": float(f"{min(h_nodes):.3e}"),
+                     "max_rel_err_state": float(f"{stats['state'][0]:.3e}"),
+                     "max_rel_err_nodes": float(f"{stats['nodes'][0]:.3e}"),
+                     "refined_state": stats["state"][1], "unresolved_state": stats["state"][2],
+                     "refined_nodes": stats["nodes"][1], "unresolved_nodes": stats["nodes"][2],
+                     "method": "central differences of lambda . R on six-ring sub-meshes (GPU residual)"},
+    }
+
+
 def host_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -120,6 +248,7 @@ def main():
                     help="wall time of each bounded CPU-baseline sample (whole steps, at least one)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-2r", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="Euler kernel variant (0 = library default)")
     ap.add_argument("--block", type=int, default=0, help="Euler workgroup size (0 = library default)")
     ap.add_argument("--sweep", action="store_true", help="time every kernel variant and block size first")
@@ -226,6 +355,11 @@ def main():
         out["sweep_ms_per_step"] = sweep
     if probe:
         out["copy_probe"] = probe
+    if not args.no_adjoint:
+        done = 2 * args.steps + args.warmup
+        out["adjoint"] = adjoint_leg(s, m, bcs, params, ic, done)
+        print(f"[bench] adjoint {out['adjoint']['vjp_device_ms']} ms vs residual "
+              f"{out['adjoint']['primal_residual_device_ms']} ms", file=sys.stderr, flush=True)
     del s
 
     # reference mode 2R on the GPU: first-order HLLC, forward Euler, global dt

@@ -19,7 +19,6 @@ int launch_visc_residual_eval(fvk_solver* s, double* R_dev);
 int launch_diffusion_residual_eval(fvk_solver* s, double* R_dev);
 int launch_wall_distance_segments(const double* cx, const double* cy, int nc, const double* seg, int nw, double* out,
                                   hipStream_t st);
-int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
 
 namespace {
 thread_local std::string g_err;
@@ -138,6 +137,7 @@ void fvk_destroy(fvk_solver* s) {
     if (!s) return;
     if (s->stream) hipStreamSynchronize(s->stream);
     timers_collect(s);
+    fvk::adjoint_release(s);
     for (void* p : s->allocations) hipFree(p);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
@@ -588,6 +588,54 @@ int fvk_wall_forces(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref,
     }
     *cd = drag / q;
     *cl = lift / q;
+    return FVK_OK;
+}
+
+int fvk_wall_forces_vjp(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref, double w_cl,
+                        double w_cd, double* U_bar, double* X_bar) {
+    if (!s || (!U_bar && !X_bar)) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (s->p.equation == FVK_EQ_DIFFUSION) return set_error(FVK_ERR_UNSUPPORTED, "no pressure for diffusion");
+    // J = w_cl Cl + w_cd Cd = sum over wall faces of p_cell (A n) . e, with
+    // A n = (ey, -ex) for the edge e = node2 - node1 and
+    // e = (w_cl lift_dir + w_cd drag_dir) / (0.5 rho_ref |V|^2 L_ref)
+    std::vector<double> U(4 * (size_t)s->nc);
+    int rc = fvk_get_field(s, U.data(), nullptr, s->nc);
+    if (rc != FVK_OK) return rc;
+    const double vm = std::hypot(vx_ref, vy_ref);
+    const double dx = vx_ref / vm, dy = vy_ref / vm, lx = -dy, ly = dx;
+    const double q = 0.5 * rho_ref * vm * vm * length_ref;
+    const double ex = (w_cl * lx + w_cd * dx) / q, ey = (w_cl * ly + w_cd * dy) / q;
+    const double g = s->p.gamma, gm = g - 1.0;
+    if (U_bar) std::fill(U_bar, U_bar + 4 * s->nc, 0.0);
+    if (X_bar) std::fill(X_bar, X_bar + 2 * s->nn, 0.0);
+    std::vector<int> dl(s->nf), dr(s->nf);
+    rc = hip_check(hipMemcpy(dl.data(), s->mesh.fl, sizeof(int) * s->nf, hipMemcpyDeviceToHost), "face copy");
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(dr.data(), s->mesh.fr, sizeof(int) * s->nf, hipMemcpyDeviceToHost), "face copy");
+    if (rc != FVK_OK) return rc;
+    const double* X = s->node_xy.data();
+    for (int64_t i = 0; i < s->nf; ++i) {
+        if (dr[i] >= 0 || s->bcs[-1 - dr[i]].type != FVK_BC_WALL) continue;
+        const int n1 = s->face_nodes_dev[2 * i], n2 = s->face_nodes_dev[2 * i + 1];
+        const double ax = X[2 * n2] - X[2 * n1], ay = X[2 * n2 + 1] - X[2 * n1 + 1];
+        const double an = ay * ex - ax * ey;  // (A n) . e
+        const int c = s->cell_new2old[dl[i]];
+        const double* u = &U[4 * (size_t)c];
+        const double uu = u[1] / u[0], vv = u[2] / u[0];
+        if (U_bar) {
+            // p = (g - 1) (E - 0.5 (ru^2 + rv^2) / r)
+            U_bar[4 * (size_t)c] += an * gm * 0.5 * (uu * uu + vv * vv);
+            U_bar[4 * (size_t)c + 1] += -an * gm * uu;
+            U_bar[4 * (size_t)c + 2] += -an * gm * vv;
+            U_bar[4 * (size_t)c + 3] += an * gm;
+        }
+        if (X_bar) {
+            const double p = gm * (u[3] - 0.5 * u[0] * (uu * uu + vv * vv));
+            X_bar[2 * n2 + 1] += p * ex;
+            X_bar[2 * n1 + 1] -= p * ex;
+            X_bar[2 * n2] -= p * ey;
+            X_bar[2 * n1] += p * ey;
+        }
+    }
     return FVK_OK;
 }
 

@@ -73,9 +73,7 @@ __device__ __forceinline__ State st(const double4& u) { return State{u.x, u.y, u
 
 // ghost state for the Euler equations, src/EulerFVMSolver.cpp:174-193
 __device__ __forceinline__ State euler_ghost(const DevBC& bc, const State& L, double nx, double ny, double g) {
-    if (bc.type == BC_FARFIELD) return State{bc.ff[0], bc.ff[1], bc.ff[2], bc.ff[3]};
-    if (bc.type == BC_OUTFLOW) return L;
-    return ghost_slip_wall(L, nx, ny, g);
+    return euler_ghost_state(bc.type, bc.ff, L, nx, ny, g);
 }
 
 // compute_dt candidate of one face (src/EulerFVMSolver.cpp:139-158), states in
@@ -291,21 +289,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_grad(EulerArgs a) {
     }
     a.G[2 * c] = double4{lim[0] * gx[0], lim[0] * gy[0], lim[1] * gx[1], lim[1] * gy[1]};
     a.G[2 * c + 1] = double4{lim[2] * gx[2], lim[2] * gy[2], lim[3] * gx[3], lim[3] * gy[3]};
-}
-
-// MUSCL face state (extension): the cell's own state when the increment is
-// exactly zero or the reconstructed density/pressure is not positive.
-__device__ __forceinline__ State reconstruct(const State& Uc, const double4& g0, const double4& g1, double rx,
-                                             double ry, double gam) {
-    const double d0 = g0.x * rx + g0.y * ry;
-    const double d1 = g0.z * rx + g0.w * ry;
-    const double d2 = g1.x * rx + g1.y * ry;
-    const double d3 = g1.z * rx + g1.w * ry;
-    if (d0 == 0.0 && d1 == 0.0 && d2 == 0.0 && d3 == 0.0) return Uc;
-    const double u = Uc.ru / Uc.r, v = Uc.rv / Uc.r, p = pressure(Uc, gam);
-    const double rho = Uc.r + d0, pp = p + d3;
-    if (!(rho > 0.0) || !(pp > 0.0)) return Uc;
-    return from_primitive(rho, u + d1, v + d2, pp, gam);
 }
 
 // ---------------------------------------------------------------------------
@@ -622,6 +605,22 @@ int launch_euler_residual_eval(fvk_solver* s, double* R_dev) {
     a.R = reinterpret_cast<double4*>(R_dev);
     run_stage(s, a, false);
     return hip_check(hipGetLastError(), "euler residual launch");
+}
+
+// limited gradients of the current state into the gradient buffer (the
+// first pass of the residual, shared with the adjoint)
+int launch_euler_gradients(fvk_solver* s) {
+    EulerArgs a = make_args(s);
+    a.U = reinterpret_cast<const double4*>(s->U[s->cur]);
+    hipEvent_t ev = nullptr;
+    timer_begin(s, "euler_grad", &ev);
+    const int deg = s->mesh.deg;
+    if (deg <= 3) Launch<3>::grad(s, a);
+    else if (deg <= 4) Launch<4>::grad(s, a);
+    else if (deg <= 9) Launch<9>::grad(s, a);
+    else Launch<kMaxEulerDegree>::grad(s, a);
+    timer_end(s, ev);
+    return hip_check(hipGetLastError(), "euler gradient launch");
 }
 
 // plain streaming copy (bandwidth calibration of the HBM byte counters)

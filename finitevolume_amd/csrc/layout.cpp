@@ -340,7 +340,10 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
     const int64_t ld = (nc + 255) / 256 * 256;
     int deg = 0;
     for (int64_t i = 0; i < nc; ++i) deg = std::max(deg, cptr[i + 1] - cptr[i]);
-    std::vector<int2> ell((size_t)deg * ld, int2{-1, -1});
+    // the per-cell kernels are instantiated for widths 3, 4, 9 and 16 and read
+    // every slot of their width: pad the slot arrays to it
+    const int width = deg <= 3 ? 3 : deg <= 4 ? 4 : deg <= 9 ? 9 : std::max(deg, kMaxEulerDegree);
+    std::vector<int2> ell((size_t)width * ld, int2{-1, -1});
     for (int64_t i = 0; i < nc; ++i) {
         for (int k = cptr[i]; k < cptr[i + 1]; ++k) {
             int e = cent[k];
@@ -361,7 +364,7 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
     // per ELL entry: LSQ weight times offset (w = 1/(dx^2+dy^2), GradientCalculator,
     // src/GradientReconstruction.cpp:126-143) and the face midpoint offset, both
     // in the cell's own frame (periodic shift applied), evaluated in the kernels' order
-    std::vector<double4> egeo((size_t)deg * ld, double4{0.0, 0.0, 0.0, 0.0});
+    std::vector<double4> egeo((size_t)width * ld, double4{0.0, 0.0, 0.0, 0.0});
     for (int64_t i = 0; i < nc; ++i) {
         for (int k = cptr[i]; k < cptr[i + 1]; ++k) {
             int e = cent[k];

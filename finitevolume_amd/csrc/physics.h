@@ -218,6 +218,30 @@ FVK_HD State ghost_noslip_wall(const State& L, double wu, double wv, double g) {
     return from_primitive(L.r, 2.0 * wu - u, 2.0 * wv - v, p, g);
 }
 
+// Euler ghost state by boundary type, src/EulerFVMSolver.cpp:174-193:
+// farfield -> the prescribed conserved state, outflow -> copy, wall -> slip mirror
+FVK_HD State euler_ghost_state(int type, const double* ff, const State& L, double nx, double ny, double g) {
+    if (type == BC_FARFIELD) return State{ff[0], ff[1], ff[2], ff[3]};
+    if (type == BC_OUTFLOW) return L;
+    return ghost_slip_wall(L, nx, ny, g);
+}
+
+// MUSCL face state (extension): primitive increments d_q = G_q . r from the
+// limited gradients g0 = (rho_x, rho_y, u_x, u_y), g1 = (v_x, v_y, p_x, p_y);
+// the cell's own state when every increment is exactly zero or the
+// reconstructed density or pressure is not positive.
+FVK_HD State reconstruct(const State& Uc, const double4& g0, const double4& g1, double rx, double ry, double gam) {
+    const double d0 = g0.x * rx + g0.y * ry;
+    const double d1 = g0.z * rx + g0.w * ry;
+    const double d2 = g1.x * rx + g1.y * ry;
+    const double d3 = g1.z * rx + g1.w * ry;
+    if (d0 == 0.0 && d1 == 0.0 && d2 == 0.0 && d3 == 0.0) return Uc;
+    const double u = Uc.ru / Uc.r, v = Uc.rv / Uc.r, p = pressure(Uc, gam);
+    const double rho = Uc.r + d0, pp = p + d3;
+    if (!(rho > 0.0) || !(pp > 0.0)) return Uc;
+    return from_primitive(rho, u + d1, v + d2, pp, gam);
+}
+
 // Venkatakrishnan limiter function (extension, no reference code):
 // ((d1^2 + eps2) + 2 d1 d2) / (d1^2 + 2 d2^2 + d1 d2 + eps2)
 FVK_HD double venkatakrishnan(double d1, double d2, double eps2) {

@@ -58,17 +58,19 @@ struct MeshView {
     // ---- gather-friendly copies for the cell-centric Euler kernels ----
     int deg;                // ELL width: the largest number of faces of a cell
     int64_t ld;             // padded cell count (ELL and per-cell array stride)
-    const int2* ell;        // [deg][ld] per face of the cell, in Cell::faces order:
+    const int2* ell;        // [width][ld] (width 3, 4, 9 or 16 >= deg) per face of the cell, in Cell::faces order:
                             //   x = (device face << 1) | owner, y = neighbour cell or -1 - patch; x = -1 pads
     const double4* cgeo;    // [ld] centroid x, centroid y, volume, Venkatakrishnan eps^2
     const double4* clsq;    // [ld] LSQ inverse {m11, -m01, m00}/det, 0
     const double4* fgeo;    // [nf] nx, ny, length, Euler dt length
     const double2* fmid2;   // [nf] face midpoint
     const double2* fsh2;    // [nf] periodic shift of cell_right (nullptr: none)
-    const double4* egeo;    // [deg][ld] per ELL entry: w dx, w dy (LSQ weight 1/d^2 times the offset to the
+    const double4* egeo;    // [width][ld] per ELL entry: w dx, w dy (LSQ weight 1/d^2 times the offset to the
                             //   neighbour, or to the face midpoint on a boundary), face midpoint - centroid
     const double4* frec;    // [nf] MUSCL offsets: midpoint - centroid of cell_left, of cell_right (periodic-shifted)
 };
+
+struct AdjointWork;  // device buffers and index maps of the residual VJP (adjoint.hip)
 
 struct KernelTimer {
     std::string name;
@@ -121,6 +123,9 @@ struct fvk_solver {
     double last_res[6] = {0, 0, 0, 0, 0, 0};
     int64_t device_bytes = 0;
 
+    // residual VJP workspace, built on first use
+    fvk::AdjointWork* adj = nullptr;
+
     // timing
     bool timing = false;
     std::vector<fvk::KernelTimer> timers;
@@ -145,6 +150,9 @@ int launch_gradient(fvk_solver* s, int scheme, const double* phi, const double* 
 int launch_wall_distance(fvk_solver* s, const int* wall_faces, int nw, const double* node_xy_dev,
                          const int* fnodes_dev, double* out);
 int launch_residual_eval(fvk_solver* s, double* R_dev);
+int launch_euler_gradients(fvk_solver* s);
+int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
+void adjoint_release(fvk_solver* s);
 int launch_copy_probe(double* dst, const double* src, int64_t n, hipStream_t st);
 
 // timers

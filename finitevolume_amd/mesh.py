@@ -40,6 +40,7 @@ class UnstructuredMesh:
     cell_xy: np.ndarray            # [n_cells, 2] float64
     patches: List[str] = field(default_factory=list)
     face_shift: Optional[np.ndarray] = None  # [n_faces, 2] float64, periodic pairs
+    relative_geometry: bool = False          # polygon geometry evaluated relative to each first vertex
 
     @property
     def n_nodes(self) -> int:
@@ -74,6 +75,59 @@ class UnstructuredMesh:
         nx = dy / area
         ny = -dx / area
         return mid, area, nx, ny
+
+    def with_nodes(self, node_xy: np.ndarray) -> "UnstructuredMesh":
+        """The same topology on moved nodes: cell volumes and centroids are
+        recomputed by the shoelace formulas over each cell's polygon (face
+        geometry follows from the nodes wherever it is evaluated)."""
+        node_xy = np.ascontiguousarray(node_xy, dtype=np.float64)
+        area, cxy = _polygon_geometry(node_xy, self.cell_node_ptr.astype(np.int64), self.cell_nodes.astype(np.int64),
+                                      relative=self.relative_geometry)
+        return UnstructuredMesh(node_xy=node_xy, face_nodes=self.face_nodes, face_cells=self.face_cells,
+                                face_patch=self.face_patch, cell_face_ptr=self.cell_face_ptr,
+                                cell_faces=self.cell_faces, cell_node_ptr=self.cell_node_ptr,
+                                cell_nodes=self.cell_nodes, cell_volume=np.abs(area), cell_xy=cxy,
+                                patches=list(self.patches), face_shift=self.face_shift,
+                                relative_geometry=self.relative_geometry)
+
+    def cell_rings(self, seeds, n_rings: int) -> np.ndarray:
+        """Cells within n_rings face-neighbour steps of the seed cells (sorted)."""
+        inner = self.face_cells[:, 1] >= 0
+        a, b = self.face_cells[inner, 0], self.face_cells[inner, 1]
+        mark = np.zeros(self.n_cells, dtype=bool)
+        mark[np.asarray(seeds, dtype=np.int64)] = True
+        for _ in range(n_rings):
+            grow = mark.copy()
+            grow[b[mark[a]]] = True
+            grow[a[mark[b]]] = True
+            mark = grow
+        return np.nonzero(mark)[0]
+
+    def submesh(self, cells, cut_patch: int):
+        """The polygons of `cells` as a mesh of their own: boundary edges keep
+        their patch, edges cut from the rest of the mesh get `cut_patch`.
+        Returns (mesh, node map sub -> full)."""
+        cells = np.asarray(cells, dtype=np.int64)
+        ptr = self.cell_node_ptr.astype(np.int64)
+        deg = ptr[cells + 1] - ptr[cells]
+        idx = np.concatenate([np.arange(ptr[c], ptr[c + 1]) for c in cells])
+        nodes_full = self.cell_nodes[idx].astype(np.int64)
+        used, inv = np.unique(nodes_full, return_inverse=True)
+        sub_ptr = np.concatenate([[0], np.cumsum(deg)])
+        keep = np.zeros(self.n_cells, dtype=bool)
+        keep[cells] = True
+        fl, fr = self.face_cells[:, 0], self.face_cells[:, 1]
+        inside_l = keep[fl]
+        inside_r = np.where(fr >= 0, keep[np.maximum(fr, 0)], False)
+        bnd = (inside_l & (fr < 0)) | (inside_l ^ inside_r)
+        bnd &= inside_l | inside_r
+        faces = np.nonzero(bnd)[0]
+        patch = np.where(fr[faces] < 0, self.face_patch[faces], cut_patch)
+        pos = np.searchsorted(used, self.face_nodes[faces].astype(np.int64))
+        sub = from_polygons(self.node_xy[used], sub_ptr, inv, pos, patch,
+                            list(self.patches) + ["cut"] * max(0, cut_patch + 1 - len(self.patches)),
+                            relative_geometry=self.relative_geometry)
+        return sub, used
 
     def validate(self) -> None:
         """Structural checks (the reference's reader rejects the same
@@ -199,6 +253,7 @@ def from_polygons(node_xy: np.ndarray, cell_ptr: np.ndarray, cell_nodes: np.ndar
         cell_volume=np.abs(area),
         cell_xy=cxy,
         patches=list(patch_names),
+        relative_geometry=relative_geometry,
     )
     return mesh
 

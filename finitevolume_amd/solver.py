@@ -54,6 +54,8 @@ def lib():
     L.fvk_residual_eval.argtypes = [vp, dp]
     L.fvk_residual_vjp.argtypes = [vp, dp, dp, dp]
     L.fvk_wall_forces.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
+    L.fvk_wall_forces_vjp.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      dp, dp]
     L.fvk_set_kernel_timing.argtypes = [vp, C.c_int32]
     L.fvk_kernel_times.argtypes = [vp, C.c_char_p, i64, dp, C.POINTER(i64), C.c_int32, C.POINTER(C.c_int32)]
     L.fvk_device_bytes.argtypes = [vp]
@@ -172,6 +174,14 @@ class FVKSolver:
         cl, cd = C.c_double(), C.c_double()
         _check(self.L.fvk_wall_forces(self.h, rho_ref, vx_ref, vy_ref, length_ref, C.byref(cl), C.byref(cd)))
         return cl.value, cd.value
+
+    def wall_forces_vjp(self, rho_ref, vx_ref, vy_ref, length_ref=1.0, w_cl=1.0, w_cd=0.0, want_x: bool = True):
+        """Gradient of w_cl Cl + w_cd Cd with respect to the state and (optionally) the node coordinates."""
+        Ub = np.empty((self.n, 4))
+        Xb = np.empty((self.mesh.n_nodes, 2)) if want_x else None
+        _check(self.L.fvk_wall_forces_vjp(self.h, rho_ref, vx_ref, vy_ref, length_ref, w_cl, w_cd, abi.as_dptr(Ub),
+                                          abi.as_dptr(Xb)))
+        return (Ub, Xb) if want_x else Ub
 
     def set_kernel_timing(self, on: bool):
         _check(self.L.fvk_set_kernel_timing(self.h, 1 if on else 0))

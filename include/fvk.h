@@ -24,7 +24,8 @@
  *
  * Entry points with no counterpart in the reference (extensions that
  * reduce to the reference path when switched off, see fvk_params):
- *   fvk_residual_eval, fvk_residual_vjp, fvk_wall_forces, fvk_batch_*.
+ *   fvk_residual_eval, fvk_residual_vjp, fvk_wall_forces, fvk_wall_forces_vjp,
+ *   fvk_batch_*.
  *
  * Conventions (same as the reference):
  *  - cell and face indices are the caller's (reference) order; the library
@@ -232,7 +233,10 @@ int fvk_residual_eval(fvk_solver* s, double* R);
 /* Extension: discrete adjoint (vector-Jacobian product) of R(U, X) for the
  * Euler path: given lambda [4*n_cells], returns U_bar = (dR/dU)^T lambda
  * [4*n_cells] and, if X_bar != NULL, (dR/dX)^T lambda [2*n_nodes] with X the
- * node coordinates.  Gradients and limiters are recomputed. */
+ * node coordinates (face geometry from the face's nodes; cell volume and
+ * centroid from the shoelace formulas over the polygon that the cell's
+ * faces close).  Gradients and limiters are recomputed.  Rusanov and HLLC
+ * fluxes; X_bar is not available for periodic meshes. */
 int fvk_residual_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
 
 /* Extension: pressure lift and drag coefficients on the wall patches
@@ -241,6 +245,12 @@ int fvk_residual_vjp(fvk_solver* s, const double* lambda, double* U_bar, double*
  * normalised by 0.5 rho_ref |V|^2 L_ref). */
 int fvk_wall_forces(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref,
                     double* cl, double* cd);
+
+/* Extension: gradient of J = w_cl Cl + w_cd Cd (the pressure forces of
+ * fvk_wall_forces) with respect to the state, U_bar [4*n_cells], and the
+ * node coordinates, X_bar [2*n_nodes]; either may be NULL.  Overwrites both. */
+int fvk_wall_forces_vjp(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref, double w_cl,
+                        double w_cd, double* U_bar, double* X_bar);
 
 /* Per-kernel device time accumulated while timing is on, measured with HIP
  * events on the solver's stream around each launch.  names is a
