@@ -217,6 +217,103 @@ def adjoint_leg(s, m, bcs, params, ic, steps_done, reps=5, n_check=32, h=1e-7):
     }
 
 
+def _gen_batch_case(prm):
+    from finitevolume_amd import voronoi as V
+    return V.naca_voronoi(prm["n_target"], prm["seed"])
+
+
+def batch_leg(n_cases, max_steps, drop=1e-6):
+    """Batched airfoil cases: case k a Voronoi mesh of about N_k cells around
+    NACA 0012 at its own Mach number and angle of attack (default_rng(1000 +
+    k)), config-2 numerics; every case runs until its density residual has
+    dropped by `drop` from its first-step value (or max_steps), first all
+    together through fvk_batch (one launch per kernel for all active cases,
+    steps replayed from a hipGraph), then one after the other, each alone on
+    the GPU.  Meshes are generated first, on the host, before the GPU is
+    used."""
+    from concurrent.futures import ProcessPoolExecutor
+    from finitevolume_amd import abi, mesh as M
+    from finitevolume_amd import voronoi as V
+    prms = [V.batch_case_params(k) for k in range(n_cases)]
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(max_workers=min(8, host_cores())) as ex:
+        meshes = list(ex.map(_gen_batch_case, prms))
+    t_gen = time.perf_counter() - t0
+    sizes = np.array([m.n_cells for m in meshes])
+    print(f"[bench] batch: {n_cases} meshes, {sizes.sum()} cells, generated in {t_gen:.1f}s",
+          file=sys.stderr, flush=True)
+    from finitevolume_amd.batch import FVKBatch
+    from finitevolume_amd.solver import FVKSolver
+    gamma = 1.4
+    bcs, ics = [], []
+    for prm in prms:
+        a = math.radians(prm["aoa"])
+        rho, pinf = 1.0, 1.0 / gamma
+        u, v = prm["mach"] * math.cos(a), prm["mach"] * math.sin(a)
+        ff = M.from_primitive(rho, u, v, pinf, gamma)
+        bcs.append([abi.make_bc(abi.BC_WALL), abi.make_bc(abi.BC_FARFIELD, ff)])
+        ics.append(abi.initial_condition(abi.IC_FREESTREAM, rho=rho, u=u, v=v, p=pinf))
+    params = abi.default_params(equation=abi.EQ_EULER, flux_scheme=abi.FLUX_HLLC, cfl=0.8, order=2,
+                                limiter=abi.LIMITER_VENKATAKRISHNAN, venkat_k=5.0, rk_stages=3, local_dt=1)
+    last = [time.perf_counter()]
+
+    def progress(done, active, now):
+        if time.perf_counter() - last[0] > 10.0:
+            last[0] = time.perf_counter()
+            print(f"[bench] batch: step {done}, {active} cases active, {now:.1f}s", file=sys.stderr, flush=True)
+
+    b = FVKBatch(meshes, bcs, params, ics)
+    b.set_graph(True)
+    rb = b.run_to_convergence(drop, max_steps, chunk=100, progress=progress)
+    del b
+    print(f"[bench] batch: all cases done in {rb['wall']:.1f}s ({rb['total_steps']} steps)", file=sys.stderr,
+          flush=True)
+    seq_steps = np.full(n_cases, -1)
+    seq_time = np.zeros(n_cases)
+    t_seq0 = time.perf_counter()
+    for k in range(n_cases):
+        g = FVKSolver(meshes[k], bcs[k], params, ics[k])
+        t1 = time.perf_counter()
+        first, done = None, 0
+        while done < max_steps:
+            r = g.step(100)
+            if first is None:
+                first = r[0, 0]
+            hit = np.nonzero(r[:, 0] <= drop * first)[0]
+            if hit.size:
+                seq_steps[k] = done + int(hit[0]) + 1
+                done += 100
+                break
+            done += 100
+        seq_time[k] = time.perf_counter() - t1
+        del g
+        progress(k + 1, n_cases - k - 1, time.perf_counter() - t_seq0)
+    t_seq = time.perf_counter() - t_seq0
+    conv = rb["steps"] > 0
+    useful = float(np.sum(sizes[conv] * 3 * rb["steps"][conv]))
+    tb = rb["time"][conv]
+    return {
+        "workload": "config5: %d Voronoi NACA 0012 cases (%d cells), config-2 numerics, each to a %.0e drop of "
+                    "its density residual (at most %d steps)" % (n_cases, int(sizes.sum()), drop, max_steps),
+        "cells_total": int(sizes.sum()),
+        "cells_min_max": [int(sizes.min()), int(sizes.max())],
+        "converged": int(conv.sum()),
+        "steps_to_converge_p50_max": [int(np.median(rb["steps"][conv])) if conv.any() else None,
+                                      int(rb["steps"][conv].max()) if conv.any() else None],
+        "batched_wall_s": round(rb["wall"], 2),
+        "sequential_wall_s": round(t_seq, 2),
+        "batched_over_sequential_speedup": round(t_seq / rb["wall"], 2),
+        "batched_time_to_converge_p50_p99_s": [round(float(np.percentile(tb, 50)), 2),
+                                               round(float(np.percentile(tb, 99)), 2)] if conv.any() else None,
+        "sequential_time_to_converge_p50_p99_s": [round(float(np.percentile(seq_time[conv], 50)), 3),
+                                                  round(float(np.percentile(seq_time[conv], 99)), 3)]
+        if conv.any() else None,
+        "batched_useful_Gcell_updates_per_s": round(useful / rb["wall"] / 1e9, 3),
+        "mesh_generation_s": round(t_gen, 1),
+        "hip_graph": True,
+    }
+
+
 def host_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -249,6 +346,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-2r", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--batch", action="store_true",
+                    help="run only the batched-airfoil workload (64 Voronoi cases to convergence) and print its line")
+    ap.add_argument("--batch-cases", type=int, default=64)
+    ap.add_argument("--batch-max-steps", type=int, default=20000)
     ap.add_argument("--variant", type=int, default=0, help="Euler kernel variant (0 = library default)")
     ap.add_argument("--block", type=int, default=0, help="Euler workgroup size (0 = library default)")
     ap.add_argument("--sweep", action="store_true", help="time every kernel variant and block size first")
@@ -258,6 +359,13 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding is a design, not a build, in this repository)", file=sys.stderr)
         sys.exit(2)
+
+    if args.batch:
+        line = batch_leg(args.batch_cases, args.batch_max_steps)
+        line.update({"metric": "batched airfoil cases: wall time to convergence, batched vs sequential",
+                     "n_gpus": 1, "dtype": "f64", "data": "synthetic (generated Voronoi meshes, free-stream start)"})
+        print(json.dumps(line))
+        return
 
     from finitevolume_amd.solver import FVKSolver, copy_probe
 

@@ -143,7 +143,9 @@ void fvk_destroy(fvk_solver* s) {
     delete s;
 }
 
-static int upload_state(fvk_solver* s, const double* U4, const double* turb, bool keep_turb) {
+}  // extern "C"
+
+int fvk::upload_state(fvk_solver* s, const double* U4, const double* turb, bool keep_turb) {
     const int64_t nc = s->nc, ld = s->ld;
     const int nv = s->nv;
     std::vector<double> h((size_t)nv * ld, 0.0);
@@ -180,8 +182,19 @@ static int upload_state(fvk_solver* s, const double* U4, const double* turb, boo
     return hip_check(e, "upload state");
 }
 
+extern "C" {
+
 int fvk_create(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, const fvk_params* params,
                const fvk_initial_condition* ic, fvk_solver** out) {
+    return fvk::create_solver(mesh, bcs, n_bcs, params, ic, nullptr, 0, out);
+}
+
+}  // extern "C"
+
+namespace fvk {
+
+int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, const fvk_params* params,
+                  const fvk_initial_condition* ic, const int32_t* cell_case, int32_t n_cases, fvk_solver** out) {
     if (!mesh || !params || !out) return set_error(FVK_ERR_INVALID, "NULL argument to fvk_create");
     *out = nullptr;
     if (params->equation < FVK_EQ_EULER || params->equation > FVK_EQ_DIFFUSION)
@@ -215,6 +228,10 @@ int fvk_create(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, cons
     s->nv = nvars_eq(params->equation);
     s->n_bcs = n_bcs;
     s->bcs.assign(bcs, bcs + n_bcs);
+    if (cell_case) {
+        s->n_cases = n_cases;
+        s->cell_case_old.assign(cell_case, cell_case + mesh->n_cells);
+    }
     for (int i = 0; i < n_bcs; ++i) {
         int t = bcs[i].type;
         if (t < 0 || t > 2) {
@@ -347,6 +364,10 @@ int fvk_create(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, cons
     *out = s;
     return FVK_OK;
 }
+
+}  // namespace fvk
+
+extern "C" {
 
 int fvk_set_field(fvk_solver* s, const double* U4, const double* turb, int64_t n_cells) {
     if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");

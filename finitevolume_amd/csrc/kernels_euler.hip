@@ -57,6 +57,14 @@ struct EulerArgs {
     int want_res;         // accumulate residual norms
     int slot;             // dt slot of this step
     int64_t hist_idx;
+    // batched cases (null when there is one case): case ranges in device
+    // order, per-case activity, case of the first cell of each logical block
+    const int* case_start;
+    const int* active;
+    const int* case_lo;
+    double* case_partials;
+    int n_cases;
+    int stages;
 };
 
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
@@ -67,6 +75,27 @@ __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned
 __device__ __forceinline__ int xcd_index() {
     const int per = gridDim.x / kXcds;
     return ((blockIdx.x % kXcds) * per + blockIdx.x / kXcds) * blockDim.x + threadIdx.x;
+}
+
+__device__ __forceinline__ int logical_block() {
+    const int per = gridDim.x / kXcds;
+    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+
+// case of device cell c (binary search of the case ranges)
+__device__ __forceinline__ int case_of(const EulerArgs& a, int c) {
+    int lo = 0, hi = a.n_cases;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.case_start[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// false for the cells of a frozen case
+__device__ __forceinline__ bool cell_active(const EulerArgs& a, int c) {
+    return a.active == nullptr || a.active[case_of(a, c)] != 0;
 }
 
 __device__ __forceinline__ State st(const double4& u) { return State{u.x, u.y, u.z, u.w}; }
@@ -126,6 +155,59 @@ __device__ __forceinline__ void block_partials4(const double (&v)[4], double* pa
         for (int q = 0; q < 4; ++q) partials[q * gridDim.x + blockIdx.x] = sm[q][0];
     }
 }
+
+// batched cases: the block's cells belong to case_lo[block] and possibly the
+// next case; the squared residuals are summed separately for the two (fixed
+// tree order) into case_partials[(q * 2 + side) * nblocks + logical block]
+__device__ __forceinline__ void block_partials_cases(const EulerArgs& a, int side, const double (&v)[4]) {
+    __shared__ double sm[4][kMaxBlock];
+    const int lb = logical_block();
+    for (int s2 = 0; s2 < 2; ++s2) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sm[q][threadIdx.x] = side == s2 ? v[q] : 0.0;
+        __syncthreads();
+        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a.case_partials[(q * 2 + s2) * gridDim.x + lb] = sm[q][0];
+        }
+        __syncthreads();
+    }
+}
+
+// one block per case: the case's residual norms from the block partials of
+// its cell range, into row (*step mod cap) of the per-case history
+constexpr int kCaseFinBlock = 256;
+__global__ __launch_bounds__(kCaseFinBlock) void k_case_finalize(const double* partials, int nblocks,
+                                                                 const int* case_blocks, const int* case_lo,
+                                                                 double* case_res, const unsigned int* step,
+                                                                 int cap, int n_cases) {
+    __shared__ double sm[kCaseFinBlock];
+    const int k = blockIdx.x;
+    const int b0 = case_blocks[2 * k], b1 = case_blocks[2 * k + 1];
+    const int64_t row = (int64_t)(*step % (unsigned int)cap);
+    for (int q = 0; q < 4; ++q) {
+        double acc = 0.0;
+        for (int lb = b0 + (int)threadIdx.x; lb <= b1; lb += kCaseFinBlock)
+            acc += partials[(q * 2 + (case_lo[lb] == k ? 0 : 1)) * nblocks + lb];
+        sm[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = kCaseFinBlock / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) case_res[(row * n_cases + k) * 4 + q] = sqrt(sm[0]);
+        __syncthreads();
+    }
+}
+
+__global__ void k_step_tick(unsigned int* step) { *step += 1u; }
 
 // one block: residual norms = sqrt of the sums of the partials (fixed order),
 // the step's dt into the history, and the next step's dt slot re-armed
@@ -214,7 +296,7 @@ template <int D>
 __global__ __launch_bounds__(kMaxBlock) void k_e_grad(EulerArgs a) {
     const MeshView& m = a.m;
     const int c = xcd_index();
-    if (c >= m.nc) return;
+    if (c >= m.nc || !cell_active(a, c)) return;
     const double g = a.gamma;
     const int64_t ld = m.ld;
     int2 en[D];
@@ -375,7 +457,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_face(EulerArgs a) {
     const int f = xcd_index();
     const double g = a.gamma;
     double cand = DBL_MAX;
-    if (f < m.nf) {
+    if (f < m.nf && cell_active(a, m.fl[f])) {
         const int cl = m.fl[f], cr = m.fr[f];
         const double4 fg = m.fgeo[f];
         const State L = st(a.U[cl]);
@@ -413,7 +495,17 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
     const MeshView& m = a.m;
     const int c = xcd_index();
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
-    if (c < m.nc) {
+    int side = 0;
+    bool act = true;
+    if (a.active && c < m.nc) {
+        const int k = case_of(a, c);
+        act = a.active[k] != 0;
+        side = k != a.case_lo[logical_block()];
+        // a frozen case keeps its state: forward Euler writes the other buffer
+        // of the pair, so it is copied; RK stages leave the base state alone
+        if (!act && a.stages == 1 && !a.R) a.Unew[c] = a.U[c];
+    }
+    if (c < m.nc && act) {
         const int64_t ld = m.ld;
         int2 en[D];
         double4 F[D];
@@ -441,7 +533,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
         }
         update_cell(a, c, u, r0, r1, r2, r3, dmin);
     }
-    finish_residual(a, r0, r1, r2, r3);
+    if (a.active) {
+        if (a.want_res) {
+            const double sq[4] = {r0 * r0, r1 * r1, r2 * r2, r3 * r3};
+            block_partials_cases(a, side, sq);
+        }
+    } else {
+        finish_residual(a, r0, r1, r2, r3);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -470,6 +569,14 @@ EulerArgs make_args(fvk_solver* s) {
     a.tol = s->p.exact_riemann_tol;
     a.maxit = s->p.exact_riemann_max_iter;
     a.limiter = s->p.limiter;
+    a.stages = s->p.rk_stages;
+    if (s->n_cases > 0) {
+        a.case_start = s->d_case_start;
+        a.active = s->d_active;
+        a.case_lo = s->d_case_lo;
+        a.case_partials = s->case_partials;
+        a.n_cases = s->n_cases;
+    }
     return a;
 }
 
@@ -539,8 +646,10 @@ void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
     }
     if (a.want_res) {
         timer_begin(s, "euler_finalize", &ev);
-        k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(a.partials, grid_for(s->nc, s->block), 4, a.res_hist,
-                                                     a.dt_hist, a.hist_idx, a.dtmin, a.slot);
+        if (s->n_cases > 0) launch_case_finalize(s);
+        else
+            k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(a.partials, grid_for(s->nc, s->block), 4, a.res_hist,
+                                                         a.dt_hist, a.hist_idx, a.dtmin, a.slot);
         timer_end(s, ev);
     }
 }
@@ -605,6 +714,15 @@ int launch_euler_residual_eval(fvk_solver* s, double* R_dev) {
     a.R = reinterpret_cast<double4*>(R_dev);
     run_stage(s, a, false);
     return hip_check(hipGetLastError(), "euler residual launch");
+}
+
+// per-case residual norms of the step and the step counter
+int launch_case_finalize(fvk_solver* s) {
+    k_case_finalize<<<s->n_cases, kCaseFinBlock, 0, s->stream>>>(s->case_partials, grid_for(s->nc, s->block),
+                                                                s->d_case_blocks, s->d_case_lo, s->case_res,
+                                                                s->d_step, s->hist_cap, s->n_cases);
+    k_step_tick<<<1, 1, 0, s->stream>>>(s->d_step);
+    return hip_check(hipGetLastError(), "case finalize launch");
 }
 
 // limited gradients of the current state into the gradient buffer (the

@@ -169,6 +169,15 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
         }
     }
     rcm_order((int)nc, aptr, adj, s->cell_new2old);
+    if (!s->cell_case_old.empty()) {
+        // batched cases: one contiguous range per case, RCM order within it
+        // (the cases are disconnected, so this only reorders whole components)
+        const std::vector<int>& cc = s->cell_case_old;
+        std::stable_sort(s->cell_new2old.begin(), s->cell_new2old.end(), [&](int a, int b) { return cc[a] < cc[b]; });
+        s->case_cell_start.assign(s->n_cases + 1, 0);
+        for (int64_t c = 0; c < nc; ++c) s->case_cell_start[cc[c] + 1]++;
+        for (int k = 0; k < s->n_cases; ++k) s->case_cell_start[k + 1] += s->case_cell_start[k];
+    }
     s->cell_old2new.assign(nc, 0);
     for (int64_t i = 0; i < nc; ++i) s->cell_old2new[s->cell_new2old[i]] = (int)i;
     const std::vector<int>& o2n = s->cell_old2new;

@@ -13,7 +13,7 @@
 
 namespace fvk {
 
-constexpr int kMaxPatches = 64;
+constexpr int kMaxPatches = 1024;
 constexpr int kMaxEulerDegree = 16;  // most faces of one cell the per-cell Euler kernels take
 
 // Per-patch boundary data on the device (fvk_bc reduced to what kernels read).
@@ -126,6 +126,20 @@ struct fvk_solver {
     // residual VJP workspace, built on first use
     fvk::AdjointWork* adj = nullptr;
 
+    // batched independent cases (fvk_batch_*): every case's cells are one
+    // contiguous range of the device order, so each logical block of a
+    // per-cell kernel holds at most two cases
+    int n_cases = 0;
+    std::vector<int> cell_case_old;              // [nc] case of each caller-order cell
+    std::vector<int> case_cell_start;            // [n_cases + 1] device order
+    int* d_case_start = nullptr;                 // device copy
+    int* d_active = nullptr;                     // [n_cases] 1: stepping, 0: frozen
+    int* d_case_lo = nullptr;                    // [nblocks] case of the first cell of each logical block
+    int* d_case_blocks = nullptr;                // [2 n_cases] first and last logical block of each case
+    double* case_res = nullptr;                  // [hist_cap][n_cases][4] per-case residual norms
+    double* case_partials = nullptr;             // [4][2][nblocks] block sums split at a case boundary
+    unsigned int* d_step = nullptr;              // device step counter (row of case_res)
+
     // timing
     bool timing = false;
     std::vector<fvk::KernelTimer> timers;
@@ -138,6 +152,9 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d);
 void rcm_order(int n, const std::vector<int>& ptr, const std::vector<int>& adj, std::vector<int>& order);
 
 // ---- api helpers ----
+int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, const fvk_params* params,
+                  const fvk_initial_condition* ic, const int32_t* cell_case, int32_t n_cases, fvk_solver** out);
+int upload_state(fvk_solver* s, const double* U4, const double* turb, bool keep_turb);
 int set_error(int code, const char* fmt, ...);
 int hip_check(hipError_t e, const char* what);
 void* dev_alloc(fvk_solver* s, size_t bytes);
@@ -151,6 +168,7 @@ int launch_wall_distance(fvk_solver* s, const int* wall_faces, int nw, const dou
                          const int* fnodes_dev, double* out);
 int launch_residual_eval(fvk_solver* s, double* R_dev);
 int launch_euler_gradients(fvk_solver* s);
+int launch_case_finalize(fvk_solver* s);
 int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
 void adjoint_release(fvk_solver* s);
 int launch_copy_probe(double* dst, const double* src, int64_t n, hipStream_t st);

@@ -265,6 +265,39 @@ int64_t fvk_device_bytes(const fvk_solver* s);
 /* A plain device copy of n doubles (bandwidth calibration for counters). */
 int fvk_copy_probe(int64_t n_doubles, int32_t reps, double* ms_per_copy);
 
+/* Extension (no reference counterpart): independent cases stepped together.
+ * The cases' meshes are the disconnected parts of one device mesh, each a
+ * contiguous range of cells; every kernel launch covers all cases, the
+ * residual norms are reduced per case, and a case can be frozen (its work is
+ * skipped, its state kept).  Euler only, local time steps, no periodic
+ * faces, each case at least one workgroup of cells.  bcs holds the cases'
+ * boundary conditions one after the other (n_bcs[k] for case k, whose patch
+ * ids index its own); ics[k] is case k's initial condition; params are
+ * shared. */
+typedef struct fvk_batch fvk_batch;
+int fvk_batch_create(int32_t n_cases, const fvk_mesh_desc* meshes, const fvk_bc* bcs, const int32_t* n_bcs,
+                     const fvk_params* params, const fvk_initial_condition* ics, fvk_batch** out);
+void fvk_batch_destroy(fvk_batch* b);
+/* nsteps steps of every active case; case_residuals (may be NULL) receives
+ * [nsteps][n_cases][4] L2 norms (a frozen case's rows are 0). */
+int fvk_batch_step(fvk_batch* b, int32_t nsteps, double* case_residuals);
+/* active[n_cases]: 1 steps the case, 0 freezes it. */
+int fvk_batch_set_active(fvk_batch* b, const int32_t* active);
+/* Replay each step from a captured hipGraph (1) or launch it directly (0). */
+int fvk_batch_set_graph(fvk_batch* b, int32_t enable);
+/* Case k's state, AoS [4 * n_cells of case k] in its own cell order. */
+int fvk_batch_get_field(fvk_batch* b, int32_t k, double* U);
+/* The solver over the union mesh (kernel timing, device bytes). */
+fvk_solver* fvk_batch_solver(fvk_batch* b);
+
+/* Mesh-generation helper (host only, no reference counterpart): Bridson
+ * Poisson-disk samples of radius `radius` in [x0,x1) x [y0,y1), outside a
+ * symmetric NACA 4-digit section of the given thickness (chord 1, leading
+ * edge at the origin) dilated by `margin`, after the n_seed caller points.
+ * Deterministic for `seed`.  out_xy [2*cap]; the count is *n_out. */
+int fvk_poisson_disk(double x0, double y0, double x1, double y1, double radius, uint64_t seed, const double* seed_xy,
+                     int64_t n_seed, double thickness, double margin, double* out_xy, int64_t cap, int64_t* n_out);
+
 const char* fvk_last_error(void);
 
 #ifdef __cplusplus
