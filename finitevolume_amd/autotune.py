@@ -1,0 +1,128 @@
+"""Launch-shape autotuner for the Euler stage kernels.
+
+For the mesh and numerics at hand it times every kernel variant (face sweep
++ gather, or fused per cell) and workgroup size (64 to 512 threads, one
+cell or face per thread) for a few steps, and keeps the fastest.  Choices
+are cached in a JSON file (FVK_TUNE_CACHE, default
+~/.cache/fvk/autotune.json) under a key made of the device name, the mesh
+size and largest cell degree, and the numerics that change the kernels, so
+a later run on the same kind of case reuses them.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import time
+from typing import Dict, Iterable, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import abi
+
+VARIANTS = (1, 2)
+BLOCKS = (64, 128, 256, 512)
+
+
+def _cache_path() -> str:
+    return os.environ.get("FVK_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "fvk", "autotune.json"))
+
+
+def _device_name() -> str:
+    try:
+        with open("/sys/class/kfd/kfd/topology/nodes/1/name") as f:
+            return f.read().strip() or "gpu"
+    except OSError:
+        return "gpu"
+
+
+def signature(n_cells: int, n_faces: int, max_degree: int, params: abi.ParamsC, batch: bool = False) -> str:
+    key = (_device_name(), int(n_cells), int(n_faces), int(max_degree), params.equation, params.flux_scheme,
+           params.order, params.limiter, params.rk_stages, params.local_dt, batch)
+    return hashlib.sha1(repr(key).encode()).hexdigest()[:16]
+
+
+def _load() -> Dict:
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(db: Dict) -> None:
+    path = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(db, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # a read-only home only loses the cache
+
+
+def _time_solver(make, steps: int, warmup: int) -> float:
+    s = make()
+    s.step(warmup)
+    t0 = time.perf_counter()
+    s.step(steps)
+    return (time.perf_counter() - t0) / steps
+
+
+def tune(mesh, bcs, params: abi.ParamsC, ic, steps: int = 20, warmup: int = 3,
+         variants: Iterable[int] = VARIANTS, blocks: Iterable[int] = BLOCKS, use_cache: bool = True) -> Dict:
+    """Fastest (variant, block) for one case; sets them on `params` too."""
+    from .solver import FVKSolver
+    deg = int(np.diff(mesh.cell_face_ptr).max())
+    key = signature(mesh.n_cells, mesh.n_faces, deg, params)
+    db = _load() if use_cache else {}
+    if key in db:
+        best = db[key]
+    else:
+        table = {}
+        for v in variants:
+            for b in blocks:
+                p = abi.ParamsC.from_buffer_copy(params)
+                p.kernel_variant, p.block_size = v, b
+                table[f"v{v}_b{b}"] = _time_solver(lambda: FVKSolver(mesh, bcs, p, ic), steps, warmup) * 1e3
+        name = min(table, key=table.get)
+        best = {"variant": int(name[1]), "block": int(name.split("_b")[1]), "ms_per_step": table[name],
+                "table_ms_per_step": {k: round(x, 4) for k, x in table.items()}}
+        if use_cache:
+            db[key] = best
+            _store(db)
+    params.kernel_variant, params.block_size = best["variant"], best["block"]
+    return best
+
+
+def tune_batch(meshes: Sequence, bcs: Sequence, params: abi.ParamsC, ics: Sequence, steps: int = 10,
+               warmup: int = 2, blocks: Iterable[int] = (128, 256, 512), use_cache: bool = True) -> Dict:
+    """Fastest workgroup size for a batch (the batch runs the face sweep +
+    gather variant); sets it on `params` too.  Workgroups larger than the
+    smallest case are skipped."""
+    from .batch import FVKBatch
+    n = sum(m.n_cells for m in meshes)
+    nf = sum(m.n_faces for m in meshes)
+    deg = max(int(np.diff(m.cell_face_ptr).max()) for m in meshes)
+    key = signature(n, nf, deg, params, batch=True)
+    db = _load() if use_cache else {}
+    if key in db:
+        best = db[key]
+    else:
+        smallest = min(m.n_cells for m in meshes)
+        table = {}
+        for b in blocks:
+            if b > smallest:
+                continue
+            p = abi.ParamsC.from_buffer_copy(params)
+            p.kernel_variant, p.block_size = 1, b
+            table[f"b{b}"] = _time_solver(lambda: FVKBatch(meshes, bcs, p, ics), steps, warmup) * 1e3
+        name = min(table, key=table.get)
+        best = {"variant": 1, "block": int(name[1:]), "ms_per_step": table[name],
+                "table_ms_per_step": {k: round(x, 4) for k, x in table.items()}}
+        if use_cache:
+            db[key] = best
+            _store(db)
+    params.kernel_variant, params.block_size = best["variant"], best["block"]
+    return best

@@ -408,6 +408,38 @@ int fvk_get_field(const fvk_solver* sc, double* U4, double* turb, int64_t n_cell
     return FVK_OK;
 }
 
+int fvk_set_state(fvk_solver* s, const double* U_aos, int64_t n_cells) {
+    if (!s || !U_aos) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (n_cells != s->nc)
+        return set_error(FVK_ERR_INVALID, "set_state: %lld cells given, mesh has %lld", (long long)n_cells, (long long)s->nc);
+    const int nv = s->nv;
+    if (nv <= 4) return fvk_set_field(s, U_aos, nullptr, n_cells);
+    const int nt = nv - 4;
+    std::vector<double> U(4 * (size_t)n_cells), T((size_t)nt * n_cells);
+    for (int64_t c = 0; c < n_cells; ++c) {
+        for (int q = 0; q < 4; ++q) U[4 * c + q] = U_aos[nv * c + q];
+        for (int q = 0; q < nt; ++q) T[nt * c + q] = U_aos[nv * c + 4 + q];
+    }
+    return fvk_set_field(s, U.data(), T.data(), n_cells);
+}
+
+int fvk_get_state(const fvk_solver* s, double* U_aos, int64_t n_cells) {
+    if (!s || !U_aos) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (n_cells != s->nc)
+        return set_error(FVK_ERR_INVALID, "get_state: %lld cells given, mesh has %lld", (long long)n_cells, (long long)s->nc);
+    const int nv = s->nv;
+    if (nv <= 4) return fvk_get_field(s, U_aos, nullptr, n_cells);
+    const int nt = nv - 4;
+    std::vector<double> U(4 * (size_t)n_cells), T((size_t)nt * n_cells);
+    int rc = fvk_get_field(s, U.data(), T.data(), n_cells);
+    if (rc != FVK_OK) return rc;
+    for (int64_t c = 0; c < n_cells; ++c) {
+        for (int q = 0; q < 4; ++q) U_aos[nv * c + q] = U[4 * c + q];
+        for (int q = 0; q < nt; ++q) U_aos[nv * c + 4 + q] = T[nt * c + q];
+    }
+    return FVK_OK;
+}
+
 int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hist) {
     if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
     if (nsteps < 0) return set_error(FVK_ERR_INVALID, "nsteps < 0");

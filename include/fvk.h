@@ -209,6 +209,11 @@ int fvk_residual(const fvk_solver* s, double* norms);
  * set_field(U) overloads. */
 int fvk_set_field(fvk_solver* s, const double* U4, const double* turb, int64_t n_cells);
 int fvk_get_field(const fvk_solver* s, double* U4, double* turb, int64_t n_cells);
+/* The same two calls under the names of a state-vector binding: the nvars
+ * doubles of each cell AoS, [nvars*n_cells] (Euler/NS: 4; SA: 5 with nut
+ * last; SST: 6 with k, omega last; diffusion: 1). */
+int fvk_set_state(fvk_solver* s, const double* U_aos, int64_t n_cells);
+int fvk_get_state(const fvk_solver* s, double* U_aos, int64_t n_cells);
 
 int fvk_set_cfl(fvk_solver* s, double cfl);
 double fvk_gas_gamma(const fvk_solver* s);

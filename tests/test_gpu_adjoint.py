@@ -178,3 +178,4 @@ def test_exact_flux_rejected():
     g = FVKSolver(m, bcs, p, ic)
     with pytest.raises(Exception, match="exact Riemann"):
         g.residual_vjp(np.zeros((m.n_cells, 4)), want_x=False)
+

@@ -86,3 +86,23 @@ def test_forward_euler_graph_pairs():
     a.step(4)
     for k in range(2):
         assert np.array_equal(a.field(k), c.field(k)), k
+
+
+def test_autotuner_picks_a_valid_shape(three, tmp_path, monkeypatch):
+    from finitevolume_amd import autotune
+    monkeypatch.setenv("FVK_TUNE_CACHE", str(tmp_path / "tune.json"))
+    meshes, bcs, params, ics = three
+    p = abi.ParamsC.from_buffer_copy(params)
+    best = autotune.tune(meshes[0], bcs[0], p, ics[0], steps=5, warmup=1)
+    assert (p.kernel_variant, p.block_size) == (best["variant"], best["block"])
+    assert best["block"] in autotune.BLOCKS and best["variant"] in autotune.VARIANTS
+    again = autotune.tune(meshes[0], bcs[0], abi.ParamsC.from_buffer_copy(params), ics[0])
+    assert again == best  # served from the cache
+    g = FVKSolver(meshes[0], bcs[0], p, ics[0])
+    ref = FVKSolver(meshes[0], bcs[0], params, ics[0])
+    g.step(10)
+    ref.step(10)
+    assert np.array_equal(g.field(), ref.field())
+    pb = abi.ParamsC.from_buffer_copy(params)
+    bb = autotune.tune_batch(meshes, bcs, pb, ics, steps=3, warmup=1)
+    assert pb.block_size == bb["block"] and pb.kernel_variant == 1
