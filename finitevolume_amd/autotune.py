@@ -20,7 +20,7 @@ import numpy as np
 
 from . import abi
 
-VARIANTS = (1, 2)
+VARIANTS = (1, 2, 3)
 BLOCKS = (64, 128, 256, 512)
 
 
@@ -80,12 +80,16 @@ def tune(mesh, bcs, params: abi.ParamsC, ic, steps: int = 20, warmup: int = 3,
     if key in db:
         best = db[key]
     else:
+        from .solver import FVKError
         table = {}
         for v in variants:
             for b in blocks:
                 p = abi.ParamsC.from_buffer_copy(params)
                 p.kernel_variant, p.block_size = v, b
-                table[f"v{v}_b{b}"] = _time_solver(lambda: FVKSolver(mesh, bcs, p, ic), steps, warmup) * 1e3
+                try:
+                    table[f"v{v}_b{b}"] = _time_solver(lambda: FVKSolver(mesh, bcs, p, ic), steps, warmup) * 1e3
+                except FVKError:
+                    continue  # not built for this case (e.g. a tile too large for LDS)
         name = min(table, key=table.get)
         best = {"variant": int(name[1]), "block": int(name.split("_b")[1]), "ms_per_step": table[name],
                 "table_ms_per_step": {k: round(x, 4) for k, x in table.items()}}

@@ -214,8 +214,10 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
     if (n_bcs < 0 || n_bcs > kMaxPatches) return set_error(FVK_ERR_INVALID, "n_bcs out of range (%d)", n_bcs);
     if (n_bcs > 0 && !bcs) return set_error(FVK_ERR_INVALID, "bcs is NULL");
 
-    if (params->kernel_variant < 0 || params->kernel_variant > 2)
-        return set_error(FVK_ERR_INVALID, "kernel_variant must be 0, 1 or 2 (got %d)", params->kernel_variant);
+    if (params->kernel_variant < 0 || params->kernel_variant > 3)
+        return set_error(FVK_ERR_INVALID, "kernel_variant must be 0, 1, 2 or 3 (got %d)", params->kernel_variant);
+    if (params->kernel_variant == 3 && (params->equation != FVK_EQ_EULER || cell_case))
+        return set_error(FVK_ERR_UNSUPPORTED, "the fused tile kernels (variant 3) are built for single Euler cases");
     if (params->block_size != 0 && params->block_size != 64 && params->block_size != 128 &&
         params->block_size != 256 && params->block_size != 512)
         return set_error(FVK_ERR_INVALID, "block_size must be 0, 64, 128, 256 or 512 (got %d)", params->block_size);
@@ -224,7 +226,7 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
     if (!s) return set_error(FVK_ERR_NOMEM, "host allocation failed");
     s->p = *params;
     s->variant = params->kernel_variant == 0 ? 1 : params->kernel_variant;
-    s->block = params->block_size == 0 ? 512 : params->block_size;
+    s->block = params->block_size != 0 ? params->block_size : (s->variant == 3 ? 256 : 512);
     s->nv = nvars_eq(params->equation);
     s->n_bcs = n_bcs;
     s->bcs.assign(bcs, bcs + n_bcs);

@@ -544,6 +544,239 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// fused cell-tile pass (variant 3): one workgroup per tile of graph-adjacent
+// cells.  The tile's states and its halo (one ring for the fluxes, a second
+// for the neighbours' gradients) are staged in LDS; gradients of the owned
+// cells and the first ring, the fluxes of every face of the owned cells and
+// the gather + update then read LDS only.  Every value is computed by the
+// same operations as in the face sweep + gather kernels, so the states are
+// bit-identical; faces on a tile's edge are evaluated by both tiles.
+// ---------------------------------------------------------------------------
+template <int D, int SCHEME, int ORDER>
+__global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
+    extern __shared__ double lds[];
+    const MeshView& m = a.m;
+    const int t = logical_block();
+    const double g = a.gamma;
+    const int bs = blockDim.x, tid = threadIdx.x;
+    const int MG = m.tmax_g;
+    double* sU = lds;           // [MG][4] conserved state, owned cells then ring 1
+    double* sG = lds + 4 * MG;  // [MG][8] limited gradients
+    double* sW = lds + 12 * MG; // [tmax_l][4] primitives of all local cells; then
+    double* sF = sW;            //   [tmax_f][4] face flux x length
+    double* sDt = sW + 4 * m.tmax_f;  // [tmax_f] dt candidates
+    int c0 = 0, n = 0, hp = 0, h1 = 0, nh = 0, f0 = 0, nft = 0;
+    if (t < m.ntiles) {
+        c0 = m.tstart[t];
+        n = m.tstart[t + 1] - c0;
+        hp = m.thptr[t];
+        h1 = m.th1[t];
+        nh = m.thptr[t + 1] - hp;
+        f0 = m.tfptr[t];
+        nft = m.tfptr[t + 1] - f0;
+    }
+    const int ng = n + h1, nl = n + nh;
+    const int2* te = m.tell + (int64_t)t * D * MG;
+
+    // 1. states (and primitives) of the tile and its halo
+    for (int j = tid; j < nl; j += bs) {
+        const int gc = j < n ? c0 + j : m.thalo[hp + j - n];
+        const double4 u = a.U[gc];
+        if (j < ng) {
+            sU[4 * j] = u.x;
+            sU[4 * j + 1] = u.y;
+            sU[4 * j + 2] = u.z;
+            sU[4 * j + 3] = u.w;
+        }
+        if (ORDER == 2) {
+            const State U = st(u);
+            sW[4 * j] = U.r;
+            sW[4 * j + 1] = U.ru / U.r;
+            sW[4 * j + 2] = U.rv / U.r;
+            sW[4 * j + 3] = pressure(U, g);
+        }
+    }
+    __syncthreads();
+
+    // 2. limited LSQ gradients of the owned cells and ring 1 (as k_e_grad)
+    if (ORDER == 2) {
+        for (int j = tid; j < ng; j += bs) {
+            const int gc = j < n ? c0 + j : m.thalo[hp + j - n];
+            int2 en[D];
+            double4 eg[D];
+#pragma unroll
+            for (int k = 0; k < D; ++k) en[k] = te[k * MG + j];
+#pragma unroll
+            for (int k = 0; k < D; ++k) eg[k] = m.egeo[k * m.ld + gc];
+            const State Uc{sU[4 * j], sU[4 * j + 1], sU[4 * j + 2], sU[4 * j + 3]};
+            const double wc[4] = {sW[4 * j], sW[4 * j + 1], sW[4 * j + 2], sW[4 * j + 3]};
+            double bx[4] = {0.0, 0.0, 0.0, 0.0}, by[4] = {0.0, 0.0, 0.0, 0.0};
+            double wmin[4], wmax[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) wmin[q] = wmax[q] = wc[q];
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                if (en[k].x < 0) continue;
+                double wn[4];
+                if (en[k].y < 0) {
+                    const double4 fg = m.fgeo[en[k].x >> 1];
+                    const State Gs = euler_ghost(m.bc[-1 - en[k].y], Uc, fg.x, fg.y, g);
+                    const double wg0 = Gs.r, wg1 = Gs.ru / Gs.r, wg2 = Gs.rv / Gs.r, wg3 = pressure(Gs, g);
+                    wn[0] = 0.5 * (wc[0] + wg0);
+                    wn[1] = 0.5 * (wc[1] + wg1);
+                    wn[2] = 0.5 * (wc[2] + wg2);
+                    wn[3] = 0.5 * (wc[3] + wg3);
+                } else {
+                    const int l = en[k].y;
+                    wn[0] = sW[4 * l];
+                    wn[1] = sW[4 * l + 1];
+                    wn[2] = sW[4 * l + 2];
+                    wn[3] = sW[4 * l + 3];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double dphi = wn[q] - wc[q];
+                    bx[q] += eg[k].x * dphi;
+                    by[q] += eg[k].y * dphi;
+                    wmin[q] = smin(wmin[q], wn[q]);
+                    wmax[q] = smax(wmax[q], wn[q]);
+                }
+            }
+            const double4 iv = m.clsq[gc];
+            double gx[4], gy[4], lim[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                gx[q] = iv.x * bx[q] + iv.y * by[q];
+                gy[q] = iv.y * bx[q] + iv.z * by[q];
+                lim[q] = 1.0;
+            }
+            if (a.limiter == FVK_LIMITER_VENKATAKRISHNAN) {
+                const double eps2 = iv.w;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    if (en[k].x < 0) continue;
+                    const double rx = eg[k].z, ry = eg[k].w;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const double d2 = gx[q] * rx + gy[q] * ry;
+                        double ph = 1.0;
+                        if (d2 > 0.0) ph = venkatakrishnan(wmax[q] - wc[q], d2, eps2);
+                        else if (d2 < 0.0) ph = venkatakrishnan(wmin[q] - wc[q], d2, eps2);
+                        lim[q] = smin(lim[q], ph);
+                    }
+                }
+            } else if (a.limiter == FVK_LIMITER_FIRST_ORDER) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) lim[q] = 0.0;
+            }
+            double* gj = sG + 8 * j;
+            gj[0] = lim[0] * gx[0]; gj[1] = lim[0] * gy[0]; gj[2] = lim[1] * gx[1]; gj[3] = lim[1] * gy[1];
+            gj[4] = lim[2] * gx[2]; gj[5] = lim[2] * gy[2]; gj[6] = lim[3] * gx[3]; gj[7] = lim[3] * gy[3];
+        }
+    }
+    __syncthreads();  // the primitives are dead from here: their space takes the fluxes
+
+    // 3. fluxes of the tile's faces (as k_e_face)
+    double cand_min = DBL_MAX;
+    for (int e = tid; e < nft; e += bs) {
+        const int4 fd = m.tface[f0 + e];
+        const int f = fd.x, l = fd.y, r = fd.z;
+        const double4 fg = m.fgeo[f];
+        const State L{sU[4 * l], sU[4 * l + 1], sU[4 * l + 2], sU[4 * l + 3]};
+        State R = L, Lf = L, Rf;
+        if (ORDER == 2) {
+            const double4 ro = m.frec[f];
+            const double* gl = sG + 8 * l;
+            Lf = reconstruct(L, double4{gl[0], gl[1], gl[2], gl[3]}, double4{gl[4], gl[5], gl[6], gl[7]}, ro.x, ro.y, g);
+            if (r >= 0) {
+                R = State{sU[4 * r], sU[4 * r + 1], sU[4 * r + 2], sU[4 * r + 3]};
+                const double* gr = sG + 8 * r;
+                Rf = reconstruct(R, double4{gr[0], gr[1], gr[2], gr[3]}, double4{gr[4], gr[5], gr[6], gr[7]}, ro.z,
+                                 ro.w, g);
+            } else {
+                Rf = euler_ghost(m.bc[-1 - r], Lf, fg.x, fg.y, g);
+            }
+        } else {
+            if (r >= 0) {
+                R = State{sU[4 * r], sU[4 * r + 1], sU[4 * r + 2], sU[4 * r + 3]};
+                Rf = R;
+            } else {
+                Rf = euler_ghost(m.bc[-1 - r], L, fg.x, fg.y, g);
+            }
+        }
+        const State F = num_flux<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
+        const double A = fg.z;
+        sF[4 * e] = A * F.r;
+        sF[4 * e + 1] = A * F.ru;
+        sF[4 * e + 2] = A * F.rv;
+        sF[4 * e + 3] = A * F.E;
+        if (a.face_dt) {
+            const double cand = dt_candidate(L, r >= 0 ? &R : nullptr, fg.x, fg.y, fg.w, g, a.cfl);
+            sDt[e] = cand;
+            cand_min = smin(cand_min, cand);
+        }
+    }
+    __syncthreads();
+
+    // 4. gather in Cell::faces order, owner minus, and the update (as k_e_gather)
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    if (tid < n) {
+        const short* ts = m.tslot + (int64_t)t * D * bs;
+        int2 en[D];
+        int lf[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) en[k] = te[k * MG + tid];
+#pragma unroll
+        for (int k = 0; k < D; ++k) lf[k] = ts[k * bs + tid];
+        double dmin = DBL_MAX;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            if (en[k].x < 0) continue;
+            const double* F = sF + 4 * lf[k];
+            if (en[k].x & 1) {
+                r0 = r0 - F[0]; r1 = r1 - F[1]; r2 = r2 - F[2]; r3 = r3 - F[3];
+            } else {
+                r0 += F[0]; r1 += F[1]; r2 += F[2]; r3 += F[3];
+            }
+            if (a.dt_src == 2) dmin = smin(dmin, sDt[lf[k]]);
+        }
+        const double4 u{sU[4 * tid], sU[4 * tid + 1], sU[4 * tid + 2], sU[4 * tid + 3]};
+        update_cell(a, c0 + tid, u, r0, r1, r2, r3, dmin);
+    }
+
+    // 5. block reductions in the (now free) LDS: residual partials, dt minimum
+    if (a.want_res) {
+        __syncthreads();
+        lds[tid] = r0 * r0;
+        lds[bs + tid] = r1 * r1;
+        lds[2 * bs + tid] = r2 * r2;
+        lds[3 * bs + tid] = r3 * r3;
+        __syncthreads();
+        for (int s2 = bs / 2; s2 > 0; s2 >>= 1) {
+            if (tid < s2) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) lds[q * bs + tid] += lds[q * bs + tid + s2];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a.partials[q * gridDim.x + blockIdx.x] = lds[q * bs];
+        }
+    }
+    if (a.face_dt) {
+        __syncthreads();
+        lds[tid] = cand_min;
+        __syncthreads();
+        for (int s2 = bs / 2; s2 > 0; s2 >>= 1) {
+            if (tid < s2) lds[tid] = smin(lds[tid], lds[tid + s2]);
+            __syncthreads();
+        }
+        if (tid == 0) atomicMin(&a.dtmin[a.slot], dbits(lds[0]));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
 namespace {
@@ -601,6 +834,19 @@ struct Launch {
     static void gather(fvk_solver* s, const EulerArgs& a) {
         k_e_gather<D><<<grid_for(s->nc, s->block), s->block, 0, s->stream>>>(a);
     }
+    template <int ORDER>
+    static void tile_o(fvk_solver* s, const EulerArgs& a, int grid) {
+        const size_t lds = s->tile_lds;
+        switch (s->p.flux_scheme) {
+            case FLUX_HLLC: k_e_tile<D, FLUX_HLLC, ORDER><<<grid, s->block, lds, s->stream>>>(a); break;
+            case FLUX_EXACT: k_e_tile<D, FLUX_EXACT, ORDER><<<grid, s->block, lds, s->stream>>>(a); break;
+            default: k_e_tile<D, FLUX_RUSANOV, ORDER><<<grid, s->block, lds, s->stream>>>(a); break;
+        }
+    }
+    static void tile(fvk_solver* s, const EulerArgs& a, int grid) {
+        if (s->p.order == 2) tile_o<2>(s, a, grid);
+        else tile_o<1>(s, a, grid);
+    }
 };
 
 template <int ORDER>
@@ -616,6 +862,24 @@ void face_o(fvk_solver* s, const EulerArgs& a) {
 template <int D>
 void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
     hipEvent_t ev = nullptr;
+    if (s->variant == 3) {
+        if (dt_pass) {
+            timer_begin(s, "euler_dt", &ev);
+            k_e_dt<<<grid_for(s->nf, s->block), s->block, 0, s->stream>>>(a);
+            timer_end(s, ev);
+        }
+        const int grid = (s->mesh.ntiles + kXcds - 1) / kXcds * kXcds;
+        timer_begin(s, "euler_tile", &ev);
+        Launch<D>::tile(s, a, grid);
+        timer_end(s, ev);
+        if (a.want_res) {
+            timer_begin(s, "euler_finalize", &ev);
+            k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(a.partials, grid, 4, a.res_hist, a.dt_hist, a.hist_idx,
+                                                         a.dtmin, a.slot);
+            timer_end(s, ev);
+        }
+        return;
+    }
     const bool split = s->variant == 1;
     if (dt_pass && !split) {
         timer_begin(s, "euler_dt", &ev);

@@ -103,6 +103,163 @@ T* upload(fvk_solver* s, const std::vector<T>& h) {
 
 }  // namespace
 
+// Graph-grown cell tiles: seeds taken in RCM order, each tile the first T
+// cells of a breadth-first search from its seed over cells not yet taken.
+// The device order becomes tile by tile (breadth-first order inside a tile).
+void tile_order(fvk_solver* s, int n, const std::vector<int>& aptr, const std::vector<int>& adj, int T) {
+    std::vector<int>& order = s->cell_new2old;
+    std::vector<char> taken(n, 0);
+    std::vector<int> out, queue;
+    out.reserve(n);
+    s->tile_start.clear();
+    for (int seed : order) {
+        if (taken[seed]) continue;
+        s->tile_start.push_back((int)out.size());
+        queue.clear();
+        queue.push_back(seed);
+        taken[seed] = 1;
+        size_t head = 0;
+        while (head < queue.size() && (int)head < T) {
+            const int c = queue[head++];
+            out.push_back(c);
+            for (int e = aptr[c]; e < aptr[c + 1]; ++e) {
+                const int nb = adj[e];
+                if (!taken[nb]) {
+                    taken[nb] = 1;
+                    queue.push_back(nb);
+                }
+            }
+        }
+        for (size_t i = head; i < queue.size(); ++i) taken[queue[i]] = 0;  // queued past the tile: released
+    }
+    s->tile_start.push_back(n);
+    order.swap(out);
+}
+
+// Per tile: the halo rings, the tile-local copies of the ELL entries of the
+// cells that need gradients, the tile's face list and each owned cell's
+// slots as indices into it.  Returns the LDS bytes a tile needs at most.
+int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<int>& cent, const std::vector<int>& fl,
+                const std::vector<int>& fr, int width, bool second_order, size_t* lds_bytes) {
+    const int nt = (int)s->tile_start.size() - 1;
+    const int T = s->block;
+    const int64_t nc = s->nc, nf = s->nf;
+    std::vector<int> loc(nc, -1), fmark(nf, -1);
+    std::vector<int> thptr(1, 0), thalo, th1(nt), tfptr(1, 0);
+    std::vector<int4> tface;
+    struct TileInfo { int n, h1, h2; };
+    std::vector<TileInfo> info(nt);
+    int mg = 0, ml = 0, mf = 0;
+    // pass 1: rings and faces
+    std::vector<int> cells;
+    for (int t = 0; t < nt; ++t) {
+        const int c0 = s->tile_start[t], n = s->tile_start[t + 1] - c0;
+        if (n > T) return set_error(FVK_ERR_INVALID, "tile %d has %d cells (more than %d)", t, n, T);
+        cells.clear();
+        for (int j = 0; j < n; ++j) { loc[c0 + j] = j; cells.push_back(c0 + j); }
+        auto add_ring = [&](int from, int to) {
+            for (int j = from; j < to; ++j) {
+                const int c = cells[j];
+                for (int k = cptr[c]; k < cptr[c + 1]; ++k) {
+                    const int f = cent[k] >> 1;
+                    if (fr[f] < 0) continue;
+                    const int nb = (cent[k] & 1) ? fr[f] : fl[f];
+                    if (loc[nb] < 0) { loc[nb] = (int)cells.size(); cells.push_back(nb); }
+                }
+            }
+        };
+        add_ring(0, n);
+        const int h1 = (int)cells.size() - n;
+        if (second_order) add_ring(n, n + h1);
+        const int h2 = (int)cells.size() - n - h1;
+        info[t] = TileInfo{n, h1, h2};
+        for (size_t j = n; j < cells.size(); ++j) thalo.push_back(cells[j]);
+        thptr.push_back((int)thalo.size());
+        th1[t] = h1;
+        // faces of the owned cells, each once
+        int nft = 0;
+        for (int j = 0; j < n; ++j) {
+            const int c = c0 + j;
+            for (int k = cptr[c]; k < cptr[c + 1]; ++k) {
+                const int f = cent[k] >> 1;
+                if (fmark[f] == t) continue;
+                fmark[f] = t;
+                const int r = fr[f] >= 0 ? loc[fr[f]] : fr[f];
+                tface.push_back(int4{f, loc[fl[f]], r, 0});
+                ++nft;
+            }
+        }
+        tfptr.push_back((int)tface.size());
+        mg = std::max(mg, n + h1);
+        ml = std::max(ml, n + h1 + h2);
+        mf = std::max(mf, nft);
+        for (int c : cells) loc[c] = -1;
+    }
+    // pass 2: tile-local ELL and owned slots (need the maxima for the strides)
+    std::vector<int2> tell((size_t)nt * width * mg, int2{-1, -1});
+    std::vector<short> tslot((size_t)nt * width * T, (short)-1);
+    for (int t = 0; t < nt; ++t) {
+        const int c0 = s->tile_start[t];
+        const TileInfo& ti = info[t];
+        cells.clear();
+        for (int j = 0; j < ti.n; ++j) cells.push_back(c0 + j);
+        for (int j = thptr[t]; j < thptr[t + 1]; ++j) cells.push_back(thalo[j]);
+        for (size_t j = 0; j < cells.size(); ++j) loc[cells[j]] = (int)j;
+        for (int e = tfptr[t]; e < tfptr[t + 1]; ++e) fmark[tface[e].x] = e - tfptr[t];
+        // first order reads the local entries of the owned cells only (no ring-2 halo)
+        const int nell = second_order ? ti.n + ti.h1 : ti.n;
+        for (int j = 0; j < nell; ++j) {
+            const int c = cells[j];
+            for (int k = cptr[c]; k < cptr[c + 1]; ++k) {
+                const int slot = k - cptr[c];
+                const int e = cent[k], f = e >> 1;
+                int nb;
+                if (fr[f] < 0) nb = fr[f];
+                else {
+                    const int g = (e & 1) ? fr[f] : fl[f];
+                    nb = loc[g];
+                    if (nb < 0) return set_error(FVK_ERR_INVALID, "tile %d: a neighbour is outside its halo", t);
+                }
+                if (nb >= ti.n + ti.h1 + ti.h2) return set_error(FVK_ERR_INVALID, "tile %d: local index out of range", t);
+                tell[((size_t)t * width + slot) * mg + j] = int2{e, nb};
+                if (j < ti.n) {
+                    const int lf = fmark[f];
+                    if (lf < 0 || lf >= tfptr[t + 1] - tfptr[t])
+                        return set_error(FVK_ERR_INVALID, "tile %d: face slot out of range", t);
+                    tslot[((size_t)t * width + slot) * T + j] = (short)lf;
+                }
+            }
+        }
+        // every face of the tile reads cells with staged states (owned or ring 1)
+        for (int e = tfptr[t]; e < tfptr[t + 1]; ++e) {
+            const int4 fd = tface[e];
+            if (fd.y < 0 || fd.y >= ti.n + ti.h1 || fd.z >= ti.n + ti.h1 || (fd.z < 0 && -1 - fd.z >= s->n_bcs))
+                return set_error(FVK_ERR_INVALID, "tile %d: face %d has a cell outside the staged states", t, fd.x);
+        }
+        for (int c : cells) loc[c] = -1;
+    }
+    MeshView& m = s->mesh;
+    m.ntiles = nt;
+    m.tmax_g = mg;
+    m.tmax_l = ml;
+    m.tmax_f = mf;
+    m.tstart = upload(s, s->tile_start);
+    m.thptr = upload(s, thptr);
+    m.thalo = thalo.empty() ? nullptr : upload(s, thalo);
+    m.th1 = upload(s, th1);
+    m.tell = upload(s, tell);
+    m.tfptr = upload(s, tfptr);
+    m.tface = upload(s, tface);
+    m.tslot = upload(s, tslot);
+    if (!m.tstart || !m.thptr || (!thalo.empty() && !m.thalo) || !m.th1 || !m.tell || !m.tfptr || !m.tface || !m.tslot)
+        return set_error(FVK_ERR_NOMEM, "device allocation for the tiles failed");
+    // U (4) and G (8) of the cells with gradients, then W (4) of all local
+    // cells, later reused for the face fluxes (4) and dt candidates (1)
+    *lds_bytes = sizeof(double) * std::max((size_t)12 * mg + std::max((size_t)4 * ml, (size_t)5 * mf),
+                                           (size_t)4 * T);  // at least the block reductions' 4 T doubles
+    return FVK_OK;
+}
+
 int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
     const int64_t nc = d->n_cells, nf = d->n_faces, nn = d->n_nodes;
     if (nc <= 0 || nf <= 0 || nn <= 0 || nc > (1LL << 30) || nf > (1LL << 30))
@@ -178,6 +335,7 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
         for (int64_t c = 0; c < nc; ++c) s->case_cell_start[cc[c] + 1]++;
         for (int k = 0; k < s->n_cases; ++k) s->case_cell_start[k + 1] += s->case_cell_start[k];
     }
+    if (s->variant == 3) tile_order(s, (int)nc, aptr, adj, s->block);
     s->cell_old2new.assign(nc, 0);
     for (int64_t i = 0; i < nc; ++i) s->cell_old2new[s->cell_new2old[i]] = (int)i;
     const std::vector<int>& o2n = s->cell_old2new;
@@ -434,6 +592,15 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
     m.frec = upload(s, frec);
     if (!m.ell || !m.cgeo || !m.clsq || !m.fgeo || !m.fmid2 || (sh && !m.fsh2) || !m.egeo || !m.frec)
         return set_error(FVK_ERR_NOMEM, "device allocation for the mesh failed");
+    if (s->variant == 3) {
+        size_t lds = 0;
+        int rc = build_tiles(s, cptr, cent, fl, fr, width, s->p.order == 2, &lds);
+        if (rc != FVK_OK) return rc;
+        if (lds > kMaxTileLds)
+            return set_error(FVK_ERR_UNSUPPORTED, "a tile of %d cells needs %zu B of LDS (limit %d): use smaller tiles",
+                             s->block, lds, kMaxTileLds);
+        s->tile_lds = lds;
+    }
     return FVK_OK;
 }
 

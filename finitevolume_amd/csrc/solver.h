@@ -15,6 +15,7 @@ namespace fvk {
 
 constexpr int kMaxPatches = 1024;
 constexpr int kMaxEulerDegree = 16;  // most faces of one cell the per-cell Euler kernels take
+constexpr int kMaxTileLds = 65536;   // LDS bytes one tile of the fused Euler stage may use
 
 // Per-patch boundary data on the device (fvk_bc reduced to what kernels read).
 struct DevBC {
@@ -68,6 +69,21 @@ struct MeshView {
     const double4* egeo;    // [width][ld] per ELL entry: w dx, w dy (LSQ weight 1/d^2 times the offset to the
                             //   neighbour, or to the face midpoint on a boundary), face midpoint - centroid
     const double4* frec;    // [nf] MUSCL offsets: midpoint - centroid of cell_left, of cell_right (periodic-shifted)
+    // ---- cell tiles of the fused Euler stage (kernel variant 3) ----
+    // Tile t owns the device cells [tstart[t], tstart[t+1]) (graph-grown, at
+    // most one workgroup of cells).  Its local cells are the owned ones, then
+    // the first ring of neighbours, then (second order) the second ring.
+    int ntiles;
+    int tmax_g, tmax_l, tmax_f; // most local cells with gradients (owned + ring 1), local cells, faces in a tile
+    const int* tstart;      // [ntiles + 1]
+    const int* thptr;       // [ntiles + 1] offsets into thalo
+    const int* thalo;       // ring-1 then ring-2 cells of each tile (device ids)
+    const int* th1;         // [ntiles] ring-1 count
+    const int2* tell;       // [ntiles][width][tmax_g] per local cell with gradients, slot k:
+                            //   {face << 1 | owner, local neighbour or -1 - patch}; x = -1 pads
+    const int* tfptr;       // [ntiles + 1] offsets into tface
+    const int4* tface;      // per tile face: {device face, local cell_left, local cell_right or -1 - patch, 0}
+    const short* tslot;     // [ntiles][width][tile size] tile-local face index of each owned cell's slot
 };
 
 struct AdjointWork;  // device buffers and index maps of the residual VJP (adjoint.hip)
@@ -85,8 +101,10 @@ struct fvk_solver {
     fvk_params p;
     int nv = 4;                 // variables per cell
     int n_bcs = 0;
-    int variant = 1;            // Euler kernels: 1 face sweep + gather, 2 fused per cell
-    int block = 512;            // Euler workgroup size
+    int variant = 1;            // Euler kernels: 1 face sweep + gather, 2 fused per cell, 3 fused per cell tile
+    int block = 512;            // Euler workgroup size (variant 3: cells per tile)
+    std::vector<int> tile_start; // variant 3: [ntiles + 1] device cell ranges of the tiles
+    size_t tile_lds = 0;         // variant 3: dynamic LDS bytes per tile
     int64_t nc = 0, nf = 0, nn = 0;
     int nfi = 0, nnz = 0;
     bool periodic = false;

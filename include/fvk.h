@@ -158,8 +158,10 @@ typedef struct fvk_params {
     int32_t rk_stages;             /* 1: forward Euler (reference); 2: SSP-RK2 (Heun); 3: SSP-RK3 (Shu-Osher) */
     int32_t local_dt;              /* 0: one global CFL dt (reference); 1: per-cell local dt */
     /* ---- execution tuning (same results bit for bit; 0 = library default) ---- */
-    int32_t kernel_variant;        /* Euler: 1 face sweep + per-cell gather, 2 per-cell fused flux and update */
-    int32_t block_size;            /* threads per workgroup of the Euler kernels: 64, 128, 256 or 512 (0: 512) */
+    int32_t kernel_variant;        /* Euler: 1 face sweep + per-cell gather, 2 per-cell fused flux and update,
+                                      3 one fused pass per cell tile with its halo staged in LDS (0: 1) */
+    int32_t block_size;            /* threads per workgroup of the Euler kernels: 64, 128, 256 or 512
+                                      (0: 512; variant 3: cells per tile, 0: 256) */
 } fvk_params;
 
 /* Initial condition (EulerInitialCondition, include/EulerFVMSolver.h:59-67,

@@ -142,8 +142,8 @@ def test_kernel_variants_bit_identical(order):
     ext = dict(order=order, limiter=abi.LIMITER_VENKATAKRISHNAN, rk_stages=3, local_dt=1) if order == 2 else {}
     ref = None
     by_bs = {}
-    for variant in (1, 2):
-        for bs in (64, 128, 256, 512):
+    for variant in (1, 2, 3):
+        for bs in ((64, 128, 256) if variant == 3 else (64, 128, 256, 512)):
             m, bcs, p, ic, _ = cases.case_naca(64, 32, kernel_variant=variant, block_size=bs, **ext)
             g = FVKSolver(m, bcs, p, ic)
             r = g.step(10)
@@ -158,8 +158,30 @@ def test_kernel_variants_bit_identical(order):
         assert np.array_equal(rs[0], rs[1]), bs
 
 
+@pytest.mark.parametrize("case", ["1r_o2_periodic", "sod_exact", "naca_o1"])
+def test_tile_variant_bit_identical(case):
+    """The fused tile kernels against the face sweep + gather on periodic,
+    exact-Riemann and first-order cases, several tile sizes."""
+    for bs in (64, 256):
+        out = []
+        for variant in (1, 3):
+            if case == "1r_o2_periodic":
+                c = cases.case_1r(flux=abi.FLUX_HLLC, order=2, limiter=abi.LIMITER_VENKATAKRISHNAN, rk_stages=2,
+                                  kernel_variant=variant, block_size=bs)
+            elif case == "sod_exact":
+                c = cases.case_sod(flux=abi.FLUX_EXACT)
+                c[2].kernel_variant, c[2].block_size = variant, bs
+            else:
+                c = cases.case_naca(64, 32, local_dt=1, kernel_variant=variant, block_size=bs)
+            g, _ = pair(c)
+            r = g.step(20)
+            out.append((g.field(), r))
+        assert np.array_equal(out[0][0], out[1][0]), (case, bs)
+        assert np.allclose(out[0][1], out[1][1], rtol=1e-13, atol=0.0), (case, bs)
+
+
 def test_global_dt_variants_match_oracle():
-    for variant in (1, 2):
+    for variant in (1, 2, 3):
         g, o = pair(cases.case_naca(64, 32, kernel_variant=variant))
         rg, dg = g.step(20, with_dt=True)
         ro, do = o.step(20, with_dt=True)
