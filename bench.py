@@ -137,6 +137,14 @@ def adjoint_leg(s, m, bcs, params, ic, steps_done, reps=5, n_check=32, h=1e-7):
         g.set_field(Us)
         return float(np.sum(ls * g.residual_eval()))
 
+    if n_check <= 0:
+        return {
+            "workload": "config4: VJP of the config-2 residual (state + node coordinates) at the 1000-step state",
+            "vjp_device_ms": round(dev_a, 4), "primal_residual_device_ms": round(dev_p, 4),
+            "adjoint_over_primal": round(dev_a / dev_p, 3), "vjp_state_only_device_ms": round(dev_u, 4),
+            "adjoint_state_only_over_primal": round(dev_u / dev_p, 3),
+            "vjp_kernel_avg_ms": {k: round(tot / cnt, 5) for k, (tot, cnt) in sorted(ta.items()) if cnt},
+            "fd_check": None}
     rng = np.random.default_rng(1)
     cells = rng.choice(nc, n_check, replace=False)
     nodes = rng.choice(m.n_nodes, n_check, replace=False)
@@ -354,6 +362,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-2r", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-fd-points", type=int, default=32,
+                    help="cells and nodes of the adjoint's finite-difference check (0: skip it)")
     ap.add_argument("--batch", action="store_true",
                     help="run only the batched-airfoil workload (64 Voronoi cases to convergence) and print its line")
     ap.add_argument("--batch-cases", type=int, default=64)
@@ -468,7 +478,7 @@ def main():
         out["copy_probe"] = probe
     if not args.no_adjoint:
         done = 2 * args.steps + args.warmup
-        out["adjoint"] = adjoint_leg(s, m, bcs, params, ic, done)
+        out["adjoint"] = adjoint_leg(s, m, bcs, params, ic, done, n_check=args.adjoint_fd_points)
         print(f"[bench] adjoint {out['adjoint']['vjp_device_ms']} ms vs residual "
               f"{out['adjoint']['primal_residual_device_ms']} ms", file=sys.stderr, flush=True)
     del s

@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_cell(AdjArgs a) {
 #pragma unroll
     for (int k = 0; k < D; ++k) en[k] = m.ell[k * ld + c];
 #pragma unroll
-    for (int k = 0; k < D; ++k) eg[k] = m.egeo[k * ld + c];
+    for (int k = 0; k < D; ++k) eg[k] = en[k].x >= 0 ? m.egeo[k * ld + c] : double4{0.0, 0.0, 0.0, 0.0};
     const State Uc = st4(a.U[c]);
     State Ub{0, 0, 0, 0};
     double Gb[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};

@@ -467,10 +467,20 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
             s->step_count++;
             (void)i;
         }
-        hipError_t e = hipMemcpyAsync(hr.data(), s->res_hist, sizeof(double) * nv * s->hist_cap, hipMemcpyDeviceToHost,
-                                      s->stream);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(hd.data(), s->dt_hist, sizeof(double) * s->hist_cap, hipMemcpyDeviceToHost, s->stream);
+        // copy back only the rows this chunk wrote (a ring of hist_cap rows)
+        hipError_t e = hipSuccess;
+        {
+            const int64_t first = s->p.equation == FVK_EQ_DIFFUSION ? 0 : (s->step_count - chunk) % s->hist_cap;
+            const int64_t n1 = std::min<int64_t>(chunk, s->hist_cap - first), n2 = chunk - n1;
+            e = hipMemcpyAsync(&hr[first * nv], s->res_hist + first * nv, sizeof(double) * nv * n1,
+                               hipMemcpyDeviceToHost, s->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(&hd[first], s->dt_hist + first, sizeof(double) * n1, hipMemcpyDeviceToHost, s->stream);
+            if (e == hipSuccess && n2 > 0)
+                e = hipMemcpyAsync(hr.data(), s->res_hist, sizeof(double) * nv * n2, hipMemcpyDeviceToHost, s->stream);
+            if (e == hipSuccess && n2 > 0)
+                e = hipMemcpyAsync(hd.data(), s->dt_hist, sizeof(double) * n2, hipMemcpyDeviceToHost, s->stream);
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) return hip_check(e, "step");
         if (s->timing) timers_collect(s);

@@ -211,14 +211,22 @@ __global__ void k_step_tick(unsigned int* step) { *step += 1u; }
 
 // one block: residual norms = sqrt of the sums of the partials (fixed order),
 // the step's dt into the history, and the next step's dt slot re-armed
-constexpr int kFinBlock = 256;
+constexpr int kFinBlock = 1024;
 __global__ __launch_bounds__(kFinBlock) void k_e_finalize(const double* partials, int nblocks, int nv,
                                                           double* res_hist, double* dt_hist, int64_t hist_idx,
                                                           unsigned long long* dtmin, int slot) {
     __shared__ double sm[kFinBlock];
     for (int q = 0; q < nv; ++q) {
+        // four independent loads in flight per thread, summed in a fixed order
+        const double* p = partials + (int64_t)q * nblocks;
         double acc = 0.0;
-        for (int i = threadIdx.x; i < nblocks; i += kFinBlock) acc += partials[q * nblocks + i];
+        for (int i = threadIdx.x; i < nblocks; i += 4 * kFinBlock) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = i + u * kFinBlock < nblocks ? p[i + u * kFinBlock] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += v[u];
+        }
         sm[threadIdx.x] = acc;
         __syncthreads();
         for (int s = kFinBlock / 2; s > 0; s >>= 1) {
@@ -304,7 +312,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_grad(EulerArgs a) {
 #pragma unroll
     for (int k = 0; k < D; ++k) en[k] = m.ell[k * ld + c];
 #pragma unroll
-    for (int k = 0; k < D; ++k) eg[k] = m.egeo[k * ld + c];  // w dx, w dy, face offset (precomputed, same bits)
+    for (int k = 0; k < D; ++k)  // w dx, w dy, face offset (precomputed, same bits); padding slots are not read
+        eg[k] = en[k].x >= 0 ? m.egeo[k * ld + c] : double4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < D; ++k)
         if (en[k].x >= 0 && en[k].y >= 0) un[k] = a.U[en[k].y];
@@ -543,6 +552,135 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
     }
 }
 
+// LDS views of one tile (structure of arrays: consecutive local cells or
+// faces are consecutive 8-byte words, so a wavefront's own-cell accesses
+// do not conflict on the banks)
+struct TileLds {
+    double* U;   // [4][MG] conserved state, owned cells then ring 1
+    double* G;   // [8][MG] limited gradients
+    double* W;   // [4][ML] primitives of all local cells (dead after the gradients)
+    double* F;   // [4][MF] face flux x length (in W's space)
+    double* Dt;  // [MF] dt candidates
+    int MG, ML, MF;
+};
+
+// limited LSQ gradient of local cell j (as k_e_grad), from LDS
+template <int D>
+__device__ __forceinline__ void tile_grad(const EulerArgs& a, const TileLds& L, int j, const int2 (&en)[D],
+                                          const double4 (&eg)[D], const double4& iv) {
+    const MeshView& m = a.m;
+    const double g = a.gamma;
+    const int MG = L.MG, ML = L.ML;
+    const State Uc{L.U[j], L.U[MG + j], L.U[2 * MG + j], L.U[3 * MG + j]};
+    const double wc[4] = {L.W[j], L.W[ML + j], L.W[2 * ML + j], L.W[3 * ML + j]};
+    double bx[4] = {0.0, 0.0, 0.0, 0.0}, by[4] = {0.0, 0.0, 0.0, 0.0};
+    double wmin[4], wmax[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wmin[q] = wmax[q] = wc[q];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        if (en[k].x < 0) continue;
+        double wn[4];
+        if (en[k].y < 0) {
+            const double4 fg = m.fgeo[en[k].x >> 1];
+            const State Gs = euler_ghost(m.bc[-1 - en[k].y], Uc, fg.x, fg.y, g);
+            const double wg0 = Gs.r, wg1 = Gs.ru / Gs.r, wg2 = Gs.rv / Gs.r, wg3 = pressure(Gs, g);
+            wn[0] = 0.5 * (wc[0] + wg0);
+            wn[1] = 0.5 * (wc[1] + wg1);
+            wn[2] = 0.5 * (wc[2] + wg2);
+            wn[3] = 0.5 * (wc[3] + wg3);
+        } else {
+            const int l = en[k].y;
+            wn[0] = L.W[l];
+            wn[1] = L.W[ML + l];
+            wn[2] = L.W[2 * ML + l];
+            wn[3] = L.W[3 * ML + l];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double dphi = wn[q] - wc[q];
+            bx[q] += eg[k].x * dphi;
+            by[q] += eg[k].y * dphi;
+            wmin[q] = smin(wmin[q], wn[q]);
+            wmax[q] = smax(wmax[q], wn[q]);
+        }
+    }
+    double gx[4], gy[4], lim[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        gx[q] = iv.x * bx[q] + iv.y * by[q];
+        gy[q] = iv.y * bx[q] + iv.z * by[q];
+        lim[q] = 1.0;
+    }
+    if (a.limiter == FVK_LIMITER_VENKATAKRISHNAN) {
+        const double eps2 = iv.w;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            if (en[k].x < 0) continue;
+            const double rx = eg[k].z, ry = eg[k].w;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double d2 = gx[q] * rx + gy[q] * ry;
+                double ph = 1.0;
+                if (d2 > 0.0) ph = venkatakrishnan(wmax[q] - wc[q], d2, eps2);
+                else if (d2 < 0.0) ph = venkatakrishnan(wmin[q] - wc[q], d2, eps2);
+                lim[q] = smin(lim[q], ph);
+            }
+        }
+    } else if (a.limiter == FVK_LIMITER_FIRST_ORDER) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lim[q] = 0.0;
+    }
+    double* gj = L.G + j;
+    gj[0] = lim[0] * gx[0]; gj[MG] = lim[0] * gy[0]; gj[2 * MG] = lim[1] * gx[1]; gj[3 * MG] = lim[1] * gy[1];
+    gj[4 * MG] = lim[2] * gx[2]; gj[5 * MG] = lim[2] * gy[2]; gj[6 * MG] = lim[3] * gx[3];
+    gj[7 * MG] = lim[3] * gy[3];
+}
+
+// flux x length of tile face e (as k_e_face), from LDS; returns its dt candidate
+template <int SCHEME, int ORDER>
+__device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L, int e, const int4& fd,
+                                            const double4& fg, const double4& ro) {
+    const MeshView& m = a.m;
+    const double g = a.gamma;
+    const int MG = L.MG, MF = L.MF;
+    const int l = fd.y, r = fd.z;
+    const State Ls{L.U[l], L.U[MG + l], L.U[2 * MG + l], L.U[3 * MG + l]};
+    State R = Ls, Lf = Ls, Rf;
+    if (ORDER == 2) {
+        const double* gl = L.G + l;
+        Lf = reconstruct(Ls, double4{gl[0], gl[MG], gl[2 * MG], gl[3 * MG]},
+                         double4{gl[4 * MG], gl[5 * MG], gl[6 * MG], gl[7 * MG]}, ro.x, ro.y, g);
+        if (r >= 0) {
+            R = State{L.U[r], L.U[MG + r], L.U[2 * MG + r], L.U[3 * MG + r]};
+            const double* gr = L.G + r;
+            Rf = reconstruct(R, double4{gr[0], gr[MG], gr[2 * MG], gr[3 * MG]},
+                             double4{gr[4 * MG], gr[5 * MG], gr[6 * MG], gr[7 * MG]}, ro.z, ro.w, g);
+        } else {
+            Rf = euler_ghost(m.bc[-1 - r], Lf, fg.x, fg.y, g);
+        }
+    } else {
+        if (r >= 0) {
+            R = State{L.U[r], L.U[MG + r], L.U[2 * MG + r], L.U[3 * MG + r]};
+            Rf = R;
+        } else {
+            Rf = euler_ghost(m.bc[-1 - r], Ls, fg.x, fg.y, g);
+        }
+    }
+    const State F = num_flux<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
+    const double A = fg.z;
+    L.F[e] = A * F.r;
+    L.F[MF + e] = A * F.ru;
+    L.F[2 * MF + e] = A * F.rv;
+    L.F[3 * MF + e] = A * F.E;
+    double cand = DBL_MAX;
+    if (a.face_dt) {
+        cand = dt_candidate(Ls, r >= 0 ? &R : nullptr, fg.x, fg.y, fg.w, g, a.cfl);
+        L.Dt[e] = cand;
+    }
+    return cand;
+}
+
 // ---------------------------------------------------------------------------
 // fused cell-tile pass (variant 3): one workgroup per tile of graph-adjacent
 // cells.  The tile's states and its halo (one ring for the fluxes, a second
@@ -550,7 +688,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
 // cells and the first ring, the fluxes of every face of the owned cells and
 // the gather + update then read LDS only.  Every value is computed by the
 // same operations as in the face sweep + gather kernels, so the states are
-// bit-identical; faces on a tile's edge are evaluated by both tiles.
+// bit-identical; faces on a tile's edge are evaluated by both tiles.  The
+// mesh data of each thread's first gradient cell and first face are loaded
+// before the staging barrier, so their latency overlaps the staging.
 // ---------------------------------------------------------------------------
 template <int D, int SCHEME, int ORDER>
 __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
@@ -559,12 +699,16 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     const int t = logical_block();
     const double g = a.gamma;
     const int bs = blockDim.x, tid = threadIdx.x;
-    const int MG = m.tmax_g;
-    double* sU = lds;           // [MG][4] conserved state, owned cells then ring 1
-    double* sG = lds + 4 * MG;  // [MG][8] limited gradients
-    double* sW = lds + 12 * MG; // [tmax_l][4] primitives of all local cells; then
-    double* sF = sW;            //   [tmax_f][4] face flux x length
-    double* sDt = sW + 4 * m.tmax_f;  // [tmax_f] dt candidates
+    TileLds L;
+    L.MG = m.tmax_g;
+    L.ML = m.tmax_l;
+    L.MF = m.tmax_f;
+    const int MG = L.MG, ML = L.ML, MF = L.MF;
+    L.U = lds;
+    L.G = lds + 4 * MG;
+    L.W = lds + 12 * MG;
+    L.F = L.W;
+    L.Dt = L.W + 4 * MF;
     int c0 = 0, n = 0, hp = 0, h1 = 0, nh = 0, f0 = 0, nft = 0;
     if (t < m.ntiles) {
         c0 = m.tstart[t];
@@ -578,143 +722,68 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     const int ng = n + h1, nl = n + nh;
     const int2* te = m.tell + (int64_t)t * D * MG;
 
+    // 0. prefetch the mesh data of the first gradient cell and face
+    int2 pen[D];
+    double4 peg[D], piv{0, 0, 0, 0};
+    if (ORDER == 2 && tid < ng) {
+        const int gc = tid < n ? c0 + tid : m.thalo[hp + tid - n];
+#pragma unroll
+        for (int k = 0; k < D; ++k) pen[k] = te[k * MG + tid];
+#pragma unroll
+        for (int k = 0; k < D; ++k) peg[k] = pen[k].x >= 0 ? m.egeo[k * m.ld + gc] : double4{0.0, 0.0, 0.0, 0.0};
+        piv = m.clsq[gc];
+    }
+    int4 pfd{0, 0, 0, 0};
+    double4 pfg{0, 0, 0, 0}, pro{0, 0, 0, 0};
+    if (tid < nft) {
+        pfd = m.tface[f0 + tid];
+        pfg = m.fgeo[pfd.x];
+        if (ORDER == 2) pro = m.frec[pfd.x];
+    }
+
     // 1. states (and primitives) of the tile and its halo
     for (int j = tid; j < nl; j += bs) {
         const int gc = j < n ? c0 + j : m.thalo[hp + j - n];
         const double4 u = a.U[gc];
         if (j < ng) {
-            sU[4 * j] = u.x;
-            sU[4 * j + 1] = u.y;
-            sU[4 * j + 2] = u.z;
-            sU[4 * j + 3] = u.w;
+            L.U[j] = u.x;
+            L.U[MG + j] = u.y;
+            L.U[2 * MG + j] = u.z;
+            L.U[3 * MG + j] = u.w;
         }
         if (ORDER == 2) {
             const State U = st(u);
-            sW[4 * j] = U.r;
-            sW[4 * j + 1] = U.ru / U.r;
-            sW[4 * j + 2] = U.rv / U.r;
-            sW[4 * j + 3] = pressure(U, g);
+            L.W[j] = U.r;
+            L.W[ML + j] = U.ru / U.r;
+            L.W[2 * ML + j] = U.rv / U.r;
+            L.W[3 * ML + j] = pressure(U, g);
         }
     }
     __syncthreads();
 
-    // 2. limited LSQ gradients of the owned cells and ring 1 (as k_e_grad)
+    // 2. limited LSQ gradients of the owned cells and ring 1
     if (ORDER == 2) {
-        for (int j = tid; j < ng; j += bs) {
+        if (tid < ng) tile_grad<D>(a, L, tid, pen, peg, piv);
+        for (int j = tid + bs; j < ng; j += bs) {
             const int gc = j < n ? c0 + j : m.thalo[hp + j - n];
             int2 en[D];
             double4 eg[D];
 #pragma unroll
             for (int k = 0; k < D; ++k) en[k] = te[k * MG + j];
 #pragma unroll
-            for (int k = 0; k < D; ++k) eg[k] = m.egeo[k * m.ld + gc];
-            const State Uc{sU[4 * j], sU[4 * j + 1], sU[4 * j + 2], sU[4 * j + 3]};
-            const double wc[4] = {sW[4 * j], sW[4 * j + 1], sW[4 * j + 2], sW[4 * j + 3]};
-            double bx[4] = {0.0, 0.0, 0.0, 0.0}, by[4] = {0.0, 0.0, 0.0, 0.0};
-            double wmin[4], wmax[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) wmin[q] = wmax[q] = wc[q];
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                if (en[k].x < 0) continue;
-                double wn[4];
-                if (en[k].y < 0) {
-                    const double4 fg = m.fgeo[en[k].x >> 1];
-                    const State Gs = euler_ghost(m.bc[-1 - en[k].y], Uc, fg.x, fg.y, g);
-                    const double wg0 = Gs.r, wg1 = Gs.ru / Gs.r, wg2 = Gs.rv / Gs.r, wg3 = pressure(Gs, g);
-                    wn[0] = 0.5 * (wc[0] + wg0);
-                    wn[1] = 0.5 * (wc[1] + wg1);
-                    wn[2] = 0.5 * (wc[2] + wg2);
-                    wn[3] = 0.5 * (wc[3] + wg3);
-                } else {
-                    const int l = en[k].y;
-                    wn[0] = sW[4 * l];
-                    wn[1] = sW[4 * l + 1];
-                    wn[2] = sW[4 * l + 2];
-                    wn[3] = sW[4 * l + 3];
-                }
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const double dphi = wn[q] - wc[q];
-                    bx[q] += eg[k].x * dphi;
-                    by[q] += eg[k].y * dphi;
-                    wmin[q] = smin(wmin[q], wn[q]);
-                    wmax[q] = smax(wmax[q], wn[q]);
-                }
-            }
-            const double4 iv = m.clsq[gc];
-            double gx[4], gy[4], lim[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                gx[q] = iv.x * bx[q] + iv.y * by[q];
-                gy[q] = iv.y * bx[q] + iv.z * by[q];
-                lim[q] = 1.0;
-            }
-            if (a.limiter == FVK_LIMITER_VENKATAKRISHNAN) {
-                const double eps2 = iv.w;
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    if (en[k].x < 0) continue;
-                    const double rx = eg[k].z, ry = eg[k].w;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const double d2 = gx[q] * rx + gy[q] * ry;
-                        double ph = 1.0;
-                        if (d2 > 0.0) ph = venkatakrishnan(wmax[q] - wc[q], d2, eps2);
-                        else if (d2 < 0.0) ph = venkatakrishnan(wmin[q] - wc[q], d2, eps2);
-                        lim[q] = smin(lim[q], ph);
-                    }
-                }
-            } else if (a.limiter == FVK_LIMITER_FIRST_ORDER) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) lim[q] = 0.0;
-            }
-            double* gj = sG + 8 * j;
-            gj[0] = lim[0] * gx[0]; gj[1] = lim[0] * gy[0]; gj[2] = lim[1] * gx[1]; gj[3] = lim[1] * gy[1];
-            gj[4] = lim[2] * gx[2]; gj[5] = lim[2] * gy[2]; gj[6] = lim[3] * gx[3]; gj[7] = lim[3] * gy[3];
+            for (int k = 0; k < D; ++k) eg[k] = en[k].x >= 0 ? m.egeo[k * m.ld + gc] : double4{0.0, 0.0, 0.0, 0.0};
+            tile_grad<D>(a, L, j, en, eg, m.clsq[gc]);
         }
     }
     __syncthreads();  // the primitives are dead from here: their space takes the fluxes
 
-    // 3. fluxes of the tile's faces (as k_e_face)
+    // 3. fluxes of the tile's faces
     double cand_min = DBL_MAX;
-    for (int e = tid; e < nft; e += bs) {
+    if (tid < nft) cand_min = smin(cand_min, tile_face<SCHEME, ORDER>(a, L, tid, pfd, pfg, pro));
+    for (int e = tid + bs; e < nft; e += bs) {
         const int4 fd = m.tface[f0 + e];
-        const int f = fd.x, l = fd.y, r = fd.z;
-        const double4 fg = m.fgeo[f];
-        const State L{sU[4 * l], sU[4 * l + 1], sU[4 * l + 2], sU[4 * l + 3]};
-        State R = L, Lf = L, Rf;
-        if (ORDER == 2) {
-            const double4 ro = m.frec[f];
-            const double* gl = sG + 8 * l;
-            Lf = reconstruct(L, double4{gl[0], gl[1], gl[2], gl[3]}, double4{gl[4], gl[5], gl[6], gl[7]}, ro.x, ro.y, g);
-            if (r >= 0) {
-                R = State{sU[4 * r], sU[4 * r + 1], sU[4 * r + 2], sU[4 * r + 3]};
-                const double* gr = sG + 8 * r;
-                Rf = reconstruct(R, double4{gr[0], gr[1], gr[2], gr[3]}, double4{gr[4], gr[5], gr[6], gr[7]}, ro.z,
-                                 ro.w, g);
-            } else {
-                Rf = euler_ghost(m.bc[-1 - r], Lf, fg.x, fg.y, g);
-            }
-        } else {
-            if (r >= 0) {
-                R = State{sU[4 * r], sU[4 * r + 1], sU[4 * r + 2], sU[4 * r + 3]};
-                Rf = R;
-            } else {
-                Rf = euler_ghost(m.bc[-1 - r], L, fg.x, fg.y, g);
-            }
-        }
-        const State F = num_flux<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
-        const double A = fg.z;
-        sF[4 * e] = A * F.r;
-        sF[4 * e + 1] = A * F.ru;
-        sF[4 * e + 2] = A * F.rv;
-        sF[4 * e + 3] = A * F.E;
-        if (a.face_dt) {
-            const double cand = dt_candidate(L, r >= 0 ? &R : nullptr, fg.x, fg.y, fg.w, g, a.cfl);
-            sDt[e] = cand;
-            cand_min = smin(cand_min, cand);
-        }
+        const double4 ro = ORDER == 2 ? m.frec[fd.x] : double4{0, 0, 0, 0};
+        cand_min = smin(cand_min, tile_face<SCHEME, ORDER>(a, L, e, fd, m.fgeo[fd.x], ro));
     }
     __syncthreads();
 
@@ -732,15 +801,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             if (en[k].x < 0) continue;
-            const double* F = sF + 4 * lf[k];
+            const double* F = L.F + lf[k];
             if (en[k].x & 1) {
-                r0 = r0 - F[0]; r1 = r1 - F[1]; r2 = r2 - F[2]; r3 = r3 - F[3];
+                r0 = r0 - F[0]; r1 = r1 - F[MF]; r2 = r2 - F[2 * MF]; r3 = r3 - F[3 * MF];
             } else {
-                r0 += F[0]; r1 += F[1]; r2 += F[2]; r3 += F[3];
+                r0 += F[0]; r1 += F[MF]; r2 += F[2 * MF]; r3 += F[3 * MF];
             }
-            if (a.dt_src == 2) dmin = smin(dmin, sDt[lf[k]]);
+            if (a.dt_src == 2) dmin = smin(dmin, L.Dt[lf[k]]);
         }
-        const double4 u{sU[4 * tid], sU[4 * tid + 1], sU[4 * tid + 2], sU[4 * tid + 3]};
+        const double4 u{L.U[tid], L.U[MG + tid], L.U[2 * MG + tid], L.U[3 * MG + tid]};
         update_cell(a, c0 + tid, u, r0, r1, r2, r3, dmin);
     }
 

@@ -19,6 +19,8 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <queue>
+#include <tuple>
 
 #include "solver.h"
 
@@ -103,37 +105,110 @@ T* upload(fvk_solver* s, const std::vector<T>& h) {
 
 }  // namespace
 
-// Graph-grown cell tiles: seeds taken in RCM order, each tile the first T
-// cells of a breadth-first search from its seed over cells not yet taken.
-// The device order becomes tile by tile (breadth-first order inside a tile).
+// Graph-grown cell tiles.  Each tile grows breadth-first from its seed,
+// taking within one breadth-first depth first the cells with the most
+// neighbours already tiled (so holes fill instead of fragmenting), up to
+// 7/8 of the capacity T; the next seed is the untiled cell with the most
+// tiled neighbours (the lowest RCM position first).  Tiles below T/2 are
+// then merged into the neighbouring tile they share most faces with, where
+// it has room.  The device order becomes tile by tile.
 void tile_order(fvk_solver* s, int n, const std::vector<int>& aptr, const std::vector<int>& adj, int T) {
-    std::vector<int>& order = s->cell_new2old;
-    std::vector<char> taken(n, 0);
-    std::vector<int> out, queue;
-    out.reserve(n);
-    s->tile_start.clear();
-    for (int seed : order) {
-        if (taken[seed]) continue;
-        s->tile_start.push_back((int)out.size());
-        queue.clear();
-        queue.push_back(seed);
-        taken[seed] = 1;
-        size_t head = 0;
-        while (head < queue.size() && (int)head < T) {
-            const int c = queue[head++];
-            out.push_back(c);
+    const std::vector<int>& rcm = s->cell_new2old;
+    std::vector<int> pos(n);
+    for (int i = 0; i < n; ++i) pos[rcm[i]] = i;
+    std::vector<int> tile(n, -1), ntaken(n, 0), depth(n, -1);
+    // seed queue: (-tiled neighbours, RCM position)
+    using Key = std::pair<int, int>;
+    std::priority_queue<Key, std::vector<Key>, std::greater<Key>> seeds;
+    for (int i = 0; i < n; ++i) seeds.push({0, i});
+    const int target = std::max(1, T - T / 8);
+    std::vector<int> size;
+    std::vector<int> touched;
+    struct Item { int d, k, p, c; bool operator>(const Item& o) const { return std::tie(d, k, p) > std::tie(o.d, o.k, o.p); } };
+    for (;;) {
+        int seed = -1;
+        while (!seeds.empty()) {
+            const Key k = seeds.top();
+            seeds.pop();
+            const int c = rcm[k.second];
+            if (tile[c] < 0 && -k.first == ntaken[c]) { seed = c; break; }
+        }
+        if (seed < 0) break;
+        const int tid = (int)size.size();
+        std::priority_queue<Item, std::vector<Item>, std::greater<Item>> front;
+        touched.clear();
+        depth[seed] = 0;
+        touched.push_back(seed);
+        front.push({0, 0, pos[seed], seed});
+        int cnt = 0;
+        while (!front.empty() && cnt < target) {
+            const Item it = front.top();
+            front.pop();
+            const int c = it.c;
+            if (tile[c] >= 0) continue;
+            tile[c] = tid;
+            ++cnt;
             for (int e = aptr[c]; e < aptr[c + 1]; ++e) {
                 const int nb = adj[e];
-                if (!taken[nb]) {
-                    taken[nb] = 1;
-                    queue.push_back(nb);
-                }
+                if (tile[nb] >= 0) continue;
+                ++ntaken[nb];
+                seeds.push({-ntaken[nb], pos[nb]});
+                if (depth[nb] < 0) { depth[nb] = it.d + 1; touched.push_back(nb); }
+                front.push({depth[nb], -ntaken[nb], pos[nb], nb});
             }
         }
-        for (size_t i = head; i < queue.size(); ++i) taken[queue[i]] = 0;  // queued past the tile: released
+        for (int c : touched) depth[c] = -1;
+        size.push_back(cnt);
+    }
+    // merge small tiles into a neighbour with room
+    const int nt = (int)size.size();
+    std::vector<std::vector<int>> members(nt);
+    for (int i = 0; i < n; ++i) members[tile[rcm[i]]].push_back(rcm[i]);
+    std::vector<int> by_size(nt);
+    std::iota(by_size.begin(), by_size.end(), 0);
+    std::stable_sort(by_size.begin(), by_size.end(), [&](int a, int b) { return size[a] < size[b]; });
+    std::vector<int> share(nt, 0);
+    for (int t : by_size) {
+        if (size[t] == 0 || size[t] >= T / 2) continue;
+        std::vector<int> cand;
+        for (int c : members[t])
+            for (int e = aptr[c]; e < aptr[c + 1]; ++e) {
+                const int u = tile[adj[e]];
+                if (u == t) continue;
+                if (share[u]++ == 0) cand.push_back(u);
+            }
+        std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) { return share[a] > share[b]; });
+        for (int u : cand) {
+            if (size[u] + size[t] <= T) {
+                for (int c : members[t]) tile[c] = u;
+                members[u].insert(members[u].end(), members[t].begin(), members[t].end());
+                members[t].clear();
+                size[u] += size[t];
+                size[t] = 0;
+                break;
+            }
+        }
+        for (int u : cand) share[u] = 0;
+    }
+    // device order: tiles in the order of their first cell in RCM order,
+    // each tile's cells in RCM order
+    std::vector<int> first(nt, n);
+    for (int i = 0; i < n; ++i) first[tile[rcm[i]]] = std::min(first[tile[rcm[i]]], i);
+    std::vector<int> tord;
+    for (int t = 0; t < nt; ++t)
+        if (size[t] > 0) tord.push_back(t);
+    std::stable_sort(tord.begin(), tord.end(), [&](int a, int b) { return first[a] < first[b]; });
+    std::vector<int> out;
+    out.reserve(n);
+    s->tile_start.clear();
+    for (int t : tord) {
+        s->tile_start.push_back((int)out.size());
+        std::vector<int>& mm = members[t];
+        std::sort(mm.begin(), mm.end(), [&](int a, int b) { return pos[a] < pos[b]; });
+        out.insert(out.end(), mm.begin(), mm.end());
     }
     s->tile_start.push_back(n);
-    order.swap(out);
+    s->cell_new2old.swap(out);
 }
 
 // Per tile: the halo rings, the tile-local copies of the ELL entries of the
