@@ -330,6 +330,86 @@ def batch_leg(n_cases, max_steps, drop=1e-6):
     }
 
 
+def rans_case(reference_mode=False):
+    """Config 3: Spalart-Allmaras RANS around NACA 4412 at Mach 0.2, Re 6e6 per
+    chord, 8 degrees: a 1024-point O-grid with 145 quad layers from a 5e-6
+    chord wall spacing (growth 1.1) and triangle pairs beyond, 366 layers to 50
+    chords (601,088 cells).  HLLC, Green-Gauss gradients with the Jasak face
+    gradient, Pr 0.72, Pr_t 0.9, R = 1/gamma (T = 1), nut0 = nut_farfield =
+    3 nu; adiabatic no-slip wall.  SSP-RK3 with local dt at CFL 0.5, or (the
+    reference mode) forward Euler with a global dt."""
+    m = M.naca_ogrid(1024, 366, code="4412", ratio=1.1, first=5e-6, quad_layers=145)
+    gamma = 1.4
+    mach, re, aoa = 0.2, 6e6, math.radians(8.0)
+    rho, pinf = 1.0, 1.0 / gamma
+    u, v = mach * math.cos(aoa), mach * math.sin(aoa)
+    mu = rho * mach * 1.0 / re
+    nut = 3.0 * mu / rho
+    ff = M.from_primitive(rho, u, v, pinf, gamma)
+    wall = abi.make_bc(abi.BC_WALL)
+    wall.is_isothermal_wall = 0
+    far = abi.make_bc(abi.BC_FARFIELD, ff)
+    for b in (wall, far):
+        b.farfield_nut = nut
+    ext = dict(rk_stages=1, local_dt=0) if reference_mode else dict(rk_stages=3, local_dt=1)
+    params = abi.default_params(equation=abi.EQ_RANS_SA, flux_scheme=abi.FLUX_HLLC,
+                                gradient_scheme=abi.GRAD_GREEN_GAUSS, mu=mu, prandtl=0.72, prandtl_t=0.9,
+                                gas_constant=1.0 / gamma, cfl=0.5 if not reference_mode else 0.8, **ext)
+    ic = abi.initial_condition(abi.IC_FREESTREAM, rho=rho, u=u, v=v, p=pinf, initial_nut=nut)
+    return m, [wall, far], params, ic
+
+
+def rans_leg(steps, warmup, cpu_seconds, no_cpu):
+    """Config 3 throughput (SA, RK3 + local dt) and reference mode 3R on the GPU,
+    with the reference solver's 3R on the host cores beside it."""
+    from finitevolume_amd.solver import FVKSolver
+    m, bcs, params, ic = rans_case()
+    nc, fpc = m.n_cells, m.n_faces / m.n_cells
+    s = FVKSolver(m, bcs, params, ic)
+    wall = time_gpu(s, steps, warmup, per_step=True)
+    s.set_kernel_timing(True)
+    s.step(steps)
+    times = s.kernel_times()
+    s.set_kernel_timing(False)
+    stage_ms = sum(tot for tot, _ in times.values()) / (steps * 3)
+    b_alg = 149.3 + 48.0 * fpc      # SA stage bytes: 40+40+8+16+8+48 F/N+8, +40 on stages 2-3, +8 on stage 1
+    cups = nc * 3 * steps / wall
+    out = {"workload": "config3: SA RANS, NACA 4412 hybrid O-grid (601,088 cells), HLLC + Green-Gauss/Jasak, "
+                       "SSP-RK3 local dt CFL 0.5", "cells": nc, "faces": m.n_faces,
+           "value": round(cups / 1e9, 4), "unit": "Gcell-updates/s", "ms_per_step": round(wall / steps * 1e3, 4),
+           "roofline": {"bound": "hbm", "achieved": round(b_alg * nc / (stage_ms * 1e-3) / 1e9, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(b_alg * nc / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "b_alg_per_cell_update": round(b_alg, 1), "stage_device_ms": round(stage_ms, 4),
+                        "kernel_avg_ms": {k: round(tot / cnt, 5) for k, (tot, cnt) in sorted(times.items()) if cnt}}}
+    del s
+    m3, bcs3, p3, ic3 = rans_case(reference_mode=True)
+    s3 = FVKSolver(m3, bcs3, p3, ic3)
+    w3 = time_gpu(s3, steps, warmup, per_step=True)
+    del s3
+    out["gpu_3R"] = {"value": round(nc * steps / w3 / 1e9, 4), "unit": "Gcell-updates/s",
+                     "ms_per_step": round(w3 / steps * 1e3, 4),
+                     "roofline_frac": round((112.0 + 48.0 * fpc) * nc * steps / w3 / 1e9 / HBM_PEAK_GBS, 4)}
+    if not no_cpu:
+        cores = host_cores()
+        try:
+            from oracle import pyref
+            if pyref.available("fast"):
+                r = pyref.RefSolver(m3, bcs3, p3, ic3, flavour="fast")
+                r.step(1)
+                n2, t2 = timed_steps(r, cpu_seconds)
+                ref = nc * n2 / t2
+                out["cpu_baseline_reference_3R"] = {
+                    "value": round(ref / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores, "kind": "reference",
+                    "sample": f"reference RANSTurbulenceSASolver built from its sources with its Release flags "
+                              f"({cores} threads), mode 3R on the same mesh, {n2} steps ({t2:.2f} s)"}
+                out["gpu_3R_over_reference_3R"] = round(out["gpu_3R"]["value"] * 1e9 / ref, 1)
+                del r
+        except Exception as e:
+            out["cpu_baseline_reference_3R"] = {"value": None, "sample": f"unavailable: {e}"}
+    return out
+
+
 def host_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -367,6 +447,8 @@ def main():
     ap.add_argument("--batch", action="store_true",
                     help="run only the batched-airfoil workload (64 Voronoi cases to convergence) and print its line")
     ap.add_argument("--batch-cases", type=int, default=64)
+    ap.add_argument("--rans", action="store_true",
+                    help="run only the RANS SA workload (and its reference mode) and print its line")
     ap.add_argument("--batch-max-steps", type=int, default=20000)
     ap.add_argument("--variant", type=int, default=0, help="Euler kernel variant (0 = library default)")
     ap.add_argument("--block", type=int, default=0, help="Euler workgroup size (0 = library default)")
@@ -382,6 +464,13 @@ def main():
         line = batch_leg(args.batch_cases, args.batch_max_steps)
         line.update({"metric": "batched airfoil cases: wall time to convergence, batched vs sequential",
                      "n_gpus": 1, "dtype": "f64", "data": "synthetic (generated Voronoi meshes, free-stream start)"})
+        print(json.dumps(line))
+        return
+
+    if args.rans:
+        line = rans_leg(args.steps, args.warmup, args.cpu_seconds, args.no_cpu)
+        line.update({"metric": "FP64 RANS SA residual Gcell-updates/s (1 GPU, % HBM roofline)", "n_gpus": 1,
+                     "dtype": "f64", "data": "synthetic (generated NACA 4412 hybrid O-grid, free-stream start)"})
         print(json.dumps(line))
         return
 

@@ -279,13 +279,12 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
     // launch grids (workgroups of 64 threads or more) are rounded up to a multiple of the 8 XCDs
     int64_t max_blocks = (std::max(s->nc, s->nf) + 63) / 64 + 8;
     s->partials = static_cast<double*>(dev_alloc(s, sizeof(double) * 6 * max_blocks));
-    s->ticket = static_cast<unsigned int*>(dev_alloc(s, 256));
     s->dtmin = static_cast<unsigned long long*>(dev_alloc(s, 256));
     s->hist_cap = 1024;
     s->res_hist = static_cast<double*>(dev_alloc(s, sizeof(double) * 6 * s->hist_cap));
     s->dt_hist = static_cast<double*>(dev_alloc(s, sizeof(double) * s->hist_cap));
     double* wd = static_cast<double*>(dev_alloc(s, sizeof(double) * s->ld));
-    if (!s->U[0] || ((!euler || s->variant == 1) && (!s->flux || !s->dtf)) || !s->dtc || (ng && !s->grad) || !s->partials || !s->ticket || !s->dtmin ||
+    if (!s->U[0] || ((!euler || s->variant == 1) && (!s->flux || !s->dtf)) || !s->dtc || (ng && !s->grad) || !s->partials || !s->dtmin ||
         !s->res_hist || !s->dt_hist || !wd || (nbuf == 3 && (!s->U[1] || !s->U[2]))) {
         fvk_destroy(s);
         return set_error(FVK_ERR_NOMEM, "device allocation failed");
@@ -296,7 +295,6 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
     std::memcpy(&init[0], &dm, 8);
     init[1] = init[0];
     rc = hip_check(hipMemcpy(s->dtmin, init, sizeof init, hipMemcpyHostToDevice), "init dt slots");
-    if (rc == FVK_OK) rc = hip_check(hipMemset(s->ticket, 0, 256), "init ticket");
     if (rc == FVK_OK) rc = hip_check(hipMemset(s->U[0], 0, sizeof(double) * nv * s->ld), "init state");
     if (rc == FVK_OK && s->grad) rc = hip_check(hipMemset(s->grad, 0, sizeof(double) * ng * s->ld), "init gradients");
     // wall distance to the no-slip walls (RANS constructors, src/RANSTurbulenceSASolver.cpp:67)

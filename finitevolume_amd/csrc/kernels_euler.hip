@@ -214,7 +214,8 @@ __global__ void k_step_tick(unsigned int* step) { *step += 1u; }
 constexpr int kFinBlock = 1024;
 __global__ __launch_bounds__(kFinBlock) void k_e_finalize(const double* partials, int nblocks, int nv,
                                                           double* res_hist, double* dt_hist, int64_t hist_idx,
-                                                          unsigned long long* dtmin, int slot) {
+                                                          unsigned long long* dtmin, int slot,
+                                                          double fixed_dt = -1.0) {
     __shared__ double sm[kFinBlock];
     for (int q = 0; q < nv; ++q) {
         // four independent loads in flight per thread, summed in a fixed order
@@ -237,8 +238,12 @@ __global__ __launch_bounds__(kFinBlock) void k_e_finalize(const double* partials
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        dt_hist[hist_idx] = __longlong_as_double((long long)dtmin[slot]);
-        dtmin[slot ^ 1] = dbits(DBL_MAX);
+        if (fixed_dt >= 0.0) {
+            dt_hist[hist_idx] = fixed_dt;
+        } else {
+            dt_hist[hist_idx] = __longlong_as_double((long long)dtmin[slot]);
+            dtmin[slot ^ 1] = dbits(DBL_MAX);
+        }
     }
 }
 
@@ -1047,6 +1052,14 @@ int launch_euler_residual_eval(fvk_solver* s, double* R_dev) {
     a.R = reinterpret_cast<double4*>(R_dev);
     run_stage(s, a, false);
     return hip_check(hipGetLastError(), "euler residual launch");
+}
+
+// residual norms of nv components from nblocks block partials, the step's dt
+// into the history (fixed_dt >= 0: that value) and the next dt slot re-armed
+int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, int slot, double fixed_dt) {
+    k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(s->partials, nblocks, nv, s->res_hist, s->dt_hist, hist_idx,
+                                                 s->dtmin, slot, fixed_dt);
+    return hip_check(hipGetLastError(), "finalize launch");
 }
 
 // per-case residual norms of the step and the step counter

@@ -30,7 +30,6 @@ struct ViscArgs {
     double* R;
     unsigned long long* dtmin;
     double* partials;
-    unsigned int* ticket;
     double* res_hist;
     double* dt_hist;
     int64_t ld, ldf;
@@ -474,11 +473,12 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
     }
 }
 
+// deterministic block sum of the squared residual components into
+// partials[q * gridDim.x + blockIdx.x]; a one-block finalize kernel reduces
+// them (fixed order) once the kernel has ended
 template <int NV>
-__device__ __forceinline__ bool vblock_partials(const double (&v)[NV], double* partials, unsigned int* ticket,
-                                                double (&tot)[NV]) {
+__device__ __forceinline__ void vblock_partials(const double (&v)[NV], double* partials) {
     __shared__ double sm[NV][kVB];
-    __shared__ bool last;
 #pragma unroll
     for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] = v[q];
     __syncthreads();
@@ -492,35 +492,7 @@ __device__ __forceinline__ bool vblock_partials(const double (&v)[NV], double* p
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) partials[q * gridDim.x + blockIdx.x] = sm[q][0];
-        __threadfence();
-        unsigned int t = atomicAdd(ticket, 1u);
-        last = (t == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) acc[q] = 0.0;
-    for (unsigned int i = threadIdx.x; i < gridDim.x; i += kVB) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q)
-            acc[q] += __hip_atomic_load(&partials[q * gridDim.x + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-#pragma unroll
-    for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] = acc[q];
-    __syncthreads();
-    for (int s = kVB / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-#pragma unroll
-            for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < NV; ++q) tot[q] = sm[q][0];
-    if (threadIdx.x == 0) *ticket = 0u;
-    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -563,21 +535,28 @@ __global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
             else dt = a.dtc[c];
             double h = dt / V;
             double rho_old = a.U[c];
+            // the last RK stage writes the buffer that holds the base state:
+            // read all of this cell's base values before any write
+            double u0[NV];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) u0[q] = a.first ? 0.0 : a.U0[q * ld + c];
+            double rho_new = 0.0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 double u = a.U[q * ld + c];
-                a.Unew[q * ld + c] = a.first ? u + h * r[q] : a.a * a.U0[q * ld + c] + a.b * (u + h * r[q]);
+                const double un = a.first ? u + h * r[q] : a.a * u0[q] + a.b * (u + h * r[q]);
+                if (q == 0) rho_new = un;
+                a.Unew[q * ld + c] = un;
             }
 #pragma unroll
             for (int q = 4; q < NV; ++q) {
                 double t = a.U[q * ld + c];
                 if (EQ == EQ_SST) {
                     double cur = rho_old * t;
-                    double nw = a.first ? cur + h * r[q]
-                                        : a.a * (a.U0[c] * a.U0[q * ld + c]) + a.b * (cur + h * r[q]);
-                    a.Unew[q * ld + c] = nw / a.Unew[c];
+                    double nw = a.first ? cur + h * r[q] : a.a * (u0[0] * u0[q]) + a.b * (cur + h * r[q]);
+                    a.Unew[q * ld + c] = nw / rho_new;
                 } else {
-                    a.Unew[q * ld + c] = a.first ? t + h * r[q] : a.a * a.U0[q * ld + c] + a.b * (t + h * r[q]);
+                    a.Unew[q * ld + c] = a.first ? t + h * r[q] : a.a * u0[q] + a.b * (t + h * r[q]);
                 }
             }
         }
@@ -586,13 +565,7 @@ __global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
     double sq[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) sq[q] = r[q] * r[q];
-    double tot[NV];
-    if (vblock_partials<NV>(sq, a.partials, a.ticket, tot) && threadIdx.x == 0) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) a.res_hist[a.hist_idx * NV + q] = sqrt(tot[q]);
-        a.dt_hist[a.hist_idx] = __longlong_as_double((long long)a.dtmin[a.slot]);
-        a.dtmin[a.slot ^ 1] = dbits_v(DBL_MAX);
-    }
+    vblock_partials<NV>(sq, a.partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -639,11 +612,7 @@ __global__ __launch_bounds__(kVB) void k_diff_cell(ViscArgs a) {
     }
     if (!a.want_res) return;
     double sq[1] = {r[0] * r[0]};
-    double tot[1];
-    if (vblock_partials<1>(sq, a.partials, a.ticket, tot) && threadIdx.x == 0) {
-        a.res_hist[a.hist_idx] = sqrt(tot[0]);
-        a.dt_hist[a.hist_idx] = a.fixed_dt;
-    }
+    vblock_partials<1>(sq, a.partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -745,7 +714,6 @@ ViscArgs make_vargs(fvk_solver* s) {
     a.dtc = s->dtc;
     a.dtmin = s->dtmin;
     a.partials = s->partials;
-    a.ticket = s->ticket;
     a.res_hist = s->res_hist;
     a.dt_hist = s->dt_hist;
     a.ld = s->ld;
@@ -812,6 +780,11 @@ void run_visc(fvk_solver* s, ViscArgs& a) {
         default: k_visc_cell<EQ_SST><<<gc, kVB, 0, s->stream>>>(a); break;
     }
     timer_end(s, ev);
+    if (a.want_res) {
+        timer_begin(s, "visc_finalize", &ev);
+        launch_norm_finalize(s, gc, s->nv, a.hist_idx, a.slot, -1.0);
+        timer_end(s, ev);
+    }
 }
 
 }  // namespace
@@ -861,6 +834,7 @@ int launch_diffusion_step(fvk_solver* s, int want_res, int64_t step) {
     timer_begin(s, "diffusion_cell", &ev);
     k_diff_cell<<<gc, kVB, 0, s->stream>>>(a);
     timer_end(s, ev);
+    if (want_res) launch_norm_finalize(s, gc, 1, a.hist_idx, 0, s->p.fixed_dt);
     return hip_check(hipGetLastError(), "diffusion launch");
 }
 

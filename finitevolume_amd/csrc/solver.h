@@ -132,7 +132,6 @@ struct fvk_solver {
     double* grad = nullptr;                      // [ng][ld] cell gradients
     double* aux = nullptr;                       // [na][ld] per-cell auxiliaries (sources, nu_t, F1 ...)
     double* partials = nullptr;                  // [nv][max_blocks]
-    unsigned int* ticket = nullptr;
     unsigned long long* dtmin = nullptr;         // [2] global dt (bits) per step parity
     double* res_hist = nullptr;                  // [hist_cap][nv]
     double* dt_hist = nullptr;                   // [hist_cap]
@@ -187,6 +186,7 @@ int launch_wall_distance(fvk_solver* s, const int* wall_faces, int nw, const dou
 int launch_residual_eval(fvk_solver* s, double* R_dev);
 int launch_euler_gradients(fvk_solver* s);
 int launch_case_finalize(fvk_solver* s);
+int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, int slot, double fixed_dt);
 int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
 void adjoint_release(fvk_solver* s);
 int launch_copy_probe(double* dst, const double* src, int64_t n, hipStream_t st);

@@ -462,12 +462,14 @@ def radial_spacing(n_layers: int, r_far: float, ratio: float, first: Optional[fl
 
 
 def naca_ogrid(n_surface: int = 1024, n_layers: int = 512, code: str = "0012", r_far: float = 50.0,
-               ratio: float = 1.05, first: Optional[float] = None, triangles: bool = True) -> UnstructuredMesh:
+               ratio: float = 1.05, first: Optional[float] = None, triangles: bool = True,
+               quad_layers: int = 0) -> UnstructuredMesh:
     """O-grid around a NACA 4-digit section: n_surface cosine-clustered
     surface points (trailing edge at parameter 0), n_layers layers out to a
     circle of radius r_far chords around mid-chord, geometric wall-normal
     growth (see radial_spacing).  triangles=True splits every quad on the
-    same diagonal: 2 n_surface n_layers triangles.  Patches: 0 wall,
+    same diagonal: 2 n_surface n_layers triangles (with quad_layers > 0 the
+    first quad_layers layers stay quads).  Patches: 0 wall,
     1 farfield.  Cells are numbered layer by layer from the wall."""
     th = 2.0 * np.pi * np.arange(n_surface) / n_surface
     xs = 0.5 * (1.0 + np.cos(th))
@@ -488,7 +490,20 @@ def naca_ogrid(n_surface: int = 1024, n_layers: int = 512, code: str = "0012", r
     # i runs trailing edge -> upper -> leading edge -> lower (clockwise seen
     # from outside along j); (i,j),(i,j+1),(i+1,j+1),(i+1,j) is counter-clockwise
     a, b, c, d = nid(ci, cj), nid(ci, cj + 1), nid(ci + 1, cj + 1), nid(ci + 1, cj)
-    if triangles:
+    if triangles and quad_layers > 0:
+        # hybrid: quads in the first quad_layers layers, triangle pairs beyond
+        quad = cj < quad_layers
+        polys = []
+        for k in range(n_surface * n_layers):
+            if quad[k]:
+                polys.append((a[k], b[k], c[k], d[k]))
+            else:
+                polys.append((a[k], b[k], c[k]))
+                polys.append((a[k], c[k], d[k]))
+        deg = np.fromiter((len(p) for p in polys), dtype=np.int64, count=len(polys))
+        cells = np.fromiter((v for p in polys for v in p), dtype=np.int64, count=int(deg.sum()))
+        ptr = np.concatenate([[0], np.cumsum(deg)])
+    elif triangles:
         cells = np.stack([np.stack([a, b, c], 1), np.stack([a, c, d], 1)], axis=1).reshape(-1, 3)
         ptr = np.arange(0, 3 * cells.shape[0] + 1, 3)
     else:

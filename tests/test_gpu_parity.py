@@ -187,3 +187,18 @@ def test_global_dt_variants_match_oracle():
         ro, do = o.step(20, with_dt=True)
         assert np.array_equal(dg, do)
         assert_bitwise(g, o)
+
+
+@pytest.mark.parametrize("equation", [abi.EQ_NAVIER_STOKES, abi.EQ_RANS_SA, abi.EQ_RANS_SST])
+def test_viscous_rk3_local_dt(equation):
+    """SSP-RK3 with local time steps on the viscous paths (as the RANS
+    workload runs them) against the oracle."""
+    if equation == abi.EQ_NAVIER_STOKES:
+        case = cases.case_ns_wavy(rk_stages=3, local_dt=1)
+    else:
+        case = cases.case_rans_wavy(equation, rk_stages=3, local_dt=1)
+    g, o = pair(case)
+    rg = g.step(60)
+    ro = o.step(60)
+    assert cases.residual_close(rg, ro, 1e-10)
+    assert_close(g, o, 1e-12)
