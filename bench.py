@@ -452,6 +452,7 @@ def main():
     ap.add_argument("--batch-max-steps", type=int, default=20000)
     ap.add_argument("--variant", type=int, default=0, help="Euler kernel variant (0 = library default)")
     ap.add_argument("--block", type=int, default=0, help="Euler workgroup size (0 = library default)")
+    ap.add_argument("--tile-cells", type=int, default=0, help="cells per tile of kernel variant 3 (0 = block)")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner (library defaults)")
     ap.add_argument("--probe", action="store_true", help="run a plain 256 MiB device copy first (counter calibration)")
     args = ap.parse_args()
@@ -486,6 +487,7 @@ def main():
     m, bcs, params, ic = naca_case(args.ni, args.nj, order2=True)
     params.kernel_variant = args.variant
     params.block_size = args.block
+    params.tile_cells = args.tile_cells
     nc, nf = m.n_cells, m.n_faces
     sweep = {}
     if not args.no_tune and not (args.variant or args.block):
@@ -539,9 +541,10 @@ def main():
     prof = os.path.join(REPO, "profiles", "traffic_config2.json")
     if os.path.exists(prof):
         with open(prof) as f:
-            tr = json.load(f)
-        roofline["traffic"] = tr.get("stage_bytes")
-        roofline["traffic_per_cell_update"] = tr.get("bytes_per_cell_update")
+            tr = json.load(f).get("variants", {}).get(str(params.kernel_variant or 1))
+        if tr:  # measured for this kernel variant on this workload (rocprofv3 FETCH_SIZE, WRITE_SIZE)
+            roofline["traffic"] = tr.get("stage_bytes")
+            roofline["traffic_per_cell_update"] = tr.get("bytes_per_cell_update")
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -558,7 +561,8 @@ def main():
         "config": {"workload": "config2: 2nd-order Euler HLLC + LSQ/Venkatakrishnan K=5, SSP-RK3, local dt CFL 0.8, "
                                "NACA 0012 O-grid 1024x512 split to triangles",
                    "cells": nc, "faces": nf, "rk_stages": stages, "ni": args.ni, "nj": args.nj,
-                   "kernel_variant": params.kernel_variant, "block_size": params.block_size},
+                   "kernel_variant": params.kernel_variant, "block_size": params.block_size,
+                   "tile_cells": params.tile_cells},
         "roofline": roofline,
     }
     if sweep:

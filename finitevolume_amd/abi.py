@@ -66,7 +66,8 @@ class ParamsC(C.Structure):
         ("alpha", C.c_double), ("fixed_dt", C.c_double),
         ("order", C.c_int32), ("limiter", C.c_int32), ("venkat_k", C.c_double),
         ("rk_stages", C.c_int32), ("local_dt", C.c_int32),
-        ("kernel_variant", C.c_int32), ("block_size", C.c_int32),
+        ("kernel_variant", C.c_int32), ("block_size", C.c_int32), ("tile_cells", C.c_int32),
+        ("reserved1", C.c_int32),
     ]
 
 

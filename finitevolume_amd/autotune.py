@@ -81,22 +81,28 @@ def tune(mesh, bcs, params: abi.ParamsC, ic, steps: int = 20, warmup: int = 3,
         best = db[key]
     else:
         from .solver import FVKError
-        table = {}
+        table, shapes = {}, {}
         for v in variants:
             for b in blocks:
-                p = abi.ParamsC.from_buffer_copy(params)
-                p.kernel_variant, p.block_size = v, b
-                try:
-                    table[f"v{v}_b{b}"] = _time_solver(lambda: FVKSolver(mesh, bcs, p, ic), steps, warmup) * 1e3
-                except FVKError:
-                    continue  # not built for this case (e.g. a tile too large for LDS)
+                # the tile variant also tries tiles smaller than its workgroup (less LDS, more tiles per CU)
+                for tc in ((b, b * 7 // 8, b * 3 // 4) if v == 3 else (0,)):
+                    p = abi.ParamsC.from_buffer_copy(params)
+                    p.kernel_variant, p.block_size, p.tile_cells = v, b, tc
+                    name = f"v{v}_b{b}" + (f"_t{tc}" if v == 3 else "")
+                    try:
+                        table[name] = _time_solver(lambda: FVKSolver(mesh, bcs, p, ic), steps, warmup) * 1e3
+                        shapes[name] = (v, b, tc)
+                    except FVKError:
+                        continue  # not built for this case (e.g. a tile too large for LDS)
         name = min(table, key=table.get)
-        best = {"variant": int(name[1]), "block": int(name.split("_b")[1]), "ms_per_step": table[name],
+        v, b, tc = shapes[name]
+        best = {"variant": v, "block": b, "tile_cells": tc, "ms_per_step": table[name],
                 "table_ms_per_step": {k: round(x, 4) for k, x in table.items()}}
         if use_cache:
             db[key] = best
             _store(db)
     params.kernel_variant, params.block_size = best["variant"], best["block"]
+    params.tile_cells = best.get("tile_cells", 0)
     return best
 
 

@@ -227,6 +227,11 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
     s->p = *params;
     s->variant = params->kernel_variant == 0 ? 1 : params->kernel_variant;
     s->block = params->block_size != 0 ? params->block_size : (s->variant == 3 ? 256 : 512);
+    if (params->tile_cells < 0 || params->tile_cells > s->block) {
+        delete s;
+        return set_error(FVK_ERR_INVALID, "tile_cells must be 0..block_size (got %d)", params->tile_cells);
+    }
+    s->tile_cells = params->tile_cells != 0 ? params->tile_cells : s->block;
     s->nv = nvars_eq(params->equation);
     s->n_bcs = n_bcs;
     s->bcs.assign(bcs, bcs + n_bcs);

@@ -217,7 +217,7 @@ void tile_order(fvk_solver* s, int n, const std::vector<int>& aptr, const std::v
 int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<int>& cent, const std::vector<int>& fl,
                 const std::vector<int>& fr, int width, bool second_order, size_t* lds_bytes) {
     const int nt = (int)s->tile_start.size() - 1;
-    const int T = s->block;
+    const int T = s->block;  // tslot stride: the workgroup size (tiles hold at most tile_cells <= T cells)
     const int64_t nc = s->nc, nf = s->nf;
     std::vector<int> loc(nc, -1), fmark(nf, -1);
     std::vector<int> thptr(1, 0), thalo, th1(nt), tfptr(1, 0);
@@ -410,7 +410,7 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
         for (int64_t c = 0; c < nc; ++c) s->case_cell_start[cc[c] + 1]++;
         for (int k = 0; k < s->n_cases; ++k) s->case_cell_start[k + 1] += s->case_cell_start[k];
     }
-    if (s->variant == 3) tile_order(s, (int)nc, aptr, adj, s->block);
+    if (s->variant == 3) tile_order(s, (int)nc, aptr, adj, s->tile_cells);
     s->cell_old2new.assign(nc, 0);
     for (int64_t i = 0; i < nc; ++i) s->cell_old2new[s->cell_new2old[i]] = (int)i;
     const std::vector<int>& o2n = s->cell_old2new;

@@ -102,7 +102,8 @@ struct fvk_solver {
     int nv = 4;                 // variables per cell
     int n_bcs = 0;
     int variant = 1;            // Euler kernels: 1 face sweep + gather, 2 fused per cell, 3 fused per cell tile
-    int block = 512;            // Euler workgroup size (variant 3: cells per tile)
+    int block = 512;            // Euler workgroup size
+    int tile_cells = 256;       // variant 3: most cells of one tile (at most block)
     std::vector<int> tile_start; // variant 3: [ntiles + 1] device cell ranges of the tiles
     size_t tile_lds = 0;         // variant 3: dynamic LDS bytes per tile
     int64_t nc = 0, nf = 0, nn = 0;

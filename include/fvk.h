@@ -161,7 +161,9 @@ typedef struct fvk_params {
     int32_t kernel_variant;        /* Euler: 1 face sweep + per-cell gather, 2 per-cell fused flux and update,
                                       3 one fused pass per cell tile with its halo staged in LDS (0: 1) */
     int32_t block_size;            /* threads per workgroup of the Euler kernels: 64, 128, 256 or 512
-                                      (0: 512; variant 3: cells per tile, 0: 256) */
+                                      (0: 512; variant 3: 256) */
+    int32_t tile_cells;            /* variant 3: most cells of one tile, at most block_size (0: block_size) */
+    int32_t reserved1;
 } fvk_params;
 
 /* Initial condition (EulerInitialCondition, include/EulerFVMSolver.h:59-67,
