@@ -562,7 +562,7 @@ def main():
                                "NACA 0012 O-grid 1024x512 split to triangles",
                    "cells": nc, "faces": nf, "rk_stages": stages, "ni": args.ni, "nj": args.nj,
                    "kernel_variant": params.kernel_variant, "block_size": params.block_size,
-                   "tile_cells": params.tile_cells},
+                   "tile_cells": params.tile_cells, "launch_shape": s.launch_shape()},
         "roofline": roofline,
     }
     if sweep:

@@ -71,6 +71,14 @@ class ParamsC(C.Structure):
     ]
 
 
+class LaunchShapeC(C.Structure):
+    _fields_ = [
+        ("kernel_variant", C.c_int32), ("block_size", C.c_int32), ("tile_cells", C.c_int32),
+        ("n_tiles", C.c_int32), ("tile_max_staged", C.c_int32), ("tile_max_ring2", C.c_int32),
+        ("tile_max_faces", C.c_int32), ("reserved0", C.c_int32), ("tile_lds_bytes", C.c_int64),
+    ]
+
+
 class InitialConditionC(C.Structure):
     _fields_ = [("mode", C.c_int32), ("reserved0", C.c_int32)] + [
         (n, C.c_double) for n in ("rho", "u", "v", "p", "rho_l", "u_l", "v_l", "p_l", "rho_r", "u_r", "v_r", "p_r",

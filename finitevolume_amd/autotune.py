@@ -1,8 +1,9 @@
 """Launch-shape autotuner for the Euler stage kernels.
 
 For the mesh and numerics at hand it times every kernel variant (face sweep
-+ gather, or fused per cell) and workgroup size (64 to 512 threads, one
-cell or face per thread) for a few steps, and keeps the fastest.  Choices
++ gather, fused per cell, fused per cell tile) and workgroup size (64 to
+512 threads; for tiles, pairs of workgroup size and tile capacity) for a
+few steps, and keeps the fastest.  Choices
 are cached in a JSON file (FVK_TUNE_CACHE, default
 ~/.cache/fvk/autotune.json) under a key made of the device name, the mesh
 size and largest cell degree, and the numerics that change the kernels, so
@@ -22,6 +23,12 @@ from . import abi
 
 VARIANTS = (1, 2, 3)
 BLOCKS = (64, 128, 256, 512)
+# variant 3 (fused tiles): workgroup size and tile capacity.  A tile of
+# capacity T holds about 7/8 T cells, its owned and ring-1 cells about
+# 1.2 times that and its faces about 1.6 times, so these pairs cover one
+# pass per phase (staging, gradients, faces) up to two passes.
+TILE_SHAPES = ((128, 112), (192, 160), (256, 192), (256, 224), (256, 256), (320, 224), (320, 256),
+               (384, 256), (384, 320))
 
 
 def _cache_path() -> str:
@@ -83,17 +90,15 @@ def tune(mesh, bcs, params: abi.ParamsC, ic, steps: int = 20, warmup: int = 3,
         from .solver import FVKError
         table, shapes = {}, {}
         for v in variants:
-            for b in blocks:
-                # the tile variant also tries tiles smaller than its workgroup (less LDS, more tiles per CU)
-                for tc in ((b, b * 7 // 8, b * 3 // 4) if v == 3 else (0,)):
-                    p = abi.ParamsC.from_buffer_copy(params)
-                    p.kernel_variant, p.block_size, p.tile_cells = v, b, tc
-                    name = f"v{v}_b{b}" + (f"_t{tc}" if v == 3 else "")
-                    try:
-                        table[name] = _time_solver(lambda: FVKSolver(mesh, bcs, p, ic), steps, warmup) * 1e3
-                        shapes[name] = (v, b, tc)
-                    except FVKError:
-                        continue  # not built for this case (e.g. a tile too large for LDS)
+            for b, tc in (TILE_SHAPES if v == 3 else [(b, 0) for b in blocks]):
+                p = abi.ParamsC.from_buffer_copy(params)
+                p.kernel_variant, p.block_size, p.tile_cells = v, b, tc
+                name = f"v{v}_b{b}" + (f"_t{tc}" if v == 3 else "")
+                try:
+                    table[name] = _time_solver(lambda: FVKSolver(mesh, bcs, p, ic), steps, warmup) * 1e3
+                    shapes[name] = (v, b, tc)
+                except FVKError:
+                    continue  # not built for this case (e.g. a tile too large for LDS)
         name = min(table, key=table.get)
         v, b, tc = shapes[name]
         best = {"variant": v, "block": b, "tile_cells": tc, "ms_per_step": table[name],

@@ -250,18 +250,19 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_cell(AdjArgs a) {
             lim[q] = 1.0;
         }
         if (a.limiter == FVK_LIMITER_VENKATAKRISHNAN) {
+            VenkatMin vm[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) vm[q] = VenkatMin{1.0, 1.0};
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 if (en[k].x < 0) continue;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const double d2 = gx[q] * eg[k].z + gy[q] * eg[k].w;
-                    double ph = 1.0;
-                    if (d2 > 0.0) ph = venkatakrishnan(wmax[q] - wcq[q], d2, eps2);
-                    else if (d2 < 0.0) ph = venkatakrishnan(wmin[q] - wcq[q], d2, eps2);
-                    if (ph < lim[q]) { lim[q] = ph; kl[q] = k; }
-                }
+                for (int q = 0; q < 4; ++q)
+                    if (venkat_pick(vm[q], wmin[q], wmax[q], wcq[q], gx[q] * eg[k].z + gy[q] * eg[k].w, eps2))
+                        kl[q] = k;
             }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lim[q] = venkat_value(vm[q]);
         }
         // ---- reverse: G = lim g ----
         double gbx[4], gby[4], wcb[4] = {0, 0, 0, 0}, wnb[D][4], eps2b = 0.0;

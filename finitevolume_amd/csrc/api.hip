@@ -218,9 +218,16 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
         return set_error(FVK_ERR_INVALID, "kernel_variant must be 0, 1, 2 or 3 (got %d)", params->kernel_variant);
     if (params->kernel_variant == 3 && (params->equation != FVK_EQ_EULER || cell_case))
         return set_error(FVK_ERR_UNSUPPORTED, "the fused tile kernels (variant 3) are built for single Euler cases");
-    if (params->block_size != 0 && params->block_size != 64 && params->block_size != 128 &&
-        params->block_size != 256 && params->block_size != 512)
-        return set_error(FVK_ERR_INVALID, "block_size must be 0, 64, 128, 256 or 512 (got %d)", params->block_size);
+    {
+        const int b = params->block_size;
+        const bool pow2 = b == 64 || b == 128 || b == 256 || b == 512;
+        // the tile kernels take any whole number of wavefronts (their passes then fit the tile)
+        const bool waves = params->kernel_variant == 3 && b >= 64 && b <= 512 && b % 64 == 0;
+        if (b != 0 && !pow2 && !waves)
+            return set_error(FVK_ERR_INVALID,
+                             "block_size must be 0, 64, 128, 256 or 512, or a multiple of 64 up to 512 for "
+                             "kernel_variant 3 (got %d)", b);
+    }
 
     fvk_solver* s = new (std::nothrow) fvk_solver;
     if (!s) return set_error(FVK_ERR_NOMEM, "host allocation failed");
@@ -513,6 +520,23 @@ int fvk_set_cfl(fvk_solver* s, double cfl) {
 }
 
 double fvk_gas_gamma(const fvk_solver* s) { return s ? s->p.gamma : NAN; }
+
+int fvk_get_launch_shape(const fvk_solver* s, fvk_launch_shape* shape) {
+    if (!s || !shape) return set_error(FVK_ERR_INVALID, "NULL argument");
+    fvk_launch_shape o{};
+    o.kernel_variant = s->variant;
+    o.block_size = s->block;
+    if (s->variant == 3) {
+        o.tile_cells = s->tile_cells;
+        o.n_tiles = s->mesh.ntiles;
+        o.tile_max_staged = s->mesh.tmax_g;
+        o.tile_max_ring2 = s->mesh.tmax_2;
+        o.tile_max_faces = s->mesh.tmax_f;
+        o.tile_lds_bytes = (int64_t)s->tile_lds;
+    }
+    *shape = o;
+    return FVK_OK;
+}
 
 int64_t fvk_device_bytes(const fvk_solver* s) { return s ? s->device_bytes : 0; }
 

@@ -250,24 +250,41 @@ __global__ __launch_bounds__(kFinBlock) void k_e_finalize(const double* partials
 // stage update of one cell: U' = U + dt/V R (src/EulerFVMSolver.cpp:103) on
 // the first stage, a U0 + b (U + dt/V R) on later SSP-RK stages; or the
 // residual itself for residual_eval
-__device__ __forceinline__ void update_cell(const EulerArgs& a, int c, const double4& u, double r0, double r1,
-                                            double r2, double r3, double dmin) {
+// (the loads split from the arithmetic, so a kernel can issue them early)
+struct UpdateInputs {
+    double4 u0;      // stage base state (later SSP-RK stages)
+    double vol, dt;  // cell volume; dt unless it is the cell's own new local dt
+};
+__device__ __forceinline__ UpdateInputs load_update_inputs(const EulerArgs& a, int c) {
+    UpdateInputs x{};
+    if (a.R) return x;
+    x.vol = a.m.vol[c];
+    if (a.dt_src == 1) x.dt = __longlong_as_double((long long)a.dtmin[a.slot]);
+    else if (a.dt_src == 0) x.dt = a.dtc[c];
+    if (!a.first) x.u0 = a.U0[c];
+    return x;
+}
+__device__ __forceinline__ void update_cell_with(const EulerArgs& a, int c, const UpdateInputs& x,
+                                                 const double4& u, double r0, double r1, double r2, double r3,
+                                                 double dmin) {
     if (a.R) {
         a.R[c] = double4{r0, r1, r2, r3};
         return;
     }
-    double dt;
-    if (a.dt_src == 1) dt = __longlong_as_double((long long)a.dtmin[a.slot]);
-    else if (a.dt_src == 2) { dt = dmin; a.dtc[c] = dmin; }
-    else dt = a.dtc[c];
-    const double h = dt / a.m.vol[c];
+    double dt = x.dt;
+    if (a.dt_src == 2) { dt = dmin; a.dtc[c] = dmin; }
+    const double h = dt / x.vol;
     if (a.first) {
         a.Unew[c] = double4{u.x + h * r0, u.y + h * r1, u.z + h * r2, u.w + h * r3};
     } else {
-        const double4 u0 = a.U0[c];
+        const double4 u0 = x.u0;
         a.Unew[c] = double4{a.a * u0.x + a.b * (u.x + h * r0), a.a * u0.y + a.b * (u.y + h * r1),
                             a.a * u0.z + a.b * (u.z + h * r2), a.a * u0.w + a.b * (u.w + h * r3)};
     }
+}
+__device__ __forceinline__ void update_cell(const EulerArgs& a, int c, const double4& u, double r0, double r1,
+                                            double r2, double r3, double dmin) {
+    update_cell_with(a, c, load_update_inputs(a, c), u, r0, r1, r2, r3, dmin);
 }
 
 // per-block partial sums of the squared residual (L2 norms of the stage)
@@ -366,19 +383,18 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_grad(EulerArgs a) {
     }
     if (a.limiter == FVK_LIMITER_VENKATAKRISHNAN) {
         const double eps2 = iv.w;
+        VenkatMin vm[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vm[q] = VenkatMin{1.0, 1.0};
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             if (en[k].x < 0) continue;
             const double rx = eg[k].z, ry = eg[k].w;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const double d2 = gx[q] * rx + gy[q] * ry;
-                double ph = 1.0;
-                if (d2 > 0.0) ph = venkatakrishnan(wmax[q] - wc[q], d2, eps2);
-                else if (d2 < 0.0) ph = venkatakrishnan(wmin[q] - wc[q], d2, eps2);
-                lim[q] = smin(lim[q], ph);
-            }
+            for (int q = 0; q < 4; ++q) venkat_pick(vm[q], wmin[q], wmax[q], wc[q], gx[q] * rx + gy[q] * ry, eps2);
         }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lim[q] = venkat_value(vm[q]);
     } else if (a.limiter == FVK_LIMITER_FIRST_ORDER) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) lim[q] = 0.0;
@@ -562,11 +578,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
 // do not conflict on the banks)
 struct TileLds {
     double* U;   // [4][MG] conserved state, owned cells then ring 1
-    double* G;   // [8][MG] limited gradients
-    double* W;   // [4][ML] primitives of all local cells (dead after the gradients)
-    double* F;   // [4][MF] face flux x length (in W's space)
+    double* G;   // [8][MG] limited gradients (second order)
+    double* W;   // [4][MG] primitives (rho, u, v, p) of the owned and ring-1 cells (second order)
+    double* W2;  // [4][M2] primitives of ring 2 (dead after the gradients)
+    double* F;   // [4][MF] face flux x length (in W2's space for second order)
     double* Dt;  // [MF] dt candidates
-    int MG, ML, MF;
+    int MG, M2, MF;
+    int ng;      // this tile's owned + ring-1 cells: local indices below it are in W, the rest in W2
 };
 
 // limited LSQ gradient of local cell j (as k_e_grad), from LDS
@@ -575,9 +593,9 @@ __device__ __forceinline__ void tile_grad(const EulerArgs& a, const TileLds& L, 
                                           const double4 (&eg)[D], const double4& iv) {
     const MeshView& m = a.m;
     const double g = a.gamma;
-    const int MG = L.MG, ML = L.ML;
+    const int MG = L.MG;
     const State Uc{L.U[j], L.U[MG + j], L.U[2 * MG + j], L.U[3 * MG + j]};
-    const double wc[4] = {L.W[j], L.W[ML + j], L.W[2 * ML + j], L.W[3 * ML + j]};
+    const double wc[4] = {L.W[j], L.W[MG + j], L.W[2 * MG + j], L.W[3 * MG + j]};
     double bx[4] = {0.0, 0.0, 0.0, 0.0}, by[4] = {0.0, 0.0, 0.0, 0.0};
     double wmin[4], wmax[4];
 #pragma unroll
@@ -596,10 +614,13 @@ __device__ __forceinline__ void tile_grad(const EulerArgs& a, const TileLds& L, 
             wn[3] = 0.5 * (wc[3] + wg3);
         } else {
             const int l = en[k].y;
-            wn[0] = L.W[l];
-            wn[1] = L.W[ML + l];
-            wn[2] = L.W[2 * ML + l];
-            wn[3] = L.W[3 * ML + l];
+            const bool r1 = l < L.ng;
+            const double* w = r1 ? L.W + l : L.W2 + (l - L.ng);
+            const int sw = r1 ? MG : L.M2;
+            wn[0] = w[0];
+            wn[1] = w[sw];
+            wn[2] = w[2 * sw];
+            wn[3] = w[3 * sw];
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -619,19 +640,18 @@ __device__ __forceinline__ void tile_grad(const EulerArgs& a, const TileLds& L, 
     }
     if (a.limiter == FVK_LIMITER_VENKATAKRISHNAN) {
         const double eps2 = iv.w;
+        VenkatMin vm[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vm[q] = VenkatMin{1.0, 1.0};
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             if (en[k].x < 0) continue;
             const double rx = eg[k].z, ry = eg[k].w;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const double d2 = gx[q] * rx + gy[q] * ry;
-                double ph = 1.0;
-                if (d2 > 0.0) ph = venkatakrishnan(wmax[q] - wc[q], d2, eps2);
-                else if (d2 < 0.0) ph = venkatakrishnan(wmin[q] - wc[q], d2, eps2);
-                lim[q] = smin(lim[q], ph);
-            }
+            for (int q = 0; q < 4; ++q) venkat_pick(vm[q], wmin[q], wmax[q], wc[q], gx[q] * rx + gy[q] * ry, eps2);
         }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lim[q] = venkat_value(vm[q]);
     } else if (a.limiter == FVK_LIMITER_FIRST_ORDER) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) lim[q] = 0.0;
@@ -654,13 +674,17 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
     State R = Ls, Lf = Ls, Rf;
     if (ORDER == 2) {
         const double* gl = L.G + l;
-        Lf = reconstruct(Ls, double4{gl[0], gl[MG], gl[2 * MG], gl[3 * MG]},
-                         double4{gl[4 * MG], gl[5 * MG], gl[6 * MG], gl[7 * MG]}, ro.x, ro.y, g);
+        const double* wl = L.W + l;
+        Lf = reconstruct_w(Ls, double4{wl[0], wl[MG], wl[2 * MG], wl[3 * MG]},
+                           double4{gl[0], gl[MG], gl[2 * MG], gl[3 * MG]},
+                           double4{gl[4 * MG], gl[5 * MG], gl[6 * MG], gl[7 * MG]}, ro.x, ro.y, g);
         if (r >= 0) {
             R = State{L.U[r], L.U[MG + r], L.U[2 * MG + r], L.U[3 * MG + r]};
             const double* gr = L.G + r;
-            Rf = reconstruct(R, double4{gr[0], gr[MG], gr[2 * MG], gr[3 * MG]},
-                             double4{gr[4 * MG], gr[5 * MG], gr[6 * MG], gr[7 * MG]}, ro.z, ro.w, g);
+            const double* wr = L.W + r;
+            Rf = reconstruct_w(R, double4{wr[0], wr[MG], wr[2 * MG], wr[3 * MG]},
+                               double4{gr[0], gr[MG], gr[2 * MG], gr[3 * MG]},
+                               double4{gr[4 * MG], gr[5 * MG], gr[6 * MG], gr[7 * MG]}, ro.z, ro.w, g);
         } else {
             Rf = euler_ghost(m.bc[-1 - r], Lf, fg.x, fg.y, g);
         }
@@ -697,8 +721,13 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
 // mesh data of each thread's first gradient cell and first face are loaded
 // before the staging barrier, so their latency overlaps the staging.
 // ---------------------------------------------------------------------------
+#ifdef FVK_TILE_WPE
+#define FVK_TILE_ATTR __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FVK_TILE_WPE)))
+#else
+#define FVK_TILE_ATTR __launch_bounds__(kMaxBlock)
+#endif
 template <int D, int SCHEME, int ORDER>
-__global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
+__global__ FVK_TILE_ATTR void k_e_tile(EulerArgs a) {
     extern __shared__ double lds[];
     const MeshView& m = a.m;
     const int t = logical_block();
@@ -706,14 +735,20 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     const int bs = blockDim.x, tid = threadIdx.x;
     TileLds L;
     L.MG = m.tmax_g;
-    L.ML = m.tmax_l;
+    L.M2 = m.tmax_2;
     L.MF = m.tmax_f;
-    const int MG = L.MG, ML = L.ML, MF = L.MF;
+    const int MG = L.MG, M2 = L.M2, MF = L.MF;
     L.U = lds;
-    L.G = lds + 4 * MG;
-    L.W = lds + 12 * MG;
-    L.F = L.W;
-    L.Dt = L.W + 4 * MF;
+    if (ORDER == 2) {
+        L.G = lds + 4 * MG;
+        L.W = lds + 12 * MG;
+        L.W2 = lds + 16 * MG;
+        L.F = L.W2;
+    } else {
+        L.G = L.W = L.W2 = nullptr;
+        L.F = lds + 4 * MG;
+    }
+    L.Dt = L.F + 4 * MF;
     int c0 = 0, n = 0, hp = 0, h1 = 0, nh = 0, f0 = 0, nft = 0;
     if (t < m.ntiles) {
         c0 = m.tstart[t];
@@ -725,31 +760,39 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
         nft = m.tfptr[t + 1] - f0;
     }
     const int ng = n + h1, nl = n + nh;
+    L.ng = ng;
     const int2* te = m.tell + (int64_t)t * D * MG;
 
-    // 0. prefetch the mesh data of the first gradient cell and face
+    // 0. loads issued before the staging barrier, so that their latency
+    // overlaps it: the local ELL of cell tid (its gradient, and its gather
+    // when owned), the gradient mesh data of cell tid, the mesh data of the
+    // first two faces of this thread
     int2 pen[D];
     double4 peg[D], piv{0, 0, 0, 0};
-    if (ORDER == 2 && tid < ng) {
-        const int gc = tid < n ? c0 + tid : m.thalo[hp + tid - n];
+    if (tid < (ORDER == 2 ? ng : n)) {
 #pragma unroll
         for (int k = 0; k < D; ++k) pen[k] = te[k * MG + tid];
+    }
+    if (ORDER == 2 && tid < ng) {
+        const int gc = tid < n ? c0 + tid : m.thalo[hp + tid - n];
 #pragma unroll
         for (int k = 0; k < D; ++k) peg[k] = pen[k].x >= 0 ? m.egeo[k * m.ld + gc] : double4{0.0, 0.0, 0.0, 0.0};
         piv = m.clsq[gc];
     }
-    int4 pfd{0, 0, 0, 0};
-    double4 pfg{0, 0, 0, 0}, pro{0, 0, 0, 0};
-    if (tid < nft) {
-        pfd = m.tface[f0 + tid];
-        pfg = m.fgeo[pfd.x];
-        if (ORDER == 2) pro = m.frec[pfd.x];
+    int4 pfd[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    double4 pfg[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, pro[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        if (tid + i * bs < nft) {
+            pfd[i] = m.tface[f0 + tid + i * bs];
+            pfg[i] = m.fgeo[pfd[i].x];
+            if (ORDER == 2) pro[i] = m.frec[pfd[i].x];
+        }
     }
 
-    // 1. states (and primitives) of the tile and its halo
-    for (int j = tid; j < nl; j += bs) {
-        const int gc = j < n ? c0 + j : m.thalo[hp + j - n];
-        const double4 u = a.U[gc];
+    // 1. states (and primitives) of the tile and its halo; the first two
+    // passes load before they store
+    auto stage_cell = [&](int j, const double4& u) {
         if (j < ng) {
             L.U[j] = u.x;
             L.U[MG + j] = u.y;
@@ -758,12 +801,25 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
         }
         if (ORDER == 2) {
             const State U = st(u);
-            L.W[j] = U.r;
-            L.W[ML + j] = U.ru / U.r;
-            L.W[2 * ML + j] = U.rv / U.r;
-            L.W[3 * ML + j] = pressure(U, g);
+            double* w = j < ng ? L.W + j : L.W2 + (j - ng);
+            const int sw = j < ng ? MG : M2;
+            w[0] = U.r;
+            w[sw] = U.ru / U.r;
+            w[2 * sw] = U.rv / U.r;
+            w[3 * sw] = pressure(U, g);
         }
+    };
+    {
+        const int j0 = tid, j1 = tid + bs;
+        const int g0 = j0 < n ? c0 + j0 : (j0 < nl ? m.thalo[hp + j0 - n] : -1);
+        const int g1 = j1 < n ? c0 + j1 : (j1 < nl ? m.thalo[hp + j1 - n] : -1);
+        double4 u0{}, u1{};
+        if (g0 >= 0) u0 = a.U[g0];
+        if (g1 >= 0) u1 = a.U[g1];
+        if (g0 >= 0) stage_cell(j0, u0);
+        if (g1 >= 0) stage_cell(j1, u1);
     }
+    for (int j = tid + 2 * bs; j < nl; j += bs) stage_cell(j, a.U[j < n ? c0 + j : m.thalo[hp + j - n]]);
     __syncthreads();
 
     // 2. limited LSQ gradients of the owned cells and ring 1
@@ -780,34 +836,41 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
             tile_grad<D>(a, L, j, en, eg, m.clsq[gc]);
         }
     }
-    __syncthreads();  // the primitives are dead from here: their space takes the fluxes
+    __syncthreads();  // ring 2's primitives are dead from here: their space takes the fluxes
+
+    // the gather's face slots and the update's inputs of owned cell tid
+    const short* ts = m.tslot + (int64_t)t * D * bs;
+    int lf[D];
+    UpdateInputs upd{};
+    if (tid < n) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) lf[k] = ts[k * bs + tid];
+        upd = load_update_inputs(a, c0 + tid);
+    }
 
     // 3. fluxes of the tile's faces
     double cand_min = DBL_MAX;
-    if (tid < nft) cand_min = smin(cand_min, tile_face<SCHEME, ORDER>(a, L, tid, pfd, pfg, pro));
-    for (int e = tid + bs; e < nft; e += bs) {
-        const int4 fd = m.tface[f0 + e];
-        const double4 ro = ORDER == 2 ? m.frec[fd.x] : double4{0, 0, 0, 0};
-        cand_min = smin(cand_min, tile_face<SCHEME, ORDER>(a, L, e, fd, m.fgeo[fd.x], ro));
+    for (int i = 0, e = tid; e < nft; ++i, e += bs) {
+        int4 fd = i == 0 ? pfd[0] : pfd[1];
+        double4 fg = i == 0 ? pfg[0] : pfg[1], ro = i == 0 ? pro[0] : pro[1];
+        if (i >= 2) {
+            fd = m.tface[f0 + e];
+            fg = m.fgeo[fd.x];
+            if (ORDER == 2) ro = m.frec[fd.x];
+        }
+        cand_min = smin(cand_min, tile_face<SCHEME, ORDER>(a, L, e, fd, fg, ro));
     }
     __syncthreads();
 
     // 4. gather in Cell::faces order, owner minus, and the update (as k_e_gather)
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
     if (tid < n) {
-        const short* ts = m.tslot + (int64_t)t * D * bs;
-        int2 en[D];
-        int lf[D];
-#pragma unroll
-        for (int k = 0; k < D; ++k) en[k] = te[k * MG + tid];
-#pragma unroll
-        for (int k = 0; k < D; ++k) lf[k] = ts[k * bs + tid];
         double dmin = DBL_MAX;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            if (en[k].x < 0) continue;
+            if (pen[k].x < 0) continue;
             const double* F = L.F + lf[k];
-            if (en[k].x & 1) {
+            if (pen[k].x & 1) {
                 r0 = r0 - F[0]; r1 = r1 - F[MF]; r2 = r2 - F[2 * MF]; r3 = r3 - F[3 * MF];
             } else {
                 r0 += F[0]; r1 += F[MF]; r2 += F[2 * MF]; r3 += F[3 * MF];
@@ -815,10 +878,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
             if (a.dt_src == 2) dmin = smin(dmin, L.Dt[lf[k]]);
         }
         const double4 u{L.U[tid], L.U[MG + tid], L.U[2 * MG + tid], L.U[3 * MG + tid]};
-        update_cell(a, c0 + tid, u, r0, r1, r2, r3, dmin);
+        update_cell_with(a, c0 + tid, upd, u, r0, r1, r2, r3, dmin);
     }
 
     // 5. block reductions in the (now free) LDS: residual partials, dt minimum
+    int p2 = 64;  // the largest power of two not above the workgroup size
+    while (2 * p2 <= bs) p2 *= 2;
     if (a.want_res) {
         __syncthreads();
         lds[tid] = r0 * r0;
@@ -826,7 +891,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
         lds[2 * bs + tid] = r2 * r2;
         lds[3 * bs + tid] = r3 * r3;
         __syncthreads();
-        for (int s2 = bs / 2; s2 > 0; s2 >>= 1) {
+        if (tid < bs - p2) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lds[q * bs + tid] += lds[q * bs + tid + p2];
+        }
+        __syncthreads();
+        for (int s2 = p2 / 2; s2 > 0; s2 >>= 1) {
             if (tid < s2) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) lds[q * bs + tid] += lds[q * bs + tid + s2];
@@ -842,7 +912,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
         __syncthreads();
         lds[tid] = cand_min;
         __syncthreads();
-        for (int s2 = bs / 2; s2 > 0; s2 >>= 1) {
+        if (tid < bs - p2) lds[tid] = smin(lds[tid], lds[tid + p2]);
+        __syncthreads();
+        for (int s2 = p2 / 2; s2 > 0; s2 >>= 1) {
             if (tid < s2) lds[tid] = smin(lds[tid], lds[tid + s2]);
             __syncthreads();
         }
@@ -854,6 +926,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
 // host launchers
 // ---------------------------------------------------------------------------
 namespace {
+
+constexpr int kDtBlock = 256;  // global-dt pass beside the tile kernels
 
 int grid_for(int64_t n, int bs) {
     int64_t g = (n + bs - 1) / bs;
@@ -937,9 +1011,9 @@ template <int D>
 void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
     hipEvent_t ev = nullptr;
     if (s->variant == 3) {
-        if (dt_pass) {
+        if (dt_pass) {  // its block minimum needs a power-of-two workgroup
             timer_begin(s, "euler_dt", &ev);
-            k_e_dt<<<grid_for(s->nf, s->block), s->block, 0, s->stream>>>(a);
+            k_e_dt<<<grid_for(s->nf, kDtBlock), kDtBlock, 0, s->stream>>>(a);
             timer_end(s, ev);
         }
         const int grid = (s->mesh.ntiles + kXcds - 1) / kXcds * kXcds;

@@ -224,7 +224,7 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     std::vector<int4> tface;
     struct TileInfo { int n, h1, h2; };
     std::vector<TileInfo> info(nt);
-    int mg = 0, ml = 0, mf = 0;
+    int mg = 0, ml = 0, mf = 0, m2 = 0;
     // pass 1: rings and faces
     std::vector<int> cells;
     for (int t = 0; t < nt; ++t) {
@@ -267,6 +267,7 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
         tfptr.push_back((int)tface.size());
         mg = std::max(mg, n + h1);
         ml = std::max(ml, n + h1 + h2);
+        m2 = std::max(m2, h2);
         mf = std::max(mf, nft);
         for (int c : cells) loc[c] = -1;
     }
@@ -318,6 +319,7 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     m.tmax_g = mg;
     m.tmax_l = ml;
     m.tmax_f = mf;
+    m.tmax_2 = m2;
     m.tstart = upload(s, s->tile_start);
     m.thptr = upload(s, thptr);
     m.thalo = thalo.empty() ? nullptr : upload(s, thalo);
@@ -328,10 +330,13 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     m.tslot = upload(s, tslot);
     if (!m.tstart || !m.thptr || (!thalo.empty() && !m.thalo) || !m.th1 || !m.tell || !m.tfptr || !m.tface || !m.tslot)
         return set_error(FVK_ERR_NOMEM, "device allocation for the tiles failed");
-    // U (4) and G (8) of the cells with gradients, then W (4) of all local
-    // cells, later reused for the face fluxes (4) and dt candidates (1)
-    *lds_bytes = sizeof(double) * std::max((size_t)12 * mg + std::max((size_t)4 * ml, (size_t)5 * mf),
-                                           (size_t)4 * T);  // at least the block reductions' 4 T doubles
+    // second order: U (4), G (8) and W (4) of the owned and ring-1 cells, then
+    // W (4) of ring 2, whose space the face fluxes (4) and dt candidates (1)
+    // take after the gradients; first order: U (4) of the owned and ring-1
+    // cells, then the fluxes and dt candidates
+    const size_t words = second_order ? (size_t)16 * mg + std::max((size_t)4 * m2, (size_t)5 * mf)
+                                      : (size_t)4 * mg + (size_t)5 * mf;
+    *lds_bytes = sizeof(double) * std::max(words, (size_t)4 * T);  // at least the block reductions' 4 T doubles
     return FVK_OK;
 }
 

@@ -242,11 +242,47 @@ FVK_HD State reconstruct(const State& Uc, const double4& g0, const double4& g1, 
     return from_primitive(rho, u + d1, v + d2, pp, gam);
 }
 
+// reconstruct() with the cell's primitives w = (rho, u, v, p) given, as
+// computed from Uc by the same operations (u = ru / rho, v = rv / rho,
+// p = pressure(Uc)): the same result bits without the divisions.
+FVK_HD State reconstruct_w(const State& Uc, const double4& w, const double4& g0, const double4& g1, double rx,
+                           double ry, double gam) {
+    const double d0 = g0.x * rx + g0.y * ry;
+    const double d1 = g0.z * rx + g0.w * ry;
+    const double d2 = g1.x * rx + g1.y * ry;
+    const double d3 = g1.z * rx + g1.w * ry;
+    if (d0 == 0.0 && d1 == 0.0 && d2 == 0.0 && d3 == 0.0) return Uc;
+    const double rho = Uc.r + d0, pp = w.w + d3;
+    if (!(rho > 0.0) || !(pp > 0.0)) return Uc;
+    return from_primitive(rho, w.y + d1, w.z + d2, pp, gam);
+}
+
 // Venkatakrishnan limiter function (extension, no reference code):
 // ((d1^2 + eps2) + 2 d1 d2) / (d1^2 + 2 d2^2 + d1 d2 + eps2)
 FVK_HD double venkatakrishnan(double d1, double d2, double eps2) {
     double d11 = d1 * d1;
     return ((d11 + eps2) + 2.0 * d1 * d2) / (d11 + 2.0 * d2 * d2 + d1 * d2 + eps2);
 }
+
+// phi = min(1, min over the cell's faces with d2 != 0 of venkatakrishnan(d1,
+// d2, eps2)), d1 the allowed increment on d2's side (wmax - wc for d2 > 0,
+// wmin - wc for d2 < 0).  Then d1 d2 >= 0, so the numerator is >= 0 and the
+// denominator > 0, and the smallest ratio is found by cross-multiplication:
+// one division per variable instead of one per face.  Start from {1, 1};
+// venkat_pick returns true when the face becomes the minimiser (a strict
+// decrease, as smin keeps the earlier face on a tie).
+struct VenkatMin {
+    double num, den;
+};
+FVK_HD bool venkat_pick(VenkatMin& b, double wlo, double whi, double wc, double d2, double eps2) {
+    const double d1 = (d2 > 0.0 ? whi : wlo) - wc;
+    const double d11 = d1 * d1;
+    const double num = (d11 + eps2) + 2.0 * d1 * d2;
+    const double den = d11 + 2.0 * d2 * d2 + d1 * d2 + eps2;
+    const bool take = d2 != 0.0 && num * b.den < b.num * den;
+    if (take) b = VenkatMin{num, den};
+    return take;
+}
+FVK_HD double venkat_value(const VenkatMin& b) { return b.num / b.den; }
 
 }  // namespace fvk

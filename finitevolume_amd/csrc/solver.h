@@ -75,6 +75,7 @@ struct MeshView {
     // the first ring of neighbours, then (second order) the second ring.
     int ntiles;
     int tmax_g, tmax_l, tmax_f; // most local cells with gradients (owned + ring 1), local cells, faces in a tile
+    int tmax_2;                 // most ring-2 cells of a tile
     const int* tstart;      // [ntiles + 1]
     const int* thptr;       // [ntiles + 1] offsets into thalo
     const int* thalo;       // ring-1 then ring-2 cells of each tile (device ids)

@@ -48,6 +48,7 @@ def lib():
     L.fvk_set_cfl.argtypes = [vp, C.c_double]
     L.fvk_gas_gamma.argtypes = [vp]
     L.fvk_gas_gamma.restype = C.c_double
+    L.fvk_get_launch_shape.argtypes = [vp, C.POINTER(abi.LaunchShapeC)]
     L.fvk_gradient.argtypes = [vp, C.c_int32, dp, dp, dp]
     L.fvk_wall_distance.argtypes = [vp, dp]
     L.fvk_wall_distance_faces.argtypes = [C.POINTER(abi.MeshDescC), ip, i64, dp]
@@ -145,6 +146,12 @@ class FVKSolver:
 
     def gas_gamma(self) -> float:
         return self.L.fvk_gas_gamma(self.h)
+
+    def launch_shape(self) -> dict:
+        """Kernel variant, workgroup size and (variant 3) tile layout in use."""
+        sh = abi.LaunchShapeC()
+        _check(self.L.fvk_get_launch_shape(self.h, C.byref(sh)))
+        return {f: getattr(sh, f) for f, _ in abi.LaunchShapeC._fields_ if f != "reserved0"}
 
     # ---- extensions ----
     def residual_eval(self) -> np.ndarray:

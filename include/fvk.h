@@ -160,8 +160,8 @@ typedef struct fvk_params {
     /* ---- execution tuning (same results bit for bit; 0 = library default) ---- */
     int32_t kernel_variant;        /* Euler: 1 face sweep + per-cell gather, 2 per-cell fused flux and update,
                                       3 one fused pass per cell tile with its halo staged in LDS (0: 1) */
-    int32_t block_size;            /* threads per workgroup of the Euler kernels: 64, 128, 256 or 512
-                                      (0: 512; variant 3: 256) */
+    int32_t block_size;            /* threads per workgroup of the Euler kernels: 64, 128, 256 or 512,
+                                      for variant 3 any multiple of 64 up to 512 (0: 512; variant 3: 256) */
     int32_t tile_cells;            /* variant 3: most cells of one tile, at most block_size (0: block_size) */
     int32_t reserved1;
 } fvk_params;
@@ -221,6 +221,22 @@ int fvk_get_state(const fvk_solver* s, double* U_aos, int64_t n_cells);
 
 int fvk_set_cfl(fvk_solver* s, double cfl);
 double fvk_gas_gamma(const fvk_solver* s);
+
+/* Launch shape the solver runs its Euler stage with (no counterpart in the
+ * reference: it reports what the defaults or a tuner chose, and the tile
+ * layout of variant 3). */
+typedef struct fvk_launch_shape {
+    int32_t kernel_variant;        /* 1, 2 or 3 */
+    int32_t block_size;            /* threads per workgroup */
+    int32_t tile_cells;            /* variant 3: most owned cells of a tile (else 0) */
+    int32_t n_tiles;               /* variant 3: number of tiles (else 0) */
+    int32_t tile_max_staged;       /* variant 3: most owned + ring-1 cells of a tile */
+    int32_t tile_max_ring2;        /* variant 3, order 2: most ring-2 cells of a tile */
+    int32_t tile_max_faces;        /* variant 3: most faces of a tile */
+    int32_t reserved0;
+    int64_t tile_lds_bytes;        /* variant 3: dynamic LDS per workgroup */
+} fvk_launch_shape;
+int fvk_get_launch_shape(const fvk_solver* s, fvk_launch_shape* shape);
 
 /* GradientCalculator::compute for this solver's mesh and scheme: phi
  * [n_cells], boundary_phi [n_faces] (reference face order; only boundary

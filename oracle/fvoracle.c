@@ -745,18 +745,24 @@ static void limited_gradients(fvo_solver* s, const double* U) {
         if (s->p.limiter == FVK_LIMITER_VENKATAKRISHNAN) {
             double kh = s->p.venkat_k * sqrt(s->vol[c]);
             double eps2 = kh * kh * kh;
+            /* phi_q = min(1, min over faces with d2 != 0 of num/den): d1 d2 >= 0
+             * makes num >= 0 and den > 0, so the smallest ratio is selected by
+             * cross-multiplication (a strict decrease replaces it) and divided once */
+            double bn[4] = {1.0, 1.0, 1.0, 1.0}, bd[4] = {1.0, 1.0, 1.0, 1.0};
             for (int k = s->cptr[c]; k < s->cptr[c + 1]; ++k) {
                 int f = s->cface[k];
                 vec2 m = mid_in(s, f, (int)c);
                 double rx = m.x - s->cxy[2 * c], ry = m.y - s->cxy[2 * c + 1];
                 for (int q = 0; q < 4; ++q) {
                     double d2 = gx[q] * rx + gy[q] * ry;
-                    double ph = 1.0;
-                    if (d2 > 0.0) ph = fvo_venkatakrishnan(wmax[q] - wc[q], d2, eps2);
-                    else if (d2 < 0.0) ph = fvo_venkatakrishnan(wmin[q] - wc[q], d2, eps2);
-                    lim[q] = smin(lim[q], ph);
+                    double d1 = (d2 > 0.0 ? wmax[q] : wmin[q]) - wc[q];
+                    double d11 = d1 * d1;
+                    double num = (d11 + eps2) + 2.0 * d1 * d2;
+                    double den = d11 + 2.0 * d2 * d2 + d1 * d2 + eps2;
+                    if (d2 != 0.0 && num * bd[q] < bn[q] * den) { bn[q] = num; bd[q] = den; }
                 }
             }
+            for (int q = 0; q < 4; ++q) lim[q] = bn[q] / bd[q];
         }
         if (s->p.limiter == FVK_LIMITER_FIRST_ORDER)
             for (int q = 0; q < 4; ++q) lim[q] = 0.0;

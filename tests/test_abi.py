@@ -24,6 +24,7 @@ STRUCTS = {
     "fvk_sst_constants": abi.SSTConstantsC,
     "fvk_params": abi.ParamsC,
     "fvk_initial_condition": abi.InitialConditionC,
+    "fvk_launch_shape": abi.LaunchShapeC,
 }
 
 

@@ -361,6 +361,53 @@ def test_venkatakrishnan_range():
     assert abs(pyoracle.venkatakrishnan(0.3, 0.1, 1e6) - 1.0) < 1e-6
 
 
+def test_venkatakrishnan_limiter_brute_force():
+    """phi per cell and variable is min(1, min over the cell's faces of the
+    limiter function), recomputed here face by face from the unlimited
+    gradients.  Outflow boundaries make the ghost equal to the cell, so a
+    boundary face adds its midpoint to the minimum but no new extreme."""
+    m = M.tri_box(14, 12)
+    rng = np.random.default_rng(5)
+    x, y = m.cell_xy[:, 0], m.cell_xy[:, 1]
+    n = m.n_cells
+    W = np.stack([1.0 + 0.3 * np.sin(5 * x) + 0.05 * rng.standard_normal(n),
+                  0.3 + 0.2 * np.cos(4 * y) + 0.05 * rng.standard_normal(n),
+                  -0.1 + 0.2 * x * y + 0.05 * rng.standard_normal(n),
+                  1.0 + 0.2 * np.sin(3 * x + 2 * y) + 0.05 * rng.standard_normal(n)], 1)
+    U = M.from_primitive(W[:, 0], W[:, 1], W[:, 2], W[:, 3], 1.4)
+    bcs = [abi.make_bc(abi.BC_OUTFLOW)] * 4
+    out = {}
+    for lim in (abi.LIMITER_NONE, abi.LIMITER_VENKATAKRISHNAN):
+        o = pyoracle.OracleSolver(m, bcs, abi.default_params(equation=abi.EQ_EULER, order=2, limiter=lim,
+                                                             venkat_k=0.3))
+        o.set_field(U)
+        o.residual_eval()
+        out[lim] = o.last_gradients()
+    g0, _ = out[abi.LIMITER_NONE]
+    g, phi = out[abi.LIMITER_VENKATAKRISHNAN]
+    Wc = np.stack([U[:, 0], U[:, 1] / U[:, 0], U[:, 2] / U[:, 0], 0.4 * (U[:, 3] - 0.5 * U[:, 0] * (
+        (U[:, 1] / U[:, 0]) ** 2 + (U[:, 2] / U[:, 0]) ** 2))], 1)
+    want = np.ones((n, 4))
+    for c in range(n):
+        faces = m.cell_faces[m.cell_face_ptr[c]:m.cell_face_ptr[c + 1]]
+        nb = [int(a if b == c else b) for a, b in m.face_cells[faces] if a >= 0 and b >= 0]
+        wmin = np.min(Wc[[c] + nb], axis=0)
+        wmax = np.max(Wc[[c] + nb], axis=0)
+        eps2 = (0.3 * math.sqrt(m.cell_volume[c])) ** 3
+        for f in faces:
+            r = 0.5 * (m.node_xy[m.face_nodes[f, 0]] + m.node_xy[m.face_nodes[f, 1]]) - m.cell_xy[c]
+            for q in range(4):
+                d2 = g0[c, 2 * q] * r[0] + g0[c, 2 * q + 1] * r[1]
+                if d2 == 0.0:
+                    continue
+                d1 = (wmax[q] if d2 > 0 else wmin[q]) - Wc[c, q]
+                ph = (d1 * d1 + eps2 + 2 * d1 * d2) / (d1 * d1 + 2 * d2 * d2 + d1 * d2 + eps2)
+                want[c, q] = min(want[c, q], ph)
+    assert (want < 0.999).mean() > 0.3  # the limiter is active in most cells
+    assert np.abs(phi - want).max() <= 1e-12
+    assert np.array_equal(g, np.repeat(phi, 2, axis=1) * g0)
+
+
 def _vortex_error(n, order, stages):
     """L1 density error of the isentropic vortex after it is advected a
     fixed time (an exact translating solution)."""
