@@ -65,6 +65,10 @@ struct EulerArgs {
     double* case_partials;
     int n_cases;
     int stages;
+    // finalize of the previous stage's norms, run by one spare workgroup of
+    // the tile kernel (fin_nblocks == 0: none)
+    int fin_nblocks, fin_slot;
+    int64_t fin_hist_idx;
 };
 
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
@@ -209,35 +213,38 @@ __global__ __launch_bounds__(kCaseFinBlock) void k_case_finalize(const double* p
 
 __global__ void k_step_tick(unsigned int* step) { *step += 1u; }
 
-// one block: residual norms = sqrt of the sums of the partials (fixed order),
-// the step's dt into the history, and the next step's dt slot re-armed
-constexpr int kFinBlock = 1024;
-__global__ __launch_bounds__(kFinBlock) void k_e_finalize(const double* partials, int nblocks, int nv,
-                                                          double* res_hist, double* dt_hist, int64_t hist_idx,
-                                                          unsigned long long* dtmin, int slot,
-                                                          double fixed_dt = -1.0) {
-    __shared__ double sm[kFinBlock];
+// one block: residual norms = sqrt of the sums of the partials (fixed order
+// for a given workgroup size), the step's dt into the history, and the next
+// step's dt slot re-armed; sm holds blockDim.x doubles
+__device__ __forceinline__ void finalize_norms(const double* partials, int nblocks, int nv, double* res_hist,
+                                               double* dt_hist, int64_t hist_idx, unsigned long long* dtmin,
+                                               int slot, double fixed_dt, double* sm) {
+    const int nt = blockDim.x, tid = threadIdx.x;
+    int p2 = 64;  // the largest power of two not above the workgroup size
+    while (2 * p2 <= nt) p2 *= 2;
     for (int q = 0; q < nv; ++q) {
         // four independent loads in flight per thread, summed in a fixed order
         const double* p = partials + (int64_t)q * nblocks;
         double acc = 0.0;
-        for (int i = threadIdx.x; i < nblocks; i += 4 * kFinBlock) {
+        for (int i = tid; i < nblocks; i += 4 * nt) {
             double v[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = i + u * kFinBlock < nblocks ? p[i + u * kFinBlock] : 0.0;
+            for (int u = 0; u < 4; ++u) v[u] = i + u * nt < nblocks ? p[i + u * nt] : 0.0;
 #pragma unroll
             for (int u = 0; u < 4; ++u) acc += v[u];
         }
-        sm[threadIdx.x] = acc;
+        sm[tid] = acc;
         __syncthreads();
-        for (int s = kFinBlock / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+        if (tid < nt - p2) sm[tid] += sm[tid + p2];
+        __syncthreads();
+        for (int s = p2 / 2; s > 0; s >>= 1) {
+            if (tid < s) sm[tid] += sm[tid + s];
             __syncthreads();
         }
-        if (threadIdx.x == 0) res_hist[hist_idx * nv + q] = sqrt(sm[0]);
+        if (tid == 0) res_hist[hist_idx * nv + q] = sqrt(sm[0]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         if (fixed_dt >= 0.0) {
             dt_hist[hist_idx] = fixed_dt;
         } else {
@@ -245,6 +252,15 @@ __global__ __launch_bounds__(kFinBlock) void k_e_finalize(const double* partials
             dtmin[slot ^ 1] = dbits(DBL_MAX);
         }
     }
+}
+
+constexpr int kFinBlock = 1024;
+__global__ __launch_bounds__(kFinBlock) void k_e_finalize(const double* partials, int nblocks, int nv,
+                                                          double* res_hist, double* dt_hist, int64_t hist_idx,
+                                                          unsigned long long* dtmin, int slot,
+                                                          double fixed_dt = -1.0) {
+    __shared__ double sm[kFinBlock];
+    finalize_norms(partials, nblocks, nv, res_hist, dt_hist, hist_idx, dtmin, slot, fixed_dt, sm);
 }
 
 // stage update of one cell: U' = U + dt/V R (src/EulerFVMSolver.cpp:103) on
@@ -731,6 +747,11 @@ __global__ FVK_TILE_ATTR void k_e_tile(EulerArgs a) {
     extern __shared__ double lds[];
     const MeshView& m = a.m;
     const int t = logical_block();
+    if (a.fin_nblocks > 0 && t == m.ntiles) {  // the spare workgroup: the previous stage's norms
+        finalize_norms(a.partials, a.fin_nblocks, 4, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin, a.fin_slot,
+                       -1.0, lds);
+        return;
+    }
     const double g = a.gamma;
     const int bs = blockDim.x, tid = threadIdx.x;
     TileLds L;
@@ -1016,15 +1037,23 @@ void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
             k_e_dt<<<grid_for(s->nf, kDtBlock), kDtBlock, 0, s->stream>>>(a);
             timer_end(s, ev);
         }
-        const int grid = (s->mesh.ntiles + kXcds - 1) / kXcds * kXcds;
+        // a pending finalize takes a spare workgroup (logical block ntiles)
+        const int grid = (s->mesh.ntiles + (a.fin_nblocks > 0 ? 1 : 0) + kXcds - 1) / kXcds * kXcds;
         timer_begin(s, "euler_tile", &ev);
         Launch<D>::tile(s, a, grid);
         timer_end(s, ev);
         if (a.want_res) {
-            timer_begin(s, "euler_finalize", &ev);
-            k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(a.partials, grid, 4, a.res_hist, a.dt_hist, a.hist_idx,
-                                                         a.dtmin, a.slot);
-            timer_end(s, ev);
+            if (a.stages >= 2 && !a.R) {
+                // the next stage's tile kernel reduces this stage's partials
+                s->fin_nblocks = grid;
+                s->fin_slot = a.slot;
+                s->fin_hist_idx = a.hist_idx;
+            } else {
+                timer_begin(s, "euler_finalize", &ev);
+                k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(a.partials, grid, 4, a.res_hist, a.dt_hist,
+                                                             a.hist_idx, a.dtmin, a.slot);
+                timer_end(s, ev);
+            }
         }
         return;
     }
@@ -1103,6 +1132,13 @@ int launch_euler_stage(fvk_solver* s, int stage, int stages, int /*unused*/, int
     a.want_res = want_res && stage == 0;
     a.slot = (int)(step & 1);
     a.hist_idx = step % s->hist_cap;
+    if (stage == 0) s->fin_nblocks = 0;  // nothing left from an interrupted step
+    if (s->variant == 3 && stage > 0 && s->fin_nblocks > 0) {
+        a.fin_nblocks = s->fin_nblocks;
+        a.fin_slot = s->fin_slot;
+        a.fin_hist_idx = s->fin_hist_idx;
+        s->fin_nblocks = 0;
+    }
     bool dt_pass = false;
     if (s->p.local_dt) {
         a.face_dt = stage == 0;

@@ -107,6 +107,10 @@ struct fvk_solver {
     int tile_cells = 256;       // variant 3: most cells of one tile (at most block)
     std::vector<int> tile_start; // variant 3: [ntiles + 1] device cell ranges of the tiles
     size_t tile_lds = 0;         // variant 3: dynamic LDS bytes per tile
+    // variant 3, SSP-RK: the first stage's norm partials, reduced by the next
+    // stage's tile kernel (fin_nblocks == 0: nothing pending)
+    int fin_nblocks = 0, fin_slot = 0;
+    int64_t fin_hist_idx = 0;
     int64_t nc = 0, nf = 0, nn = 0;
     int nfi = 0, nnz = 0;
     bool periodic = false;
