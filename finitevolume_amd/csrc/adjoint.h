@@ -24,8 +24,10 @@ struct Prim {
 FVK_HD Prim to_prim(const State& s, double g) { return Prim{s.r, s.ru / s.r, s.rv / s.r, pressure(s, g)}; }
 
 // adjoint of to_prim: sb += (dW/dU)^T wb
+// (reverse-only arithmetic: one reciprocal of rho instead of five divisions)
 FVK_HD void prim_rev(const State& s, const Prim& wb, State& sb, double g) {
-    const double u = s.ru / s.r, v = s.rv / s.r;
+    const double ir = 1.0 / s.r;
+    const double u = s.ru * ir, v = s.rv * ir;
     const double gm = g - 1.0;
     // p = (g - 1) (E - 0.5 r (u^2 + v^2))
     sb.E += gm * wb.p;
@@ -33,9 +35,9 @@ FVK_HD void prim_rev(const State& s, const Prim& wb, State& sb, double g) {
     const double ub = wb.u - gm * s.r * u * wb.p;
     const double vb = wb.v - gm * s.r * v * wb.p;
     // u = ru / r, v = rv / r
-    sb.ru += ub / s.r;
-    sb.rv += vb / s.r;
-    rb -= (ub * u + vb * v) / s.r;
+    sb.ru += ub * ir;
+    sb.rv += vb * ir;
+    rb -= (ub * u + vb * v) * ir;
     sb.r += rb;
 }
 
@@ -44,7 +46,7 @@ FVK_HD void from_prim_rev(const Prim& w, const State& ob, Prim& wb, double g) {
     wb.r += ob.r + ob.ru * w.u + ob.rv * w.v + ob.E * 0.5 * (w.u * w.u + w.v * w.v);
     wb.u += ob.ru * w.r + ob.E * w.r * w.u;
     wb.v += ob.rv * w.r + ob.E * w.r * w.v;
-    wb.p += ob.E / (g - 1.0);
+    wb.p += ob.E * (1.0 / (g - 1.0));
 }
 
 // adjoint of phys_flux(s, nx, ny)
@@ -66,8 +68,9 @@ FVK_HD void phys_flux_rev(const State& s, double nx, double ny, double g, const 
 
 // adjoint of c = sound_speed(s) = sqrt(g p / rho), as primitive adjoints
 FVK_HD void sound_speed_rev(double r, double c, double cb, Prim& wb, double g) {
-    wb.p += cb * g / (2.0 * c * r);
-    wb.r -= cb * c / (2.0 * r);
+    const double t = cb / (2.0 * c * r);  // dc/dp = g / (2 c r), dc/drho = -c / (2 r) = -(c^2 / g) dc/dp
+    wb.p += t * g;
+    wb.r -= t * c * c;
 }
 
 // adjoint of rusanov(L, R, nx, ny)
@@ -91,11 +94,12 @@ FVK_HD void rusanov_rev(const State& L, const State& R, double nx, double ny, do
     State& Kb = takeR ? Rb : Lb;
     const double VnK = takeR ? VnR : VnL, cK = takeR ? cR : cL;
     const double Vnb = VnK >= 0.0 ? Sb : -Sb;
-    Kb.ru += Vnb * nx / K.r;
-    Kb.rv += Vnb * ny / K.r;
-    Kb.r -= Vnb * VnK / K.r;
-    nxb += Vnb * K.ru / K.r;
-    nyb += Vnb * K.rv / K.r;
+    const double iK = 1.0 / K.r;
+    Kb.ru += Vnb * nx * iK;
+    Kb.rv += Vnb * ny * iK;
+    Kb.r -= Vnb * VnK * iK;
+    nxb += Vnb * K.ru * iK;
+    nyb += Vnb * K.rv * iK;
     Prim wb{0.0, 0.0, 0.0, 0.0};
     sound_speed_rev(K.r, cK, Sb, wb, g);
     prim_rev(K, wb, Kb, g);
@@ -149,22 +153,25 @@ FVK_HD void hllc_rev(const State& L, const State& R, double nx, double ny, doubl
     // Es = factor Q, Q = E_K / r_K + (Ss - Vn_K) T, T = Ss + p_K / D
     factorb += Esb * Q;
     const double Qb = Esb * factor;
-    UKb.E += Qb / rK;
-    double rKb = -Qb * UK.E / (rK * rK);
+    const double irK = 1.0 / rK;
+    UKb.E += Qb * irK;
+    double rKb = -Qb * UK.E * irK * irK;
     double Ssb = Qb * T;
     double VnKb = -Qb * T;
     const double Tb = Qb * (Ss - VnK);
     Ssb += Tb;
-    const double pKb = Tb / D;
-    double Db = -Tb * pK / (D * D);
+    const double iD = 1.0 / D;
+    const double pKb = Tb * iD;
+    double Db = -Tb * pK * iD * iD;
     // mn = factor Ss, mt = factor Vt_K
     factorb += mnb * Ss;
     Ssb += mnb * factor;
     factorb += mtb * VtK;
     const double VtKb = mtb * factor;
     // factor = D / (SK - Ss)
-    Db += factorb / E2;
-    const double E2b = -factorb * factor / E2;
+    const double iE2 = 1.0 / E2;
+    Db += factorb * iE2;
+    const double E2b = -factorb * factor * iE2;
     SKb += E2b;
     Ssb -= E2b;
     // D = r_K (SK - Vn_K)
@@ -172,8 +179,9 @@ FVK_HD void hllc_rev(const State& L, const State& R, double nx, double ny, doubl
     SKb += Db * rK;
     VnKb -= Db * rK;
     // Ss = num / den
-    const double numb = Ssb / den;
-    const double denb = -Ssb * Ss / den;
+    const double iden = 1.0 / den;
+    const double numb = Ssb * iden;
+    const double denb = -Ssb * Ss * iden;
     double pLb = -numb, pRb = numb;
     double rLb = numb * VnL * (SL - VnL) + denb * (SL - VnL);
     double rRb = -numb * VnR * (SR - VnR) - denb * (SR - VnR);
@@ -263,19 +271,20 @@ FVK_HD void reconstruct_rev(const State& Uc, const double4& g0, const double4& g
 FVK_HD void venkatakrishnan_grad(double d1, double d2, double eps2, double& p1, double& p2, double& pe) {
     const double N = d1 * d1 + eps2 + 2.0 * d1 * d2;
     const double D = d1 * d1 + 2.0 * d2 * d2 + d1 * d2 + eps2;
-    const double D2 = D * D;
-    p1 = ((2.0 * d1 + 2.0 * d2) * D - N * (2.0 * d1 + d2)) / D2;
-    p2 = (2.0 * d1 * D - N * (4.0 * d2 + d1)) / D2;
-    pe = (D - N) / D2;
+    const double iD2 = 1.0 / (D * D);
+    p1 = ((2.0 * d1 + 2.0 * d2) * D - N * (2.0 * d1 + d2)) * iD2;
+    p2 = (2.0 * d1 * D - N * (4.0 * d2 + d1)) * iD2;
+    pe = (D - N) * iD2;
 }
 
 // adjoint of the face length and unit normal of an edge e = node2 - node1:
 // A = |e|, n = (e_y, -e_x) / A.  Adds the adjoint of e.
 FVK_HD void edge_rev(double nx, double ny, double A, double nxb, double nyb, double Ab, double& exb, double& eyb) {
     // e = A (-ny, nx)
-    const double At = Ab - (nxb * nx + nyb * ny) / A;
-    exb += -nyb / A - At * ny;
-    eyb += nxb / A + At * nx;
+    const double iA = 1.0 / A;
+    const double At = Ab - (nxb * nx + nyb * ny) * iA;
+    exb += -nyb * iA - At * ny;
+    eyb += nxb * iA + At * nx;
 }
 
 }  // namespace fvk

@@ -358,7 +358,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_cell(AdjArgs a) {
                 if (en[k].x < 0) continue;
                 // offset d to the neighbour point, weight w = 1/|d|^2; (w dx, w dy) is stored
                 const double w = eg[k].x * eg[k].x + eg[k].y * eg[k].y;
-                const double dx = eg[k].x / w, dy = eg[k].y / w;
+                const double iw = 1.0 / w;
+                const double dx = eg[k].x * iw, dy = eg[k].y * iw;
                 const double sq = M00 * dx * dx + 2.0 * Ms * dx * dy + M11 * dy * dy;
                 const double wq = wdxb[k] * dx + wdyb[k] * dy;
                 const double w2 = w * w;
@@ -463,11 +464,12 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_gather(AdjArgs a) {
         Sx += (x[i] + xj) * cr;
         Sy += (y[i] + yj) * cr;
     }
-    const double cx = Sx / (3.0 * A2), cy = Sy / (3.0 * A2);
+    const double i3A2 = 1.0 / (3.0 * A2);
+    const double cx = Sx * i3A2, cy = Sy * i3A2;
     const double4 cb = a.cb[c];
     const double Cbx = cb.x + cbx, Cby = cb.y + cby;
-    const double Sxb = Cbx / (3.0 * A2), Syb = Cby / (3.0 * A2);
-    const double A2b = -(Cbx * cx + Cby * cy) / A2 + 0.5 * (A2 > 0.0 ? 1.0 : -1.0) * cb.z;
+    const double Sxb = Cbx * i3A2, Syb = Cby * i3A2;
+    const double A2b = -(Cbx * cx + Cby * cy) * (3.0 * i3A2) + 0.5 * (A2 > 0.0 ? 1.0 : -1.0) * cb.z;
     double xb[D], yb[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) xb[k] = yb[k] = 0.0;

@@ -288,8 +288,9 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
     if (ng) s->grad = static_cast<double*>(dev_alloc(s, sizeof(double) * ng * s->ld));
     if (params->equation == FVK_EQ_RANS_SA || params->equation == FVK_EQ_RANS_SST)
         s->aux = static_cast<double*>(dev_alloc(s, sizeof(double) * 4 * s->ld));
-    // launch grids (workgroups of 64 threads or more) are rounded up to a multiple of the 8 XCDs
-    int64_t max_blocks = (std::max(s->nc, s->nf) + 63) / 64 + 8;
+    // launch grids (workgroups of 64 threads or more, or one per tile plus a
+    // spare) are rounded up to a multiple of the 8 XCDs
+    int64_t max_blocks = std::max((std::max(s->nc, s->nf) + 63) / 64, (int64_t)s->mesh.ntiles + 1) + 8;
     s->partials = static_cast<double*>(dev_alloc(s, sizeof(double) * 6 * max_blocks));
     s->dtmin = static_cast<unsigned long long*>(dev_alloc(s, 256));
     s->hist_cap = 1024;

@@ -31,7 +31,6 @@
 
 namespace fvk {
 
-constexpr int kXcds = 8;
 constexpr int kMaxBlock = 512;
 
 struct EulerArgs {
@@ -73,18 +72,6 @@ struct EulerArgs {
 
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 
-// Workgroups are dispatched round-robin over the XCDs; remap so that XCD x
-// gets the contiguous range [x * per, (x + 1) * per) of logical blocks.
-// The grid is a multiple of kXcds.
-__device__ __forceinline__ int xcd_index() {
-    const int per = gridDim.x / kXcds;
-    return ((blockIdx.x % kXcds) * per + blockIdx.x / kXcds) * blockDim.x + threadIdx.x;
-}
-
-__device__ __forceinline__ int logical_block() {
-    const int per = gridDim.x / kXcds;
-    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
-}
 
 // case of device cell c (binary search of the case ranges)
 __device__ __forceinline__ int case_of(const EulerArgs& a, int c) {
@@ -737,13 +724,8 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
 // mesh data of each thread's first gradient cell and first face are loaded
 // before the staging barrier, so their latency overlaps the staging.
 // ---------------------------------------------------------------------------
-#ifdef FVK_TILE_WPE
-#define FVK_TILE_ATTR __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FVK_TILE_WPE)))
-#else
-#define FVK_TILE_ATTR __launch_bounds__(kMaxBlock)
-#endif
 template <int D, int SCHEME, int ORDER>
-__global__ FVK_TILE_ATTR void k_e_tile(EulerArgs a) {
+__global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     extern __shared__ double lds[];
     const MeshView& m = a.m;
     const int t = logical_block();
@@ -950,10 +932,6 @@ namespace {
 
 constexpr int kDtBlock = 256;  // global-dt pass beside the tile kernels
 
-int grid_for(int64_t n, int bs) {
-    int64_t g = (n + bs - 1) / bs;
-    return (int)((g + kXcds - 1) / kXcds * kXcds);
-}
 
 EulerArgs make_args(fvk_solver* s) {
     EulerArgs a{};

@@ -178,7 +178,7 @@ template <int EQ, int GS, bool PERIODIC>
 __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
     constexpr int NF = EQ == EQ_SST ? 5 : (EQ == EQ_SA ? 4 : 3);
     const MeshView& m = a.m;
-    int c = blockIdx.x * kVB + threadIdx.x;
+    int c = xcd_index();
     if (c >= m.nc) return;
     const double g = a.gamma;
     State Uc = ldS(a.U, a.ld, c);
@@ -310,7 +310,7 @@ __device__ __forceinline__ void corrected(const FG& fg, double nx, double ny, do
 template <int EQ, int SCHEME>
 __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
     const MeshView& m = a.m;
-    int f = blockIdx.x * kVB + threadIdx.x;
+    int f = xcd_index();
     double cand = DBL_MAX;
     if (f < m.nf) {
         const double g = a.gamma;
@@ -503,7 +503,7 @@ template <int EQ>
 __global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
     constexpr int NV = EQ == EQ_SST ? 6 : (EQ == EQ_SA ? 5 : 4);
     const MeshView& m = a.m;
-    int c = blockIdx.x * kVB + threadIdx.x;
+    int c = xcd_index();
     double r[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) r[q] = 0.0;
@@ -573,7 +573,7 @@ __global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kVB) void k_diff_face(ViscArgs a) {
     const MeshView& m = a.m;
-    int f = blockIdx.x * kVB + threadIdx.x;
+    int f = xcd_index();
     if (f >= m.nf) return;
     int cl = m.fl[f], cr = m.fr[f];
     double phiL = a.U[cl];
@@ -599,7 +599,7 @@ __global__ __launch_bounds__(kVB) void k_diff_face(ViscArgs a) {
 
 __global__ __launch_bounds__(kVB) void k_diff_cell(ViscArgs a) {
     const MeshView& m = a.m;
-    int c = blockIdx.x * kVB + threadIdx.x;
+    int c = xcd_index();
     double r[1] = {0.0};
     if (c < m.nc) {
         for (int k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
@@ -622,7 +622,7 @@ __global__ __launch_bounds__(kVB) void k_diff_cell(ViscArgs a) {
 template <int GS>
 __global__ __launch_bounds__(kVB) void k_scalar_grad(MeshView m, const double* phi, const double* bphi, double* grad,
                                                      int periodic) {
-    int c = blockIdx.x * kVB + threadIdx.x;
+    int c = xcd_index();
     if (c >= m.nc) return;
     double pc = phi[c], cx = m.cx[c], cy = m.cy[c];
     double sx = 0.0, sy = 0.0;
@@ -674,7 +674,7 @@ __global__ __launch_bounds__(kVB) void k_wall_distance(const double* cx, const d
                                                        const double* seg /* [4][nw] x1 y1 x2 y2 */, int nw,
                                                        double* out) {
     __shared__ double s[4][kVB];
-    int c = blockIdx.x * kVB + threadIdx.x;
+    int c = xcd_index();
     double px = c < nc ? cx[c] : 0.0, py = c < nc ? cy[c] : 0.0;
     double best = INFINITY;
     for (int base = 0; base < nw; base += kVB) {
@@ -756,8 +756,8 @@ void launch_face_eq(fvk_solver* s, int grid, const ViscArgs& a) {
 }
 
 void run_visc(fvk_solver* s, ViscArgs& a) {
-    const int gc = (int)((s->nc + kVB - 1) / kVB);
-    const int gf = (int)((s->nf + kVB - 1) / kVB);
+    const int gc = grid_for(s->nc, kVB);
+    const int gf = grid_for(s->nf, kVB);
     hipEvent_t ev = nullptr;
     timer_begin(s, "visc_grad", &ev);
     switch (s->p.equation) {
@@ -825,8 +825,8 @@ int launch_diffusion_step(fvk_solver* s, int want_res, int64_t step) {
     a.Unew = s->U[s->cur];
     a.want_res = want_res;
     a.hist_idx = step % s->hist_cap;
-    const int gc = (int)((s->nc + kVB - 1) / kVB);
-    const int gf = (int)((s->nf + kVB - 1) / kVB);
+    const int gc = grid_for(s->nc, kVB);
+    const int gf = grid_for(s->nf, kVB);
     hipEvent_t ev = nullptr;
     timer_begin(s, "diffusion_face", &ev);
     k_diff_face<<<gf, kVB, 0, s->stream>>>(a);
@@ -842,15 +842,15 @@ int launch_diffusion_residual_eval(fvk_solver* s, double* R_dev) {
     ViscArgs a = make_vargs(s);
     a.U = s->U[s->cur];
     a.R = R_dev;
-    const int gc = (int)((s->nc + kVB - 1) / kVB);
-    const int gf = (int)((s->nf + kVB - 1) / kVB);
+    const int gc = grid_for(s->nc, kVB);
+    const int gf = grid_for(s->nf, kVB);
     k_diff_face<<<gf, kVB, 0, s->stream>>>(a);
     k_diff_cell<<<gc, kVB, 0, s->stream>>>(a);
     return hip_check(hipGetLastError(), "diffusion residual launch");
 }
 
 int launch_gradient(fvk_solver* s, int scheme, const double* phi, const double* bphi_dev, double* grad) {
-    const int gc = (int)((s->nc + kVB - 1) / kVB);
+    const int gc = grid_for(s->nc, kVB);
     if (scheme == FVK_GRAD_LEAST_SQUARES)
         k_scalar_grad<FVK_GRAD_LEAST_SQUARES><<<gc, kVB, 0, s->stream>>>(s->mesh, phi, bphi_dev, grad, s->periodic);
     else
@@ -860,7 +860,7 @@ int launch_gradient(fvk_solver* s, int scheme, const double* phi, const double* 
 
 int launch_wall_distance_segments(const double* cx, const double* cy, int nc, const double* seg, int nw, double* out,
                                   hipStream_t st) {
-    int grid = (nc + kVB - 1) / kVB;
+    int grid = grid_for(nc, kVB);
     k_wall_distance<<<grid, kVB, 0, st>>>(cx, cy, nc, seg, nw, out);
     return hip_check(hipGetLastError(), "wall distance launch");
 }

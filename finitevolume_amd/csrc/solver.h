@@ -16,6 +16,21 @@ namespace fvk {
 constexpr int kMaxPatches = 1024;
 constexpr int kMaxEulerDegree = 16;  // most faces of one cell the per-cell Euler kernels take
 constexpr int kMaxTileLds = 65536;   // LDS bytes one tile of the fused Euler stage may use
+constexpr int kXcds = 8;             // accelerator complex dies of an MI355X, one L2 each
+
+// Workgroups are dispatched round-robin over the XCDs; the remap below gives
+// XCD x the contiguous range [x * per, (x + 1) * per) of logical blocks, so
+// that neighbouring cells and faces share an L2.  The grid must be a
+// multiple of kXcds (grid_for).
+__device__ __forceinline__ int logical_block() {
+    const int per = gridDim.x / kXcds;
+    return (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+}
+__device__ __forceinline__ int xcd_index() { return logical_block() * blockDim.x + threadIdx.x; }
+inline int grid_for(int64_t n, int bs) {
+    const int64_t g = (n + bs - 1) / bs;
+    return (int)((g + kXcds - 1) / kXcds * kXcds);
+}
 
 // Per-patch boundary data on the device (fvk_bc reduced to what kernels read).
 struct DevBC {

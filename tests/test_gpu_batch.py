@@ -95,7 +95,8 @@ def test_autotuner_picks_a_valid_shape(three, tmp_path, monkeypatch):
     p = abi.ParamsC.from_buffer_copy(params)
     best = autotune.tune(meshes[0], bcs[0], p, ics[0], steps=5, warmup=1)
     assert (p.kernel_variant, p.block_size) == (best["variant"], best["block"])
-    assert best["block"] in autotune.BLOCKS and best["variant"] in autotune.VARIANTS
+    shapes = {(v, b, 0) for v in (1, 2) for b in autotune.BLOCKS} | {(3, b, t) for b, t in autotune.TILE_SHAPES}
+    assert (best["variant"], best["block"], best["tile_cells"]) in shapes
     again = autotune.tune(meshes[0], bcs[0], abi.ParamsC.from_buffer_copy(params), ics[0])
     assert again == best  # served from the cache
     g = FVKSolver(meshes[0], bcs[0], p, ics[0])
