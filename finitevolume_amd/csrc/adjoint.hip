@@ -77,37 +77,24 @@ __device__ __forceinline__ double4 d4(const State& s) { return double4{s.r, s.ru
 // ---------------------------------------------------------------------------
 // face pass
 // ---------------------------------------------------------------------------
+// The adjoint of one face: its two states (R unused on a boundary face,
+// cr < 0 the boundary code), their limited gradients, the face geometry and
+// the lambdas of its cells.  Writes both sides' state and MUSCL-increment
+// adjoints and the edge-vector adjoint.
 template <int SCHEME, int ORDER>
-__global__ __launch_bounds__(kAdjBlock) void k_adj_face(AdjArgs a) {
+__device__ __forceinline__ void adj_face_body(const AdjArgs& a, int f, int cr, const State& L, const State& R,
+                                              const double4& gl0, const double4& gl1, const double4& gr0,
+                                              const double4& gr1, const double4& ro, const double4& fg,
+                                              const double4& ll, const double4& lr) {
     const MeshView& m = a.m;
-    const int f = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (f >= m.nf) return;
     const double g = a.gamma;
-    const int cl = m.fl[f], cr = m.fr[f];
-    const double4 fg = m.fgeo[f];
     const double nx = fg.x, ny = fg.y, A = fg.z;
-    const State L = st4(a.U[cl]);
-    double4 gl0{0, 0, 0, 0}, gl1{0, 0, 0, 0}, gr0{0, 0, 0, 0}, gr1{0, 0, 0, 0}, ro{0, 0, 0, 0};
-    if (ORDER == 2) {
-        ro = m.frec[f];
-        gl0 = a.G[2 * cl];
-        gl1 = a.G[2 * cl + 1];
-    }
     const State Lf = ORDER == 2 ? reconstruct(L, gl0, gl1, ro.x, ro.y, g) : L;
-    State R{0, 0, 0, 0}, Rf;
+    State Rf;
     int type = -1;
-    const double4 ll = a.lam[cl];
     State Fb;
     if (cr >= 0) {
-        R = st4(a.U[cr]);
-        if (ORDER == 2) {
-            gr0 = a.G[2 * cr];
-            gr1 = a.G[2 * cr + 1];
-            Rf = reconstruct(R, gr0, gr1, ro.z, ro.w, g);
-        } else {
-            Rf = R;
-        }
-        const double4 lr = a.lam[cr];
+        Rf = ORDER == 2 ? reconstruct(R, gr0, gr1, ro.z, ro.w, g) : R;
         Fb = State{lr.x - ll.x, lr.y - ll.y, lr.z - ll.z, lr.w - ll.w};
     } else {
         const DevBC& bc = m.bc[-1 - cr];
@@ -144,34 +131,52 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_face(AdjArgs a) {
     a.fe[f] = double2{exb, eyb};
 }
 
+template <int SCHEME, int ORDER>
+__global__ __launch_bounds__(kAdjBlock) void k_adj_face(AdjArgs a) {
+    const MeshView& m = a.m;
+    const int f = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (f >= m.nf) return;
+    const int cl = m.fl[f], cr = m.fr[f];
+    const double4 fg = m.fgeo[f];
+    const State L = st4(a.U[cl]);
+    double4 gl0{0, 0, 0, 0}, gl1{0, 0, 0, 0}, gr0{0, 0, 0, 0}, gr1{0, 0, 0, 0}, ro{0, 0, 0, 0};
+    if (ORDER == 2) {
+        ro = m.frec[f];
+        gl0 = a.G[2 * cl];
+        gl1 = a.G[2 * cl + 1];
+    }
+    State R{0, 0, 0, 0};
+    const double4 ll = a.lam[cl];
+    double4 lr{0, 0, 0, 0};
+    if (cr >= 0) {
+        R = st4(a.U[cr]);
+        if (ORDER == 2) {
+            gr0 = a.G[2 * cr];
+            gr1 = a.G[2 * cr + 1];
+        }
+        lr = a.lam[cr];
+    }
+    adj_face_body<SCHEME, ORDER>(a, f, cr, L, R, gl0, gl1, gr0, gr1, ro, fg, ll, lr);
+}
+
 // ---------------------------------------------------------------------------
 // cell pass: face sides, then the reverse of the limited LSQ gradient
 // (recomputed exactly as the forward gradient kernel evaluates it)
 // ---------------------------------------------------------------------------
-template <int D, bool WX>
-__global__ __launch_bounds__(kAdjBlock) void k_adj_cell(AdjArgs a) {
+// The cell reverse of cell c (device index): its ELL entries en (.x the
+// face entry; .y the neighbour, whose primitives nbr(en.y, w) returns, or a
+// boundary code), the entries' geometry eg, its state and limited gradients.
+template <int D, bool WX, class Nbr>
+__device__ __forceinline__ void adj_cell_body(const AdjArgs& a, int c, const int2 (&en)[D], const double4 (&eg)[D],
+                                              const State& Uc, const double4& G0, const double4& G1, Nbr nbr) {
     const MeshView& m = a.m;
-    const int c = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (c >= m.nc) return;
     const double g = a.gamma;
     const int64_t ld = m.ld;
-    int2 en[D];
-    double4 eg[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) en[k] = m.ell[k * ld + c];
-#pragma unroll
-    for (int k = 0; k < D; ++k) eg[k] = en[k].x >= 0 ? m.egeo[k * ld + c] : double4{0.0, 0.0, 0.0, 0.0};
-    const State Uc = st4(a.U[c]);
     State Ub{0, 0, 0, 0};
     double Gb[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     double rb[D][2];
 #pragma unroll
     for (int k = 0; k < D; ++k) rb[k][0] = rb[k][1] = 0.0;
-    double4 G0{0, 0, 0, 0}, G1{0, 0, 0, 0};
-    if (a.order == 2) {
-        G0 = a.G[2 * c];
-        G1 = a.G[2 * c + 1];
-    }
     // the face sides of this cell
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -216,11 +221,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_cell(AdjArgs a) {
                 wn[k][2] = 0.5 * (wcq[2] + wg2);
                 wn[k][3] = 0.5 * (wcq[3] + wg3);
             } else {
-                const State Uo = st4(a.U[en[k].y]);
-                wn[k][0] = Uo.r;
-                wn[k][1] = Uo.ru / Uo.r;
-                wn[k][2] = Uo.rv / Uo.r;
-                wn[k][3] = pressure(Uo, g);
+                nbr(en[k].y, wn[k]);
             }
         }
         double bx[4] = {0, 0, 0, 0}, by[4] = {0, 0, 0, 0}, wmin[4], wmax[4];
@@ -393,6 +394,33 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_cell(AdjArgs a) {
         }
         a.cb[c] = double4{Cbx, Cby, Vb, 0.0};
     }
+}
+
+template <int D, bool WX>
+__global__ __launch_bounds__(kAdjBlock) void k_adj_cell(AdjArgs a) {
+    const MeshView& m = a.m;
+    const int c = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (c >= m.nc) return;
+    const int64_t ld = m.ld;
+    int2 en[D];
+    double4 eg[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) en[k] = m.ell[k * ld + c];
+#pragma unroll
+    for (int k = 0; k < D; ++k) eg[k] = en[k].x >= 0 ? m.egeo[k * ld + c] : double4{0.0, 0.0, 0.0, 0.0};
+    double4 G0{0, 0, 0, 0}, G1{0, 0, 0, 0};
+    if (a.order == 2) {
+        G0 = a.G[2 * c];
+        G1 = a.G[2 * c + 1];
+    }
+    const double g = a.gamma;
+    adj_cell_body<D, WX>(a, c, en, eg, st4(a.U[c]), G0, G1, [&](int o, double (&w)[4]) {
+        const State Uo = st4(a.U[o]);
+        w[0] = Uo.r;
+        w[1] = Uo.ru / Uo.r;
+        w[2] = Uo.rv / Uo.r;
+        w[3] = pressure(Uo, g);
+    });
 }
 
 // ---------------------------------------------------------------------------

@@ -28,6 +28,7 @@
 
 #include "physics.h"
 #include "solver.h"
+#include "tile.h"
 
 namespace fvk {
 
@@ -576,95 +577,6 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
     }
 }
 
-// LDS views of one tile (structure of arrays: consecutive local cells or
-// faces are consecutive 8-byte words, so a wavefront's own-cell accesses
-// do not conflict on the banks)
-struct TileLds {
-    double* U;   // [4][MG] conserved state, owned cells then ring 1
-    double* G;   // [8][MG] limited gradients (second order)
-    double* W;   // [4][MG] primitives (rho, u, v, p) of the owned and ring-1 cells (second order)
-    double* W2;  // [4][M2] primitives of ring 2 (dead after the gradients)
-    double* F;   // [4][MF] face flux x length (in W2's space for second order)
-    double* Dt;  // [MF] dt candidates
-    int MG, M2, MF;
-    int ng;      // this tile's owned + ring-1 cells: local indices below it are in W, the rest in W2
-};
-
-// limited LSQ gradient of local cell j (as k_e_grad), from LDS
-template <int D>
-__device__ __forceinline__ void tile_grad(const EulerArgs& a, const TileLds& L, int j, const int2 (&en)[D],
-                                          const double4 (&eg)[D], const double4& iv) {
-    const MeshView& m = a.m;
-    const double g = a.gamma;
-    const int MG = L.MG;
-    const State Uc{L.U[j], L.U[MG + j], L.U[2 * MG + j], L.U[3 * MG + j]};
-    const double wc[4] = {L.W[j], L.W[MG + j], L.W[2 * MG + j], L.W[3 * MG + j]};
-    double bx[4] = {0.0, 0.0, 0.0, 0.0}, by[4] = {0.0, 0.0, 0.0, 0.0};
-    double wmin[4], wmax[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) wmin[q] = wmax[q] = wc[q];
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        if (en[k].x < 0) continue;
-        double wn[4];
-        if (en[k].y < 0) {
-            const double4 fg = m.fgeo[en[k].x >> 1];
-            const State Gs = euler_ghost(m.bc[-1 - en[k].y], Uc, fg.x, fg.y, g);
-            const double wg0 = Gs.r, wg1 = Gs.ru / Gs.r, wg2 = Gs.rv / Gs.r, wg3 = pressure(Gs, g);
-            wn[0] = 0.5 * (wc[0] + wg0);
-            wn[1] = 0.5 * (wc[1] + wg1);
-            wn[2] = 0.5 * (wc[2] + wg2);
-            wn[3] = 0.5 * (wc[3] + wg3);
-        } else {
-            const int l = en[k].y;
-            const bool r1 = l < L.ng;
-            const double* w = r1 ? L.W + l : L.W2 + (l - L.ng);
-            const int sw = r1 ? MG : L.M2;
-            wn[0] = w[0];
-            wn[1] = w[sw];
-            wn[2] = w[2 * sw];
-            wn[3] = w[3 * sw];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double dphi = wn[q] - wc[q];
-            bx[q] += eg[k].x * dphi;
-            by[q] += eg[k].y * dphi;
-            wmin[q] = smin(wmin[q], wn[q]);
-            wmax[q] = smax(wmax[q], wn[q]);
-        }
-    }
-    double gx[4], gy[4], lim[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        gx[q] = iv.x * bx[q] + iv.y * by[q];
-        gy[q] = iv.y * bx[q] + iv.z * by[q];
-        lim[q] = 1.0;
-    }
-    if (a.limiter == FVK_LIMITER_VENKATAKRISHNAN) {
-        const double eps2 = iv.w;
-        VenkatMin vm[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) vm[q] = VenkatMin{1.0, 1.0};
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            if (en[k].x < 0) continue;
-            const double rx = eg[k].z, ry = eg[k].w;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) venkat_pick(vm[q], wmin[q], wmax[q], wc[q], gx[q] * rx + gy[q] * ry, eps2);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) lim[q] = venkat_value(vm[q]);
-    } else if (a.limiter == FVK_LIMITER_FIRST_ORDER) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) lim[q] = 0.0;
-    }
-    double* gj = L.G + j;
-    gj[0] = lim[0] * gx[0]; gj[MG] = lim[0] * gy[0]; gj[2 * MG] = lim[1] * gx[1]; gj[3 * MG] = lim[1] * gy[1];
-    gj[4 * MG] = lim[2] * gx[2]; gj[5 * MG] = lim[2] * gy[2]; gj[6 * MG] = lim[3] * gx[3];
-    gj[7 * MG] = lim[3] * gy[3];
-}
-
 // flux x length of tile face e (as k_e_face), from LDS; returns its dt candidate
 template <int SCHEME, int ORDER>
 __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L, int e, const int4& fd,
@@ -795,23 +707,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
 
     // 1. states (and primitives) of the tile and its halo; the first two
     // passes load before they store
-    auto stage_cell = [&](int j, const double4& u) {
-        if (j < ng) {
-            L.U[j] = u.x;
-            L.U[MG + j] = u.y;
-            L.U[2 * MG + j] = u.z;
-            L.U[3 * MG + j] = u.w;
-        }
-        if (ORDER == 2) {
-            const State U = st(u);
-            double* w = j < ng ? L.W + j : L.W2 + (j - ng);
-            const int sw = j < ng ? MG : M2;
-            w[0] = U.r;
-            w[sw] = U.ru / U.r;
-            w[2 * sw] = U.rv / U.r;
-            w[3 * sw] = pressure(U, g);
-        }
-    };
+    auto stage_cell = [&](int j, const double4& u) { tile_stage(L, j, u, g, ORDER == 2); };
     {
         const int j0 = tid, j1 = tid + bs;
         const int g0 = j0 < n ? c0 + j0 : (j0 < nl ? m.thalo[hp + j0 - n] : -1);
@@ -827,7 +723,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
 
     // 2. limited LSQ gradients of the owned cells and ring 1
     if (ORDER == 2) {
-        if (tid < ng) tile_grad<D>(a, L, tid, pen, peg, piv);
+        if (tid < ng) tile_grad<D>(m, g, a.limiter, L, tid, pen, peg, piv);
         for (int j = tid + bs; j < ng; j += bs) {
             const int gc = j < n ? c0 + j : m.thalo[hp + j - n];
             int2 en[D];
@@ -836,7 +732,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
             for (int k = 0; k < D; ++k) en[k] = te[k * MG + j];
 #pragma unroll
             for (int k = 0; k < D; ++k) eg[k] = en[k].x >= 0 ? m.egeo[k * m.ld + gc] : double4{0.0, 0.0, 0.0, 0.0};
-            tile_grad<D>(a, L, j, en, eg, m.clsq[gc]);
+            tile_grad<D>(m, g, a.limiter, L, j, en, eg, m.clsq[gc]);
         }
     }
     __syncthreads();  // ring 2's primitives are dead from here: their space takes the fluxes

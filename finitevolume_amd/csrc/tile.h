@@ -1,0 +1,121 @@
+// Cell tiles in LDS, shared by the fused Euler stage and the fused adjoint:
+// a tile is a range of graph-adjacent cells (the owned cells) with two halo
+// rings; its states, primitives and limited gradients are staged in LDS
+// structure-of-arrays, so every neighbour access inside a tile is an LDS read.
+#pragma once
+
+#include "physics.h"
+#include "solver.h"
+
+namespace fvk {
+
+// LDS views of one tile (structure of arrays: consecutive local cells or
+// faces are consecutive 8-byte words, so a wavefront's own-cell accesses
+// do not conflict on the banks)
+struct TileLds {
+    double* U;   // [4][MG] conserved state, owned cells then ring 1
+    double* G;   // [8][MG] limited gradients (second order)
+    double* W;   // [4][MG] primitives (rho, u, v, p) of the owned and ring-1 cells (second order)
+    double* W2;  // [4][M2] primitives of ring 2 (dead after the gradients)
+    double* F;   // [4][MF] face flux x length (in W2's space for second order)
+    double* Dt;  // [MF] dt candidates
+    int MG, M2, MF;
+    int ng;      // this tile's owned + ring-1 cells: local indices below it are in W, the rest in W2
+};
+
+// limited LSQ gradient of local cell j (as the per-cell gradient kernel), from LDS
+template <int D>
+__device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limiter, const TileLds& L, int j,
+                                          const int2 (&en)[D], const double4 (&eg)[D], const double4& iv) {
+    const int MG = L.MG;
+    const State Uc{L.U[j], L.U[MG + j], L.U[2 * MG + j], L.U[3 * MG + j]};
+    const double wc[4] = {L.W[j], L.W[MG + j], L.W[2 * MG + j], L.W[3 * MG + j]};
+    double bx[4] = {0.0, 0.0, 0.0, 0.0}, by[4] = {0.0, 0.0, 0.0, 0.0};
+    double wmin[4], wmax[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wmin[q] = wmax[q] = wc[q];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        if (en[k].x < 0) continue;
+        double wn[4];
+        if (en[k].y < 0) {
+            const double4 fg = m.fgeo[en[k].x >> 1];
+            const DevBC& bc = m.bc[-1 - en[k].y];
+            const State Gs = euler_ghost_state(bc.type, bc.ff, Uc, fg.x, fg.y, g);
+            const double wg0 = Gs.r, wg1 = Gs.ru / Gs.r, wg2 = Gs.rv / Gs.r, wg3 = pressure(Gs, g);
+            wn[0] = 0.5 * (wc[0] + wg0);
+            wn[1] = 0.5 * (wc[1] + wg1);
+            wn[2] = 0.5 * (wc[2] + wg2);
+            wn[3] = 0.5 * (wc[3] + wg3);
+        } else {
+            const int l = en[k].y;
+            const bool r1 = l < L.ng;
+            const double* w = r1 ? L.W + l : L.W2 + (l - L.ng);
+            const int sw = r1 ? MG : L.M2;
+            wn[0] = w[0];
+            wn[1] = w[sw];
+            wn[2] = w[2 * sw];
+            wn[3] = w[3 * sw];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double dphi = wn[q] - wc[q];
+            bx[q] += eg[k].x * dphi;
+            by[q] += eg[k].y * dphi;
+            wmin[q] = smin(wmin[q], wn[q]);
+            wmax[q] = smax(wmax[q], wn[q]);
+        }
+    }
+    double gx[4], gy[4], lim[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        gx[q] = iv.x * bx[q] + iv.y * by[q];
+        gy[q] = iv.y * bx[q] + iv.z * by[q];
+        lim[q] = 1.0;
+    }
+    if (limiter == FVK_LIMITER_VENKATAKRISHNAN) {
+        const double eps2 = iv.w;
+        VenkatMin vm[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vm[q] = VenkatMin{1.0, 1.0};
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            if (en[k].x < 0) continue;
+            const double rx = eg[k].z, ry = eg[k].w;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) venkat_pick(vm[q], wmin[q], wmax[q], wc[q], gx[q] * rx + gy[q] * ry, eps2);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lim[q] = venkat_value(vm[q]);
+    } else if (limiter == FVK_LIMITER_FIRST_ORDER) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) lim[q] = 0.0;
+    }
+    double* gj = L.G + j;
+    gj[0] = lim[0] * gx[0]; gj[MG] = lim[0] * gy[0]; gj[2 * MG] = lim[1] * gx[1]; gj[3 * MG] = lim[1] * gy[1];
+    gj[4 * MG] = lim[2] * gx[2]; gj[5 * MG] = lim[2] * gy[2]; gj[6 * MG] = lim[3] * gx[3];
+    gj[7 * MG] = lim[3] * gy[3];
+}
+
+// stage local cell j: conserved state (owned and ring-1 cells) and, for
+// second order, its primitives (W for owned and ring 1, W2 for ring 2)
+__device__ __forceinline__ void tile_stage(const TileLds& L, int j, const double4& u, double g, bool second_order) {
+    const int MG = L.MG, ng = L.ng;
+    if (j < ng) {
+        L.U[j] = u.x;
+        L.U[MG + j] = u.y;
+        L.U[2 * MG + j] = u.z;
+        L.U[3 * MG + j] = u.w;
+    }
+    if (second_order) {
+        const State U{u.x, u.y, u.z, u.w};
+        double* w = j < ng ? L.W + j : L.W2 + (j - ng);
+        const int sw = j < ng ? MG : L.M2;
+        w[0] = U.r;
+        w[sw] = U.ru / U.r;
+        w[2 * sw] = U.rv / U.r;
+        w[3 * sw] = pressure(U, g);
+    }
+}
+
+}  // namespace fvk

@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 H = 1e-7
 
 
-def naca_case(flux=abi.FLUX_HLLC, order=2):
+def naca_case(flux=abi.FLUX_HLLC, order=2, **launch):
     """Config-2 numerics on a small NACA 0012 O-grid (40 x 20 triangles pairs,
     farfield at 6 chords, growth 1.2: cells of moderate aspect ratio)."""
     m = M.naca_ogrid(40, 20, r_far=6.0, ratio=1.2)
@@ -28,12 +28,12 @@ def naca_case(flux=abi.FLUX_HLLC, order=2):
     ff = M.from_primitive(rho, u, v, pinf, cases.GAMMA)
     bcs = [abi.make_bc(abi.BC_WALL), abi.make_bc(abi.BC_FARFIELD, ff)]
     ext = dict(order=2, limiter=abi.LIMITER_VENKATAKRISHNAN, rk_stages=3, local_dt=1) if order == 2 else {}
-    p = abi.default_params(equation=abi.EQ_EULER, flux_scheme=flux, cfl=0.8, **ext)
+    p = abi.default_params(equation=abi.EQ_EULER, flux_scheme=flux, cfl=0.8, **ext, **launch)
     ic = abi.initial_condition(abi.IC_FREESTREAM, rho=rho, u=u, v=v, p=pinf)
     return m, bcs, p, ic, None
 
 
-def wavy_outflow_case(order=2):
+def wavy_outflow_case(order=2, **launch):
     """Skewed quads, slip wall below, farfield left, outflow top and right,
     with a smooth non-uniform state (every ghost type, four-face cells)."""
     m = M.skewed(12, 0.15)
@@ -41,7 +41,7 @@ def wavy_outflow_case(order=2):
     bcs = [abi.make_bc(abi.BC_WALL), abi.make_bc(abi.BC_FARFIELD, ff), abi.make_bc(abi.BC_OUTFLOW),
            abi.make_bc(abi.BC_OUTFLOW)]
     ext = dict(order=2, limiter=abi.LIMITER_VENKATAKRISHNAN) if order == 2 else {}
-    p = abi.default_params(equation=abi.EQ_EULER, flux_scheme=abi.FLUX_HLLC, cfl=0.5, **ext)
+    p = abi.default_params(equation=abi.EQ_EULER, flux_scheme=abi.FLUX_HLLC, cfl=0.5, **ext, **launch)
     ic = abi.initial_condition(abi.IC_FREESTREAM, rho=1.0, u=0.4, v=0.1, p=1.0)
     x, y = m.cell_xy[:, 0], m.cell_xy[:, 1]
     U0 = M.from_primitive(1.0 + 0.2 * np.sin(3 * x) * np.cos(2 * y), 0.4 + 0.1 * np.cos(4 * y), 0.1 * np.sin(5 * x),
@@ -76,6 +76,8 @@ CASES = {
     "naca_rusanov_o2": lambda: naca_case(abi.FLUX_RUSANOV, 2),
     "naca_hllc_o1": lambda: naca_case(abi.FLUX_HLLC, 1),
     "wavy_outflow_o2": lambda: wavy_outflow_case(2),
+    # a solver with cell tiles (the adjoint itself runs the same passes)
+    "naca_hllc_o2_tiles": lambda: naca_case(abi.FLUX_HLLC, 2, kernel_variant=3, block_size=128),
 }
 
 
