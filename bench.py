@@ -579,11 +579,18 @@ def main():
     # reference mode 2R on the GPU: first-order HLLC, forward Euler, global dt
     if not args.no_2r:
         m2, bcs2, p2, ic2 = naca_case(args.ni, args.nj, order2=False)
+        tuned2 = None
+        if not args.no_tune and not (args.variant or args.block):
+            from finitevolume_amd import autotune
+            tuned2 = autotune.tune(m2, bcs2, p2, ic2, steps=50, use_cache=False)
         s2 = FVKSolver(m2, bcs2, p2, ic2)
         w2 = time_gpu(s2, args.steps, args.warmup, per_step=True)
         out["gpu_2R"] = {"value": round(nc * args.steps / w2 / 1e9, 4), "unit": "Gcell-updates/s",
                          "ms_per_step": round(w2 / args.steps * 1e3, 4),
-                         "roofline_frac": round(b_alg_2r(fpc) * nc * args.steps / w2 / 1e9 / HBM_PEAK_GBS, 4)}
+                         "roofline_frac": round(b_alg_2r(fpc) * nc * args.steps / w2 / 1e9 / HBM_PEAK_GBS, 4),
+                         "launch_shape": s2.launch_shape()}
+        if tuned2:
+            out["gpu_2R"]["autotune_ms_per_step"] = tuned2["table_ms_per_step"]
         del s2
 
     if not args.no_cpu:

@@ -48,6 +48,21 @@ void* dev_alloc(fvk_solver* s, size_t bytes) {
     return p;
 }
 
+// pinned host memory mapped into the device's address space; *dev receives
+// the device view (nullptr on failure)
+void* host_mapped_alloc(fvk_solver* s, size_t bytes, void** dev) {
+    void* h = nullptr;
+    *dev = nullptr;
+    if (hipHostMalloc(&h, bytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) return nullptr;
+    s->host_allocations.push_back(h);
+    std::memset(h, 0, bytes);
+    if (hipHostGetDevicePointer(dev, h, 0) != hipSuccess) {
+        *dev = nullptr;
+        return nullptr;
+    }
+    return h;
+}
+
 void timer_begin(fvk_solver* s, const char* name, hipEvent_t* ev_stop) {
     *ev_stop = nullptr;
     if (!s->timing) return;
@@ -139,6 +154,7 @@ void fvk_destroy(fvk_solver* s) {
     timers_collect(s);
     fvk::adjoint_release(s);
     for (void* p : s->allocations) hipFree(p);
+    for (void* p : s->host_allocations) hipHostFree(p);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
 }
@@ -294,8 +310,10 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
     s->partials = static_cast<double*>(dev_alloc(s, sizeof(double) * 6 * max_blocks));
     s->dtmin = static_cast<unsigned long long*>(dev_alloc(s, 256));
     s->hist_cap = 1024;
-    s->res_hist = static_cast<double*>(dev_alloc(s, sizeof(double) * 6 * s->hist_cap));
-    s->dt_hist = static_cast<double*>(dev_alloc(s, sizeof(double) * s->hist_cap));
+    s->res_hist_host = static_cast<double*>(host_mapped_alloc(s, sizeof(double) * 6 * s->hist_cap,
+                                                              reinterpret_cast<void**>(&s->res_hist)));
+    s->dt_hist_host = static_cast<double*>(host_mapped_alloc(s, sizeof(double) * s->hist_cap,
+                                                             reinterpret_cast<void**>(&s->dt_hist)));
     double* wd = static_cast<double*>(dev_alloc(s, sizeof(double) * s->ld));
     if (!s->U[0] || ((!euler || s->variant == 1) && (!s->flux || !s->dtf)) || !s->dtc || (ng && !s->grad) || !s->partials || !s->dtmin ||
         !s->res_hist || !s->dt_hist || !wd || (nbuf == 3 && (!s->U[1] || !s->U[2]))) {
@@ -458,7 +476,8 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
     if (nsteps < 0) return set_error(FVK_ERR_INVALID, "nsteps < 0");
     const int nv = s->nv;
     const int stages = s->p.equation == FVK_EQ_DIFFUSION ? 1 : s->p.rk_stages;
-    std::vector<double> hr((size_t)s->hist_cap * nv), hd((size_t)s->hist_cap);
+    const double* hr = s->res_hist_host;
+    const double* hd = s->dt_hist_host;
     int done = 0;
     while (done < nsteps) {
         int chunk = std::min(s->hist_cap, nsteps - done);
@@ -478,21 +497,8 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
             s->step_count++;
             (void)i;
         }
-        // copy back only the rows this chunk wrote (a ring of hist_cap rows)
-        hipError_t e = hipSuccess;
-        {
-            const int64_t first = s->p.equation == FVK_EQ_DIFFUSION ? 0 : (s->step_count - chunk) % s->hist_cap;
-            const int64_t n1 = std::min<int64_t>(chunk, s->hist_cap - first), n2 = chunk - n1;
-            e = hipMemcpyAsync(&hr[first * nv], s->res_hist + first * nv, sizeof(double) * nv * n1,
-                               hipMemcpyDeviceToHost, s->stream);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(&hd[first], s->dt_hist + first, sizeof(double) * n1, hipMemcpyDeviceToHost, s->stream);
-            if (e == hipSuccess && n2 > 0)
-                e = hipMemcpyAsync(hr.data(), s->res_hist, sizeof(double) * nv * n2, hipMemcpyDeviceToHost, s->stream);
-            if (e == hipSuccess && n2 > 0)
-                e = hipMemcpyAsync(hd.data(), s->dt_hist, sizeof(double) * n2, hipMemcpyDeviceToHost, s->stream);
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        // the finalize kernels wrote the rows into mapped host memory: one sync per chunk
+        hipError_t e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) return hip_check(e, "step");
         if (s->timing) timers_collect(s);
         // rows were written at index (global step % hist_cap)

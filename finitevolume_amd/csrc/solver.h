@@ -154,8 +154,13 @@ struct fvk_solver {
     double* aux = nullptr;                       // [na][ld] per-cell auxiliaries (sources, nu_t, F1 ...)
     double* partials = nullptr;                  // [nv][max_blocks]
     unsigned long long* dtmin = nullptr;         // [2] global dt (bits) per step parity
-    double* res_hist = nullptr;                  // [hist_cap][nv]
-    double* dt_hist = nullptr;                   // [hist_cap]
+    double* res_hist = nullptr;                  // [hist_cap][nv] device view of res_hist_host
+    double* dt_hist = nullptr;                   // [hist_cap] device view of dt_hist_host
+    // the step history lives in pinned, mapped host memory: the finalize
+    // kernels write it directly and the host reads it after the stream sync
+    double* res_hist_host = nullptr;
+    double* dt_hist_host = nullptr;
+    std::vector<void*> host_allocations;         // hipHostMalloc'd, freed by fvk_destroy
     int hist_cap = 0;
     int64_t step_count = 0;
     double last_res[6] = {0, 0, 0, 0, 0, 0};
