@@ -497,6 +497,10 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
             s->step_count++;
             (void)i;
         }
+        if (s->p.equation == FVK_EQ_EULER) {
+            int rc = flush_euler_finalize(s);
+            if (rc != FVK_OK) return rc;
+        }
         // the finalize kernels wrote the rows into mapped host memory: one sync per chunk
         hipError_t e = hipStreamSynchronize(s->stream);
         if (e != hipSuccess) return hip_check(e, "step");

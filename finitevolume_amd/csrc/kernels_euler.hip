@@ -65,10 +65,12 @@ struct EulerArgs {
     double* case_partials;
     int n_cases;
     int stages;
-    // finalize of the previous stage's norms, run by one spare workgroup of
-    // the tile kernel (fin_nblocks == 0: none)
-    int fin_nblocks, fin_slot;
+    // finalize of the previous stage's (or step's) norms, run by one spare
+    // workgroup of the tile or face kernel (fin_nblocks == 0: none); with
+    // fin_rearm it also re-arms the next step's dt slot
+    int fin_nblocks, fin_slot, fin_rearm;
     int64_t fin_hist_idx;
+    int rearm_slot;       // >= 0: the gather re-arms this dt slot (forward Euler with a folded finalize)
 };
 
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
@@ -206,7 +208,7 @@ __global__ void k_step_tick(unsigned int* step) { *step += 1u; }
 // step's dt slot re-armed; sm holds blockDim.x doubles
 __device__ __forceinline__ void finalize_norms(const double* partials, int nblocks, int nv, double* res_hist,
                                                double* dt_hist, int64_t hist_idx, unsigned long long* dtmin,
-                                               int slot, double fixed_dt, double* sm) {
+                                               int slot, double fixed_dt, double* sm, bool rearm = true) {
     const int nt = blockDim.x, tid = threadIdx.x;
     int p2 = 64;  // the largest power of two not above the workgroup size
     while (2 * p2 <= nt) p2 *= 2;
@@ -237,7 +239,7 @@ __device__ __forceinline__ void finalize_norms(const double* partials, int nbloc
             dt_hist[hist_idx] = fixed_dt;
         } else {
             dt_hist[hist_idx] = __longlong_as_double((long long)dtmin[slot]);
-            dtmin[slot ^ 1] = dbits(DBL_MAX);
+            if (rearm) dtmin[slot ^ 1] = dbits(DBL_MAX);
         }
     }
 }
@@ -246,9 +248,9 @@ constexpr int kFinBlock = 1024;
 __global__ __launch_bounds__(kFinBlock) void k_e_finalize(const double* partials, int nblocks, int nv,
                                                           double* res_hist, double* dt_hist, int64_t hist_idx,
                                                           unsigned long long* dtmin, int slot,
-                                                          double fixed_dt = -1.0) {
+                                                          double fixed_dt = -1.0, int rearm = 1) {
     __shared__ double sm[kFinBlock];
-    finalize_norms(partials, nblocks, nv, res_hist, dt_hist, hist_idx, dtmin, slot, fixed_dt, sm);
+    finalize_norms(partials, nblocks, nv, res_hist, dt_hist, hist_idx, dtmin, slot, fixed_dt, sm, rearm != 0);
 }
 
 // stage update of one cell: U' = U + dt/V R (src/EulerFVMSolver.cpp:103) on
@@ -488,6 +490,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_cell(EulerArgs a) {
 template <int SCHEME, int ORDER>
 __global__ __launch_bounds__(kMaxBlock) void k_e_face(EulerArgs a) {
     const MeshView& m = a.m;
+    if (a.fin_nblocks > 0 && logical_block() == (int)gridDim.x - 1) {  // the spare workgroup (no faces)
+        __shared__ double fsm[kMaxBlock];
+        finalize_norms(a.partials, a.fin_nblocks, 4, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin, a.fin_slot,
+                       -1.0, fsm, a.fin_rearm != 0);
+        return;
+    }
     const int f = xcd_index();
     const double g = a.gamma;
     double cand = DBL_MAX;
@@ -528,6 +536,7 @@ template <int D>
 __global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
     const MeshView& m = a.m;
     const int c = xcd_index();
+    if (a.rearm_slot >= 0 && c == 0) a.dtmin[a.rearm_slot] = dbits(DBL_MAX);  // unused until the next step
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
     int side = 0;
     bool act = true;
@@ -643,7 +652,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     const int t = logical_block();
     if (a.fin_nblocks > 0 && t == m.ntiles) {  // the spare workgroup: the previous stage's norms
         finalize_norms(a.partials, a.fin_nblocks, 4, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin, a.fin_slot,
-                       -1.0, lds);
+                       -1.0, lds, a.fin_rearm != 0);
         return;
     }
     const double g = a.gamma;
@@ -831,6 +840,7 @@ constexpr int kDtBlock = 256;  // global-dt pass beside the tile kernels
 
 EulerArgs make_args(fvk_solver* s) {
     EulerArgs a{};
+    a.rearm_slot = -1;
     a.m = s->mesh;
     a.G = reinterpret_cast<double4*>(s->grad);
     a.flux = reinterpret_cast<double4*>(s->flux);
@@ -894,7 +904,8 @@ struct Launch {
 
 template <int ORDER>
 void face_o(fvk_solver* s, const EulerArgs& a) {
-    int g = grid_for(s->nf, s->block);
+    // a pending finalize takes the last logical workgroup of one more XCD round
+    int g = grid_for(s->nf, s->block) + (a.fin_nblocks > 0 ? kXcds : 0);
     switch (s->p.flux_scheme) {
         case FLUX_HLLC: k_e_face<FLUX_HLLC, ORDER><<<g, s->block, 0, s->stream>>>(a); break;
         case FLUX_EXACT: k_e_face<FLUX_EXACT, ORDER><<<g, s->block, 0, s->stream>>>(a); break;
@@ -922,6 +933,7 @@ void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
                 s->fin_nblocks = grid;
                 s->fin_slot = a.slot;
                 s->fin_hist_idx = a.hist_idx;
+                s->fin_rearm = 1;
             } else {
                 timer_begin(s, "euler_finalize", &ev);
                 k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(a.partials, grid, 4, a.res_hist, a.dt_hist,
@@ -951,9 +963,21 @@ void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
         timer_end(s, ev);
         EulerArgs ag = a;
         ag.face_dt = 0;
+        ag.fin_nblocks = 0;
+        // the norms of this stage go to the next face launch (next stage, or
+        // for forward Euler the next step, whose dt slot the gather re-arms)
+        const bool fold = a.want_res && s->n_cases == 0 && !a.R;
+        if (fold && a.stages == 1) ag.rearm_slot = a.slot ^ 1;
         timer_begin(s, "euler_gather", &ev);
         Launch<D>::gather(s, ag);
         timer_end(s, ev);
+        if (fold) {
+            s->fin_nblocks = grid_for(s->nc, s->block);
+            s->fin_slot = a.slot;
+            s->fin_hist_idx = a.hist_idx;
+            s->fin_rearm = a.stages >= 2;
+            return;
+        }
     } else {
         timer_begin(s, "euler_cell", &ev);
         Launch<D>::cell(s, a);
@@ -1006,11 +1030,11 @@ int launch_euler_stage(fvk_solver* s, int stage, int stages, int /*unused*/, int
     a.want_res = want_res && stage == 0;
     a.slot = (int)(step & 1);
     a.hist_idx = step % s->hist_cap;
-    if (stage == 0) s->fin_nblocks = 0;  // nothing left from an interrupted step
-    if (s->variant == 3 && stage > 0 && s->fin_nblocks > 0) {
+    if (s->fin_nblocks > 0) {  // the previous stage's (or step's) norms, folded into this launch
         a.fin_nblocks = s->fin_nblocks;
         a.fin_slot = s->fin_slot;
         a.fin_hist_idx = s->fin_hist_idx;
+        a.fin_rearm = s->fin_rearm;
         s->fin_nblocks = 0;
     }
     bool dt_pass = false;
@@ -1043,6 +1067,15 @@ int launch_euler_residual_eval(fvk_solver* s, double* R_dev) {
 int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, int slot, double fixed_dt) {
     k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(s->partials, nblocks, nv, s->res_hist, s->dt_hist, hist_idx,
                                                  s->dtmin, slot, fixed_dt);
+    return hip_check(hipGetLastError(), "finalize launch");
+}
+
+// a finalize still pending after the last stage launch of a call
+int flush_euler_finalize(fvk_solver* s) {
+    if (s->fin_nblocks <= 0) return FVK_OK;
+    k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(s->partials, s->fin_nblocks, 4, s->res_hist, s->dt_hist,
+                                                 s->fin_hist_idx, s->dtmin, s->fin_slot, -1.0, s->fin_rearm);
+    s->fin_nblocks = 0;
     return hip_check(hipGetLastError(), "finalize launch");
 }
 

@@ -122,9 +122,11 @@ struct fvk_solver {
     int tile_cells = 256;       // variant 3: most cells of one tile (at most block)
     std::vector<int> tile_start; // variant 3: [ntiles + 1] device cell ranges of the tiles
     size_t tile_lds = 0;         // variant 3: dynamic LDS bytes per tile
-    // variant 3, SSP-RK: the first stage's norm partials, reduced by the next
-    // stage's tile kernel (fin_nblocks == 0: nothing pending)
-    int fin_nblocks = 0, fin_slot = 0;
+    // a stage's norm partials waiting for the next stage's (or, forward
+    // Euler, the next step's) tile or face launch to reduce them with a
+    // spare workgroup (fin_nblocks == 0: nothing pending; fvk_step flushes
+    // what is left); fin_rearm: that finalize re-arms the next dt slot
+    int fin_nblocks = 0, fin_slot = 0, fin_rearm = 1;
     int64_t fin_hist_idx = 0;
     int64_t nc = 0, nf = 0, nn = 0;
     int nfi = 0, nnz = 0;
@@ -212,6 +214,7 @@ int launch_wall_distance(fvk_solver* s, const int* wall_faces, int nw, const dou
 int launch_residual_eval(fvk_solver* s, double* R_dev);
 int launch_euler_gradients(fvk_solver* s);
 int launch_case_finalize(fvk_solver* s);
+int flush_euler_finalize(fvk_solver* s);
 int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, int slot, double fixed_dt);
 int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
 void adjoint_release(fvk_solver* s);
