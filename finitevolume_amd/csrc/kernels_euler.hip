@@ -103,15 +103,15 @@ __device__ __forceinline__ State euler_ghost(const DevBC& bc, const State& L, do
 // the face's orientation; R == nullptr on a boundary face.
 __device__ __forceinline__ double dt_candidate(const State& L, const State* R, double nx, double ny, double len,
                                                double g, double cfl) {
-    double uL = L.ru / L.r, vL = L.rv / L.r;
+    double uL = fdiv(L.ru, L.r), vL = fdiv(L.rv, L.r);
     double VnL = uL * nx + vL * ny;
     double Smax = fabs(VnL) + sound_speed(L, g);
     if (R) {
-        double uR = R->ru / R->r, vR = R->rv / R->r;
+        double uR = fdiv(R->ru, R->r), vR = fdiv(R->rv, R->r);
         double VnR = uR * nx + vR * ny;
         Smax = smax(Smax, fabs(VnR) + sound_speed(*R, g));
     }
-    return cfl * len / Smax;
+    return fdiv(cfl * len, Smax);
 }
 
 // block minimum, then one atomic per block.  Positive doubles order like
@@ -279,7 +279,7 @@ __device__ __forceinline__ void update_cell_with(const EulerArgs& a, int c, cons
     }
     double dt = x.dt;
     if (a.dt_src == 2) { dt = dmin; a.dtc[c] = dmin; }
-    const double h = dt / x.vol;
+    const double h = fdiv(dt, x.vol);
     if (a.first) {
         a.Unew[c] = double4{u.x + h * r0, u.y + h * r1, u.z + h * r2, u.w + h * r3};
     } else {
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_grad(EulerArgs a) {
     for (int k = 0; k < D; ++k)
         if (en[k].x >= 0 && en[k].y >= 0) un[k] = a.U[en[k].y];
     const State Uc = st(a.U[c]);
-    const double wc[4] = {Uc.r, Uc.ru / Uc.r, Uc.rv / Uc.r, pressure(Uc, g)};
+    const double wc[4] = {Uc.r, fdiv(Uc.ru, Uc.r), fdiv(Uc.rv, Uc.r), pressure(Uc, g)};
     double bx[4] = {0.0, 0.0, 0.0, 0.0}, by[4] = {0.0, 0.0, 0.0, 0.0};
     double wmin[4], wmax[4];
 #pragma unroll
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_grad(EulerArgs a) {
         if (en[k].y < 0) {
             const double4 fg = m.fgeo[en[k].x >> 1];
             const State Gs = euler_ghost(m.bc[-1 - en[k].y], Uc, fg.x, fg.y, g);
-            const double wg0 = Gs.r, wg1 = Gs.ru / Gs.r, wg2 = Gs.rv / Gs.r, wg3 = pressure(Gs, g);
+            const double wg0 = Gs.r, wg1 = fdiv(Gs.ru, Gs.r), wg2 = fdiv(Gs.rv, Gs.r), wg3 = pressure(Gs, g);
             wn[0] = 0.5 * (wc[0] + wg0);
             wn[1] = 0.5 * (wc[1] + wg1);
             wn[2] = 0.5 * (wc[2] + wg2);
@@ -366,8 +366,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_grad(EulerArgs a) {
         } else {
             const State Uo = st(un[k]);
             wn[0] = Uo.r;
-            wn[1] = Uo.ru / Uo.r;
-            wn[2] = Uo.rv / Uo.r;
+            wn[1] = fdiv(Uo.ru, Uo.r);
+            wn[2] = fdiv(Uo.rv, Uo.r);
             wn[3] = pressure(Uo, g);
         }
 #pragma unroll

@@ -42,7 +42,7 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
             const double4 fg = m.fgeo[en[k].x >> 1];
             const DevBC& bc = m.bc[-1 - en[k].y];
             const State Gs = euler_ghost_state(bc.type, bc.ff, Uc, fg.x, fg.y, g);
-            const double wg0 = Gs.r, wg1 = Gs.ru / Gs.r, wg2 = Gs.rv / Gs.r, wg3 = pressure(Gs, g);
+            const double wg0 = Gs.r, wg1 = fdiv(Gs.ru, Gs.r), wg2 = fdiv(Gs.rv, Gs.r), wg3 = pressure(Gs, g);
             wn[0] = 0.5 * (wc[0] + wg0);
             wn[1] = 0.5 * (wc[1] + wg1);
             wn[2] = 0.5 * (wc[2] + wg2);
@@ -112,8 +112,8 @@ __device__ __forceinline__ void tile_stage(const TileLds& L, int j, const double
         double* w = j < ng ? L.W + j : L.W2 + (j - ng);
         const int sw = j < ng ? MG : L.M2;
         w[0] = U.r;
-        w[sw] = U.ru / U.r;
-        w[2 * sw] = U.rv / U.r;
+        w[sw] = fdiv(U.ru, U.r);
+        w[2 * sw] = fdiv(U.rv, U.r);
         w[3 * sw] = pressure(U, g);
     }
 }
