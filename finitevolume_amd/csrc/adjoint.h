@@ -21,12 +21,12 @@ struct Prim {
     double r, u, v, p;
 };
 
-FVK_HD Prim to_prim(const State& s, double g) { return Prim{s.r, s.ru / s.r, s.rv / s.r, pressure(s, g)}; }
+FVK_HD Prim to_prim(const State& s, double g) { return Prim{s.r, fdiv(s.ru, s.r), fdiv(s.rv, s.r), pressure(s, g)}; }
 
 // adjoint of to_prim: sb += (dW/dU)^T wb
 // (reverse-only arithmetic: one reciprocal of rho instead of five divisions)
 FVK_HD void prim_rev(const State& s, const Prim& wb, State& sb, double g) {
-    const double ir = 1.0 / s.r;
+    const double ir = fdiv(1.0, s.r);
     const double u = s.ru * ir, v = s.rv * ir;
     const double gm = g - 1.0;
     // p = (g - 1) (E - 0.5 r (u^2 + v^2))
@@ -52,7 +52,7 @@ FVK_HD void from_prim_rev(const Prim& w, const State& ob, Prim& wb, double g) {
 // adjoint of phys_flux(s, nx, ny)
 FVK_HD void phys_flux_rev(const State& s, double nx, double ny, double g, const State& Fb, State& sb, double& nxb,
                           double& nyb) {
-    const double u = s.ru / s.r, v = s.rv / s.r;
+    const double u = fdiv(s.ru, s.r), v = fdiv(s.rv, s.r);
     const double Vn = u * nx + v * ny;
     const double p = pressure(s, g);
     const double Vnb = Fb.r * s.r + Fb.ru * s.ru + Fb.rv * s.rv + Fb.E * (s.E + p);
@@ -94,7 +94,7 @@ FVK_HD void rusanov_rev(const State& L, const State& R, double nx, double ny, do
     State& Kb = takeR ? Rb : Lb;
     const double VnK = takeR ? VnR : VnL, cK = takeR ? cR : cL;
     const double Vnb = VnK >= 0.0 ? Sb : -Sb;
-    const double iK = 1.0 / K.r;
+    const double iK = fdiv(1.0, K.r);
     Kb.ru += Vnb * nx * iK;
     Kb.rv += Vnb * ny * iK;
     Kb.r -= Vnb * VnK * iK;
@@ -110,7 +110,7 @@ FVK_HD void hllc_rev(const State& L, const State& R, double nx, double ny, doubl
                      State& Rb, double& nxb, double& nyb) {
     const double tx = -ny, ty = nx;
     const double rL = L.r, rR = R.r;
-    const double uL = L.ru / rL, vL = L.rv / rL, uR = R.ru / rR, vR = R.rv / rR;
+    const double uL = fdiv(L.ru, rL), vL = fdiv(L.rv, rL), uR = fdiv(R.ru, rR), vR = fdiv(R.rv, rR);
     const double VnL = uL * nx + vL * ny, VnR = uR * nx + vR * ny;
     const double VtL = uL * tx + vL * ty, VtR = uR * tx + vR * ty;
     const double pL = pressure(L, g), pR = pressure(R, g);
@@ -125,17 +125,17 @@ FVK_HD void hllc_rev(const State& L, const State& R, double nx, double ny, doubl
     if (SR <= 0.0) { phys_flux_rev(R, nx, ny, g, Ob, Rb, nxb, nyb); return; }
     const double num = pR - pL + rL * VnL * (SL - VnL) - rR * VnR * (SR - VnR);
     const double den = rL * (SL - VnL) - rR * (SR - VnR);
-    const double Ss = num / den;
+    const double Ss = fdiv(num, den);
     const bool left = Ss >= 0.0;
     const double SK = left ? SL : SR, rK = left ? rL : rR, VnK = left ? VnL : VnR, VtK = left ? VtL : VtR;
     const double pK = left ? pL : pR;
     const State& UK = left ? L : R;
     const double D = rK * (SK - VnK);
     const double E2 = SK - Ss;
-    const double factor = D / E2;
+    const double factor = fdiv(D, E2);
     const double mn = factor * Ss, mt = factor * VtK;
-    const double T = Ss + pK / D;
-    const double Q = UK.E / rK + (Ss - VnK) * T;
+    const double T = Ss + fdiv(pK, D);
+    const double Q = fdiv(UK.E, rK) + (Ss - VnK) * T;
     const double Es = factor * Q;
 
     // out = F(UK) + SK (U* - UK)
@@ -153,14 +153,14 @@ FVK_HD void hllc_rev(const State& L, const State& R, double nx, double ny, doubl
     // Es = factor Q, Q = E_K / r_K + (Ss - Vn_K) T, T = Ss + p_K / D
     factorb += Esb * Q;
     const double Qb = Esb * factor;
-    const double irK = 1.0 / rK;
+    const double irK = fdiv(1.0, rK);
     UKb.E += Qb * irK;
     double rKb = -Qb * UK.E * irK * irK;
     double Ssb = Qb * T;
     double VnKb = -Qb * T;
     const double Tb = Qb * (Ss - VnK);
     Ssb += Tb;
-    const double iD = 1.0 / D;
+    const double iD = fdiv(1.0, D);
     const double pKb = Tb * iD;
     double Db = -Tb * pK * iD * iD;
     // mn = factor Ss, mt = factor Vt_K
@@ -169,7 +169,7 @@ FVK_HD void hllc_rev(const State& L, const State& R, double nx, double ny, doubl
     factorb += mtb * VtK;
     const double VtKb = mtb * factor;
     // factor = D / (SK - Ss)
-    const double iE2 = 1.0 / E2;
+    const double iE2 = fdiv(1.0, E2);
     Db += factorb * iE2;
     const double E2b = -factorb * factor * iE2;
     SKb += E2b;
@@ -179,7 +179,7 @@ FVK_HD void hllc_rev(const State& L, const State& R, double nx, double ny, doubl
     SKb += Db * rK;
     VnKb -= Db * rK;
     // Ss = num / den
-    const double iden = 1.0 / den;
+    const double iden = fdiv(1.0, den);
     const double numb = Ssb * iden;
     const double denb = -Ssb * Ss * iden;
     double pLb = -numb, pRb = numb;
@@ -221,7 +221,7 @@ FVK_HD void num_flux_rev(const State& L, const State& R, double nx, double ny, d
 // adjoint of ghost_slip_wall(L, n)
 FVK_HD void ghost_slip_wall_rev(const State& L, double nx, double ny, double g, const State& Gb, State& Lb,
                                 double& nxb, double& nyb) {
-    const double u = L.ru / L.r, v = L.rv / L.r;
+    const double u = fdiv(L.ru, L.r), v = fdiv(L.rv, L.r);
     const double Vn = u * nx + v * ny;
     const Prim wg{L.r, u - 2.0 * Vn * nx, v - 2.0 * Vn * ny, pressure(L, g)};
     Prim wgb{0.0, 0.0, 0.0, 0.0};
@@ -254,7 +254,7 @@ FVK_HD void reconstruct_rev(const State& Uc, const double4& g0, const double4& g
     const double d1 = g0.z * rx + g0.w * ry;
     const double d2 = g1.x * rx + g1.y * ry;
     const double d3 = g1.z * rx + g1.w * ry;
-    const double u = Uc.ru / Uc.r, v = Uc.rv / Uc.r, p = pressure(Uc, gam);
+    const double u = fdiv(Uc.ru, Uc.r), v = fdiv(Uc.rv, Uc.r), p = pressure(Uc, gam);
     const double rho = Uc.r + d0, pp = p + d3;
     if (!(rho > 0.0) || !(pp > 0.0)) {
         Ucb.r += Ob.r; Ucb.ru += Ob.ru; Ucb.rv += Ob.rv; Ucb.E += Ob.E;
@@ -271,7 +271,7 @@ FVK_HD void reconstruct_rev(const State& Uc, const double4& g0, const double4& g
 FVK_HD void venkatakrishnan_grad(double d1, double d2, double eps2, double& p1, double& p2, double& pe) {
     const double N = d1 * d1 + eps2 + 2.0 * d1 * d2;
     const double D = d1 * d1 + 2.0 * d2 * d2 + d1 * d2 + eps2;
-    const double iD2 = 1.0 / (D * D);
+    const double iD2 = fdiv(1.0, D * D);
     p1 = ((2.0 * d1 + 2.0 * d2) * D - N * (2.0 * d1 + d2)) * iD2;
     p2 = (2.0 * d1 * D - N * (4.0 * d2 + d1)) * iD2;
     pe = (D - N) * iD2;
@@ -281,7 +281,7 @@ FVK_HD void venkatakrishnan_grad(double d1, double d2, double eps2, double& p1, 
 // A = |e|, n = (e_y, -e_x) / A.  Adds the adjoint of e.
 FVK_HD void edge_rev(double nx, double ny, double A, double nxb, double nyb, double Ab, double& exb, double& eyb) {
     // e = A (-ny, nx)
-    const double iA = 1.0 / A;
+    const double iA = fdiv(1.0, A);
     const double At = Ab - (nxb * nx + nyb * ny) * iA;
     exb += -nyb * iA - At * ny;
     eyb += nxb * iA + At * nx;

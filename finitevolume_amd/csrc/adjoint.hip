@@ -215,7 +215,7 @@ __device__ __forceinline__ void adj_cell_body(const AdjArgs& a, int c, const int
                 const double4 fg = m.fgeo[en[k].x >> 1];
                 const DevBC& bc = m.bc[-1 - en[k].y];
                 const State Gs = euler_ghost_state(bc.type, bc.ff, Uc, fg.x, fg.y, g);
-                const double wg0 = Gs.r, wg1 = Gs.ru / Gs.r, wg2 = Gs.rv / Gs.r, wg3 = pressure(Gs, g);
+                const double wg0 = Gs.r, wg1 = fdiv(Gs.ru, Gs.r), wg2 = fdiv(Gs.rv, Gs.r), wg3 = pressure(Gs, g);
                 wn[k][0] = 0.5 * (wcq[0] + wg0);
                 wn[k][1] = 0.5 * (wcq[1] + wg1);
                 wn[k][2] = 0.5 * (wcq[2] + wg2);
@@ -359,7 +359,7 @@ __device__ __forceinline__ void adj_cell_body(const AdjArgs& a, int c, const int
                 if (en[k].x < 0) continue;
                 // offset d to the neighbour point, weight w = 1/|d|^2; (w dx, w dy) is stored
                 const double w = eg[k].x * eg[k].x + eg[k].y * eg[k].y;
-                const double iw = 1.0 / w;
+                const double iw = fdiv(1.0, w);
                 const double dx = eg[k].x * iw, dy = eg[k].y * iw;
                 const double sq = M00 * dx * dx + 2.0 * Ms * dx * dy + M11 * dy * dy;
                 const double wq = wdxb[k] * dx + wdyb[k] * dy;
@@ -417,8 +417,8 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_cell(AdjArgs a) {
     adj_cell_body<D, WX>(a, c, en, eg, st4(a.U[c]), G0, G1, [&](int o, double (&w)[4]) {
         const State Uo = st4(a.U[o]);
         w[0] = Uo.r;
-        w[1] = Uo.ru / Uo.r;
-        w[2] = Uo.rv / Uo.r;
+        w[1] = fdiv(Uo.ru, Uo.r);
+        w[2] = fdiv(Uo.rv, Uo.r);
         w[3] = pressure(Uo, g);
     });
 }
@@ -492,7 +492,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_gather(AdjArgs a) {
         Sx += (x[i] + xj) * cr;
         Sy += (y[i] + yj) * cr;
     }
-    const double i3A2 = 1.0 / (3.0 * A2);
+    const double i3A2 = fdiv(1.0, 3.0 * A2);
     const double cx = Sx * i3A2, cy = Sy * i3A2;
     const double4 cb = a.cb[c];
     const double Cbx = cb.x + cbx, Cby = cb.y + cby;

@@ -52,7 +52,7 @@ __device__ __forceinline__ unsigned long long dbits_v(double x) { return (unsign
 // ---- Spalart-Allmaras, src/SpalartAllmaras.cpp:8-69 ----
 __device__ __forceinline__ double sa_fv1(double nut, double nu, const fvk_sa_constants& c) {
     if (nut <= 0.0) return 0.0;
-    double chi = nut / nu, chi3 = chi * chi * chi;
+    double chi = fdiv(nut, nu), chi3 = chi * chi * chi;
     return chi3 / (chi3 + c.cv1 * c.cv1 * c.cv1);
 }
 __device__ __forceinline__ double sa_eddy(double nut, double nu, const fvk_sa_constants& c) {
@@ -60,7 +60,7 @@ __device__ __forceinline__ double sa_eddy(double nut, double nu, const fvk_sa_co
 }
 __device__ double sa_source(double nut, double nu, double omega, double d, double gx, double gy,
                             const fvk_sa_constants& c) {
-    double chi = (nut > 0.0) ? nut / nu : 0.0;
+    double chi = (nut > 0.0) ? fdiv(nut, nu) : 0.0;
     double fv1 = sa_fv1(nut, nu, c);
     double fv2 = 1.0 - chi / (1.0 + chi * fv1);
     double d2 = d * d;
@@ -75,11 +75,11 @@ __device__ double sa_source(double nut, double nu, double omega, double d, doubl
         double r = smin(nut / (sts * c.kappa * c.kappa * d2), 10.0);
         double gg = r + c.cw2 * (pow(r, 6.0) - r);
         double fw = gg * pow((1.0 + pow(c.cw3, 6.0)) / (pow(gg, 6.0) + pow(c.cw3, 6.0)), 1.0 / 6.0);
-        double nod = nut / d;
+        double nod = fdiv(nut, d);
         double cw1 = c.cb1 / (c.kappa * c.kappa) + (1.0 + c.cb2) / c.sigma;
         D = cw1 * fw * nod * nod;
     }
-    double CD = (c.cb2 / c.sigma) * (gx * gx + gy * gy);
+    double CD = (fdiv(c.cb2, c.sigma)) * (gx * gx + gy * gy);
     return P - D + CD;
 }
 
@@ -116,8 +116,8 @@ __device__ void sst_source(double rho, double k, double w, double nu, double d, 
     double clip = (variant == FVK_SST_VORTICITY) ? 20.0 : 10.0;
     double kp = smin(mut * prs, clip * bsrwk);
     double kd = bsrwk;
-    double g1 = c.beta1 / c.beta_star - c.sigma_omega1 * c.kappa * c.kappa / sqrt(c.beta_star);
-    double g2 = c.beta2 / c.beta_star - c.sigma_omega2 * c.kappa * c.kappa / sqrt(c.beta_star);
+    double g1 = fdiv(c.beta1, c.beta_star) - c.sigma_omega1 * c.kappa * c.kappa / sqrt(c.beta_star);
+    double g2 = fdiv(c.beta2, c.beta_star) - c.sigma_omega2 * c.kappa * c.kappa / sqrt(c.beta_star);
     double gam = F1 * g1 + (1.0 - F1) * g2;
     double beta = F1 * c.beta1 + (1.0 - F1) * c.beta2;
     double wp = gam * rho * prs;
@@ -148,7 +148,7 @@ __device__ __forceinline__ void boundary_values(const ViscArgs& a, const DevBC& 
     if (EQ == EQ_SST) {
         if (bc.type == BC_WALL) {
             b[3] = 0.0;
-            double nu1 = a.mu / L.r;
+            double nu1 = fdiv(a.mu, L.r);
             double d1 = a.m.fdc[f];
             b[4] = 60.0 * nu1 / (a.sst.beta1 * d1 * d1);
             b[5] = 0.0;
@@ -183,8 +183,8 @@ __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
     const double g = a.gamma;
     State Uc = ldS(a.U, a.ld, c);
     double pc[NF];
-    pc[0] = Uc.ru / Uc.r;
-    pc[1] = Uc.rv / Uc.r;
+    pc[0] = fdiv(Uc.ru, Uc.r);
+    pc[1] = fdiv(Uc.rv, Uc.r);
     pc[2] = temperature(Uc, g, a.Rgas);
     double t0c = 0.0, t1c = 0.0;
     if (EQ == EQ_SA) { t0c = a.U[4 * a.ld + c]; pc[3] = t0c; }
@@ -211,8 +211,8 @@ __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
         } else {
             int o = owner ? cr : m.fl[f];
             State Uo = ldS(a.U, a.ld, o);
-            pn[0] = Uo.ru / Uo.r;
-            pn[1] = Uo.rv / Uo.r;
+            pn[0] = fdiv(Uo.ru, Uo.r);
+            pn[1] = fdiv(Uo.rv, Uo.r);
             pn[2] = temperature(Uo, g, a.Rgas);
             if (EQ == EQ_SA) pn[3] = a.U[4 * a.ld + o];
             if (EQ == EQ_SST) { pn[3] = a.U[4 * a.ld + o]; pn[NF - 1] = a.U[5 * a.ld + o]; }
@@ -257,20 +257,20 @@ __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
         const double V = m.vol[c];
 #pragma unroll
         for (int q = 0; q < NF; ++q) {
-            gr[2 * q] = sx[q] / V;
-            gr[2 * q + 1] = sy[q] / V;
+            gr[2 * q] = fdiv(sx[q], V);
+            gr[2 * q + 1] = fdiv(sy[q], V);
         }
     }
 #pragma unroll
     for (int q = 0; q < 2 * NF; ++q) a.G[q * a.ld + c] = gr[q];
     if (EQ == EQ_SA) {
-        double nul = a.mu / Uc.r;
+        double nul = fdiv(a.mu, Uc.r);
         double om = fabs(gr[2] - gr[1]);
         a.aux[c] = sa_source(t0c, nul, om, m.wdist[c], gr[6], gr[7], a.sa);
     }
     if (EQ == EQ_SST) {
         double o4[4];
-        double nul = a.mu / Uc.r;
+        double nul = fdiv(a.mu, Uc.r);
         sst_source(Uc.r, t0c, t1c, nul, m.wdist[c], gr, gr + 2, gr + 6, gr + 8, a.sst_variant, a.kato, a.sst, o4);
 #pragma unroll
         for (int q = 0; q < 4; ++q) a.aux[q * a.ld + c] = o4[q];
@@ -323,9 +323,9 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
         State R = b ? L : ldS(a.U, ld, cr);
         State Rs = b ? visc_ghost(*bc, L, g) : R;
         State inv = num_flux<SCHEME>(L, Rs, nx, ny, g, a.tol, a.maxit);
-        double uL = L.ru / L.r, vL = L.rv / L.r, TL = temperature(L, g, a.Rgas);
+        double uL = fdiv(L.ru, L.r), vL = fdiv(L.rv, L.r), TL = temperature(L, g, a.Rgas);
         double uR = 0.0, vR = 0.0, TR = 0.0;
-        if (!b) { uR = R.ru / R.r; vR = R.rv / R.r; TR = temperature(R, g, a.Rgas); }
+        if (!b) { uR = fdiv(R.ru, R.r); vR = fdiv(R.rv, R.r); TR = temperature(R, g, a.Rgas); }
         double t0L = 0.0, t1L = 0.0, t0R = 0.0, t1R = 0.0;
         if (EQ == EQ_SA || EQ == EQ_SST) { t0L = a.U[4 * ld + cl]; if (!b) t0R = a.U[4 * ld + cr]; }
         if (EQ == EQ_SST) { t1L = a.U[5 * ld + cl]; if (!b) t1R = a.U[5 * ld + cr]; }
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
         double vx = 0.0, vy = 0.0, vE = 0.0, adv0 = 0.0, adv1 = 0.0, dif0 = 0.0, dif1 = 0.0;
         double Vn = 0.0;
         if (EQ != EQ_NS) {
-            double uRg = Rs.ru / Rs.r, vRg = Rs.rv / Rs.r;
+            double uRg = fdiv(Rs.ru, Rs.r), vRg = fdiv(Rs.rv, Rs.r);
             Vn = 0.5 * ((uL + uRg) * nx + (vL + vRg) * ny);
         }
         if (EQ == EQ_SA) {
@@ -368,9 +368,9 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
             double mu_eff = a.mu, k_eff = a.mu * cp / a.pr;
             double tdvf = 0.0, nulL = 0.0, nulR = 0.0, ntL = 0.0, ntR = 0.0;
             if (EQ == EQ_SA) {
-                nulL = a.mu / L.r;
+                nulL = fdiv(a.mu, L.r);
                 ntL = sa_eddy(t0L, nulL, a.sa);
-                nulR = b ? nulL : a.mu / R.r;
+                nulR = b ? nulL : fdiv(a.mu, R.r);
                 ntR = b ? sa_eddy(bv[3], nulR, a.sa) : sa_eddy(t0R, nulR, a.sa);
                 double rR = b ? L.r : R.r;
                 tdvf = 0.5 * (L.r * ntL + rR * ntR);
@@ -431,20 +431,20 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
             double Smax = fabs(VnL) + sound_speed(L, g);
             double rmin = L.r;
             double ntmax = 0.0, wmax = 0.0;
-            if (EQ == EQ_SA) ntmax = sa_eddy(t0L, a.mu / L.r, a.sa);
-            if (EQ == EQ_SST) { ntmax = (t0L > 0.0 && t1L > 0.0) ? t0L / t1L : 0.0; wmax = t1L; }
+            if (EQ == EQ_SA) ntmax = sa_eddy(t0L, fdiv(a.mu, L.r), a.sa);
+            if (EQ == EQ_SST) { ntmax = (t0L > 0.0 && t1L > 0.0) ? fdiv(t0L, t1L) : 0.0; wmax = t1L; }
             if (!b) {
                 double VnR = uR * nx + vR * ny;
                 Smax = smax(Smax, fabs(VnR) + sound_speed(R, g));
                 rmin = smin(rmin, R.r);
-                if (EQ == EQ_SA) ntmax = smax(ntmax, sa_eddy(t0R, a.mu / R.r, a.sa));
+                if (EQ == EQ_SA) ntmax = smax(ntmax, sa_eddy(t0R, fdiv(a.mu, R.r), a.sa));
                 if (EQ == EQ_SST) {
-                    ntmax = smax(ntmax, (t0R > 0.0 && t1R > 0.0) ? t0R / t1R : 0.0);
+                    ntmax = smax(ntmax, (t0R > 0.0 && t1R > 0.0) ? fdiv(t0R, t1R) : 0.0);
                     wmax = smax(wmax, t1R);
                 }
             }
             double len = m.fdc[f];
-            double nu = (a.mu > 0.0) ? a.mu / rmin : 0.0;
+            double nu = (a.mu > 0.0) ? fdiv(a.mu, rmin) : 0.0;
             if (EQ == EQ_NS) {
                 double St = Smax + 2.0 * nu / len;
                 cand = a.cfl * len / St;
@@ -533,7 +533,7 @@ __global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
             if (a.dt_src == 1) dt = __longlong_as_double((long long)a.dtmin[a.slot]);
             else if (a.dt_src == 2) { dt = dmin; a.dtc[c] = dmin; }
             else dt = a.dtc[c];
-            double h = dt / V;
+            double h = fdiv(dt, V);
             double rho_old = a.U[c];
             // the last RK stage writes the buffer that holds the base state:
             // read all of this cell's base values before any write
@@ -554,7 +554,7 @@ __global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
                 if (EQ == EQ_SST) {
                     double cur = rho_old * t;
                     double nw = a.first ? cur + h * r[q] : a.a * (u0[0] * u0[q]) + a.b * (cur + h * r[q]);
-                    a.Unew[q * ld + c] = nw / rho_new;
+                    a.Unew[q * ld + c] = fdiv(nw, rho_new);
                 } else {
                     a.Unew[q * ld + c] = a.first ? t + h * r[q] : a.a * u0[q] + a.b * (t + h * r[q]);
                 }
@@ -608,7 +608,7 @@ __global__ __launch_bounds__(kVB) void k_diff_cell(ViscArgs a) {
             else r[0] += a.flux[e >> 1];
         }
         if (a.R) a.R[c] = r[0];
-        else a.Unew[c] = a.U[c] + (a.fixed_dt / m.vol[c]) * r[0];
+        else a.Unew[c] = a.U[c] + (fdiv(a.fixed_dt, m.vol[c])) * r[0];
     }
     if (!a.want_res) return;
     double sq[1] = {r[0] * r[0]};
@@ -663,8 +663,8 @@ __global__ __launch_bounds__(kVB) void k_scalar_grad(MeshView m, const double* p
         grad[2 * c] = i0 * sx + i1 * sy;
         grad[2 * c + 1] = i1 * sx + i2 * sy;
     } else {
-        grad[2 * c] = sx / m.vol[c];
-        grad[2 * c + 1] = sy / m.vol[c];
+        grad[2 * c] = fdiv(sx, m.vol[c]);
+        grad[2 * c + 1] = fdiv(sy, m.vol[c]);
     }
 }
 
