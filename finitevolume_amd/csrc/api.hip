@@ -497,8 +497,8 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
             s->step_count++;
             (void)i;
         }
-        if (s->p.equation == FVK_EQ_EULER) {
-            int rc = flush_euler_finalize(s);
+        {
+            int rc = flush_pending_finalize(s);
             if (rc != FVK_OK) return rc;
         }
         // the finalize kernels wrote the rows into mapped host memory: one sync per chunk

@@ -29,6 +29,7 @@
 #include "physics.h"
 #include "solver.h"
 #include "tile.h"
+#include "norms.h"
 
 namespace fvk {
 
@@ -73,7 +74,6 @@ struct EulerArgs {
     int rearm_slot;       // >= 0: the gather re-arms this dt slot (forward Euler with a folded finalize)
 };
 
-__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 
 
 // case of device cell c (binary search of the case ranges)
@@ -202,47 +202,6 @@ __global__ __launch_bounds__(kCaseFinBlock) void k_case_finalize(const double* p
 }
 
 __global__ void k_step_tick(unsigned int* step) { *step += 1u; }
-
-// one block: residual norms = sqrt of the sums of the partials (fixed order
-// for a given workgroup size), the step's dt into the history, and the next
-// step's dt slot re-armed; sm holds blockDim.x doubles
-__device__ __forceinline__ void finalize_norms(const double* partials, int nblocks, int nv, double* res_hist,
-                                               double* dt_hist, int64_t hist_idx, unsigned long long* dtmin,
-                                               int slot, double fixed_dt, double* sm, bool rearm = true) {
-    const int nt = blockDim.x, tid = threadIdx.x;
-    int p2 = 64;  // the largest power of two not above the workgroup size
-    while (2 * p2 <= nt) p2 *= 2;
-    for (int q = 0; q < nv; ++q) {
-        // four independent loads in flight per thread, summed in a fixed order
-        const double* p = partials + (int64_t)q * nblocks;
-        double acc = 0.0;
-        for (int i = tid; i < nblocks; i += 4 * nt) {
-            double v[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) v[u] = i + u * nt < nblocks ? p[i + u * nt] : 0.0;
-#pragma unroll
-            for (int u = 0; u < 4; ++u) acc += v[u];
-        }
-        sm[tid] = acc;
-        __syncthreads();
-        if (tid < nt - p2) sm[tid] += sm[tid + p2];
-        __syncthreads();
-        for (int s = p2 / 2; s > 0; s >>= 1) {
-            if (tid < s) sm[tid] += sm[tid + s];
-            __syncthreads();
-        }
-        if (tid == 0) res_hist[hist_idx * nv + q] = sqrt(sm[0]);
-        __syncthreads();
-    }
-    if (tid == 0) {
-        if (fixed_dt >= 0.0) {
-            dt_hist[hist_idx] = fixed_dt;
-        } else {
-            dt_hist[hist_idx] = __longlong_as_double((long long)dtmin[slot]);
-            if (rearm) dtmin[slot ^ 1] = dbits(DBL_MAX);
-        }
-    }
-}
 
 constexpr int kFinBlock = 1024;
 __global__ __launch_bounds__(kFinBlock) void k_e_finalize(const double* partials, int nblocks, int nv,
@@ -1070,10 +1029,10 @@ int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, i
     return hip_check(hipGetLastError(), "finalize launch");
 }
 
-// a finalize still pending after the last stage launch of a call
-int flush_euler_finalize(fvk_solver* s) {
+// a finalize still pending after the last stage launch of a call (Euler or viscous)
+int flush_pending_finalize(fvk_solver* s) {
     if (s->fin_nblocks <= 0) return FVK_OK;
-    k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(s->partials, s->fin_nblocks, 4, s->res_hist, s->dt_hist,
+    k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(s->partials, s->fin_nblocks, s->nv, s->res_hist, s->dt_hist,
                                                  s->fin_hist_idx, s->dtmin, s->fin_slot, -1.0, s->fin_rearm);
     s->fin_nblocks = 0;
     return hip_check(hipGetLastError(), "finalize launch");

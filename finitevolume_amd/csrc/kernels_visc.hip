@@ -10,6 +10,7 @@
 //                stage update + residual partials
 #include <cfloat>
 
+#include "norms.h"
 #include "physics.h"
 #include "solver.h"
 
@@ -40,6 +41,10 @@ struct ViscArgs {
     double a, b;
     int face_dt, dt_src, first, want_res, slot;
     int64_t hist_idx;
+    // the previous stage's norms, reduced by a spare workgroup of the
+    // gradient launch (fin_nblocks == 0: none)
+    int fin_nblocks, fin_slot, fin_nv;
+    int64_t fin_hist_idx;
 };
 
 enum : int { EQ_NS = 1, EQ_SA = 2, EQ_SST = 3 };
@@ -47,7 +52,6 @@ enum : int { EQ_NS = 1, EQ_SA = 2, EQ_SST = 3 };
 __device__ __forceinline__ State ldS(const double* U, int64_t ld, int c) {
     return State{U[c], U[ld + c], U[2 * ld + c], U[3 * ld + c]};
 }
-__device__ __forceinline__ unsigned long long dbits_v(double x) { return (unsigned long long)__double_as_longlong(x); }
 
 // ---- Spalart-Allmaras, src/SpalartAllmaras.cpp:8-69 ----
 __device__ __forceinline__ double sa_fv1(double nut, double nu, const fvk_sa_constants& c) {
@@ -178,6 +182,12 @@ template <int EQ, int GS, bool PERIODIC>
 __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
     constexpr int NF = EQ == EQ_SST ? 5 : (EQ == EQ_SA ? 4 : 3);
     const MeshView& m = a.m;
+    if (a.fin_nblocks > 0 && logical_block() == (int)gridDim.x - 1) {  // the spare workgroup (no cells)
+        __shared__ double fsm[kVB];
+        finalize_norms(a.partials, a.fin_nblocks, a.fin_nv, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin,
+                       a.fin_slot, -1.0, fsm, true);
+        return;
+    }
     int c = xcd_index();
     if (c >= m.nc) return;
     const double g = a.gamma;
@@ -469,7 +479,7 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
             if (threadIdx.x < s) sm[threadIdx.x] = smin(sm[threadIdx.x], sm[threadIdx.x + s]);
             __syncthreads();
         }
-        if (threadIdx.x == 0) atomicMin(&a.dtmin[a.slot], dbits_v(sm[0]));
+        if (threadIdx.x == 0) atomicMin(&a.dtmin[a.slot], dbits(sm[0]));
     }
 }
 
@@ -758,12 +768,14 @@ void launch_face_eq(fvk_solver* s, int grid, const ViscArgs& a) {
 void run_visc(fvk_solver* s, ViscArgs& a) {
     const int gc = grid_for(s->nc, kVB);
     const int gf = grid_for(s->nf, kVB);
+    // a pending finalize takes the last logical workgroup of one more XCD round
+    const int gg = gc + (a.fin_nblocks > 0 ? kXcds : 0);
     hipEvent_t ev = nullptr;
     timer_begin(s, "visc_grad", &ev);
     switch (s->p.equation) {
-        case FVK_EQ_NAVIER_STOKES: launch_grad_eq<EQ_NS>(s, gc, a); break;
-        case FVK_EQ_RANS_SA: launch_grad_eq<EQ_SA>(s, gc, a); break;
-        default: launch_grad_eq<EQ_SST>(s, gc, a); break;
+        case FVK_EQ_NAVIER_STOKES: launch_grad_eq<EQ_NS>(s, gg, a); break;
+        case FVK_EQ_RANS_SA: launch_grad_eq<EQ_SA>(s, gg, a); break;
+        default: launch_grad_eq<EQ_SST>(s, gg, a); break;
     }
     timer_end(s, ev);
     timer_begin(s, "visc_face", &ev);
@@ -781,6 +793,14 @@ void run_visc(fvk_solver* s, ViscArgs& a) {
     }
     timer_end(s, ev);
     if (a.want_res) {
+        if (s->p.rk_stages >= 2 && !a.R) {
+            // the next stage's gradient launch reduces this stage's partials
+            s->fin_nblocks = gc;
+            s->fin_slot = a.slot;
+            s->fin_hist_idx = a.hist_idx;
+            s->fin_rearm = 1;
+            return;
+        }
         timer_begin(s, "visc_finalize", &ev);
         launch_norm_finalize(s, gc, s->nv, a.hist_idx, a.slot, -1.0);
         timer_end(s, ev);
@@ -807,6 +827,13 @@ int launch_visc_stage(fvk_solver* s, int stage, int stages, int want_res, int64_
     a.want_res = want_res && stage == 0;
     a.slot = (int)(step & 1);
     a.hist_idx = step % s->hist_cap;
+    if (s->fin_nblocks > 0) {  // the previous stage's norms, folded into this stage's gradient launch
+        a.fin_nblocks = s->fin_nblocks;
+        a.fin_slot = s->fin_slot;
+        a.fin_hist_idx = s->fin_hist_idx;
+        a.fin_nv = s->nv;
+        s->fin_nblocks = 0;
+    }
     run_visc(s, a);
     return hip_check(hipGetLastError(), "viscous stage launch");
 }

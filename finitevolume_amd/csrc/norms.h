@@ -1,0 +1,58 @@
+// Residual-norm finalize shared by the Euler and viscous kernels: one
+// workgroup reduces the per-block partial sums of the squared residuals in a
+// fixed order, writes the step's norms and dt into the history, and re-arms
+// the next step's dt slot.  It runs either as its own one-block kernel or as
+// a spare workgroup of the next launch on the stream.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace fvk {
+
+__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+
+// one block: residual norms = sqrt of the sums of the partials (fixed order
+// for a given workgroup size), the step's dt into the history, and the next
+// step's dt slot re-armed; sm holds blockDim.x doubles
+__device__ __forceinline__ void finalize_norms(const double* partials, int nblocks, int nv, double* res_hist,
+                                               double* dt_hist, int64_t hist_idx, unsigned long long* dtmin,
+                                               int slot, double fixed_dt, double* sm, bool rearm = true) {
+    const int nt = blockDim.x, tid = threadIdx.x;
+    int p2 = 64;  // the largest power of two not above the workgroup size
+    while (2 * p2 <= nt) p2 *= 2;
+    for (int q = 0; q < nv; ++q) {
+        // four independent loads in flight per thread, summed in a fixed order
+        const double* p = partials + (int64_t)q * nblocks;
+        double acc = 0.0;
+        for (int i = tid; i < nblocks; i += 4 * nt) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = i + u * nt < nblocks ? p[i + u * nt] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc += v[u];
+        }
+        sm[tid] = acc;
+        __syncthreads();
+        if (tid < nt - p2) sm[tid] += sm[tid + p2];
+        __syncthreads();
+        for (int s = p2 / 2; s > 0; s >>= 1) {
+            if (tid < s) sm[tid] += sm[tid + s];
+            __syncthreads();
+        }
+        if (tid == 0) res_hist[hist_idx * nv + q] = sqrt(sm[0]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (fixed_dt >= 0.0) {
+            dt_hist[hist_idx] = fixed_dt;
+        } else {
+            dt_hist[hist_idx] = __longlong_as_double((long long)dtmin[slot]);
+            if (rearm) dtmin[slot ^ 1] = dbits(DBL_MAX);
+        }
+    }
+}
+
+}  // namespace fvk

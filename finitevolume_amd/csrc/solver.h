@@ -214,7 +214,7 @@ int launch_wall_distance(fvk_solver* s, const int* wall_faces, int nw, const dou
 int launch_residual_eval(fvk_solver* s, double* R_dev);
 int launch_euler_gradients(fvk_solver* s);
 int launch_case_finalize(fvk_solver* s);
-int flush_euler_finalize(fvk_solver* s);
+int flush_pending_finalize(fvk_solver* s);
 int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, int slot, double fixed_dt);
 int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
 void adjoint_release(fvk_solver* s);
