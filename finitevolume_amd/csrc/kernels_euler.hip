@@ -656,11 +656,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
 #pragma unroll
         for (int k = 0; k < D; ++k) pen[k] = te[k * MG + tid];
     }
+    const double4* tg = m.tegeo + (int64_t)t * D * MG;  // tile-ordered geometry: no index load first
     if (ORDER == 2 && tid < ng) {
-        const int gc = tid < n ? c0 + tid : m.thalo[hp + tid - n];
 #pragma unroll
-        for (int k = 0; k < D; ++k) peg[k] = pen[k].x >= 0 ? m.egeo[k * m.ld + gc] : double4{0.0, 0.0, 0.0, 0.0};
-        piv = m.clsq[gc];
+        for (int k = 0; k < D; ++k) peg[k] = tg[k * MG + tid];
+        piv = m.tclsq[(int64_t)t * MG + tid];
     }
     int4 pfd[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     double4 pfg[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, pro[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
@@ -668,8 +668,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     for (int i = 0; i < 2; ++i) {
         if (tid + i * bs < nft) {
             pfd[i] = m.tface[f0 + tid + i * bs];
-            pfg[i] = m.fgeo[pfd[i].x];
-            if (ORDER == 2) pro[i] = m.frec[pfd[i].x];
+            pfg[i] = m.tfgeo[f0 + tid + i * bs];
+            if (ORDER == 2) pro[i] = m.tfrec[f0 + tid + i * bs];
         }
     }
 
@@ -693,14 +693,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     if (ORDER == 2) {
         if (tid < ng) tile_grad<D>(m, g, a.limiter, L, tid, pen, peg, piv);
         for (int j = tid + bs; j < ng; j += bs) {
-            const int gc = j < n ? c0 + j : m.thalo[hp + j - n];
             int2 en[D];
             double4 eg[D];
 #pragma unroll
             for (int k = 0; k < D; ++k) en[k] = te[k * MG + j];
 #pragma unroll
-            for (int k = 0; k < D; ++k) eg[k] = en[k].x >= 0 ? m.egeo[k * m.ld + gc] : double4{0.0, 0.0, 0.0, 0.0};
-            tile_grad<D>(m, g, a.limiter, L, j, en, eg, m.clsq[gc]);
+            for (int k = 0; k < D; ++k) eg[k] = tg[k * MG + j];
+            tile_grad<D>(m, g, a.limiter, L, j, en, eg, m.tclsq[(int64_t)t * MG + j]);
         }
     }
     __syncthreads();  // ring 2's primitives are dead from here: their space takes the fluxes
@@ -722,8 +721,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
         double4 fg = i == 0 ? pfg[0] : pfg[1], ro = i == 0 ? pro[0] : pro[1];
         if (i >= 2) {
             fd = m.tface[f0 + e];
-            fg = m.fgeo[fd.x];
-            if (ORDER == 2) ro = m.frec[fd.x];
+            fg = m.tfgeo[f0 + e];
+            if (ORDER == 2) ro = m.tfrec[f0 + e];
         }
         cand_min = smin(cand_min, tile_face<SCHEME, ORDER>(a, L, e, fd, fg, ro));
     }

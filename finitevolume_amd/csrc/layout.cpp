@@ -213,9 +213,14 @@ void tile_order(fvk_solver* s, int n, const std::vector<int>& aptr, const std::v
 
 // Per tile: the halo rings, the tile-local copies of the ELL entries of the
 // cells that need gradients, the tile's face list and each owned cell's
-// slots as indices into it.  Returns the LDS bytes a tile needs at most.
+// slots as indices into it.  The static geometry the tile kernel reads
+// (LSQ and limiter offsets, LSQ inverse, face geometry and MUSCL offsets)
+// is also copied in tile order, so its loads are contiguous and need no
+// index loaded first.  Returns the LDS bytes a tile needs at most.
 int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<int>& cent, const std::vector<int>& fl,
-                const std::vector<int>& fr, int width, bool second_order, size_t* lds_bytes) {
+                const std::vector<int>& fr, int width, bool second_order, const std::vector<double4>& egeo,
+                const std::vector<double4>& clsq, const std::vector<double4>& fgeo, const std::vector<double4>& frec,
+                size_t* lds_bytes) {
     const int nt = (int)s->tile_start.size() - 1;
     const int T = s->block;  // tslot stride: the workgroup size (tiles hold at most tile_cells <= T cells)
     const int64_t nc = s->nc, nf = s->nf;
@@ -273,6 +278,12 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     }
     // pass 2: tile-local ELL and owned slots (need the maxima for the strides)
     std::vector<int2> tell((size_t)nt * width * mg, int2{-1, -1});
+    std::vector<double4> tegeo, tclsq;
+    if (second_order) {
+        tegeo.assign((size_t)nt * width * mg, double4{0.0, 0.0, 0.0, 0.0});
+        tclsq.assign((size_t)nt * mg, double4{0.0, 0.0, 0.0, 0.0});
+    }
+    const int64_t ld = s->mesh.ld;
     std::vector<short> tslot((size_t)nt * width * T, (short)-1);
     for (int t = 0; t < nt; ++t) {
         const int c0 = s->tile_start[t];
@@ -286,6 +297,11 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
         const int nell = second_order ? ti.n + ti.h1 : ti.n;
         for (int j = 0; j < nell; ++j) {
             const int c = cells[j];
+            if (second_order) {
+                tclsq[(size_t)t * mg + j] = clsq[c];
+                for (int k = 0; k < cptr[c + 1] - cptr[c]; ++k)
+                    tegeo[((size_t)t * width + k) * mg + j] = egeo[(size_t)k * ld + c];
+            }
             for (int k = cptr[c]; k < cptr[c + 1]; ++k) {
                 const int slot = k - cptr[c];
                 const int e = cent[k], f = e >> 1;
@@ -328,7 +344,18 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     m.tfptr = upload(s, tfptr);
     m.tface = upload(s, tface);
     m.tslot = upload(s, tslot);
-    if (!m.tstart || !m.thptr || (!thalo.empty() && !m.thalo) || !m.th1 || !m.tell || !m.tfptr || !m.tface || !m.tslot)
+    std::vector<double4> tfgeo(tface.size()), tfrec;
+    for (size_t e = 0; e < tface.size(); ++e) tfgeo[e] = fgeo[tface[e].x];
+    if (second_order) {
+        tfrec.resize(tface.size());
+        for (size_t e = 0; e < tface.size(); ++e) tfrec[e] = frec[tface[e].x];
+    }
+    m.tfgeo = upload(s, tfgeo);
+    m.tfrec = second_order ? upload(s, tfrec) : nullptr;
+    m.tegeo = second_order ? upload(s, tegeo) : nullptr;
+    m.tclsq = second_order ? upload(s, tclsq) : nullptr;
+    if (!m.tstart || !m.thptr || (!thalo.empty() && !m.thalo) || !m.th1 || !m.tell || !m.tfptr || !m.tface || !m.tslot ||
+        !m.tfgeo || (second_order && (!m.tfrec || !m.tegeo || !m.tclsq)))
         return set_error(FVK_ERR_NOMEM, "device allocation for the tiles failed");
     // second order: U (4), G (8) and W (4) of the owned and ring-1 cells, then
     // W (4) of ring 2, whose space the face fluxes (4) and dt candidates (1)
@@ -674,7 +701,7 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
         return set_error(FVK_ERR_NOMEM, "device allocation for the mesh failed");
     if (s->variant == 3) {
         size_t lds = 0;
-        int rc = build_tiles(s, cptr, cent, fl, fr, width, s->p.order == 2, &lds);
+        int rc = build_tiles(s, cptr, cent, fl, fr, width, s->p.order == 2, egeo, clsq, fgeo, frec, &lds);
         if (rc != FVK_OK) return rc;
         if (lds > kMaxTileLds)
             return set_error(FVK_ERR_UNSUPPORTED, "a tile of %d cells needs %zu B of LDS (limit %d): use smaller tiles",

@@ -100,6 +100,10 @@ struct MeshView {
     const int* tfptr;       // [ntiles + 1] offsets into tface
     const int4* tface;      // per tile face: {device face, local cell_left, local cell_right or -1 - patch, 0}
     const short* tslot;     // [ntiles][width][tile size] tile-local face index of each owned cell's slot
+    // static geometry in tile order: egeo and clsq of each tile's owned and
+    // ring-1 cells ([ntiles][width][tmax_g], [ntiles][tmax_g]; second
+    // order), fgeo and frec of each tile face (as tface)
+    const double4 *tegeo, *tclsq, *tfgeo, *tfrec;
 };
 
 struct AdjointWork;  // device buffers and index maps of the residual VJP (adjoint.hip)
