@@ -359,12 +359,19 @@ def rans_case(reference_mode=False):
     return m, [wall, far], params, ic
 
 
-def rans_leg(steps, warmup, cpu_seconds, no_cpu):
+def rans_leg(steps, warmup, cpu_seconds, no_cpu, args=None):
     """Config 3 throughput (SA, RK3 + local dt) and reference mode 3R on the GPU,
     with the reference solver's 3R on the host cores beside it."""
     from finitevolume_amd.solver import FVKSolver
     m, bcs, params, ic = rans_case()
     nc, fpc = m.n_cells, m.n_faces / m.n_cells
+    tuned = None
+    if args is not None:
+        params.kernel_variant, params.block_size, params.tile_cells = args.variant, args.block, args.tile_cells
+        if not args.no_tune and not (args.variant or args.block):
+            from finitevolume_amd import autotune
+            tuned = autotune.tune(m, bcs, params, ic, steps=10, warmup=2, use_cache=False)
+            print(f"[bench] rans autotune: {tuned['table_ms_per_step']}", file=sys.stderr, flush=True)
     s = FVKSolver(m, bcs, params, ic)
     wall = time_gpu(s, steps, warmup, per_step=True)
     s.set_kernel_timing(True)
@@ -381,7 +388,10 @@ def rans_leg(steps, warmup, cpu_seconds, no_cpu):
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(b_alg * nc / (stage_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                         "b_alg_per_cell_update": round(b_alg, 1), "stage_device_ms": round(stage_ms, 4),
-                        "kernel_avg_ms": {k: round(tot / cnt, 5) for k, (tot, cnt) in sorted(times.items()) if cnt}}}
+                        "kernel_avg_ms": {k: round(tot / cnt, 5) for k, (tot, cnt) in sorted(times.items()) if cnt}},
+           "launch_shape": s.launch_shape()}
+    if tuned:
+        out["autotune_ms_per_step"] = tuned["table_ms_per_step"]
     del s
     m3, bcs3, p3, ic3 = rans_case(reference_mode=True)
     s3 = FVKSolver(m3, bcs3, p3, ic3)
@@ -469,7 +479,7 @@ def main():
         return
 
     if args.rans:
-        line = rans_leg(args.steps, args.warmup, args.cpu_seconds, args.no_cpu)
+        line = rans_leg(args.steps, args.warmup, args.cpu_seconds, args.no_cpu, args)
         line.update({"metric": "FP64 RANS SA residual Gcell-updates/s (1 GPU, % HBM roofline)", "n_gpus": 1,
                      "dtype": "f64", "data": "synthetic (generated NACA 4412 hybrid O-grid, free-stream start)"})
         print(json.dumps(line))

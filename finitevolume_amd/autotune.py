@@ -89,6 +89,8 @@ def tune(mesh, bcs, params: abi.ParamsC, ic, steps: int = 20, warmup: int = 3,
     else:
         from .solver import FVKError
         table, shapes = {}, {}
+        if params.equation != abi.EQ_EULER:
+            variants = [v for v in variants if v != 2]  # the per-cell fused variant is Euler-only
         for v in variants:
             for b, tc in (TILE_SHAPES if v == 3 else [(b, 0) for b in blocks]):
                 p = abi.ParamsC.from_buffer_copy(params)

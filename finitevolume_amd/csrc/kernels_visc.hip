@@ -317,28 +317,29 @@ __device__ __forceinline__ void corrected(const FG& fg, double nx, double ny, do
 // RANSTurbulenceSSTSolver (src/RANSTurbulenceSSTSolver.cpp:183-307) and the
 // compute_dt candidates (NS :231-258, SA :313-346, SST :380-429)
 // ---------------------------------------------------------------------------
-template <int EQ, int SCHEME>
-__global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
+// Per-cell field access of one face's two sides (0 = left, 1 = right):
+// u(side, q) conserved field q (turbulence fields from q = 4), g(side, q)
+// gradient component q, aux(side, q) per-cell auxiliary field q.  The face
+// and tile kernels read them from HBM or LDS.
+template <int EQ, int SCHEME, class Acc>
+__device__ __forceinline__ double visc_face_body(const ViscArgs& a, int f, int cr, const Acc& acc, double (&out)[6]) {
     const MeshView& m = a.m;
-    int f = xcd_index();
     double cand = DBL_MAX;
-    if (f < m.nf) {
+    {
         const double g = a.gamma;
-        const int64_t ld = a.ld;
-        int cl = m.fl[f], cr = m.fr[f];
         bool b = cr < 0;
         const DevBC* bc = b ? &m.bc[-1 - cr] : nullptr;
         double nx = m.fnx[f], ny = m.fny[f], A = m.fa[f];
-        State L = ldS(a.U, ld, cl);
-        State R = b ? L : ldS(a.U, ld, cr);
+        State L{acc.u(0, 0), acc.u(0, 1), acc.u(0, 2), acc.u(0, 3)};
+        State R = b ? L : State{acc.u(1, 0), acc.u(1, 1), acc.u(1, 2), acc.u(1, 3)};
         State Rs = b ? visc_ghost(*bc, L, g) : R;
         State inv = num_flux<SCHEME>(L, Rs, nx, ny, g, a.tol, a.maxit);
         double uL = fdiv(L.ru, L.r), vL = fdiv(L.rv, L.r), TL = temperature(L, g, a.Rgas);
         double uR = 0.0, vR = 0.0, TR = 0.0;
         if (!b) { uR = fdiv(R.ru, R.r); vR = fdiv(R.rv, R.r); TR = temperature(R, g, a.Rgas); }
         double t0L = 0.0, t1L = 0.0, t0R = 0.0, t1R = 0.0;
-        if (EQ == EQ_SA || EQ == EQ_SST) { t0L = a.U[4 * ld + cl]; if (!b) t0R = a.U[4 * ld + cr]; }
-        if (EQ == EQ_SST) { t1L = a.U[5 * ld + cl]; if (!b) t1R = a.U[5 * ld + cr]; }
+        if (EQ == EQ_SA || EQ == EQ_SST) { t0L = acc.u(0, 4); if (!b) t0R = acc.u(1, 4); }
+        if (EQ == EQ_SST) { t1L = acc.u(0, 5); if (!b) t1R = acc.u(1, 5); }
         double bv[7] = {0, 0, 0, 0, 0, 0, 0};
         if (b) boundary_values<EQ>(a, *bc, f, L, uL, vL, TL, t0L, t1L, bv);
         double vx = 0.0, vy = 0.0, vE = 0.0, adv0 = 0.0, adv1 = 0.0, dif0 = 0.0, dif1 = 0.0;
@@ -359,13 +360,12 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
             adv1 = Vn * ((Vn >= 0.0) ? rwL : rwR) * A;
         }
         if (!b || bc->type == BC_WALL) {
-            const double* G = a.G;
-            double gLu0 = G[cl], gLu1 = G[ld + cl], gLv0 = G[2 * ld + cl], gLv1 = G[3 * ld + cl];
-            double gLT0 = G[4 * ld + cl], gLT1 = G[5 * ld + cl];
+            double gLu0 = acc.g(0, 0), gLu1 = acc.g(0, 1), gLv0 = acc.g(0, 2), gLv1 = acc.g(0, 3);
+            double gLT0 = acc.g(0, 4), gLT1 = acc.g(0, 5);
             double gRu0 = 0, gRu1 = 0, gRv0 = 0, gRv1 = 0, gRT0 = 0, gRT1 = 0;
             if (!b) {
-                gRu0 = G[cr]; gRu1 = G[ld + cr]; gRv0 = G[2 * ld + cr]; gRv1 = G[3 * ld + cr];
-                gRT0 = G[4 * ld + cr]; gRT1 = G[5 * ld + cr];
+                gRu0 = acc.g(1, 0); gRu1 = acc.g(1, 1); gRv0 = acc.g(1, 2); gRv1 = acc.g(1, 3);
+                gRT0 = acc.g(1, 4); gRT1 = acc.g(1, 5);
             }
             FG fu = face_grad(m, f, b, uL, b ? bv[0] : uR, gLu0, gLu1, gRu0, gRu1);
             FG fv = face_grad(m, f, b, vL, b ? bv[1] : vR, gLv0, gLv1, gRv0, gRv1);
@@ -388,8 +388,8 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
                 k_eff = cp * a.mu / a.pr + cp * tdvf / a.prt;
             }
             if (EQ == EQ_SST) {
-                ntL = a.aux[3 * ld + cl];
-                ntR = b ? 0.0 : a.aux[3 * ld + cr];
+                ntL = acc.aux(0, 3);
+                ntR = b ? 0.0 : acc.aux(1, 3);
                 double rR = b ? L.r : R.r;
                 tdvf = 0.5 * (L.r * ntL + rR * ntR);
                 mu_eff = a.mu + tdvf;
@@ -409,33 +409,32 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
             if (EQ == EQ_SA) {
                 double nlf = 0.5 * (nulL + nulR);
                 double ntf = 0.5 * (ntL + ntR);
-                FG fn = face_grad(m, f, b, t0L, b ? bv[3] : t0R, G[6 * ld + cl], G[7 * ld + cl],
-                                  b ? 0.0 : G[6 * ld + cr], b ? 0.0 : G[7 * ld + cr]);
+                FG fn = face_grad(m, f, b, t0L, b ? bv[3] : t0R, acc.g(0, 6), acc.g(0, 7),
+                                  b ? 0.0 : acc.g(1, 6), b ? 0.0 : acc.g(1, 7));
                 double dif = (nlf + ntf) / a.sa.sigma;
                 dif0 = -dif * fn.dn * A;
             }
             if (EQ == EQ_SST) {
-                double F1L = a.aux[2 * ld + cl];
-                double F1R = b ? F1L : a.aux[2 * ld + cr];
+                double F1L = acc.aux(0, 2);
+                double F1R = b ? F1L : acc.aux(1, 2);
                 double F1f = b ? F1L : 0.5 * (F1L + F1R);
                 double sk = F1f * a.sst.sigma_k1 + (1.0 - F1f) * a.sst.sigma_k2;
                 double sw = F1f * a.sst.sigma_omega1 + (1.0 - F1f) * a.sst.sigma_omega2;
                 double mek = a.mu + sk * tdvf, mew = a.mu + sw * tdvf;
-                FG fk = face_grad(m, f, b, t0L, b ? bv[3] : t0R, G[6 * ld + cl], G[7 * ld + cl],
-                                  b ? 0.0 : G[6 * ld + cr], b ? 0.0 : G[7 * ld + cr]);
-                FG fw = face_grad(m, f, b, t1L, b ? bv[4] : t1R, G[8 * ld + cl], G[9 * ld + cl],
-                                  b ? 0.0 : G[8 * ld + cr], b ? 0.0 : G[9 * ld + cr]);
+                FG fk = face_grad(m, f, b, t0L, b ? bv[3] : t0R, acc.g(0, 6), acc.g(0, 7),
+                                  b ? 0.0 : acc.g(1, 6), b ? 0.0 : acc.g(1, 7));
+                FG fw = face_grad(m, f, b, t1L, b ? bv[4] : t1R, acc.g(0, 8), acc.g(0, 9),
+                                  b ? 0.0 : acc.g(1, 8), b ? 0.0 : acc.g(1, 9));
                 dif0 = -mek * fk.dn * A;
                 dif1 = -mew * fw.dn * A;
             }
         }
-        const int64_t ldf = a.ldf;
-        a.flux[f] = A * (inv.r - 0.0);
-        a.flux[ldf + f] = A * (inv.ru - vx);
-        a.flux[2 * ldf + f] = A * (inv.rv - vy);
-        a.flux[3 * ldf + f] = A * (inv.E - vE);
-        if (EQ == EQ_SA) a.flux[4 * ldf + f] = adv0 + dif0;
-        if (EQ == EQ_SST) { a.flux[4 * ldf + f] = adv0 + dif0; a.flux[5 * ldf + f] = adv1 + dif1; }
+        out[0] = A * (inv.r - 0.0);
+        out[1] = A * (inv.ru - vx);
+        out[2] = A * (inv.rv - vy);
+        out[3] = A * (inv.E - vE);
+        if (EQ == EQ_SA) out[4] = adv0 + dif0;
+        if (EQ == EQ_SST) { out[4] = adv0 + dif0; out[5] = adv1 + dif1; }
         if (a.face_dt) {
             double VnL = uL * nx + vL * ny;
             double Smax = fabs(VnL) + sound_speed(L, g);
@@ -468,8 +467,35 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
                     cand = smin(cand, dts);
                 }
             }
-            if (a.dt_src == 2) a.dtf[f] = cand;
         }
+    }
+    return cand;
+}
+
+// global field access for the face sweep
+struct GlobalSides {
+    const double *U, *G, *X;
+    int64_t ld;
+    int c[2];
+    __device__ double u(int s, int q) const { return U[q * ld + c[s]]; }
+    __device__ double g(int s, int q) const { return G[q * ld + c[s]]; }
+    __device__ double aux(int s, int q) const { return X[q * ld + c[s]]; }
+};
+
+template <int EQ, int SCHEME>
+__global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
+    constexpr int NV = EQ == EQ_SST ? 6 : (EQ == EQ_SA ? 5 : 4);
+    const MeshView& m = a.m;
+    int f = xcd_index();
+    double cand = DBL_MAX;
+    if (f < m.nf) {
+        const int cl = m.fl[f], cr = m.fr[f];
+        const GlobalSides acc{a.U, a.G, a.aux, a.ld, {cl, cr < 0 ? cl : cr}};
+        double out[6];
+        cand = visc_face_body<EQ, SCHEME>(a, f, cr, acc, out);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) a.flux[q * a.ldf + f] = out[q];
+        if (a.face_dt && a.dt_src == 2) a.dtf[f] = cand;
     }
     if (a.face_dt) {
         __shared__ double sm[kVB];
@@ -509,29 +535,13 @@ __device__ __forceinline__ void vblock_partials(const double (&v)[NV], double* p
 // cell pass: gather (Cell::faces order), sources (SA :279, SST :323-325),
 // update (SST: rho k, rho omega divided by the new density, :335-353)
 // ---------------------------------------------------------------------------
-template <int EQ>
-__global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
-    constexpr int NV = EQ == EQ_SST ? 6 : (EQ == EQ_SA ? 5 : 4);
+// sources, then the stage update (or the residual output) of cell c from
+// its gathered face sum r and, for local dt, the minimum of its faces' dt
+template <int EQ, int NV>
+__device__ __forceinline__ void visc_cell_finish(const ViscArgs& a, int c, double (&r)[NV], double dmin) {
     const MeshView& m = a.m;
-    int c = xcd_index();
-    double r[NV];
-#pragma unroll
-    for (int q = 0; q < NV; ++q) r[q] = 0.0;
-    const int64_t ld = a.ld, ldf = a.ldf;
-    if (c < m.nc) {
-        double dmin = DBL_MAX;
-        for (int k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-            int e = m.cent[k];
-            int f = e >> 1;
-            if (e & 1) {
-#pragma unroll
-                for (int q = 0; q < NV; ++q) r[q] = r[q] - a.flux[q * ldf + f];
-            } else {
-#pragma unroll
-                for (int q = 0; q < NV; ++q) r[q] += a.flux[q * ldf + f];
-            }
-            if (a.dt_src == 2) dmin = smin(dmin, a.dtf[f]);
-        }
+    const int64_t ld = a.ld;
+    {
         const double V = m.vol[c];
         if (EQ == EQ_SA) r[4] += V * a.aux[c];
         if (EQ == EQ_SST) { r[4] += V * a.aux[c]; r[5] += V * a.aux[ld + c]; }
@@ -571,11 +581,175 @@ __global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
             }
         }
     }
+}
+
+template <int EQ>
+__global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
+    constexpr int NV = EQ == EQ_SST ? 6 : (EQ == EQ_SA ? 5 : 4);
+    const MeshView& m = a.m;
+    int c = xcd_index();
+    double r[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) r[q] = 0.0;
+    const int64_t ldf = a.ldf;
+    if (c < m.nc) {
+        double dmin = DBL_MAX;
+        for (int k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
+            int e = m.cent[k];
+            int f = e >> 1;
+            if (e & 1) {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) r[q] = r[q] - a.flux[q * ldf + f];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) r[q] += a.flux[q * ldf + f];
+            }
+            if (a.dt_src == 2) dmin = smin(dmin, a.dtf[f]);
+        }
+        visc_cell_finish<EQ, NV>(a, c, r, dmin);
+    }
     if (!a.want_res) return;
     double sq[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) sq[q] = r[q] * r[q];
     vblock_partials<NV>(sq, a.partials);
+}
+
+// ---------------------------------------------------------------------------
+// fused face + cell pass over the solver's cell tiles (kernel variant 3, local
+// or no dt): the states, gradients (and SST's F1 and nu_t) of the tile's
+// cells and its halo ring are staged in LDS, the fluxes of the tile's faces
+// go to LDS, then each owned cell gathers them in Cell::faces order and is
+// updated.  Same operations as k_visc_face + k_visc_cell, so the same bits;
+// a face on a tile's edge is evaluated by both tiles.
+// ---------------------------------------------------------------------------
+struct TileSides {
+    const double *U, *G, *X;
+    int MG;
+    int c[2];
+    __device__ double u(int s, int q) const { return U[q * MG + c[s]]; }
+    __device__ double g(int s, int q) const { return G[q * MG + c[s]]; }
+    __device__ double aux(int s, int q) const { return X[(q - 2) * MG + c[s]]; }  // SST: F1 (2), nu_t (3)
+};
+
+constexpr int kVTileMax = 512;
+template <int EQ, int SCHEME>
+__global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
+    constexpr int NV = EQ == EQ_SST ? 6 : (EQ == EQ_SA ? 5 : 4);
+    constexpr int NG = 2 * (NV - 1), NX = EQ == EQ_SST ? 2 : 0;
+    extern __shared__ double lds[];
+    const MeshView& m = a.m;
+    const int t = logical_block();
+    const int bs = blockDim.x, tid = threadIdx.x;
+    const int MG = m.tmax_g, MF = m.tmax_f, W = m.twidth;
+    const int64_t ld = a.ld;
+    double* U = lds;
+    double* G = U + NV * MG;
+    double* X = G + NG * MG;
+    double* F = X + NX * MG;
+    double* Dt = F + NV * MF;
+    int c0 = 0, n = 0, hp = 0, nh = 0, f0 = 0, nft = 0;
+    if (t < m.ntiles) {
+        c0 = m.tstart[t];
+        n = m.tstart[t + 1] - c0;
+        hp = m.thptr[t];
+        nh = m.thptr[t + 1] - hp;
+        f0 = m.tfptr[t];
+        nft = m.tfptr[t + 1] - f0;
+    }
+    const int ng = n + nh;  // owned cells and ring 1 (the only halo ring here)
+
+    // 1. states, gradients (and SST's F1, nu_t) of the tile and its halo
+    for (int j = tid; j < ng; j += bs) {
+        const int gc = j < n ? c0 + j : m.thalo[hp + j - n];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) U[q * MG + j] = a.U[q * ld + gc];
+#pragma unroll
+        for (int q = 0; q < NG; ++q) G[q * MG + j] = a.G[q * ld + gc];
+        if (EQ == EQ_SST) {
+            X[j] = a.aux[2 * ld + gc];
+            X[MG + j] = a.aux[3 * ld + gc];
+        }
+    }
+    __syncthreads();
+
+    // 2. fluxes (and dt candidates) of the tile's faces
+    double cand_min = DBL_MAX;
+    for (int e = tid; e < nft; e += bs) {
+        const int4 fd = m.tface[f0 + e];
+        const TileSides acc{U, G, X, MG, {fd.y, fd.z < 0 ? fd.y : fd.z}};
+        double out[6];
+        const double cand = visc_face_body<EQ, SCHEME>(a, fd.x, fd.z, acc, out);
+#pragma unroll
+        for (int q = 0; q < NV; ++q) F[q * MF + e] = out[q];
+        if (a.face_dt) {
+            Dt[e] = cand;
+            cand_min = smin(cand_min, cand);
+        }
+    }
+    __syncthreads();
+
+    // 3. gather in Cell::faces order (owner minus), sources and update of the owned cells
+    double r[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) r[q] = 0.0;
+    if (tid < n) {
+        const int2* te = m.tell + (int64_t)t * W * MG;
+        const short* ts = m.tslot + (int64_t)t * W * m.tslot_stride;
+        double dmin = DBL_MAX;
+        for (int k = 0; k < W; ++k) {
+            const int2 en = te[k * MG + tid];
+            if (en.x < 0) continue;  // ELL padding (after the cell's faces)
+            const int lf = ts[k * m.tslot_stride + tid];
+            if (en.x & 1) {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) r[q] = r[q] - F[q * MF + lf];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) r[q] += F[q * MF + lf];
+            }
+            if (a.dt_src == 2) dmin = smin(dmin, Dt[lf]);
+        }
+        visc_cell_finish<EQ, NV>(a, c0 + tid, r, dmin);
+    }
+
+    // 4. block reductions in the (now free) LDS: residual partials, dt minimum
+    int p2 = 64;  // the largest power of two not above the workgroup size
+    while (2 * p2 <= bs) p2 *= 2;
+    if (a.want_res) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NV; ++q) lds[q * bs + tid] = r[q] * r[q];
+        __syncthreads();
+        if (tid < bs - p2) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) lds[q * bs + tid] += lds[q * bs + tid + p2];
+        }
+        __syncthreads();
+        for (int s2 = p2 / 2; s2 > 0; s2 >>= 1) {
+            if (tid < s2) {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) lds[q * bs + tid] += lds[q * bs + tid + s2];
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) a.partials[q * gridDim.x + blockIdx.x] = lds[q * bs];
+        }
+    }
+    if (a.face_dt) {
+        __syncthreads();
+        lds[tid] = cand_min;
+        __syncthreads();
+        if (tid < bs - p2) lds[tid] = smin(lds[tid], lds[tid + p2]);
+        __syncthreads();
+        for (int s2 = p2 / 2; s2 > 0; s2 >>= 1) {
+            if (tid < s2) lds[tid] = smin(lds[tid], lds[tid + s2]);
+            __syncthreads();
+        }
+        if (tid == 0) atomicMin(&a.dtmin[a.slot], dbits(lds[0]));
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -759,6 +933,16 @@ void launch_grad_eq(fvk_solver* s, int grid, const ViscArgs& a) {
 }
 
 template <int EQ>
+void launch_tile_eq(fvk_solver* s, int grid, const ViscArgs& a0) {
+    ViscArgs a = a0;
+    a.fin_nblocks = 0;  // a pending finalize went to this stage's gradient launch
+    const size_t lds = s->tile_lds;
+    if (s->p.flux_scheme == FLUX_HLLC) k_visc_tile<EQ, FLUX_HLLC><<<grid, s->block, lds, s->stream>>>(a);
+    else if (s->p.flux_scheme == FLUX_EXACT) k_visc_tile<EQ, FLUX_EXACT><<<grid, s->block, lds, s->stream>>>(a);
+    else k_visc_tile<EQ, FLUX_RUSANOV><<<grid, s->block, lds, s->stream>>>(a);
+}
+
+template <int EQ>
 void launch_face_eq(fvk_solver* s, int grid, const ViscArgs& a) {
     if (s->p.flux_scheme == FLUX_HLLC) k_visc_face<EQ, FLUX_HLLC><<<grid, kVB, 0, s->stream>>>(a);
     else if (s->p.flux_scheme == FLUX_EXACT) k_visc_face<EQ, FLUX_EXACT><<<grid, kVB, 0, s->stream>>>(a);
@@ -778,31 +962,45 @@ void run_visc(fvk_solver* s, ViscArgs& a) {
         default: launch_grad_eq<EQ_SST>(s, gg, a); break;
     }
     timer_end(s, ev);
-    timer_begin(s, "visc_face", &ev);
-    switch (s->p.equation) {
-        case FVK_EQ_NAVIER_STOKES: launch_face_eq<EQ_NS>(s, gf, a); break;
-        case FVK_EQ_RANS_SA: launch_face_eq<EQ_SA>(s, gf, a); break;
-        default: launch_face_eq<EQ_SST>(s, gf, a); break;
+    int nparts = gc;
+    // the fused tile pass needs the step's dt before its updates: local or no dt only
+    if (s->variant == 3 && (a.dt_src != 1 || a.R)) {
+        const int gt = (s->mesh.ntiles + kXcds - 1) / kXcds * kXcds;
+        timer_begin(s, "visc_tile", &ev);
+        switch (s->p.equation) {
+            case FVK_EQ_NAVIER_STOKES: launch_tile_eq<EQ_NS>(s, gt, a); break;
+            case FVK_EQ_RANS_SA: launch_tile_eq<EQ_SA>(s, gt, a); break;
+            default: launch_tile_eq<EQ_SST>(s, gt, a); break;
+        }
+        timer_end(s, ev);
+        nparts = gt;
+    } else {
+        timer_begin(s, "visc_face", &ev);
+        switch (s->p.equation) {
+            case FVK_EQ_NAVIER_STOKES: launch_face_eq<EQ_NS>(s, gf, a); break;
+            case FVK_EQ_RANS_SA: launch_face_eq<EQ_SA>(s, gf, a); break;
+            default: launch_face_eq<EQ_SST>(s, gf, a); break;
+        }
+        timer_end(s, ev);
+        timer_begin(s, "visc_cell", &ev);
+        switch (s->p.equation) {
+            case FVK_EQ_NAVIER_STOKES: k_visc_cell<EQ_NS><<<gc, kVB, 0, s->stream>>>(a); break;
+            case FVK_EQ_RANS_SA: k_visc_cell<EQ_SA><<<gc, kVB, 0, s->stream>>>(a); break;
+            default: k_visc_cell<EQ_SST><<<gc, kVB, 0, s->stream>>>(a); break;
+        }
+        timer_end(s, ev);
     }
-    timer_end(s, ev);
-    timer_begin(s, "visc_cell", &ev);
-    switch (s->p.equation) {
-        case FVK_EQ_NAVIER_STOKES: k_visc_cell<EQ_NS><<<gc, kVB, 0, s->stream>>>(a); break;
-        case FVK_EQ_RANS_SA: k_visc_cell<EQ_SA><<<gc, kVB, 0, s->stream>>>(a); break;
-        default: k_visc_cell<EQ_SST><<<gc, kVB, 0, s->stream>>>(a); break;
-    }
-    timer_end(s, ev);
     if (a.want_res) {
         if (s->p.rk_stages >= 2 && !a.R) {
             // the next stage's gradient launch reduces this stage's partials
-            s->fin_nblocks = gc;
+            s->fin_nblocks = nparts;
             s->fin_slot = a.slot;
             s->fin_hist_idx = a.hist_idx;
             s->fin_rearm = 1;
             return;
         }
         timer_begin(s, "visc_finalize", &ev);
-        launch_norm_finalize(s, gc, s->nv, a.hist_idx, a.slot, -1.0);
+        launch_norm_finalize(s, nparts, s->nv, a.hist_idx, a.slot, -1.0);
         timer_end(s, ev);
     }
 }

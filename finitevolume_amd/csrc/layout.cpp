@@ -336,6 +336,8 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     m.tmax_l = ml;
     m.tmax_f = mf;
     m.tmax_2 = m2;
+    m.twidth = width;
+    m.tslot_stride = T;
     m.tstart = upload(s, s->tile_start);
     m.thptr = upload(s, thptr);
     m.thalo = thalo.empty() ? nullptr : upload(s, thalo);
@@ -701,8 +703,17 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
         return set_error(FVK_ERR_NOMEM, "device allocation for the mesh failed");
     if (s->variant == 3) {
         size_t lds = 0;
-        int rc = build_tiles(s, cptr, cent, fl, fr, width, s->p.order == 2, egeo, clsq, fgeo, frec, &lds);
+        const bool euler = s->p.equation == FVK_EQ_EULER;
+        int rc = build_tiles(s, cptr, cent, fl, fr, width, euler && s->p.order == 2, egeo, clsq, fgeo, frec, &lds);
         if (rc != FVK_OK) return rc;
+        if (!euler) {
+            // viscous tiles: state, gradients (and SST's F1, nu_t) of the owned and
+            // ring-1 cells, then the face fluxes and dt candidates; at least the
+            // residual reduction's nv doubles per thread
+            const int nv = s->nv, nf2 = 2 * (nv - 1), nx = s->p.equation == FVK_EQ_RANS_SST ? 2 : 0;
+            const size_t words = (size_t)(nv + nf2 + nx) * m.tmax_g + (size_t)(nv + 1) * m.tmax_f;
+            lds = sizeof(double) * std::max(words, (size_t)nv * s->block);
+        }
         if (lds > kMaxTileLds)
             return set_error(FVK_ERR_UNSUPPORTED, "a tile of %d cells needs %zu B of LDS (limit %d): use smaller tiles",
                              s->block, lds, kMaxTileLds);

@@ -91,6 +91,7 @@ struct MeshView {
     int ntiles;
     int tmax_g, tmax_l, tmax_f; // most local cells with gradients (owned + ring 1), local cells, faces in a tile
     int tmax_2;                 // most ring-2 cells of a tile
+    int twidth, tslot_stride;   // slot width of tell and tslot; tslot's per-slot stride (the tile capacity bound)
     const int* tstart;      // [ntiles + 1]
     const int* thptr;       // [ntiles + 1] offsets into thalo
     const int* thalo;       // ring-1 then ring-2 cells of each tile (device ids)

@@ -193,6 +193,35 @@ def test_global_dt_variants_match_oracle():
         assert_bitwise(g, o)
 
 
+@pytest.mark.parametrize("gradient", [abi.GRAD_GREEN_GAUSS, abi.GRAD_LEAST_SQUARES])
+@pytest.mark.parametrize("equation", [abi.EQ_NAVIER_STOKES, abi.EQ_RANS_SA, abi.EQ_RANS_SST])
+def test_viscous_tile_bit_identical(equation, gradient):
+    """The fused viscous tile pass (faces and cell updates per tile, local dt)
+    against the face sweep + cell pass: the same state bits at several tile
+    sizes; residual norms to round-off (their block grouping differs).  The
+    global-dt mode keeps the split passes, and must equal them too."""
+    def run(variant, bs, local_dt):
+        if equation == abi.EQ_NAVIER_STOKES:
+            m, bcs, p, ic, _ = cases.case_ns_wavy(gradient=gradient, n=24, rk_stages=3, local_dt=local_dt)
+        else:
+            m, bcs, p, ic, _ = cases.case_rans_wavy(equation, gradient=gradient, n=24, rk_stages=3,
+                                                    local_dt=local_dt)
+        p.kernel_variant, p.block_size = variant, bs
+        g = FVKSolver(m, bcs, p, ic)
+        g.set_kernel_timing(True)
+        r = g.step(40)
+        names = g.kernel_times()
+        f = g.field()
+        return np.concatenate([x.ravel() for x in (f if isinstance(f, tuple) else (f,))]), r, names
+    for local_dt in (1, 0):
+        ref, rref, _ = run(1, 256, local_dt)
+        for bs in (64, 128, 256):
+            out, r, names = run(3, bs, local_dt)
+            assert ("visc_tile" in names) == (local_dt == 1), (bs, sorted(names))
+            assert np.array_equal(out, ref), (equation, gradient, bs, local_dt)
+            assert np.allclose(r, rref, rtol=1e-13, atol=0.0), (equation, gradient, bs, local_dt)
+
+
 @pytest.mark.parametrize("equation", [abi.EQ_NAVIER_STOKES, abi.EQ_RANS_SA, abi.EQ_RANS_SST])
 def test_viscous_rk3_local_dt(equation):
     """SSP-RK3 with local time steps on the viscous paths (as the RANS
