@@ -80,8 +80,9 @@ __device__ __forceinline__ double4 d4(const State& s) { return double4{s.r, s.ru
 // The adjoint of one face: its two states (R unused on a boundary face,
 // cr < 0 the boundary code), their limited gradients, the face geometry and
 // the lambdas of its cells.  Writes both sides' state and MUSCL-increment
-// adjoints and the edge-vector adjoint.
-template <int SCHEME, int ORDER>
+// adjoints and, for node coordinates (WX), the edge-vector adjoint; without
+// it the geometry adjoints are dead code.
+template <int SCHEME, int ORDER, bool WX>
 __device__ __forceinline__ void adj_face_body(const AdjArgs& a, int f, int cr, const State& L, const State& R,
                                               const double4& gl0, const double4& gl1, const double4& gr0,
                                               const double4& gr1, const double4& ro, const double4& fg,
@@ -126,12 +127,14 @@ __device__ __forceinline__ void adj_face_body(const AdjArgs& a, int f, int cr, c
         a.ful[f] = d4(Lfb);
         if (cr >= 0) a.fur[f] = d4(Rfb);
     }
-    double exb = 0.0, eyb = 0.0;
-    edge_rev(nx, ny, A, nxb, nyb, Ab, exb, eyb);
-    a.fe[f] = double2{exb, eyb};
+    if (WX) {
+        double exb = 0.0, eyb = 0.0;
+        edge_rev(nx, ny, A, nxb, nyb, Ab, exb, eyb);
+        a.fe[f] = double2{exb, eyb};
+    }
 }
 
-template <int SCHEME, int ORDER>
+template <int SCHEME, int ORDER, bool WX>
 __global__ __launch_bounds__(kAdjBlock) void k_adj_face(AdjArgs a) {
     const MeshView& m = a.m;
     const int f = blockIdx.x * kAdjBlock + threadIdx.x;
@@ -156,7 +159,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_face(AdjArgs a) {
         }
         lr = a.lam[cr];
     }
-    adj_face_body<SCHEME, ORDER>(a, f, cr, L, R, gl0, gl1, gr0, gr1, ro, fg, ll, lr);
+    adj_face_body<SCHEME, ORDER, WX>(a, f, cr, L, R, gl0, gl1, gr0, gr1, ro, fg, ll, lr);
 }
 
 // ---------------------------------------------------------------------------
@@ -751,10 +754,15 @@ void launch_cells(fvk_solver* s, const AdjArgs& a, bool wx) {
 }
 
 template <int ORDER>
-void launch_faces(fvk_solver* s, const AdjArgs& a) {
+void launch_faces(fvk_solver* s, const AdjArgs& a, bool wx) {
     const int gf = grid_of(s->nf);
-    if (s->p.flux_scheme == FLUX_HLLC) k_adj_face<FLUX_HLLC, ORDER><<<gf, kAdjBlock, 0, s->stream>>>(a);
-    else k_adj_face<FLUX_RUSANOV, ORDER><<<gf, kAdjBlock, 0, s->stream>>>(a);
+    if (s->p.flux_scheme == FLUX_HLLC) {
+        if (wx) k_adj_face<FLUX_HLLC, ORDER, true><<<gf, kAdjBlock, 0, s->stream>>>(a);
+        else k_adj_face<FLUX_HLLC, ORDER, false><<<gf, kAdjBlock, 0, s->stream>>>(a);
+    } else {
+        if (wx) k_adj_face<FLUX_RUSANOV, ORDER, true><<<gf, kAdjBlock, 0, s->stream>>>(a);
+        else k_adj_face<FLUX_RUSANOV, ORDER, false><<<gf, kAdjBlock, 0, s->stream>>>(a);
+    }
 }
 
 }  // namespace
@@ -804,8 +812,8 @@ int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_ba
         if (rc != FVK_OK) return rc;
     }
     timer_begin(s, "adj_face", &ev);
-    if (a.order == 2) launch_faces<2>(s, a);
-    else launch_faces<1>(s, a);
+    if (a.order == 2) launch_faces<2>(s, a, wx);
+    else launch_faces<1>(s, a, wx);
     timer_end(s, ev);
     switch (w->D) {
         case 3: launch_cells<3>(s, a, wx); break;

@@ -620,7 +620,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     L.MG = m.tmax_g;
     L.M2 = m.tmax_2;
     L.MF = m.tmax_f;
-    const int MG = L.MG, M2 = L.M2, MF = L.MF;
+    const int MG = L.MG, MF = L.MF;
     L.U = lds;
     if (ORDER == 2) {
         L.G = lds + 4 * MG;
