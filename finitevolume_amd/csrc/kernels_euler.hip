@@ -656,7 +656,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
 #pragma unroll
         for (int k = 0; k < D; ++k) pen[k] = te[k * MG + tid];
     }
-    const double4* tg = m.tegeo + (int64_t)t * D * MG;  // tile-ordered geometry: no index load first
+    // tile-ordered geometry (second order only): no index load first
+    const double4* tg = ORDER == 2 ? m.tegeo + (int64_t)t * D * MG : nullptr;
     if (ORDER == 2 && tid < ng) {
 #pragma unroll
         for (int k = 0; k < D; ++k) peg[k] = tg[k * MG + tid];
