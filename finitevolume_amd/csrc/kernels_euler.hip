@@ -547,12 +547,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
 
 // flux x length of tile face e (as k_e_face), from LDS; returns its dt candidate
 template <int SCHEME, int ORDER>
-__device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L, int e, const int4& fd,
+__device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L, int e, int fd,
                                             const double4& fg, const double4& ro) {
     const MeshView& m = a.m;
     const double g = a.gamma;
     const int MG = L.MG, MF = L.MF;
-    const int l = fd.y, r = fd.z;
+    const int l = tface_left(fd), r = tface_right(fd);
     const State Ls{L.U[l], L.U[MG + l], L.U[2 * MG + l], L.U[3 * MG + l]};
     State R = Ls, Lf = Ls, Rf;
     if (ORDER == 2) {
@@ -644,13 +644,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     }
     const int ng = n + h1, nl = n + nh;
     L.ng = ng;
-    const int2* te = m.tell + (int64_t)t * D * MG;
+    const int* te = m.tell + (int64_t)t * D * MG;
 
     // 0. loads issued before the staging barrier, so that their latency
     // overlaps it: the local ELL of cell tid (its gradient, and its gather
     // when owned), the gradient mesh data of cell tid, the mesh data of the
     // first two faces of this thread
-    int2 pen[D];
+    int pen[D];
     double4 peg[D], piv{0, 0, 0, 0};
     if (tid < (ORDER == 2 ? ng : n)) {
 #pragma unroll
@@ -663,12 +663,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
         for (int k = 0; k < D; ++k) peg[k] = tg[k * MG + tid];
         piv = m.tclsq[(int64_t)t * MG + tid];
     }
-    int4 pfd[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    int pfd[2] = {0, 0};
     double4 pfg[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, pro[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         if (tid + i * bs < nft) {
-            pfd[i] = m.tface[f0 + tid + i * bs];
+            pfd[i] = m.tfc[f0 + tid + i * bs];
             pfg[i] = m.tfgeo[f0 + tid + i * bs];
             if (ORDER == 2) pro[i] = m.tfrec[f0 + tid + i * bs];
         }
@@ -694,7 +694,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     if (ORDER == 2) {
         if (tid < ng) tile_grad<D>(m, g, a.limiter, L, tid, pen, peg, piv);
         for (int j = tid + bs; j < ng; j += bs) {
-            int2 en[D];
+            int en[D];
             double4 eg[D];
 #pragma unroll
             for (int k = 0; k < D; ++k) en[k] = te[k * MG + j];
@@ -718,10 +718,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     // 3. fluxes of the tile's faces
     double cand_min = DBL_MAX;
     for (int i = 0, e = tid; e < nft; ++i, e += bs) {
-        int4 fd = i == 0 ? pfd[0] : pfd[1];
+        int fd = i == 0 ? pfd[0] : pfd[1];
         double4 fg = i == 0 ? pfg[0] : pfg[1], ro = i == 0 ? pro[0] : pro[1];
         if (i >= 2) {
-            fd = m.tface[f0 + e];
+            fd = m.tfc[f0 + e];
             fg = m.tfgeo[f0 + e];
             if (ORDER == 2) ro = m.tfrec[f0 + e];
         }
@@ -735,9 +735,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
         double dmin = DBL_MAX;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            if (pen[k].x < 0) continue;
+            if (pen[k] == -1) continue;
             const double* F = L.F + lf[k];
-            if (pen[k].x & 1) {
+            if (tell_owner(pen[k])) {
                 r0 = r0 - F[0]; r1 = r1 - F[MF]; r2 = r2 - F[2 * MF]; r3 = r3 - F[3 * MF];
             } else {
                 r0 += F[0]; r1 += F[MF]; r2 += F[2 * MF]; r3 += F[3 * MF];

@@ -694,14 +694,14 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) r[q] = 0.0;
     if (tid < n) {
-        const int2* te = m.tell + (int64_t)t * W * MG;
+        const int* te = m.tell + (int64_t)t * W * MG;
         const short* ts = m.tslot + (int64_t)t * W * m.tslot_stride;
         double dmin = DBL_MAX;
         for (int k = 0; k < W; ++k) {
-            const int2 en = te[k * MG + tid];
-            if (en.x < 0) continue;  // ELL padding (after the cell's faces)
+            const int en = te[k * MG + tid];
+            if (en == -1) continue;  // ELL padding (after the cell's faces)
             const int lf = ts[k * m.tslot_stride + tid];
-            if (en.x & 1) {
+            if (tell_owner(en)) {
 #pragma unroll
                 for (int q = 0; q < NV; ++q) r[q] = r[q] - F[q * MF + lf];
             } else {

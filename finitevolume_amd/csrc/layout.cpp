@@ -276,8 +276,12 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
         mf = std::max(mf, nft);
         for (int c : cells) loc[c] = -1;
     }
+    // the packed face cells hold local indices and patch codes in 16 bits each
+    if (mg > 32767 || s->n_bcs > 32767)
+        return set_error(FVK_ERR_INVALID, "tiles of %d staged cells or %d patches exceed the packed face format", mg,
+                         s->n_bcs);
     // pass 2: tile-local ELL and owned slots (need the maxima for the strides)
-    std::vector<int2> tell((size_t)nt * width * mg, int2{-1, -1});
+    std::vector<int> tell((size_t)nt * width * mg, -1);
     std::vector<double4> tegeo, tclsq;
     if (second_order) {
         tegeo.assign((size_t)nt * width * mg, double4{0.0, 0.0, 0.0, 0.0});
@@ -313,7 +317,7 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
                     if (nb < 0) return set_error(FVK_ERR_INVALID, "tile %d: a neighbour is outside its halo", t);
                 }
                 if (nb >= ti.n + ti.h1 + ti.h2) return set_error(FVK_ERR_INVALID, "tile %d: local index out of range", t);
-                tell[((size_t)t * width + slot) * mg + j] = int2{e, nb};
+                tell[((size_t)t * width + slot) * mg + j] = tell_pack(e, nb, f);
                 if (j < ti.n) {
                     const int lf = fmark[f];
                     if (lf < 0 || lf >= tfptr[t + 1] - tfptr[t])
@@ -345,6 +349,9 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     m.tell = upload(s, tell);
     m.tfptr = upload(s, tfptr);
     m.tface = upload(s, tface);
+    std::vector<int> tfc(tface.size());
+    for (size_t e = 0; e < tface.size(); ++e) tfc[e] = tface_pack(tface[e].y, tface[e].z);
+    m.tfc = upload(s, tfc);
     m.tslot = upload(s, tslot);
     std::vector<double4> tfgeo(tface.size()), tfrec;
     for (size_t e = 0; e < tface.size(); ++e) tfgeo[e] = fgeo[tface[e].x];
@@ -356,7 +363,8 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     m.tfrec = second_order ? upload(s, tfrec) : nullptr;
     m.tegeo = second_order ? upload(s, tegeo) : nullptr;
     m.tclsq = second_order ? upload(s, tclsq) : nullptr;
-    if (!m.tstart || !m.thptr || (!thalo.empty() && !m.thalo) || !m.th1 || !m.tell || !m.tfptr || !m.tface || !m.tslot ||
+    if (!m.tstart || !m.thptr || (!thalo.empty() && !m.thalo) || !m.th1 || !m.tell || !m.tfptr || !m.tface || !m.tfc ||
+        !m.tslot ||
         !m.tfgeo || (second_order && (!m.tfrec || !m.tegeo || !m.tclsq)))
         return set_error(FVK_ERR_NOMEM, "device allocation for the tiles failed");
     // second order: U (4), G (8) and W (4) of the owned and ring-1 cells, then

@@ -96,16 +96,29 @@ struct MeshView {
     const int* thptr;       // [ntiles + 1] offsets into thalo
     const int* thalo;       // ring-1 then ring-2 cells of each tile (device ids)
     const int* th1;         // [ntiles] ring-1 count
-    const int2* tell;       // [ntiles][width][tmax_g] per local cell with gradients, slot k:
-                            //   {face << 1 | owner, local neighbour or -1 - patch}; x = -1 pads
+    const int* tell;        // [ntiles][width][tmax_g] per local cell with gradients, slot k, packed
+                            //   (tell_pack): local neighbour << 1 | owner; -2 - device face on a boundary
+                            //   (the cell owns it); -1 pads
     const int* tfptr;       // [ntiles + 1] offsets into tface
     const int4* tface;      // per tile face: {device face, local cell_left, local cell_right or -1 - patch, 0}
+    const int* tfc;         // per tile face, packed (tface_pack): local cell_left in the low 16 bits,
+                            //   local cell_right or -1 - patch as a signed high half
     const short* tslot;     // [ntiles][width][tile size] tile-local face index of each owned cell's slot
     // static geometry in tile order: egeo and clsq of each tile's owned and
     // ring-1 cells ([ntiles][width][tmax_g], [ntiles][tmax_g]; second
     // order), fgeo and frec of each tile face (as tface)
     const double4 *tegeo, *tclsq, *tfgeo, *tfrec;
 };
+
+// packed tile ELL entry (MeshView::tell)
+__host__ __device__ inline int tell_pack(int entry, int nb, int face) {
+    return nb >= 0 ? (nb << 1) | (entry & 1) : -2 - face;
+}
+__host__ __device__ inline bool tell_owner(int p) { return p < -1 || (p & 1); }  // boundary faces belong to their cell
+// packed tile face cells (MeshView::tfc)
+__host__ __device__ inline int tface_pack(int l, int r) { return (int)(((unsigned)r << 16) | (unsigned)(l & 0xffff)); }
+__host__ __device__ inline int tface_left(int p) { return p & 0xffff; }
+__host__ __device__ inline int tface_right(int p) { return p >> 16; }
 
 struct AdjointWork;  // device buffers and index maps of the residual VJP (adjoint.hip)
 

@@ -26,7 +26,7 @@ struct TileLds {
 // limited LSQ gradient of local cell j (as the per-cell gradient kernel), from LDS
 template <int D>
 __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limiter, const TileLds& L, int j,
-                                          const int2 (&en)[D], const double4 (&eg)[D], const double4& iv) {
+                                          const int (&en)[D], const double4 (&eg)[D], const double4& iv) {
     const int MG = L.MG;
     const State Uc{L.U[j], L.U[MG + j], L.U[2 * MG + j], L.U[3 * MG + j]};
     const double wc[4] = {L.W[j], L.W[MG + j], L.W[2 * MG + j], L.W[3 * MG + j]};
@@ -36,11 +36,12 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
     for (int q = 0; q < 4; ++q) wmin[q] = wmax[q] = wc[q];
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        if (en[k].x < 0) continue;
+        if (en[k] == -1) continue;
         double wn[4];
-        if (en[k].y < 0) {
-            const double4 fg = m.fgeo[en[k].x >> 1];
-            const DevBC& bc = m.bc[-1 - en[k].y];
+        if (en[k] < -1) {  // boundary face: the ghost state
+            const int f = -2 - en[k];
+            const double4 fg = m.fgeo[f];
+            const DevBC& bc = m.bc[-1 - m.fr[f]];
             const State Gs = euler_ghost_state(bc.type, bc.ff, Uc, fg.x, fg.y, g);
             const double wg0 = Gs.r, wg1 = fdiv(Gs.ru, Gs.r), wg2 = fdiv(Gs.rv, Gs.r), wg3 = pressure(Gs, g);
             wn[0] = 0.5 * (wc[0] + wg0);
@@ -48,7 +49,7 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
             wn[2] = 0.5 * (wc[2] + wg2);
             wn[3] = 0.5 * (wc[3] + wg3);
         } else {
-            const int l = en[k].y;
+            const int l = en[k] >> 1;
             const bool r1 = l < L.ng;
             const double* w = r1 ? L.W + l : L.W2 + (l - L.ng);
             const int sw = r1 ? MG : L.M2;
@@ -80,7 +81,7 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
         for (int q = 0; q < 4; ++q) vm[q] = VenkatMin{1.0, 1.0};
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            if (en[k].x < 0) continue;
+            if (en[k] == -1) continue;
             const double rx = eg[k].z, ry = eg[k].w;
 #pragma unroll
             for (int q = 0; q < 4; ++q) venkat_pick(vm[q], wmin[q], wmax[q], wc[q], gx[q] * rx + gy[q] * ry, eps2);
