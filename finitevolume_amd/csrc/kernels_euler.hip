@@ -548,7 +548,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
 // flux x length of tile face e (as k_e_face), from LDS; returns its dt candidate
 template <int SCHEME, int ORDER>
 __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L, int e, int fd,
-                                            const double4& fg, const double4& ro) {
+                                            const double4& fg) {
     const MeshView& m = a.m;
     const double g = a.gamma;
     const int MG = L.MG, MF = L.MF;
@@ -556,18 +556,15 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
     const State Ls{L.U[l], L.U[MG + l], L.U[2 * MG + l], L.U[3 * MG + l]};
     State R = Ls, Lf = Ls, Rf;
     if (ORDER == 2) {
-        const double* gl = L.G + l;
+        const double* in = L.I + e;
         const double* wl = L.W + l;
-        Lf = reconstruct_w(Ls, double4{wl[0], wl[MG], wl[2 * MG], wl[3 * MG]},
-                           double4{gl[0], gl[MG], gl[2 * MG], gl[3 * MG]},
-                           double4{gl[4 * MG], gl[5 * MG], gl[6 * MG], gl[7 * MG]}, ro.x, ro.y, g);
+        Lf = reconstruct_inc(Ls, double4{wl[0], wl[MG], wl[2 * MG], wl[3 * MG]}, in[0], in[MF], in[2 * MF],
+                             in[3 * MF], g);
         if (r >= 0) {
             R = State{L.U[r], L.U[MG + r], L.U[2 * MG + r], L.U[3 * MG + r]};
-            const double* gr = L.G + r;
             const double* wr = L.W + r;
-            Rf = reconstruct_w(R, double4{wr[0], wr[MG], wr[2 * MG], wr[3 * MG]},
-                               double4{gr[0], gr[MG], gr[2 * MG], gr[3 * MG]},
-                               double4{gr[4 * MG], gr[5 * MG], gr[6 * MG], gr[7 * MG]}, ro.z, ro.w, g);
+            Rf = reconstruct_inc(R, double4{wr[0], wr[MG], wr[2 * MG], wr[3 * MG]}, in[4 * MF], in[5 * MF],
+                                 in[6 * MF], in[7 * MF], g);
         } else {
             Rf = euler_ghost(m.bc[-1 - r], Lf, fg.x, fg.y, g);
         }
@@ -596,9 +593,10 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
 // ---------------------------------------------------------------------------
 // fused cell-tile pass (variant 3): one workgroup per tile of graph-adjacent
 // cells.  The tile's states and its halo (one ring for the fluxes, a second
-// for the neighbours' gradients) are staged in LDS; gradients of the owned
-// cells and the first ring, the fluxes of every face of the owned cells and
-// the gather + update then read LDS only.  Every value is computed by the
+// for the neighbours' gradients) are staged in LDS.  The gradients of the
+// owned cells and the first ring leave each tile face's MUSCL increments in
+// LDS; the fluxes of every face of the owned cells and the gather + update
+// then read LDS only.  Every value is computed by the
 // same operations as in the face sweep + gather kernels, so the states are
 // bit-identical; faces on a tile's edge are evaluated by both tiles.  The
 // mesh data of each thread's first gradient cell and first face are loaded
@@ -623,12 +621,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     const int MG = L.MG, MF = L.MF;
     L.U = lds;
     if (ORDER == 2) {
-        L.G = lds + 4 * MG;
-        L.W = lds + 12 * MG;
-        L.W2 = lds + 16 * MG;
-        L.F = L.W2;
+        L.W = lds + 4 * MG;
+        L.W2 = lds + 8 * MG;
+        L.I = L.W2 + 4 * L.M2;
+        L.F = L.I;
     } else {
-        L.G = L.W = L.W2 = nullptr;
+        L.W = L.W2 = L.I = nullptr;
         L.F = lds + 4 * MG;
     }
     L.Dt = L.F + 4 * MF;
@@ -647,14 +645,19 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     const int* te = m.tell + (int64_t)t * D * MG;
 
     // 0. loads issued before the staging barrier, so that their latency
-    // overlaps it: the local ELL of cell tid (its gradient, and its gather
-    // when owned), the gradient mesh data of cell tid, the mesh data of the
-    // first two faces of this thread
-    int pen[D];
+    // overlaps it: the local ELL and face slots of cell tid (its gradient, and
+    // its gather when owned), the gradient mesh data of cell tid, the mesh
+    // data of the first two faces of this thread
+    const int TS = m.tslot_stride;
+    const short* ts = m.tslot + (int64_t)t * D * TS;
+    int pen[D], plf[D];
     double4 peg[D], piv{0, 0, 0, 0};
     if (tid < (ORDER == 2 ? ng : n)) {
 #pragma unroll
-        for (int k = 0; k < D; ++k) pen[k] = te[k * MG + tid];
+        for (int k = 0; k < D; ++k) {
+            pen[k] = te[k * MG + tid];
+            plf[k] = ts[k * TS + tid];
+        }
     }
     // tile-ordered geometry (second order only): no index load first
     const double4* tg = ORDER == 2 ? m.tegeo + (int64_t)t * D * MG : nullptr;
@@ -664,13 +667,12 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
         piv = m.tclsq[(int64_t)t * MG + tid];
     }
     int pfd[2] = {0, 0};
-    double4 pfg[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, pro[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    double4 pfg[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
         if (tid + i * bs < nft) {
             pfd[i] = m.tfc[f0 + tid + i * bs];
             pfg[i] = m.tfgeo[f0 + tid + i * bs];
-            if (ORDER == 2) pro[i] = m.tfrec[f0 + tid + i * bs];
         }
     }
 
@@ -690,42 +692,40 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     for (int j = tid + 2 * bs; j < nl; j += bs) stage_cell(j, a.U[j < n ? c0 + j : m.thalo[hp + j - n]]);
     __syncthreads();
 
-    // 2. limited LSQ gradients of the owned cells and ring 1
+    // 2. limited LSQ gradients of the owned cells and ring 1, and the MUSCL
+    // increments of the tile faces on their sides
     if (ORDER == 2) {
-        if (tid < ng) tile_grad<D>(m, g, a.limiter, L, tid, pen, peg, piv);
+        const double2* rec = m.trec ? m.trec + (int64_t)t * D * MG : nullptr;
+        if (tid < ng) tile_grad<D>(m, g, a.limiter, L, tid, pen, peg, piv, plf, rec ? rec + tid : nullptr);
         for (int j = tid + bs; j < ng; j += bs) {
-            int en[D];
+            int en[D], lf[D];
             double4 eg[D];
 #pragma unroll
-            for (int k = 0; k < D; ++k) en[k] = te[k * MG + j];
+            for (int k = 0; k < D; ++k) {
+                en[k] = te[k * MG + j];
+                lf[k] = ts[k * TS + j];
+            }
 #pragma unroll
             for (int k = 0; k < D; ++k) eg[k] = tg[k * MG + j];
-            tile_grad<D>(m, g, a.limiter, L, j, en, eg, m.tclsq[(int64_t)t * MG + j]);
+            tile_grad<D>(m, g, a.limiter, L, j, en, eg, m.tclsq[(int64_t)t * MG + j], lf, rec ? rec + j : nullptr);
         }
     }
-    __syncthreads();  // ring 2's primitives are dead from here: their space takes the fluxes
+    __syncthreads();
 
-    // the gather's face slots and the update's inputs of owned cell tid
-    const short* ts = m.tslot + (int64_t)t * D * bs;
-    int lf[D];
+    // the update's inputs of owned cell tid
     UpdateInputs upd{};
-    if (tid < n) {
-#pragma unroll
-        for (int k = 0; k < D; ++k) lf[k] = ts[k * bs + tid];
-        upd = load_update_inputs(a, c0 + tid);
-    }
+    if (tid < n) upd = load_update_inputs(a, c0 + tid);
 
     // 3. fluxes of the tile's faces
     double cand_min = DBL_MAX;
     for (int i = 0, e = tid; e < nft; ++i, e += bs) {
         int fd = i == 0 ? pfd[0] : pfd[1];
-        double4 fg = i == 0 ? pfg[0] : pfg[1], ro = i == 0 ? pro[0] : pro[1];
+        double4 fg = i == 0 ? pfg[0] : pfg[1];
         if (i >= 2) {
             fd = m.tfc[f0 + e];
             fg = m.tfgeo[f0 + e];
-            if (ORDER == 2) ro = m.tfrec[f0 + e];
         }
-        cand_min = smin(cand_min, tile_face<SCHEME, ORDER>(a, L, e, fd, fg, ro));
+        cand_min = smin(cand_min, tile_face<SCHEME, ORDER>(a, L, e, fd, fg));
     }
     __syncthreads();
 
@@ -736,13 +736,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             if (pen[k] == -1) continue;
-            const double* F = L.F + lf[k];
+            const double* F = L.F + plf[k];
             if (tell_owner(pen[k])) {
                 r0 = r0 - F[0]; r1 = r1 - F[MF]; r2 = r2 - F[2 * MF]; r3 = r3 - F[3 * MF];
             } else {
                 r0 += F[0]; r1 += F[MF]; r2 += F[2 * MF]; r3 += F[3 * MF];
             }
-            if (a.dt_src == 2) dmin = smin(dmin, L.Dt[lf[k]]);
+            if (a.dt_src == 2) dmin = smin(dmin, L.Dt[plf[k]]);
         }
         const double4 u{L.U[tid], L.U[MG + tid], L.U[2 * MG + tid], L.U[3 * MG + tid]};
         update_cell_with(a, c0 + tid, upd, u, r0, r1, r2, r3, dmin);

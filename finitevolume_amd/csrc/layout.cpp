@@ -218,9 +218,9 @@ void tile_order(fvk_solver* s, int n, const std::vector<int>& aptr, const std::v
 // is also copied in tile order, so its loads are contiguous and need no
 // index loaded first.  Returns the LDS bytes a tile needs at most.
 int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<int>& cent, const std::vector<int>& fl,
-                const std::vector<int>& fr, int width, bool second_order, const std::vector<double4>& egeo,
-                const std::vector<double4>& clsq, const std::vector<double4>& fgeo, const std::vector<double4>& frec,
-                size_t* lds_bytes) {
+                const std::vector<int>& fr, int width, bool second_order, bool periodic,
+                const std::vector<double4>& egeo, const std::vector<double4>& clsq,
+                const std::vector<double4>& fgeo, const std::vector<double4>& frec, size_t* lds_bytes) {
     const int nt = (int)s->tile_start.size() - 1;
     const int T = s->block;  // tslot stride: the workgroup size (tiles hold at most tile_cells <= T cells)
     const int64_t nc = s->nc, nf = s->nf;
@@ -280,15 +280,17 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     if (mg > 32767 || s->n_bcs > 32767)
         return set_error(FVK_ERR_INVALID, "tiles of %d staged cells or %d patches exceed the packed face format", mg,
                          s->n_bcs);
-    // pass 2: tile-local ELL and owned slots (need the maxima for the strides)
+    // pass 2: tile-local ELL and face slots (need the maxima for the strides)
     std::vector<int> tell((size_t)nt * width * mg, -1);
     std::vector<double4> tegeo, tclsq;
+    std::vector<double2> trec;
     if (second_order) {
         tegeo.assign((size_t)nt * width * mg, double4{0.0, 0.0, 0.0, 0.0});
         tclsq.assign((size_t)nt * mg, double4{0.0, 0.0, 0.0, 0.0});
+        if (periodic) trec.assign((size_t)nt * width * mg, double2{0.0, 0.0});
     }
     const int64_t ld = s->mesh.ld;
-    std::vector<short> tslot((size_t)nt * width * T, (short)-1);
+    std::vector<short> tslot((size_t)nt * width * mg, (short)-1);
     for (int t = 0; t < nt; ++t) {
         const int c0 = s->tile_start[t];
         const TileInfo& ti = info[t];
@@ -296,6 +298,7 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
         for (int j = 0; j < ti.n; ++j) cells.push_back(c0 + j);
         for (int j = thptr[t]; j < thptr[t + 1]; ++j) cells.push_back(thalo[j]);
         for (size_t j = 0; j < cells.size(); ++j) loc[cells[j]] = (int)j;
+        const int nft = tfptr[t + 1] - tfptr[t];
         for (int e = tfptr[t]; e < tfptr[t + 1]; ++e) fmark[tface[e].x] = e - tfptr[t];
         // first order reads the local entries of the owned cells only (no ring-2 halo)
         const int nell = second_order ? ti.n + ti.h1 : ti.n;
@@ -318,11 +321,18 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
                 }
                 if (nb >= ti.n + ti.h1 + ti.h2) return set_error(FVK_ERR_INVALID, "tile %d: local index out of range", t);
                 tell[((size_t)t * width + slot) * mg + j] = tell_pack(e, nb, f);
-                if (j < ti.n) {
-                    const int lf = fmark[f];
-                    if (lf < 0 || lf >= tfptr[t + 1] - tfptr[t])
-                        return set_error(FVK_ERR_INVALID, "tile %d: face slot out of range", t);
-                    tslot[((size_t)t * width + slot) * T + j] = (short)lf;
+                // the slot's face in the tile's list: every face of an owned cell,
+                // and a ring-1 cell's faces towards the owned cells
+                const int lf = fmark[f];
+                const bool ours = lf >= 0 && lf < nft && tface[tfptr[t] + lf].x == f;
+                if (j < ti.n && !ours) return set_error(FVK_ERR_INVALID, "tile %d: face slot out of range", t);
+                if (ours) tslot[((size_t)t * width + slot) * mg + j] = (short)lf;
+                // periodic: the MUSCL offset of the face's side (frec), which for
+                // cell_right differs from the limiter offset in rounding
+                if (second_order && periodic) {
+                    const double4 fr4 = frec[f];
+                    trec[((size_t)t * width + slot) * mg + j] =
+                        (e & 1) ? double2{fr4.x, fr4.y} : double2{fr4.z, fr4.w};
                 }
             }
         }
@@ -341,7 +351,7 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     m.tmax_f = mf;
     m.tmax_2 = m2;
     m.twidth = width;
-    m.tslot_stride = T;
+    m.tslot_stride = mg;
     m.tstart = upload(s, s->tile_start);
     m.thptr = upload(s, thptr);
     m.thalo = thalo.empty() ? nullptr : upload(s, thalo);
@@ -353,25 +363,21 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     for (size_t e = 0; e < tface.size(); ++e) tfc[e] = tface_pack(tface[e].y, tface[e].z);
     m.tfc = upload(s, tfc);
     m.tslot = upload(s, tslot);
-    std::vector<double4> tfgeo(tface.size()), tfrec;
+    std::vector<double4> tfgeo(tface.size());
     for (size_t e = 0; e < tface.size(); ++e) tfgeo[e] = fgeo[tface[e].x];
-    if (second_order) {
-        tfrec.resize(tface.size());
-        for (size_t e = 0; e < tface.size(); ++e) tfrec[e] = frec[tface[e].x];
-    }
     m.tfgeo = upload(s, tfgeo);
-    m.tfrec = second_order ? upload(s, tfrec) : nullptr;
     m.tegeo = second_order ? upload(s, tegeo) : nullptr;
     m.tclsq = second_order ? upload(s, tclsq) : nullptr;
+    m.trec = second_order && periodic ? upload(s, trec) : nullptr;
     if (!m.tstart || !m.thptr || (!thalo.empty() && !m.thalo) || !m.th1 || !m.tell || !m.tfptr || !m.tface || !m.tfc ||
         !m.tslot ||
-        !m.tfgeo || (second_order && (!m.tfrec || !m.tegeo || !m.tclsq)))
+        !m.tfgeo || (second_order && (!m.tegeo || !m.tclsq || (periodic && !m.trec))))
         return set_error(FVK_ERR_NOMEM, "device allocation for the tiles failed");
-    // second order: U (4), G (8) and W (4) of the owned and ring-1 cells, then
-    // W (4) of ring 2, whose space the face fluxes (4) and dt candidates (1)
-    // take after the gradients; first order: U (4) of the owned and ring-1
-    // cells, then the fluxes and dt candidates
-    const size_t words = second_order ? (size_t)16 * mg + std::max((size_t)4 * m2, (size_t)5 * mf)
+    // second order: U (4) and W (4) of the owned and ring-1 cells, W (4) of
+    // ring 2, then the MUSCL increments (4 per side) of each tile face, which
+    // each face's fluxes (4) and dt candidate (1) replace; first order: U (4)
+    // of the owned and ring-1 cells, then the fluxes and dt candidates
+    const size_t words = second_order ? (size_t)8 * mg + (size_t)4 * m2 + (size_t)8 * mf
                                       : (size_t)4 * mg + (size_t)5 * mf;
     *lds_bytes = sizeof(double) * std::max(words, (size_t)4 * T);  // at least the block reductions' 4 T doubles
     return FVK_OK;
@@ -712,7 +718,8 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
     if (s->variant == 3) {
         size_t lds = 0;
         const bool euler = s->p.equation == FVK_EQ_EULER;
-        int rc = build_tiles(s, cptr, cent, fl, fr, width, euler && s->p.order == 2, egeo, clsq, fgeo, frec, &lds);
+        int rc = build_tiles(s, cptr, cent, fl, fr, width, euler && s->p.order == 2, sh != nullptr, egeo, clsq,
+                             fgeo, frec, &lds);
         if (rc != FVK_OK) return rc;
         if (!euler) {
             // viscous tiles: state, gradients (and SST's F1, nu_t) of the owned and

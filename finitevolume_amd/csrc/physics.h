@@ -265,15 +265,12 @@ FVK_HD State reconstruct(const State& Uc, const double4& g0, const double4& g1, 
     return from_primitive(rho, u + d1, v + d2, pp, gam);
 }
 
-// reconstruct() with the cell's primitives w = (rho, u, v, p) given, as
-// computed from Uc by the same operations (u = ru / rho, v = rv / rho,
-// p = pressure(Uc)): the same result bits without the divisions.
-FVK_HD State reconstruct_w(const State& Uc, const double4& w, const double4& g0, const double4& g1, double rx,
-                           double ry, double gam) {
-    const double d0 = g0.x * rx + g0.y * ry;
-    const double d1 = g0.z * rx + g0.w * ry;
-    const double d2 = g1.x * rx + g1.y * ry;
-    const double d3 = g1.z * rx + g1.w * ry;
+// reconstruct() from its increments d_q = G_q . r, with the cell's
+// primitives w = (rho, u, v, p) given, as computed from Uc by the same
+// operations (u = ru / rho, v = rv / rho, p = pressure(Uc)): the same result
+// bits without the divisions.
+FVK_HD State reconstruct_inc(const State& Uc, const double4& w, double d0, double d1, double d2, double d3,
+                             double gam) {
     if (d0 == 0.0 && d1 == 0.0 && d2 == 0.0 && d3 == 0.0) return Uc;
     const double rho = Uc.r + d0, pp = w.w + d3;
     if (!(rho > 0.0) || !(pp > 0.0)) return Uc;

@@ -91,7 +91,7 @@ struct MeshView {
     int ntiles;
     int tmax_g, tmax_l, tmax_f; // most local cells with gradients (owned + ring 1), local cells, faces in a tile
     int tmax_2;                 // most ring-2 cells of a tile
-    int twidth, tslot_stride;   // slot width of tell and tslot; tslot's per-slot stride (the tile capacity bound)
+    int twidth, tslot_stride;   // slot width of tell and tslot; tslot's per-slot stride (tmax_g)
     const int* tstart;      // [ntiles + 1]
     const int* thptr;       // [ntiles + 1] offsets into thalo
     const int* thalo;       // ring-1 then ring-2 cells of each tile (device ids)
@@ -103,11 +103,16 @@ struct MeshView {
     const int4* tface;      // per tile face: {device face, local cell_left, local cell_right or -1 - patch, 0}
     const int* tfc;         // per tile face, packed (tface_pack): local cell_left in the low 16 bits,
                             //   local cell_right or -1 - patch as a signed high half
-    const short* tslot;     // [ntiles][width][tile size] tile-local face index of each owned cell's slot
+    const short* tslot;     // [ntiles][width][tslot_stride] per local cell with ELL entries, slot k: the
+                            //   tile-local index of the slot's face, -1 if it is not a tile face (a ring-1
+                            //   cell's faces away from the owned cells)
     // static geometry in tile order: egeo and clsq of each tile's owned and
     // ring-1 cells ([ntiles][width][tmax_g], [ntiles][tmax_g]; second
-    // order), fgeo and frec of each tile face (as tface)
-    const double4 *tegeo, *tclsq, *tfgeo, *tfrec;
+    // order), fgeo of each tile face (as tface); on a periodic mesh, the
+    // MUSCL offset (frec) of each slot's side of its face (trec, as tegeo:
+    // for cell_right it differs in rounding from egeo's midpoint offset)
+    const double4 *tegeo, *tclsq, *tfgeo;
+    const double2* trec;
 };
 
 // packed tile ELL entry (MeshView::tell)

@@ -1,6 +1,6 @@
-// Cell tiles in LDS, shared by the fused Euler stage and the fused adjoint:
-// a tile is a range of graph-adjacent cells (the owned cells) with two halo
-// rings; its states, primitives and limited gradients are staged in LDS
+// Cell tiles in LDS for the fused Euler stage: a tile is a range of
+// graph-adjacent cells (the owned cells) with two halo rings; its states and
+// primitives, and the MUSCL increments of its faces, are staged in LDS
 // structure-of-arrays, so every neighbour access inside a tile is an LDS read.
 #pragma once
 
@@ -14,19 +14,25 @@ namespace fvk {
 // do not conflict on the banks)
 struct TileLds {
     double* U;   // [4][MG] conserved state, owned cells then ring 1
-    double* G;   // [8][MG] limited gradients (second order)
     double* W;   // [4][MG] primitives (rho, u, v, p) of the owned and ring-1 cells (second order)
-    double* W2;  // [4][M2] primitives of ring 2 (dead after the gradients)
-    double* F;   // [4][MF] face flux x length (in W2's space for second order)
-    double* Dt;  // [MF] dt candidates
+    double* W2;  // [4][M2] primitives of ring 2 (second order)
+    double* I;   // [8][MF] second order: per tile face, the primitive increments G . r of cell_left
+                 //   (rows 0-3) and cell_right (rows 4-7), written by the gradients of the two cells
+    double* F;   // [4][MF] face flux x length (second order: over I's rows 0-3, each face's own column)
+    double* Dt;  // [MF] dt candidates (second order: over I's row 4)
     int MG, M2, MF;
     int ng;      // this tile's owned + ring-1 cells: local indices below it are in W, the rest in W2
 };
 
-// limited LSQ gradient of local cell j (as the per-cell gradient kernel), from LDS
+// limited LSQ gradient of local cell j (as the per-cell gradient kernel),
+// from LDS; then, for each slot whose face is a tile face (lf[k] >= 0), the
+// MUSCL increments G . r of the face's reconstruction on this cell's side
+// (as reconstruct(): r the offset to the face midpoint, frec's value; rec,
+// stride MG, holds it on a periodic mesh, else egeo's equal midpoint offset)
 template <int D>
 __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limiter, const TileLds& L, int j,
-                                          const int (&en)[D], const double4 (&eg)[D], const double4& iv) {
+                                          const int (&en)[D], const double4 (&eg)[D], const double4& iv,
+                                          const int (&lf)[D], const double2* rec) {
     const int MG = L.MG;
     const State Uc{L.U[j], L.U[MG + j], L.U[2 * MG + j], L.U[3 * MG + j]};
     const double wc[4] = {L.W[j], L.W[MG + j], L.W[2 * MG + j], L.W[3 * MG + j]};
@@ -92,10 +98,28 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
 #pragma unroll
         for (int q = 0; q < 4; ++q) lim[q] = 0.0;
     }
-    double* gj = L.G + j;
-    gj[0] = lim[0] * gx[0]; gj[MG] = lim[0] * gy[0]; gj[2 * MG] = lim[1] * gx[1]; gj[3 * MG] = lim[1] * gy[1];
-    gj[4 * MG] = lim[2] * gx[2]; gj[5 * MG] = lim[2] * gy[2]; gj[6 * MG] = lim[3] * gx[3];
-    gj[7 * MG] = lim[3] * gy[3];
+    double G[8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        G[2 * q] = lim[q] * gx[q];
+        G[2 * q + 1] = lim[q] * gy[q];
+    }
+    const int MF = L.MF;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        if (en[k] == -1 || lf[k] < 0) continue;
+        double rx = eg[k].z, ry = eg[k].w;
+        if (rec) {
+            const double2 r = rec[k * MG];
+            rx = r.x;
+            ry = r.y;
+        }
+        double* I = L.I + (tell_owner(en[k]) ? 0 : 4 * MF) + lf[k];
+        I[0] = G[0] * rx + G[1] * ry;
+        I[MF] = G[2] * rx + G[3] * ry;
+        I[2 * MF] = G[4] * rx + G[5] * ry;
+        I[3 * MF] = G[6] * rx + G[7] * ry;
+    }
 }
 
 // stage local cell j: conserved state (owned and ring-1 cells) and, for
