@@ -648,16 +648,11 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     // overlaps it: the local ELL and face slots of cell tid (its gradient, and
     // its gather when owned), the gradient mesh data of cell tid, the mesh
     // data of the first two faces of this thread
-    const int TS = m.tslot_stride;
-    const short* ts = m.tslot + (int64_t)t * D * TS;
-    int pen[D], plf[D];
+    int pen[D];
     double4 peg[D], piv{0, 0, 0, 0};
     if (tid < (ORDER == 2 ? ng : n)) {
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-            pen[k] = te[k * MG + tid];
-            plf[k] = ts[k * TS + tid];
-        }
+        for (int k = 0; k < D; ++k) pen[k] = te[k * MG + tid];
     }
     // tile-ordered geometry (second order only): no index load first
     const double4* tg = ORDER == 2 ? m.tegeo + (int64_t)t * D * MG : nullptr;
@@ -696,18 +691,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     // increments of the tile faces on their sides
     if (ORDER == 2) {
         const double2* rec = m.trec ? m.trec + (int64_t)t * D * MG : nullptr;
-        if (tid < ng) tile_grad<D>(m, g, a.limiter, L, tid, pen, peg, piv, plf, rec ? rec + tid : nullptr);
+        if (tid < ng) tile_grad<D>(m, g, a.limiter, L, tid, pen, peg, piv, f0, rec ? rec + tid : nullptr);
         for (int j = tid + bs; j < ng; j += bs) {
-            int en[D], lf[D];
+            int en[D];
             double4 eg[D];
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-                en[k] = te[k * MG + j];
-                lf[k] = ts[k * TS + j];
-            }
+            for (int k = 0; k < D; ++k) en[k] = te[k * MG + j];
 #pragma unroll
             for (int k = 0; k < D; ++k) eg[k] = tg[k * MG + j];
-            tile_grad<D>(m, g, a.limiter, L, j, en, eg, m.tclsq[(int64_t)t * MG + j], lf, rec ? rec + j : nullptr);
+            tile_grad<D>(m, g, a.limiter, L, j, en, eg, m.tclsq[(int64_t)t * MG + j], f0, rec ? rec + j : nullptr);
         }
     }
     __syncthreads();
@@ -736,13 +728,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             if (pen[k] == -1) continue;
-            const double* F = L.F + plf[k];
+            const int lf = tell_lf(pen[k]);
+            const double* F = L.F + lf;
             if (tell_owner(pen[k])) {
                 r0 = r0 - F[0]; r1 = r1 - F[MF]; r2 = r2 - F[2 * MF]; r3 = r3 - F[3 * MF];
             } else {
                 r0 += F[0]; r1 += F[MF]; r2 += F[2 * MF]; r3 += F[3 * MF];
             }
-            if (a.dt_src == 2) dmin = smin(dmin, L.Dt[plf[k]]);
+            if (a.dt_src == 2) dmin = smin(dmin, L.Dt[lf]);
         }
         const double4 u{L.U[tid], L.U[MG + tid], L.U[2 * MG + tid], L.U[3 * MG + tid]};
         update_cell_with(a, c0 + tid, upd, u, r0, r1, r2, r3, dmin);

@@ -695,12 +695,11 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
     for (int q = 0; q < NV; ++q) r[q] = 0.0;
     if (tid < n) {
         const int* te = m.tell + (int64_t)t * W * MG;
-        const short* ts = m.tslot + (int64_t)t * W * m.tslot_stride;
         double dmin = DBL_MAX;
         for (int k = 0; k < W; ++k) {
             const int en = te[k * MG + tid];
             if (en == -1) continue;  // ELL padding (after the cell's faces)
-            const int lf = ts[k * m.tslot_stride + tid];
+            const int lf = tell_lf(en);
             if (tell_owner(en)) {
 #pragma unroll
                 for (int q = 0; q < NV; ++q) r[q] = r[q] - F[q * MF + lf];

@@ -222,7 +222,7 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
                 const std::vector<double4>& egeo, const std::vector<double4>& clsq,
                 const std::vector<double4>& fgeo, const std::vector<double4>& frec, size_t* lds_bytes) {
     const int nt = (int)s->tile_start.size() - 1;
-    const int T = s->block;  // tslot stride: the workgroup size (tiles hold at most tile_cells <= T cells)
+    const int T = s->block;  // the workgroup size (tiles hold at most tile_cells <= T cells)
     const int64_t nc = s->nc, nf = s->nf;
     std::vector<int> loc(nc, -1), fmark(nf, -1);
     std::vector<int> thptr(1, 0), thalo, th1(nt), tfptr(1, 0);
@@ -276,8 +276,9 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
         mf = std::max(mf, nft);
         for (int c : cells) loc[c] = -1;
     }
-    // the packed face cells hold local indices and patch codes in 16 bits each
-    if (mg > 32767 || s->n_bcs > 32767)
+    // the packed face cells hold local indices and patch codes in 16 bits each,
+    // the packed slots local indices and tile face indices in 15
+    if (mg > 32767 || mf > 32766 || s->n_bcs > 32767)
         return set_error(FVK_ERR_INVALID, "tiles of %d staged cells or %d patches exceed the packed face format", mg,
                          s->n_bcs);
     // pass 2: tile-local ELL and face slots (need the maxima for the strides)
@@ -290,7 +291,7 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
         if (periodic) trec.assign((size_t)nt * width * mg, double2{0.0, 0.0});
     }
     const int64_t ld = s->mesh.ld;
-    std::vector<short> tslot((size_t)nt * width * mg, (short)-1);
+    if (nf >= (1 << 30) - 1) return set_error(FVK_ERR_INVALID, "%lld faces exceed the packed tile format", (long long)nf);
     for (int t = 0; t < nt; ++t) {
         const int c0 = s->tile_start[t];
         const TileInfo& ti = info[t];
@@ -320,13 +321,13 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
                     if (nb < 0) return set_error(FVK_ERR_INVALID, "tile %d: a neighbour is outside its halo", t);
                 }
                 if (nb >= ti.n + ti.h1 + ti.h2) return set_error(FVK_ERR_INVALID, "tile %d: local index out of range", t);
-                tell[((size_t)t * width + slot) * mg + j] = tell_pack(e, nb, f);
                 // the slot's face in the tile's list: every face of an owned cell,
                 // and a ring-1 cell's faces towards the owned cells
                 const int lf = fmark[f];
                 const bool ours = lf >= 0 && lf < nft && tface[tfptr[t] + lf].x == f;
                 if (j < ti.n && !ours) return set_error(FVK_ERR_INVALID, "tile %d: face slot out of range", t);
-                if (ours) tslot[((size_t)t * width + slot) * mg + j] = (short)lf;
+                tell[((size_t)t * width + slot) * mg + j] =
+                    nb >= 0 ? tell_interior(nb, e & 1, ours ? lf : -1) : tell_boundary_slot(j < ti.n, lf, f);
                 // periodic: the MUSCL offset of the face's side (frec), which for
                 // cell_right differs from the limiter offset in rounding
                 if (second_order && periodic) {
@@ -351,7 +352,6 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     m.tmax_f = mf;
     m.tmax_2 = m2;
     m.twidth = width;
-    m.tslot_stride = mg;
     m.tstart = upload(s, s->tile_start);
     m.thptr = upload(s, thptr);
     m.thalo = thalo.empty() ? nullptr : upload(s, thalo);
@@ -362,7 +362,6 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     std::vector<int> tfc(tface.size());
     for (size_t e = 0; e < tface.size(); ++e) tfc[e] = tface_pack(tface[e].y, tface[e].z);
     m.tfc = upload(s, tfc);
-    m.tslot = upload(s, tslot);
     std::vector<double4> tfgeo(tface.size());
     for (size_t e = 0; e < tface.size(); ++e) tfgeo[e] = fgeo[tface[e].x];
     m.tfgeo = upload(s, tfgeo);
@@ -370,7 +369,6 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     m.tclsq = second_order ? upload(s, tclsq) : nullptr;
     m.trec = second_order && periodic ? upload(s, trec) : nullptr;
     if (!m.tstart || !m.thptr || (!thalo.empty() && !m.thalo) || !m.th1 || !m.tell || !m.tfptr || !m.tface || !m.tfc ||
-        !m.tslot ||
         !m.tfgeo || (second_order && (!m.tegeo || !m.tclsq || (periodic && !m.trec))))
         return set_error(FVK_ERR_NOMEM, "device allocation for the tiles failed");
     // second order: U (4) and W (4) of the owned and ring-1 cells, W (4) of

@@ -91,21 +91,17 @@ struct MeshView {
     int ntiles;
     int tmax_g, tmax_l, tmax_f; // most local cells with gradients (owned + ring 1), local cells, faces in a tile
     int tmax_2;                 // most ring-2 cells of a tile
-    int twidth, tslot_stride;   // slot width of tell and tslot; tslot's per-slot stride (tmax_g)
+    int twidth;                 // slot width of tell
     const int* tstart;      // [ntiles + 1]
     const int* thptr;       // [ntiles + 1] offsets into thalo
     const int* thalo;       // ring-1 then ring-2 cells of each tile (device ids)
     const int* th1;         // [ntiles] ring-1 count
-    const int* tell;        // [ntiles][width][tmax_g] per local cell with gradients, slot k, packed
-                            //   (tell_pack): local neighbour << 1 | owner; -2 - device face on a boundary
-                            //   (the cell owns it); -1 pads
+    const int* tell;        // [ntiles][width][tmax_g] per local cell with ELL entries, slot k, packed
+                            //   (tell_*): neighbour, orientation and tile face of the slot
     const int* tfptr;       // [ntiles + 1] offsets into tface
     const int4* tface;      // per tile face: {device face, local cell_left, local cell_right or -1 - patch, 0}
     const int* tfc;         // per tile face, packed (tface_pack): local cell_left in the low 16 bits,
                             //   local cell_right or -1 - patch as a signed high half
-    const short* tslot;     // [ntiles][width][tslot_stride] per local cell with ELL entries, slot k: the
-                            //   tile-local index of the slot's face, -1 if it is not a tile face (a ring-1
-                            //   cell's faces away from the owned cells)
     // static geometry in tile order: egeo and clsq of each tile's owned and
     // ring-1 cells ([ntiles][width][tmax_g], [ntiles][tmax_g]; second
     // order), fgeo of each tile face (as tface); on a periodic mesh, the
@@ -115,11 +111,26 @@ struct MeshView {
     const double2* trec;
 };
 
-// packed tile ELL entry (MeshView::tell)
-__host__ __device__ inline int tell_pack(int entry, int nb, int face) {
-    return nb >= 0 ? (nb << 1) | (entry & 1) : -2 - face;
+// Packed tile ELL slot (MeshView::tell).  -1 pads.  An interior face: bit 31
+// clear; the local neighbour in bits 0-14, the orientation (set: the cell
+// owns the face) in bit 15, and 1 + the slot's tile face index in bits 16-30
+// (0: not a tile face, as a ring-1 cell's faces away from the owned cells).
+// A boundary face, which its cell owns: bit 31 set; bits 1-30 hold the tile
+// face index (bit 0 clear: an owned cell) or the device face (bit 0 set: a
+// ring-1 cell, whose boundary faces are never tile faces).
+__host__ __device__ inline int tell_interior(int nb, bool owner, int lf) {
+    return ((lf + 1) << 16) | ((int)owner << 15) | nb;
 }
-__host__ __device__ inline bool tell_owner(int p) { return p < -1 || (p & 1); }  // boundary faces belong to their cell
+__host__ __device__ inline int tell_boundary_slot(bool owned_cell, int lf, int face) {
+    return (int)(0x80000000u | (owned_cell ? (unsigned)lf << 1 : ((unsigned)face << 1) | 1u));
+}
+__host__ __device__ inline bool tell_is_boundary(int p) { return p < -1; }
+__host__ __device__ inline int tell_nb(int p) { return p & 0x7fff; }
+__host__ __device__ inline bool tell_owner(int p) { return p < -1 || ((p >> 15) & 1); }
+__host__ __device__ inline int tell_index(int p) { return (p & 0x7fffffff) >> 1; }  // boundary: tile or device face
+__host__ __device__ inline int tell_lf(int p) {  // the slot's tile face, -1 if none
+    return p >= 0 ? (p >> 16) - 1 : ((p & 1) ? -1 : tell_index(p));
+}
 // packed tile face cells (MeshView::tfc)
 __host__ __device__ inline int tface_pack(int l, int r) { return (int)(((unsigned)r << 16) | (unsigned)(l & 0xffff)); }
 __host__ __device__ inline int tface_left(int p) { return p & 0xffff; }

@@ -25,14 +25,15 @@ struct TileLds {
 };
 
 // limited LSQ gradient of local cell j (as the per-cell gradient kernel),
-// from LDS; then, for each slot whose face is a tile face (lf[k] >= 0), the
-// MUSCL increments G . r of the face's reconstruction on this cell's side
-// (as reconstruct(): r the offset to the face midpoint, frec's value; rec,
-// stride MG, holds it on a periodic mesh, else egeo's equal midpoint offset)
+// from LDS; then, for each slot whose face is a tile face, the MUSCL
+// increments G . r of the face's reconstruction on this cell's side (as
+// reconstruct(): r the offset to the face midpoint, frec's value; rec,
+// stride MG, holds it on a periodic mesh, else egeo's equal midpoint
+// offset).  f0: the tile's first face in the tile face arrays.
 template <int D>
 __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limiter, const TileLds& L, int j,
-                                          const int (&en)[D], const double4 (&eg)[D], const double4& iv,
-                                          const int (&lf)[D], const double2* rec) {
+                                          const int (&en)[D], const double4 (&eg)[D], const double4& iv, int f0,
+                                          const double2* rec) {
     const int MG = L.MG;
     const State Uc{L.U[j], L.U[MG + j], L.U[2 * MG + j], L.U[3 * MG + j]};
     const double wc[4] = {L.W[j], L.W[MG + j], L.W[2 * MG + j], L.W[3 * MG + j]};
@@ -44,10 +45,11 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
     for (int k = 0; k < D; ++k) {
         if (en[k] == -1) continue;
         double wn[4];
-        if (en[k] < -1) {  // boundary face: the ghost state
-            const int f = -2 - en[k];
-            const double4 fg = m.fgeo[f];
-            const DevBC& bc = m.bc[-1 - m.fr[f]];
+        if (tell_is_boundary(en[k])) {  // the ghost state; the face from the tile's list or, ring 1, the mesh
+            const int x = tell_index(en[k]);
+            const bool ring1 = en[k] & 1;
+            const double4 fg = ring1 ? m.fgeo[x] : m.tfgeo[f0 + x];
+            const DevBC& bc = m.bc[ring1 ? -1 - m.fr[x] : -1 - tface_right(m.tfc[f0 + x])];
             const State Gs = euler_ghost_state(bc.type, bc.ff, Uc, fg.x, fg.y, g);
             const double wg0 = Gs.r, wg1 = fdiv(Gs.ru, Gs.r), wg2 = fdiv(Gs.rv, Gs.r), wg3 = pressure(Gs, g);
             wn[0] = 0.5 * (wc[0] + wg0);
@@ -55,7 +57,7 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
             wn[2] = 0.5 * (wc[2] + wg2);
             wn[3] = 0.5 * (wc[3] + wg3);
         } else {
-            const int l = en[k] >> 1;
+            const int l = tell_nb(en[k]);
             const bool r1 = l < L.ng;
             const double* w = r1 ? L.W + l : L.W2 + (l - L.ng);
             const int sw = r1 ? MG : L.M2;
@@ -107,14 +109,15 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
     const int MF = L.MF;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        if (en[k] == -1 || lf[k] < 0) continue;
+        const int lf = tell_lf(en[k]);
+        if (en[k] == -1 || lf < 0) continue;
         double rx = eg[k].z, ry = eg[k].w;
         if (rec) {
             const double2 r = rec[k * MG];
             rx = r.x;
             ry = r.y;
         }
-        double* I = L.I + (tell_owner(en[k]) ? 0 : 4 * MF) + lf[k];
+        double* I = L.I + (tell_owner(en[k]) ? 0 : 4 * MF) + lf;
         I[0] = G[0] * rx + G[1] * ry;
         I[MF] = G[2] * rx + G[3] * ry;
         I[2 * MF] = G[4] * rx + G[5] * ry;
