@@ -24,6 +24,7 @@
 // gradients are arrays of per-cell vectors (double4): a gathered neighbour
 // costs one or two cache lines.  Blocks are remapped so that each XCD (own
 // L2) sweeps one contiguous range of the RCM-ordered cells or faces.
+#include <algorithm>
 #include <cfloat>
 
 #include "physics.h"
@@ -602,8 +603,15 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
 // mesh data of each thread's first gradient cell and first face are loaded
 // before the staging barrier, so their latency overlaps the staging.
 // ---------------------------------------------------------------------------
-template <int D, int SCHEME, int ORDER>
-__global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
+// WPE: the wavefronts per SIMD the kernel is held to.  At 1 (no bound) the
+// mesh data of each thread's first two faces is loaded before the staging
+// barrier too, which costs registers.  At 4 (at most 128 VGPRs; for tiles
+// small enough that four workgroups share a CU's LDS) the face data is loaded
+// after the gradients; triangles with the HLLC and Rusanov fluxes fit so
+// without spills.
+template <int D, int SCHEME, int ORDER, int WPE>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_e_tile(EulerArgs a) {
+    constexpr bool PF = WPE < 4;
     extern __shared__ double lds[];
     const MeshView& m = a.m;
     const int t = logical_block();
@@ -663,11 +671,13 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     }
     int pfd[2] = {0, 0};
     double4 pfg[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    if (PF) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        if (tid + i * bs < nft) {
-            pfd[i] = m.tfc[f0 + tid + i * bs];
-            pfg[i] = m.tfgeo[f0 + tid + i * bs];
+        for (int i = 0; i < 2; ++i) {
+            if (tid + i * bs < nft) {
+                pfd[i] = m.tfc[f0 + tid + i * bs];
+                pfg[i] = m.tfgeo[f0 + tid + i * bs];
+            }
         }
     }
 
@@ -713,7 +723,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     for (int i = 0, e = tid; e < nft; ++i, e += bs) {
         int fd = i == 0 ? pfd[0] : pfd[1];
         double4 fg = i == 0 ? pfg[0] : pfg[1];
-        if (i >= 2) {
+        if (!PF || i >= 2) {
             fd = m.tfc[f0 + e];
             fg = m.tfgeo[f0 + e];
         }
@@ -782,6 +792,8 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_tile(EulerArgs a) {
     }
 }
 
+constexpr int kCuLds = 160 * 1024;  // LDS per CU
+
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
@@ -842,10 +854,18 @@ struct Launch {
     template <int ORDER>
     static void tile_o(fvk_solver* s, const EulerArgs& a, int grid) {
         const size_t lds = s->tile_lds;
+        // four wavefronts per SIMD when the CU's LDS holds enough workgroups for them
+        const bool four = D == 3 && ORDER == 2 && s->p.flux_scheme != FLUX_EXACT &&
+                          (int64_t)(kCuLds / std::max<size_t>(lds, 1)) * s->block >= 4 * 4 * 64;
+        if (four) {
+            if (s->p.flux_scheme == FLUX_HLLC) k_e_tile<3, FLUX_HLLC, 2, 4><<<grid, s->block, lds, s->stream>>>(a);
+            else k_e_tile<3, FLUX_RUSANOV, 2, 4><<<grid, s->block, lds, s->stream>>>(a);
+            return;
+        }
         switch (s->p.flux_scheme) {
-            case FLUX_HLLC: k_e_tile<D, FLUX_HLLC, ORDER><<<grid, s->block, lds, s->stream>>>(a); break;
-            case FLUX_EXACT: k_e_tile<D, FLUX_EXACT, ORDER><<<grid, s->block, lds, s->stream>>>(a); break;
-            default: k_e_tile<D, FLUX_RUSANOV, ORDER><<<grid, s->block, lds, s->stream>>>(a); break;
+            case FLUX_HLLC: k_e_tile<D, FLUX_HLLC, ORDER, 1><<<grid, s->block, lds, s->stream>>>(a); break;
+            case FLUX_EXACT: k_e_tile<D, FLUX_EXACT, ORDER, 1><<<grid, s->block, lds, s->stream>>>(a); break;
+            default: k_e_tile<D, FLUX_RUSANOV, ORDER, 1><<<grid, s->block, lds, s->stream>>>(a); break;
         }
     }
     static void tile(fvk_solver* s, const EulerArgs& a, int grid) {
