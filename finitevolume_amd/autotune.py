@@ -27,7 +27,7 @@ BLOCKS = (64, 128, 256, 512)
 # capacity T holds about 7/8 T cells, its owned and ring-1 cells about
 # 1.2 times that and its faces about 1.6 times, so these pairs cover one
 # pass per phase (staging, gradients, faces) up to two passes.
-TILE_SHAPES = ((128, 112), (192, 160), (256, 192), (256, 200), (256, 208), (256, 224), (256, 240), (256, 256),
+TILE_SHAPES = ((128, 112), (192, 160), (256, 192), (256, 200), (256, 208), (256, 216), (256, 224), (256, 240), (256, 256),
                (320, 224), (320, 256), (384, 256), (384, 320))
 
 

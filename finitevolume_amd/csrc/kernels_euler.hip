@@ -559,13 +559,13 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
     if (ORDER == 2) {
         const double* in = L.I + e;
         const double* wl = L.W + l;
-        Lf = reconstruct_inc(Ls, double4{wl[0], wl[MG], wl[2 * MG], wl[3 * MG]}, in[0], in[MF], in[2 * MF],
-                             in[3 * MF], g);
+        Lf = reconstruct_inc(Ls, double4{Ls.r, wl[0], wl[MG], wl[2 * MG]}, in[0], in[MF], in[2 * MF], in[3 * MF],
+                             g);
         if (r >= 0) {
             R = State{L.U[r], L.U[MG + r], L.U[2 * MG + r], L.U[3 * MG + r]};
             const double* wr = L.W + r;
-            Rf = reconstruct_inc(R, double4{wr[0], wr[MG], wr[2 * MG], wr[3 * MG]}, in[4 * MF], in[5 * MF],
-                                 in[6 * MF], in[7 * MF], g);
+            Rf = reconstruct_inc(R, double4{R.r, wr[0], wr[MG], wr[2 * MG]}, in[4 * MF], in[5 * MF], in[6 * MF],
+                                 in[7 * MF], g);
         } else {
             Rf = euler_ghost(m.bc[-1 - r], Lf, fg.x, fg.y, g);
         }
@@ -630,7 +630,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
     L.U = lds;
     if (ORDER == 2) {
         L.W = lds + 4 * MG;
-        L.W2 = lds + 8 * MG;
+        L.W2 = lds + 7 * MG;
         L.I = L.W2 + 4 * L.M2;
         L.F = L.I;
     } else {

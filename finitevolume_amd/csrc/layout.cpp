@@ -371,11 +371,11 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     if (!m.tstart || !m.thptr || (!thalo.empty() && !m.thalo) || !m.th1 || !m.tell || !m.tfptr || !m.tface || !m.tfc ||
         !m.tfgeo || (second_order && (!m.tegeo || !m.tclsq || (periodic && !m.trec))))
         return set_error(FVK_ERR_NOMEM, "device allocation for the tiles failed");
-    // second order: U (4) and W (4) of the owned and ring-1 cells, W (4) of
-    // ring 2, then the MUSCL increments (4 per side) of each tile face, which
-    // each face's fluxes (4) and dt candidate (1) replace; first order: U (4)
-    // of the owned and ring-1 cells, then the fluxes and dt candidates
-    const size_t words = second_order ? (size_t)8 * mg + (size_t)4 * m2 + (size_t)8 * mf
+    // second order: U (4) and W (3: u, v, p) of the owned and ring-1 cells, W
+    // (4) of ring 2, then the MUSCL increments (4 per side) of each tile face,
+    // which each face's fluxes (4) and dt candidate (1) replace; first order:
+    // U (4) of the owned and ring-1 cells, then the fluxes and dt candidates
+    const size_t words = second_order ? (size_t)7 * mg + (size_t)4 * m2 + (size_t)8 * mf
                                       : (size_t)4 * mg + (size_t)5 * mf;
     *lds_bytes = sizeof(double) * std::max(words, (size_t)4 * T);  // at least the block reductions' 4 T doubles
     return FVK_OK;

@@ -14,8 +14,8 @@ namespace fvk {
 // do not conflict on the banks)
 struct TileLds {
     double* U;   // [4][MG] conserved state, owned cells then ring 1
-    double* W;   // [4][MG] primitives (rho, u, v, p) of the owned and ring-1 cells (second order)
-    double* W2;  // [4][M2] primitives of ring 2 (second order)
+    double* W;   // [3][MG] primitives u, v, p of the owned and ring-1 cells (second order; rho is U's row 0)
+    double* W2;  // [4][M2] primitives rho, u, v, p of ring 2 (second order)
     double* I;   // [8][MF] second order: per tile face, the primitive increments G . r of cell_left
                  //   (rows 0-3) and cell_right (rows 4-7), written by the gradients of the two cells
     double* F;   // [4][MF] face flux x length (second order: over I's rows 0-3, each face's own column)
@@ -36,7 +36,7 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
                                           const double2* rec) {
     const int MG = L.MG;
     const State Uc{L.U[j], L.U[MG + j], L.U[2 * MG + j], L.U[3 * MG + j]};
-    const double wc[4] = {L.W[j], L.W[MG + j], L.W[2 * MG + j], L.W[3 * MG + j]};
+    const double wc[4] = {L.U[j], L.W[j], L.W[MG + j], L.W[2 * MG + j]};
     double bx[4] = {0.0, 0.0, 0.0, 0.0}, by[4] = {0.0, 0.0, 0.0, 0.0};
     double wmin[4], wmax[4];
 #pragma unroll
@@ -58,13 +58,19 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
             wn[3] = 0.5 * (wc[3] + wg3);
         } else {
             const int l = tell_nb(en[k]);
-            const bool r1 = l < L.ng;
-            const double* w = r1 ? L.W + l : L.W2 + (l - L.ng);
-            const int sw = r1 ? MG : L.M2;
-            wn[0] = w[0];
-            wn[1] = w[sw];
-            wn[2] = w[2 * sw];
-            wn[3] = w[3 * sw];
+            if (l < L.ng) {
+                wn[0] = L.U[l];
+                wn[1] = L.W[l];
+                wn[2] = L.W[MG + l];
+                wn[3] = L.W[2 * MG + l];
+            } else {
+                const double* w = L.W2 + (l - L.ng);
+                const int sw = L.M2;
+                wn[0] = w[0];
+                wn[1] = w[sw];
+                wn[2] = w[2 * sw];
+                wn[3] = w[3 * sw];
+            }
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -137,12 +143,19 @@ __device__ __forceinline__ void tile_stage(const TileLds& L, int j, const double
     }
     if (second_order) {
         const State U{u.x, u.y, u.z, u.w};
-        double* w = j < ng ? L.W + j : L.W2 + (j - ng);
-        const int sw = j < ng ? MG : L.M2;
-        w[0] = U.r;
-        w[sw] = fdiv(U.ru, U.r);
-        w[2 * sw] = fdiv(U.rv, U.r);
-        w[3 * sw] = pressure(U, g);
+        const double pu = fdiv(U.ru, U.r), pv = fdiv(U.rv, U.r), pp = pressure(U, g);
+        if (j < ng) {
+            L.W[j] = pu;
+            L.W[MG + j] = pv;
+            L.W[2 * MG + j] = pp;
+        } else {
+            double* w = L.W2 + (j - ng);
+            const int sw = L.M2;
+            w[0] = U.r;
+            w[sw] = pu;
+            w[2 * sw] = pv;
+            w[3 * sw] = pp;
+        }
     }
 }
 
