@@ -606,9 +606,9 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
 // WPE: the wavefronts per SIMD the kernel is held to.  At 1 (no bound) the
 // mesh data of each thread's first two faces is loaded before the staging
 // barrier too, which costs registers.  At 4 (at most 128 VGPRs; for tiles
-// small enough that four workgroups share a CU's LDS) the face data is loaded
-// after the gradients; triangles with the HLLC and Rusanov fluxes fit so
-// without spills.
+// small enough that four workgroups share a CU's LDS) it is loaded once the
+// thread's gradients are done, before the barrier that ends them; triangles
+// with the HLLC and Rusanov fluxes fit so without spills.
 template <int D, int SCHEME, int ORDER, int WPE>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_e_tile(EulerArgs a) {
     constexpr bool PF = WPE < 4;
@@ -712,6 +712,15 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
             tile_grad<D>(m, g, a.limiter, L, j, en, eg, m.tclsq[(int64_t)t * MG + j], f0, rec ? rec + j : nullptr);
         }
     }
+    if (!PF) {  // the first two faces' data, loaded while the gradients finish
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            if (tid + i * bs < nft) {
+                pfd[i] = m.tfc[f0 + tid + i * bs];
+                pfg[i] = m.tfgeo[f0 + tid + i * bs];
+            }
+        }
+    }
     __syncthreads();
 
     // the update's inputs of owned cell tid
@@ -723,7 +732,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
     for (int i = 0, e = tid; e < nft; ++i, e += bs) {
         int fd = i == 0 ? pfd[0] : pfd[1];
         double4 fg = i == 0 ? pfg[0] : pfg[1];
-        if (!PF || i >= 2) {
+        if (i >= 2) {
             fd = m.tfc[f0 + e];
             fg = m.tfgeo[f0 + e];
         }
