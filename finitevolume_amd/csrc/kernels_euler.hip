@@ -721,11 +721,10 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
             }
         }
     }
-    __syncthreads();
-
     // the update's inputs of owned cell tid
     UpdateInputs upd{};
     if (tid < n) upd = load_update_inputs(a, c0 + tid);
+    __syncthreads();
 
     // 3. fluxes of the tile's faces
     double cand_min = DBL_MAX;
