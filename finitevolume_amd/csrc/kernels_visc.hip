@@ -23,7 +23,8 @@ struct ViscArgs {
     const double* U;   // [nv][ld] stage state: rho, rho_u, rho_v, E (, nut | k, omega)
     double* Unew;
     const double* U0;
-    double* G;         // [10][ld] gradients: u(0,1) v(2,3) T(4,5) nut|k(6,7) omega(8,9)
+    double* G;         // [ld][2 nf] gradients, cell-major (a cell's in one 16-byte-aligned run):
+                       //   u(0,1) v(2,3) T(4,5) nut|k(6,7) omega(8,9)
     double* aux;       // [4][ld]  SA: src; SST: k src, omega src, F1, nu_t
     double* flux;      // [nv][ldf]
     double* dtf;
@@ -271,8 +272,9 @@ __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
             gr[2 * q + 1] = fdiv(sy[q], V);
         }
     }
+    double2* gc = reinterpret_cast<double2*>(a.G + (int64_t)c * (2 * NF));
 #pragma unroll
-    for (int q = 0; q < 2 * NF; ++q) a.G[q * a.ld + c] = gr[q];
+    for (int q = 0; q < NF; ++q) gc[q] = double2{gr[2 * q], gr[2 * q + 1]};
     if (EQ == EQ_SA) {
         double nul = fdiv(a.mu, Uc.r);
         double om = fabs(gr[2] - gr[1]);
@@ -472,13 +474,17 @@ __device__ __forceinline__ double visc_face_body(const ViscArgs& a, int f, int c
     return cand;
 }
 
-// global field access for the face sweep
+// global field access for the face sweep (NG gradients per cell, cell-major)
+template <int NG>
 struct GlobalSides {
     const double *U, *G, *X;
     int64_t ld;
     int c[2];
     __device__ double u(int s, int q) const { return U[q * ld + c[s]]; }
-    __device__ double g(int s, int q) const { return G[q * ld + c[s]]; }
+    __device__ double g(int s, int q) const {
+        const double2 v = reinterpret_cast<const double2*>(G + (int64_t)c[s] * NG)[q >> 1];
+        return (q & 1) ? v.y : v.x;
+    }
     __device__ double aux(int s, int q) const { return X[q * ld + c[s]]; }
 };
 
@@ -490,7 +496,7 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
     double cand = DBL_MAX;
     if (f < m.nf) {
         const int cl = m.fl[f], cr = m.fr[f];
-        const GlobalSides acc{a.U, a.G, a.aux, a.ld, {cl, cr < 0 ? cl : cr}};
+        const GlobalSides<2 * (NV - 1)> acc{a.U, a.G, a.aux, a.ld, {cl, cr < 0 ? cl : cr}};
         double out[6];
         cand = visc_face_body<EQ, SCHEME>(a, f, cr, acc, out);
 #pragma unroll
@@ -664,8 +670,13 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
         const int gc = j < n ? c0 + j : m.thalo[hp + j - n];
 #pragma unroll
         for (int q = 0; q < NV; ++q) U[q * MG + j] = a.U[q * ld + gc];
+        const double2* gsrc = reinterpret_cast<const double2*>(a.G + (int64_t)gc * NG);
 #pragma unroll
-        for (int q = 0; q < NG; ++q) G[q * MG + j] = a.G[q * ld + gc];
+        for (int q = 0; q < NG / 2; ++q) {
+            const double2 v = gsrc[q];
+            G[2 * q * MG + j] = v.x;
+            G[(2 * q + 1) * MG + j] = v.y;
+        }
         if (EQ == EQ_SST) {
             X[j] = a.aux[2 * ld + gc];
             X[MG + j] = a.aux[3 * ld + gc];
