@@ -609,6 +609,9 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
     m.fky = upload(s, fky);
     m.fwl = upload(s, fwl);
     m.fwr = upload(s, fwr);
+    if (nf > 0 && (!m.fnx || !m.fny || !m.fa || !m.fmx || !m.fmy || !m.fdl || !m.fdc || !m.fkx || !m.fky || !m.fwl ||
+                   !m.fwr))
+        return set_error(FVK_ERR_NOMEM, "device allocation for the face geometry failed");
     std::vector<DevBC> hb(std::max(1, s->n_bcs));
     for (int i = 0; i < s->n_bcs; ++i) {
         const fvk_bc& b = s->bcs[i];
