@@ -600,8 +600,9 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
 // then read LDS only.  Every value is computed by the
 // same operations as in the face sweep + gather kernels, so the states are
 // bit-identical; faces on a tile's edge are evaluated by both tiles.  The
-// mesh data of each thread's first gradient cell and first face are loaded
-// before the staging barrier, so their latency overlaps the staging.
+// mesh data of each thread's first gradient cell is loaded before the
+// staging barrier, so its latency overlaps the staging; the data of its
+// first faces and its update, before the barrier that precedes their use.
 // ---------------------------------------------------------------------------
 // WPE: the wavefronts per SIMD the kernel is held to.  At 1 (no bound) the
 // mesh data of each thread's first two faces is loaded before the staging
@@ -653,9 +654,9 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
     const int* te = m.tell + (int64_t)t * D * MG;
 
     // 0. loads issued before the staging barrier, so that their latency
-    // overlaps it: the local ELL and face slots of cell tid (its gradient, and
-    // its gather when owned), the gradient mesh data of cell tid, the mesh
-    // data of the first two faces of this thread
+    // overlaps it: the local ELL of cell tid (its gradient, and its gather
+    // when owned), the gradient mesh data of cell tid and (PF) the mesh data
+    // of the first two faces of this thread
     int pen[D];
     double4 peg[D], piv{0, 0, 0, 0};
     if (tid < (ORDER == 2 ? ng : n)) {
