@@ -543,50 +543,76 @@ __device__ __forceinline__ void vblock_partials(const double (&v)[NV], double* p
 // ---------------------------------------------------------------------------
 // sources, then the stage update (or the residual output) of cell c from
 // its gathered face sum r and, for local dt, the minimum of its faces' dt
+// the update's inputs of cell c, loaded apart from the arithmetic so that a
+// kernel can issue them early: volume, source terms, dt (unless it is the
+// cell's own new local dt) and the RK base state
+template <int NV>
+struct ViscUpdateInputs {
+    double V, s0, s1, dt;
+    double u0[NV];
+};
 template <int EQ, int NV>
-__device__ __forceinline__ void visc_cell_finish(const ViscArgs& a, int c, double (&r)[NV], double dmin) {
-    const MeshView& m = a.m;
+__device__ __forceinline__ ViscUpdateInputs<NV> load_visc_update_inputs(const ViscArgs& a, int c) {
     const int64_t ld = a.ld;
-    {
-        const double V = m.vol[c];
-        if (EQ == EQ_SA) r[4] += V * a.aux[c];
-        if (EQ == EQ_SST) { r[4] += V * a.aux[c]; r[5] += V * a.aux[ld + c]; }
-        if (a.R) {
+    ViscUpdateInputs<NV> x{};
+    x.V = a.m.vol[c];
+    if (EQ == EQ_SA) x.s0 = a.aux[c];
+    if (EQ == EQ_SST) { x.s0 = a.aux[c]; x.s1 = a.aux[ld + c]; }
+    if (!a.R) {
+        if (a.dt_src == 1) x.dt = __longlong_as_double((long long)a.dtmin[a.slot]);
+        else if (a.dt_src == 0) x.dt = a.dtc[c];
 #pragma unroll
-            for (int q = 0; q < NV; ++q) a.R[q * ld + c] = r[q];
+        for (int q = 0; q < NV; ++q) x.u0[q] = a.first ? 0.0 : a.U0[q * ld + c];
+    }
+    return x;
+}
+
+// sources and the stage update of cell c from its residual r and its state u
+// (read before any write: the last RK stage writes the buffer that holds the
+// base state, whose values x already holds)
+template <int EQ, int NV>
+__device__ __forceinline__ void visc_cell_finish_with(const ViscArgs& a, int c, const ViscUpdateInputs<NV>& x,
+                                                      const double (&u)[NV], double (&r)[NV], double dmin) {
+    const int64_t ld = a.ld;
+    const double V = x.V;
+    if (EQ == EQ_SA) r[4] += V * x.s0;
+    if (EQ == EQ_SST) { r[4] += V * x.s0; r[5] += V * x.s1; }
+    if (a.R) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) a.R[q * ld + c] = r[q];
+        return;
+    }
+    double dt = x.dt;
+    if (a.dt_src == 2) { dt = dmin; a.dtc[c] = dmin; }
+    double h = fdiv(dt, V);
+    double rho_old = u[0];
+    double rho_new = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double un = a.first ? u[q] + h * r[q] : a.a * x.u0[q] + a.b * (u[q] + h * r[q]);
+        if (q == 0) rho_new = un;
+        a.Unew[q * ld + c] = un;
+    }
+#pragma unroll
+    for (int q = 4; q < NV; ++q) {
+        double t = u[q];
+        if (EQ == EQ_SST) {
+            double cur = rho_old * t;
+            double nw = a.first ? cur + h * r[q] : a.a * (x.u0[0] * x.u0[q]) + a.b * (cur + h * r[q]);
+            a.Unew[q * ld + c] = fdiv(nw, rho_new);
         } else {
-            double dt;
-            if (a.dt_src == 1) dt = __longlong_as_double((long long)a.dtmin[a.slot]);
-            else if (a.dt_src == 2) { dt = dmin; a.dtc[c] = dmin; }
-            else dt = a.dtc[c];
-            double h = fdiv(dt, V);
-            double rho_old = a.U[c];
-            // the last RK stage writes the buffer that holds the base state:
-            // read all of this cell's base values before any write
-            double u0[NV];
-#pragma unroll
-            for (int q = 0; q < NV; ++q) u0[q] = a.first ? 0.0 : a.U0[q * ld + c];
-            double rho_new = 0.0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                double u = a.U[q * ld + c];
-                const double un = a.first ? u + h * r[q] : a.a * u0[q] + a.b * (u + h * r[q]);
-                if (q == 0) rho_new = un;
-                a.Unew[q * ld + c] = un;
-            }
-#pragma unroll
-            for (int q = 4; q < NV; ++q) {
-                double t = a.U[q * ld + c];
-                if (EQ == EQ_SST) {
-                    double cur = rho_old * t;
-                    double nw = a.first ? cur + h * r[q] : a.a * (u0[0] * u0[q]) + a.b * (cur + h * r[q]);
-                    a.Unew[q * ld + c] = fdiv(nw, rho_new);
-                } else {
-                    a.Unew[q * ld + c] = a.first ? t + h * r[q] : a.a * u0[q] + a.b * (t + h * r[q]);
-                }
-            }
+            a.Unew[q * ld + c] = a.first ? t + h * r[q] : a.a * x.u0[q] + a.b * (t + h * r[q]);
         }
     }
+}
+
+template <int EQ, int NV>
+__device__ __forceinline__ void visc_cell_finish(const ViscArgs& a, int c, double (&r)[NV], double dmin) {
+    const ViscUpdateInputs<NV> x = load_visc_update_inputs<EQ, NV>(a, c);
+    double u[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) u[q] = a.U[q * a.ld + c];
+    visc_cell_finish_with<EQ, NV>(a, c, x, u, r, dmin);
 }
 
 template <int EQ>
@@ -684,6 +710,10 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
     }
     __syncthreads();
 
+    // the update's inputs of owned cell tid, loaded while the faces are computed
+    ViscUpdateInputs<NV> upd{};
+    if (tid < n) upd = load_visc_update_inputs<EQ, NV>(a, c0 + tid);
+
     // 2. fluxes (and dt candidates) of the tile's faces
     double cand_min = DBL_MAX;
     for (int e = tid; e < nft; e += bs) {
@@ -720,7 +750,10 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
             }
             if (a.dt_src == 2) dmin = smin(dmin, Dt[lf]);
         }
-        visc_cell_finish<EQ, NV>(a, c0 + tid, r, dmin);
+        double u[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) u[q] = U[q * MG + tid];
+        visc_cell_finish_with<EQ, NV>(a, c0 + tid, upd, u, r, dmin);
     }
 
     // 4. block reductions in the (now free) LDS: residual partials, dt minimum
