@@ -69,8 +69,9 @@ struct EulerArgs {
     int stages;
     // finalize of the previous stage's (or step's) norms, run by one spare
     // workgroup of the tile or face kernel (fin_nblocks == 0: none); with
-    // fin_rearm it also re-arms the next step's dt slot
-    int fin_nblocks, fin_slot, fin_rearm;
+    // fin_rearm it also re-arms the next step's dt slot; with fin_dt_row the
+    // step's dt is the minimum of the partials' fifth row (tile kernels)
+    int fin_nblocks, fin_slot, fin_rearm, fin_dt_row;
     int64_t fin_hist_idx;
     int rearm_slot;       // >= 0: the gather re-arms this dt slot (forward Euler with a folded finalize)
 };
@@ -208,9 +209,11 @@ constexpr int kFinBlock = 1024;
 __global__ __launch_bounds__(kFinBlock) void k_e_finalize(const double* partials, int nblocks, int nv,
                                                           double* res_hist, double* dt_hist, int64_t hist_idx,
                                                           unsigned long long* dtmin, int slot,
-                                                          double fixed_dt = -1.0, int rearm = 1) {
+                                                          double fixed_dt = -1.0, int rearm = 1,
+                                                          int dt_row = 0) {
     __shared__ double sm[kFinBlock];
-    finalize_norms(partials, nblocks, nv, res_hist, dt_hist, hist_idx, dtmin, slot, fixed_dt, sm, rearm != 0);
+    finalize_norms(partials, nblocks, nv, res_hist, dt_hist, hist_idx, dtmin, slot, fixed_dt, sm, rearm != 0,
+                   dt_row != 0);
 }
 
 // stage update of one cell: U' = U + dt/V R (src/EulerFVMSolver.cpp:103) on
@@ -263,20 +266,27 @@ __device__ __forceinline__ void finish_residual(const EulerArgs& a, double r0, d
 // ---------------------------------------------------------------------------
 // global dt: compute_dt over all faces (reference mode, variant 2)
 // ---------------------------------------------------------------------------
+// Each workgroup takes a contiguous range of faces and leaves one atomic
+// minimum: the grid is bounded (dt_grid), since device-wide atomics on one
+// address are serialised across the XCDs.
 __global__ __launch_bounds__(kMaxBlock) void k_e_dt(EulerArgs a) {
     const MeshView& m = a.m;
-    const int f = xcd_index();
+    const int64_t per = ((int64_t)m.nf + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)logical_block() * per;
+    const int64_t hi = lo + per < m.nf ? lo + per : m.nf;
     double cand = DBL_MAX;
-    if (f < m.nf) {
+    for (int64_t f = lo + threadIdx.x; f < hi; f += blockDim.x) {
         const int cl = m.fl[f], cr = m.fr[f];
         const double4 fg = m.fgeo[f];
         const State L = st(a.U[cl]);
+        double c;
         if (cr >= 0) {
             const State R = st(a.U[cr]);
-            cand = dt_candidate(L, &R, fg.x, fg.y, fg.w, a.gamma, a.cfl);
+            c = dt_candidate(L, &R, fg.x, fg.y, fg.w, a.gamma, a.cfl);
         } else {
-            cand = dt_candidate(L, nullptr, fg.x, fg.y, fg.w, a.gamma, a.cfl);
+            c = dt_candidate(L, nullptr, fg.x, fg.y, fg.w, a.gamma, a.cfl);
         }
+        cand = smin(cand, c);
     }
     block_min_to(cand, &a.dtmin[a.slot]);
 }
@@ -450,13 +460,19 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_cell(EulerArgs a) {
 template <int SCHEME, int ORDER>
 __global__ __launch_bounds__(kMaxBlock) void k_e_face(EulerArgs a) {
     const MeshView& m = a.m;
-    if (a.fin_nblocks > 0 && logical_block() == (int)gridDim.x - 1) {  // the spare workgroup (no faces)
-        __shared__ double fsm[kMaxBlock];
-        finalize_norms(a.partials, a.fin_nblocks, 4, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin, a.fin_slot,
-                       -1.0, fsm, a.fin_rearm != 0);
-        return;
+    // a pending finalize takes the first logical workgroup, dispatched first so
+    // that its one-workgroup reduction overlaps the faces; the faces follow it
+    int lb = logical_block();
+    if (a.fin_nblocks > 0) {
+        if (lb == 0) {
+            __shared__ double fsm[kMaxBlock];
+            finalize_norms(a.partials, a.fin_nblocks, 4, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin,
+                           a.fin_slot, -1.0, fsm, a.fin_rearm != 0, a.fin_dt_row != 0);
+            return;
+        }
+        --lb;
     }
-    const int f = xcd_index();
+    const int f = lb * blockDim.x + threadIdx.x;
     const double g = a.gamma;
     double cand = DBL_MAX;
     if (f < m.nf && cell_active(a, m.fl[f])) {
@@ -615,11 +631,17 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
     constexpr bool PF = WPE < 4;
     extern __shared__ double lds[];
     const MeshView& m = a.m;
-    const int t = logical_block();
-    if (a.fin_nblocks > 0 && t == m.ntiles) {  // the spare workgroup: the previous stage's norms
-        finalize_norms(a.partials, a.fin_nblocks, 4, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin, a.fin_slot,
-                       -1.0, lds, a.fin_rearm != 0);
-        return;
+    // a pending finalize (the previous stage's norms) takes the first logical
+    // workgroup, dispatched first so that its one-workgroup reduction overlaps
+    // the tiles; the tiles follow it
+    int t = logical_block();
+    if (a.fin_nblocks > 0) {
+        if (t == 0) {
+            finalize_norms(a.partials, a.fin_nblocks, 4, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin,
+                           a.fin_slot, -1.0, lds, a.fin_rearm != 0, a.fin_dt_row != 0);
+            return;
+        }
+        --t;
     }
     const double g = a.gamma;
     const int bs = blockDim.x, tid = threadIdx.x;
@@ -797,7 +819,9 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
             if (tid < s2) lds[tid] = smin(lds[tid], lds[tid + s2]);
             __syncthreads();
         }
-        if (tid == 0) atomicMin(&a.dtmin[a.slot], dbits(lds[0]));
+        // the step's dt (local dt: only its history) is reduced by the finalize
+        // from one entry per workgroup, not by device-wide atomics
+        if (tid == 0) a.partials[4 * gridDim.x + blockIdx.x] = lds[0];
     }
 }
 
@@ -809,6 +833,8 @@ constexpr int kCuLds = 160 * 1024;  // LDS per CU
 namespace {
 
 constexpr int kDtBlock = 256;  // global-dt pass beside the tile kernels
+// the global-dt pass's grid: a face per thread up to four workgroups per CU
+int dt_grid(int64_t nf, int bs) { return std::min(grid_for(nf, bs), 4 * 256); }
 
 
 EulerArgs make_args(fvk_solver* s) {
@@ -885,7 +911,7 @@ struct Launch {
 
 template <int ORDER>
 void face_o(fvk_solver* s, const EulerArgs& a) {
-    // a pending finalize takes the last logical workgroup of one more XCD round
+    // a pending finalize takes the first logical workgroup (one more XCD round)
     int g = grid_for(s->nf, s->block) + (a.fin_nblocks > 0 ? kXcds : 0);
     switch (s->p.flux_scheme) {
         case FLUX_HLLC: k_e_face<FLUX_HLLC, ORDER><<<g, s->block, 0, s->stream>>>(a); break;
@@ -900,10 +926,10 @@ void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
     if (s->variant == 3) {
         if (dt_pass) {  // its block minimum needs a power-of-two workgroup
             timer_begin(s, "euler_dt", &ev);
-            k_e_dt<<<grid_for(s->nf, kDtBlock), kDtBlock, 0, s->stream>>>(a);
+            k_e_dt<<<dt_grid(s->nf, kDtBlock), kDtBlock, 0, s->stream>>>(a);
             timer_end(s, ev);
         }
-        // a pending finalize takes a spare workgroup (logical block ntiles)
+        // a pending finalize takes a spare workgroup (logical block 0)
         const int grid = (s->mesh.ntiles + (a.fin_nblocks > 0 ? 1 : 0) + kXcds - 1) / kXcds * kXcds;
         timer_begin(s, "euler_tile", &ev);
         Launch<D>::tile(s, a, grid);
@@ -915,10 +941,11 @@ void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
                 s->fin_slot = a.slot;
                 s->fin_hist_idx = a.hist_idx;
                 s->fin_rearm = 1;
+                s->fin_dt_row = a.face_dt;
             } else {
                 timer_begin(s, "euler_finalize", &ev);
                 k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(a.partials, grid, 4, a.res_hist, a.dt_hist,
-                                                             a.hist_idx, a.dtmin, a.slot);
+                                                             a.hist_idx, a.dtmin, a.slot, -1.0, 1, a.face_dt);
                 timer_end(s, ev);
             }
         }
@@ -927,7 +954,7 @@ void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
     const bool split = s->variant == 1;
     if (dt_pass && !split) {
         timer_begin(s, "euler_dt", &ev);
-        k_e_dt<<<grid_for(s->nf, s->block), s->block, 0, s->stream>>>(a);
+        k_e_dt<<<dt_grid(s->nf, s->block), s->block, 0, s->stream>>>(a);
         timer_end(s, ev);
     }
     if (s->p.order == 2) {
@@ -957,6 +984,7 @@ void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
             s->fin_slot = a.slot;
             s->fin_hist_idx = a.hist_idx;
             s->fin_rearm = a.stages >= 2;
+            s->fin_dt_row = 0;
             return;
         }
     } else {
@@ -1016,6 +1044,7 @@ int launch_euler_stage(fvk_solver* s, int stage, int stages, int /*unused*/, int
         a.fin_slot = s->fin_slot;
         a.fin_hist_idx = s->fin_hist_idx;
         a.fin_rearm = s->fin_rearm;
+        a.fin_dt_row = s->fin_dt_row;
         s->fin_nblocks = 0;
     }
     bool dt_pass = false;
@@ -1055,7 +1084,8 @@ int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, i
 int flush_pending_finalize(fvk_solver* s) {
     if (s->fin_nblocks <= 0) return FVK_OK;
     k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(s->partials, s->fin_nblocks, s->nv, s->res_hist, s->dt_hist,
-                                                 s->fin_hist_idx, s->dtmin, s->fin_slot, -1.0, s->fin_rearm);
+                                                 s->fin_hist_idx, s->dtmin, s->fin_slot, -1.0, s->fin_rearm,
+                                                 s->fin_dt_row);
     s->fin_nblocks = 0;
     return hip_check(hipGetLastError(), "finalize launch");
 }

@@ -183,13 +183,19 @@ template <int EQ, int GS, bool PERIODIC>
 __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
     constexpr int NF = EQ == EQ_SST ? 5 : (EQ == EQ_SA ? 4 : 3);
     const MeshView& m = a.m;
-    if (a.fin_nblocks > 0 && logical_block() == (int)gridDim.x - 1) {  // the spare workgroup (no cells)
-        __shared__ double fsm[kVB];
-        finalize_norms(a.partials, a.fin_nblocks, a.fin_nv, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin,
-                       a.fin_slot, -1.0, fsm, true);
-        return;
+    // a pending finalize takes the first logical workgroup, dispatched first so
+    // that its one-workgroup reduction overlaps the cells; the cells follow it
+    int lb = logical_block();
+    if (a.fin_nblocks > 0) {
+        if (lb == 0) {
+            __shared__ double fsm[kVB];
+            finalize_norms(a.partials, a.fin_nblocks, a.fin_nv, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin,
+                           a.fin_slot, -1.0, fsm, true);
+            return;
+        }
+        --lb;
     }
-    int c = xcd_index();
+    int c = lb * kVB + threadIdx.x;
     if (c >= m.nc) return;
     const double g = a.gamma;
     State Uc = ldS(a.U, a.ld, c);
@@ -1040,6 +1046,7 @@ void run_visc(fvk_solver* s, ViscArgs& a) {
             s->fin_slot = a.slot;
             s->fin_hist_idx = a.hist_idx;
             s->fin_rearm = 1;
+            s->fin_dt_row = 0;
             return;
         }
         timer_begin(s, "visc_finalize", &ev);

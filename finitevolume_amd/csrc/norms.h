@@ -16,10 +16,13 @@ __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned
 
 // one block: residual norms = sqrt of the sums of the partials (fixed order
 // for a given workgroup size), the step's dt into the history, and the next
-// step's dt slot re-armed; sm holds blockDim.x doubles
+// step's dt slot re-armed; sm holds blockDim.x doubles.  dt_row: the step's
+// dt is the minimum of the partials' row nv (one entry per block) instead of
+// the atomic minimum in dtmin[slot].
 __device__ __forceinline__ void finalize_norms(const double* partials, int nblocks, int nv, double* res_hist,
                                                double* dt_hist, int64_t hist_idx, unsigned long long* dtmin,
-                                               int slot, double fixed_dt, double* sm, bool rearm = true) {
+                                               int slot, double fixed_dt, double* sm, bool rearm = true,
+                                               bool dt_row = false) {
     const int nt = blockDim.x, tid = threadIdx.x;
     int p2 = 64;  // the largest power of two not above the workgroup size
     while (2 * p2 <= nt) p2 *= 2;
@@ -45,11 +48,25 @@ __device__ __forceinline__ void finalize_norms(const double* partials, int nbloc
         if (tid == 0) res_hist[hist_idx * nv + q] = sqrt(sm[0]);
         __syncthreads();
     }
+    double dmin = DBL_MAX;
+    if (dt_row && fixed_dt < 0.0) {
+        const double* p = partials + (int64_t)nv * nblocks;
+        for (int i = tid; i < nblocks; i += nt) dmin = p[i] < dmin ? p[i] : dmin;
+        sm[tid] = dmin;
+        __syncthreads();
+        if (tid < nt - p2) sm[tid] = sm[tid + p2] < sm[tid] ? sm[tid + p2] : sm[tid];
+        __syncthreads();
+        for (int s = p2 / 2; s > 0; s >>= 1) {
+            if (tid < s) sm[tid] = sm[tid + s] < sm[tid] ? sm[tid + s] : sm[tid];
+            __syncthreads();
+        }
+        dmin = sm[0];
+    }
     if (tid == 0) {
         if (fixed_dt >= 0.0) {
             dt_hist[hist_idx] = fixed_dt;
         } else {
-            dt_hist[hist_idx] = __longlong_as_double((long long)dtmin[slot]);
+            dt_hist[hist_idx] = dt_row ? dmin : __longlong_as_double((long long)dtmin[slot]);
             if (rearm) dtmin[slot ^ 1] = dbits(DBL_MAX);
         }
     }

@@ -160,7 +160,7 @@ struct fvk_solver {
     // Euler, the next step's) tile or face launch to reduce them with a
     // spare workgroup (fin_nblocks == 0: nothing pending; fvk_step flushes
     // what is left); fin_rearm: that finalize re-arms the next dt slot
-    int fin_nblocks = 0, fin_slot = 0, fin_rearm = 1;
+    int fin_nblocks = 0, fin_slot = 0, fin_rearm = 1, fin_dt_row = 0;
     int64_t fin_hist_idx = 0;
     int64_t nc = 0, nf = 0, nn = 0;
     int nfi = 0, nnz = 0;

@@ -84,8 +84,9 @@ def test_config2_variants_identical_after_100_steps(config2):
     out = []
     for variant, block, tc in ((1, 512, 0), (2, 256, 0), (3, 256, 224), (3, 256, 208), (3, 192, 160)):
         g = FVKSolver(m, bcs, with_shape(p, variant, block, tc), ic)
-        r = g.step(100)
-        out.append((fields(g)[0], r))
-    for f, r in out[1:]:
+        r, dt = g.step(100, with_dt=True)
+        out.append((fields(g)[0], r, dt))
+    for f, r, dt in out[1:]:
         assert np.array_equal(f, out[0][0])
         assert np.allclose(r, out[0][1], rtol=1e-12, atol=0.0)
+        assert np.array_equal(dt, out[0][2])

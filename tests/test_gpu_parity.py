@@ -148,13 +148,15 @@ def test_kernel_variants_bit_identical(order):
         for bs, tc in (shapes3 if variant == 3 else ((64, 0), (128, 0), (256, 0), (512, 0))):
             m, bcs, p, ic, _ = cases.case_naca(64, 32, kernel_variant=variant, block_size=bs, tile_cells=tc, **ext)
             g = FVKSolver(m, bcs, p, ic)
-            r = g.step(10)
-            out = (g.field(), r)
+            r, dt = g.step(10, with_dt=True)
+            out = (g.field(), r, dt)
             if ref is None:
                 ref = out
             else:
                 assert np.array_equal(out[0], ref[0]), (variant, bs)
                 assert np.allclose(out[1], ref[1], rtol=1e-13, atol=0.0), (variant, bs)
+                # the step's dt: an atomic minimum (variants 1, 2) or one per tile reduced by the finalize
+                assert np.array_equal(out[2], ref[2]), (variant, bs)
             if tc in (0, 224):
                 by_bs.setdefault(bs, []).append(r)
     for bs, rs in by_bs.items():
