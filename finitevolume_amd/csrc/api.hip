@@ -308,7 +308,8 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
     // launch grids (workgroups of 64 threads or more, or one per tile plus a
     // spare) are rounded up to a multiple of the 8 XCDs
     int64_t max_blocks = std::max((std::max(s->nc, s->nf) + 63) / 64, (int64_t)s->mesh.ntiles + 1) + 8;
-    s->partials = static_cast<double*>(dev_alloc(s, sizeof(double) * 6 * max_blocks));
+    // partial sums: up to six residual rows and the dt row
+    s->partials = static_cast<double*>(dev_alloc(s, sizeof(double) * 7 * max_blocks));
     s->dtmin = static_cast<unsigned long long*>(dev_alloc(s, 256));
     s->hist_cap = 1024;
     s->res_hist_host = static_cast<double*>(host_mapped_alloc(s, sizeof(double) * 6 * s->hist_cap,

@@ -1074,9 +1074,10 @@ int launch_euler_residual_eval(fvk_solver* s, double* R_dev) {
 
 // residual norms of nv components from nblocks block partials, the step's dt
 // into the history (fixed_dt >= 0: that value) and the next dt slot re-armed
-int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, int slot, double fixed_dt) {
+int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, int slot, double fixed_dt,
+                         bool dt_row) {
     k_e_finalize<<<1, kFinBlock, 0, s->stream>>>(s->partials, nblocks, nv, s->res_hist, s->dt_hist, hist_idx,
-                                                 s->dtmin, slot, fixed_dt);
+                                                 s->dtmin, slot, fixed_dt, 1, dt_row ? 1 : 0);
     return hip_check(hipGetLastError(), "finalize launch");
 }
 

@@ -45,6 +45,7 @@ struct ViscArgs {
     // the previous stage's norms, reduced by a spare workgroup of the
     // gradient launch (fin_nblocks == 0: none)
     int fin_nblocks, fin_slot, fin_nv;
+    int fin_dt_row;    // the step's dt is the minimum of the partials' row fin_nv (local dt)
     int64_t fin_hist_idx;
 };
 
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
         if (lb == 0) {
             __shared__ double fsm[kVB];
             finalize_norms(a.partials, a.fin_nblocks, a.fin_nv, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin,
-                           a.fin_slot, -1.0, fsm, true);
+                           a.fin_slot, -1.0, fsm, true, a.fin_dt_row != 0);
             return;
         }
         --lb;
@@ -509,7 +510,7 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
         for (int q = 0; q < NV; ++q) a.flux[q * a.ldf + f] = out[q];
         if (a.face_dt && a.dt_src == 2) a.dtf[f] = cand;
     }
-    if (a.face_dt) {
+    if (a.face_dt && a.dt_src != 2) {  // local dt: the cell pass reduces the step's dt from its cells
         __shared__ double sm[kVB];
         sm[threadIdx.x] = cand;
         __syncthreads();
@@ -523,23 +524,29 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
 
 // deterministic block sum of the squared residual components into
 // partials[q * gridDim.x + blockIdx.x]; a one-block finalize kernel reduces
-// them (fixed order) once the kernel has ended
+// them (fixed order) once the kernel has ended.  with_dt: the block minimum of
+// dt goes to row NV (the step's dt, reduced by the finalize instead of by
+// device-wide atomics)
 template <int NV>
-__device__ __forceinline__ void vblock_partials(const double (&v)[NV], double* partials) {
-    __shared__ double sm[NV][kVB];
+__device__ __forceinline__ void vblock_partials(const double (&v)[NV], double* partials, bool with_dt = false,
+                                                double dt = DBL_MAX) {
+    __shared__ double sm[NV + 1][kVB];
 #pragma unroll
     for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] = v[q];
+    sm[NV][threadIdx.x] = dt;
     __syncthreads();
     for (int s = kVB / 2; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
 #pragma unroll
             for (int q = 0; q < NV; ++q) sm[q][threadIdx.x] += sm[q][threadIdx.x + s];
+            if (with_dt) sm[NV][threadIdx.x] = smin(sm[NV][threadIdx.x], sm[NV][threadIdx.x + s]);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NV; ++q) partials[q * gridDim.x + blockIdx.x] = sm[q][0];
+        if (with_dt) partials[NV * gridDim.x + blockIdx.x] = sm[NV][0];
     }
 }
 
@@ -630,8 +637,8 @@ __global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
 #pragma unroll
     for (int q = 0; q < NV; ++q) r[q] = 0.0;
     const int64_t ldf = a.ldf;
+    double dmin = DBL_MAX;
     if (c < m.nc) {
-        double dmin = DBL_MAX;
         for (int k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
             int e = m.cent[k];
             int f = e >> 1;
@@ -650,7 +657,7 @@ __global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
     double sq[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) sq[q] = r[q] * r[q];
-    vblock_partials<NV>(sq, a.partials);
+    vblock_partials<NV>(sq, a.partials, a.face_dt && a.dt_src == 2, dmin);
 }
 
 // ---------------------------------------------------------------------------
@@ -797,7 +804,9 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
             if (tid < s2) lds[tid] = smin(lds[tid], lds[tid + s2]);
             __syncthreads();
         }
-        if (tid == 0) atomicMin(&a.dtmin[a.slot], dbits(lds[0]));
+        // the step's dt (local dt: only its history), reduced by the finalize
+        // from one entry per workgroup instead of by device-wide atomics
+        if (tid == 0) a.partials[NV * gridDim.x + blockIdx.x] = lds[0];
     }
 }
 
@@ -1046,11 +1055,11 @@ void run_visc(fvk_solver* s, ViscArgs& a) {
             s->fin_slot = a.slot;
             s->fin_hist_idx = a.hist_idx;
             s->fin_rearm = 1;
-            s->fin_dt_row = 0;
+            s->fin_dt_row = a.face_dt && a.dt_src == 2;
             return;
         }
         timer_begin(s, "visc_finalize", &ev);
-        launch_norm_finalize(s, nparts, s->nv, a.hist_idx, a.slot, -1.0);
+        launch_norm_finalize(s, nparts, s->nv, a.hist_idx, a.slot, -1.0, a.face_dt && a.dt_src == 2);
         timer_end(s, ev);
     }
 }
@@ -1080,6 +1089,7 @@ int launch_visc_stage(fvk_solver* s, int stage, int stages, int want_res, int64_
         a.fin_slot = s->fin_slot;
         a.fin_hist_idx = s->fin_hist_idx;
         a.fin_nv = s->nv;
+        a.fin_dt_row = s->fin_dt_row;
         s->fin_nblocks = 0;
     }
     run_visc(s, a);

@@ -249,7 +249,8 @@ int launch_residual_eval(fvk_solver* s, double* R_dev);
 int launch_euler_gradients(fvk_solver* s);
 int launch_case_finalize(fvk_solver* s);
 int flush_pending_finalize(fvk_solver* s);
-int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, int slot, double fixed_dt);
+int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, int slot, double fixed_dt,
+                         bool dt_row = false);
 int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
 void adjoint_release(fvk_solver* s);
 int launch_copy_probe(double* dst, const double* src, int64_t n, hipStream_t st);

@@ -213,17 +213,18 @@ def test_viscous_tile_bit_identical(equation, gradient):
         p.kernel_variant, p.block_size = variant, bs
         g = FVKSolver(m, bcs, p, ic)
         g.set_kernel_timing(True)
-        r = g.step(40)
+        r, dt = g.step(40, with_dt=True)
         names = g.kernel_times()
         f = g.field()
-        return np.concatenate([x.ravel() for x in (f if isinstance(f, tuple) else (f,))]), r, names
+        return np.concatenate([x.ravel() for x in (f if isinstance(f, tuple) else (f,))]), r, names, dt
     for local_dt in (1, 0):
-        ref, rref, _ = run(1, 256, local_dt)
+        ref, rref, _, dref = run(1, 256, local_dt)
         for bs in (64, 128, 256):
-            out, r, names = run(3, bs, local_dt)
+            out, r, names, dt = run(3, bs, local_dt)
             assert ("visc_tile" in names) == (local_dt == 1), (bs, sorted(names))
             assert np.array_equal(out, ref), (equation, gradient, bs, local_dt)
             assert np.allclose(r, rref, rtol=1e-13, atol=0.0), (equation, gradient, bs, local_dt)
+            assert np.array_equal(dt, dref), (equation, gradient, bs, local_dt)
 
 
 @pytest.mark.parametrize("equation", [abi.EQ_NAVIER_STOKES, abi.EQ_RANS_SA, abi.EQ_RANS_SST])
@@ -235,7 +236,9 @@ def test_viscous_rk3_local_dt(equation):
     else:
         case = cases.case_rans_wavy(equation, rk_stages=3, local_dt=1)
     g, o = pair(case)
-    rg = g.step(60)
-    ro = o.step(60)
+    rg, dg = g.step(60, with_dt=True)
+    ro, do = o.step(60, with_dt=True)
     assert cases.residual_close(rg, ro, 1e-10)
     assert_close(g, o, 1e-12)
+    # the step's dt (the smallest local dt), reduced from per-block minima
+    assert np.allclose(dg, do, rtol=1e-12, atol=0.0)
