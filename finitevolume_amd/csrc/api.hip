@@ -10,6 +10,7 @@
 #include <new>
 #include <string>
 
+#include "norms.h"
 #include "solver.h"
 
 namespace fvk {
@@ -310,7 +311,7 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
     int64_t max_blocks = std::max((std::max(s->nc, s->nf) + 63) / 64, (int64_t)s->mesh.ntiles + 1) + 8;
     // partial sums: up to six residual rows and the dt row
     s->partials = static_cast<double*>(dev_alloc(s, sizeof(double) * 7 * max_blocks));
-    s->dtmin = static_cast<unsigned long long*>(dev_alloc(s, 256));
+    s->dtmin = static_cast<unsigned long long*>(dev_alloc(s, sizeof(unsigned long long) * kDtWords));
     s->hist_cap = 1024;
     s->res_hist_host = static_cast<double*>(host_mapped_alloc(s, sizeof(double) * 6 * s->hist_cap,
                                                               reinterpret_cast<void**>(&s->res_hist)));
@@ -323,11 +324,12 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
         return set_error(FVK_ERR_NOMEM, "device allocation failed");
     }
     s->mesh.wdist = wd;
-    unsigned long long init[2];
+    std::vector<unsigned long long> init(kDtWords);
     double dm = DBL_MAX;
     std::memcpy(&init[0], &dm, 8);
-    init[1] = init[0];
-    rc = hip_check(hipMemcpy(s->dtmin, init, sizeof init, hipMemcpyHostToDevice), "init dt slots");
+    for (auto& w : init) w = init[0];
+    rc = hip_check(hipMemcpy(s->dtmin, init.data(), sizeof(unsigned long long) * init.size(), hipMemcpyHostToDevice),
+                   "init dt slots");
     if (rc == FVK_OK) rc = hip_check(hipMemset(s->U[0], 0, sizeof(double) * nv * s->ld), "init state");
     if (rc == FVK_OK && s->grad) rc = hip_check(hipMemset(s->grad, 0, sizeof(double) * ng * s->ld), "init gradients");
     // wall distance to the no-slip walls (RANS constructors, src/RANSTurbulenceSASolver.cpp:67)

@@ -119,7 +119,8 @@ __device__ __forceinline__ double dt_candidate(const State& L, const State* R, d
 // block minimum, then one atomic per block.  Positive doubles order like
 // their bit patterns and NaN bits exceed +inf and DBL_MAX, so a NaN
 // candidate never wins (std::min semantics of the reference's reduction).
-__device__ __forceinline__ void block_min_to(double v, unsigned long long* slot) {
+// the workgroup's minimum of v into the dt slot (its XCD's sub-slot)
+__device__ __forceinline__ void block_min_to(double v, unsigned long long* dtmin, int slot) {
     __shared__ double sm[kMaxBlock];
     sm[threadIdx.x] = v;
     __syncthreads();
@@ -127,7 +128,7 @@ __device__ __forceinline__ void block_min_to(double v, unsigned long long* slot)
         if (threadIdx.x < s) sm[threadIdx.x] = smin(sm[threadIdx.x], sm[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicMin(slot, dbits(sm[0]));
+    if (threadIdx.x == 0) dt_atomic_min(dtmin, slot, sm[0]);
 }
 
 // deterministic block sum of the four squared residual components into
@@ -228,7 +229,7 @@ __device__ __forceinline__ UpdateInputs load_update_inputs(const EulerArgs& a, i
     UpdateInputs x{};
     if (a.R) return x;
     x.vol = a.m.vol[c];
-    if (a.dt_src == 1) x.dt = __longlong_as_double((long long)a.dtmin[a.slot]);
+    if (a.dt_src == 1) x.dt = dt_read(a.dtmin, a.slot);
     else if (a.dt_src == 0) x.dt = a.dtc[c];
     if (!a.first) x.u0 = a.U0[c];
     return x;
@@ -288,7 +289,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_dt(EulerArgs a) {
         }
         cand = smin(cand, c);
     }
-    block_min_to(cand, &a.dtmin[a.slot]);
+    block_min_to(cand, a.dtmin, a.slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -447,7 +448,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_cell(EulerArgs a) {
         update_cell(a, c, uc4, r0, r1, r2, r3, dmin);
     }
     // local dt: the global minimum is still reported (dt history)
-    if (a.face_dt) block_min_to(dmin, &a.dtmin[a.slot]);
+    if (a.face_dt) block_min_to(dmin, a.dtmin, a.slot);
     finish_residual(a, r0, r1, r2, r3);
 }
 
@@ -501,7 +502,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_face(EulerArgs a) {
             if (a.dt_src == 2) a.dtf[f] = cand;
         }
     }
-    if (a.face_dt) block_min_to(cand, &a.dtmin[a.slot]);
+    if (a.face_dt) block_min_to(cand, a.dtmin, a.slot);
 }
 
 // ---------------------------------------------------------------------------
@@ -512,7 +513,7 @@ template <int D>
 __global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
     const MeshView& m = a.m;
     const int c = xcd_index();
-    if (a.rearm_slot >= 0 && c == 0) a.dtmin[a.rearm_slot] = dbits(DBL_MAX);  // unused until the next step
+    if (a.rearm_slot >= 0 && c == 0) dt_arm(a.dtmin, a.rearm_slot);  // unused until the next step
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
     int side = 0;
     bool act = true;

@@ -518,7 +518,7 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
             if (threadIdx.x < s) sm[threadIdx.x] = smin(sm[threadIdx.x], sm[threadIdx.x + s]);
             __syncthreads();
         }
-        if (threadIdx.x == 0) atomicMin(&a.dtmin[a.slot], dbits(sm[0]));
+        if (threadIdx.x == 0) dt_atomic_min(a.dtmin, a.slot, sm[0]);
     }
 }
 
@@ -572,7 +572,7 @@ __device__ __forceinline__ ViscUpdateInputs<NV> load_visc_update_inputs(const Vi
     if (EQ == EQ_SA) x.s0 = a.aux[c];
     if (EQ == EQ_SST) { x.s0 = a.aux[c]; x.s1 = a.aux[ld + c]; }
     if (!a.R) {
-        if (a.dt_src == 1) x.dt = __longlong_as_double((long long)a.dtmin[a.slot]);
+        if (a.dt_src == 1) x.dt = dt_read(a.dtmin, a.slot);
         else if (a.dt_src == 0) x.dt = a.dtc[c];
 #pragma unroll
         for (int q = 0; q < NV; ++q) x.u0[q] = a.first ? 0.0 : a.U0[q * ld + c];

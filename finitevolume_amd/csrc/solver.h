@@ -189,7 +189,7 @@ struct fvk_solver {
     double* grad = nullptr;                      // [ng][ld] cell gradients
     double* aux = nullptr;                       // [na][ld] per-cell auxiliaries (sources, nu_t, F1 ...)
     double* partials = nullptr;                  // [nv][max_blocks]
-    unsigned long long* dtmin = nullptr;         // [2] global dt (bits) per step parity
+    unsigned long long* dtmin = nullptr;         // [kDtWords] global dt (bits) per step parity, per-XCD sub-slots
     double* res_hist = nullptr;                  // [hist_cap][nv] device view of res_hist_host
     double* dt_hist = nullptr;                   // [hist_cap] device view of dt_hist_host
     // the step history lives in pinned, mapped host memory: the finalize
