@@ -178,7 +178,9 @@ __device__ __forceinline__ State visc_ghost(const DevBC& bc, const State& L, dou
 
 // ---------------------------------------------------------------------------
 // gradients (GradientCalculator::compute, src/GradientReconstruction.cpp:83-150)
-// of NF fields at once, plus per-cell turbulence sources
+// of NF fields at once, plus per-cell turbulence sources.  PERIODIC selects
+// nothing any more (the per-entry LSQ data carry the periodic shift); the
+// launcher keeps the two instances
 // ---------------------------------------------------------------------------
 template <int EQ, int GS, bool PERIODIC>
 __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
@@ -207,54 +209,47 @@ __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
     double t0c = 0.0, t1c = 0.0;
     if (EQ == EQ_SA) { t0c = a.U[4 * a.ld + c]; pc[3] = t0c; }
     if (EQ == EQ_SST) { t0c = a.U[4 * a.ld + c]; t1c = a.U[5 * a.ld + c]; pc[3] = t0c; pc[NF - 1] = t1c; }
-    const double cx = m.cx[c], cy = m.cy[c];
     double sx[NF], sy[NF];
 #pragma unroll
     for (int q = 0; q < NF; ++q) { sx[q] = 0.0; sy[q] = 0.0; }
-    const int k0 = m.cptr[c], k1 = m.cptr[c + 1];
-    for (int k = k0; k < k1; ++k) {
-        int e = m.cent[k];
-        int f = e >> 1;
-        bool owner = e & 1;
-        int cr = m.fr[f];
+    // the cell's faces in Cell::faces order from the ELL (neighbour and face in
+    // one load); LSQ takes the per-entry weight times offset (egeo, periodic
+    // shift included), Green-Gauss the face's weight and normal
+    const int64_t ldm = m.ld;
+    for (int k = 0; k < m.deg; ++k) {
+        const int2 en = m.ell[k * ldm + c];
+        if (en.x < 0) break;  // ELL padding (after the cell's faces)
+        const int f = en.x >> 1;
+        const int cr = en.y;  // neighbour, or -1 - patch on a boundary
         double pn[NF];
-        double qx, qy;
         if (cr < 0) {
             double bv[7];
             boundary_values<EQ>(a, m.bc[-1 - cr], f, Uc, pc[0], pc[1], pc[2], t0c, t1c, bv);
 #pragma unroll
             for (int q = 0; q < NF; ++q) pn[q] = bv[q];
-            qx = m.fmx[f];
-            qy = m.fmy[f];
         } else {
-            int o = owner ? cr : m.fl[f];
+            const int o = cr;
             State Uo = ldS(a.U, a.ld, o);
             pn[0] = fdiv(Uo.ru, Uo.r);
             pn[1] = fdiv(Uo.rv, Uo.r);
             pn[2] = temperature(Uo, g, a.Rgas);
             if (EQ == EQ_SA) pn[3] = a.U[4 * a.ld + o];
             if (EQ == EQ_SST) { pn[3] = a.U[4 * a.ld + o]; pn[NF - 1] = a.U[5 * a.ld + o]; }
-            qx = m.cx[o];
-            qy = m.cy[o];
-            if (PERIODIC) {
-                if (owner) { qx = qx + m.fsx[f]; qy = qy + m.fsy[f]; }
-                else { qx = qx - m.fsx[f]; qy = qy - m.fsy[f]; }
-            }
         }
         if (GS == FVK_GRAD_LEAST_SQUARES) {
-            double dx = qx - cx, dy = qy - cy;
-            double w = 1.0 / (dx * dx + dy * dy);
+            const double4 eg = m.egeo[k * ldm + c];  // {w dx, w dy, ...}, w = 1/(dx^2 + dy^2)
 #pragma unroll
             for (int q = 0; q < NF; ++q) {
                 double dphi = pn[q] - pc[q];
-                sx[q] += w * dx * dphi;
-                sy[q] += w * dy * dphi;
+                sx[q] += eg.x * dphi;
+                sy[q] += eg.y * dphi;
             }
         } else {
-            double wc = owner ? m.fwl[f] : m.fwr[f];
-            double nx = owner ? m.fnx[f] : -m.fnx[f];
-            double ny = owner ? m.fny[f] : -m.fny[f];
-            double A = m.fa[f];
+            const bool owner = en.x & 1;
+            const double wc = owner ? m.fwl[f] : m.fwr[f];
+            const double nx = owner ? m.fnx[f] : -m.fnx[f];
+            const double ny = owner ? m.fny[f] : -m.fny[f];
+            const double A = m.fa[f];
 #pragma unroll
             for (int q = 0; q < NF; ++q) {
                 double pf = (cr < 0) ? pn[q] : wc * pc[q] + (1.0 - wc) * pn[q];
