@@ -634,8 +634,9 @@ __global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
     const int64_t ldf = a.ldf;
     double dmin = DBL_MAX;
     if (c < m.nc) {
-        for (int k = m.cptr[c]; k < m.cptr[c + 1]; ++k) {
-            int e = m.cent[k];
+        for (int k = 0; k < m.deg; ++k) {  // Cell::faces order, from the ELL (coalesced across cells)
+            const int e = m.ell[k * m.ld + c].x;
+            if (e < 0) break;  // ELL padding (after the cell's faces)
             int f = e >> 1;
             if (e & 1) {
 #pragma unroll
