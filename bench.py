@@ -65,6 +65,7 @@ def naca_case(ni, nj, order2=True):
 
 def time_gpu(solver, steps, warmup, per_step=True):
     solver.step(warmup)
+    solver.synchronize()
     t0 = time.perf_counter()
     if per_step:
         for _ in range(steps):
@@ -74,6 +75,7 @@ def time_gpu(solver, steps, warmup, per_step=True):
                 raise RuntimeError(f"residual not finite: {r}")
     else:
         solver.step(steps)
+    solver.synchronize()
     return time.perf_counter() - t0
 
 
@@ -297,6 +299,7 @@ def batch_leg(n_cases, max_steps, drop=1e-6):
                 done += 100
                 break
             done += 100
+        g.synchronize()
         seq_time[k] = time.perf_counter() - t1
         del g
         progress(k + 1, n_cases - k - 1, time.perf_counter() - t_seq0)

@@ -72,8 +72,10 @@ def _store(db: Dict) -> None:
 def _time_solver(make, steps: int, warmup: int) -> float:
     s = make()
     s.step(warmup)
+    s.synchronize()
     t0 = time.perf_counter()
     s.step(steps)
+    s.synchronize()
     return (time.perf_counter() - t0) / steps
 
 

@@ -65,6 +65,9 @@ class FVKBatch:
         _check(self.L.fvk_batch_step(self.h, n, abi.as_dptr(r)))
         return r
 
+    def synchronize(self) -> None:
+        """Nothing to wait for: a batch step returns with its stream drained."""
+
     def set_active(self, active) -> None:
         a = np.ascontiguousarray(active, dtype=np.int32)
         _check(self.L.fvk_batch_set_active(self.h, a.ctypes.data))

@@ -156,6 +156,7 @@ void fvk_destroy(fvk_solver* s) {
     fvk::adjoint_release(s);
     for (void* p : s->allocations) hipFree(p);
     for (void* p : s->host_allocations) hipHostFree(p);
+    if (s->row_ready) hipEventDestroy(s->row_ready);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
 }
@@ -273,6 +274,7 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
     }
     int rc = hip_check(hipGetDevice(&s->device), "hipGetDevice");
     if (rc == FVK_OK) rc = hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (rc == FVK_OK) rc = hip_check(hipEventCreateWithFlags(&s->row_ready, hipEventDisableTiming), "hipEventCreate");
     if (rc == FVK_OK) rc = build_layout(s, mesh);
     if (rc != FVK_OK) {
         std::string msg = fvk::g_err;
@@ -402,10 +404,17 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
 
 }  // namespace fvk
 
+int fvk::settle(const fvk_solver* s) {
+    if (!s->inflight) return FVK_OK;
+    s->inflight = false;
+    return hip_check(hipStreamSynchronize(s->stream), "step");
+}
+
 extern "C" {
 
 int fvk_set_field(fvk_solver* s, const double* U4, const double* turb, int64_t n_cells) {
     if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     if (n_cells != s->nc)
         return set_error(FVK_ERR_INVALID, "set_field: %lld cells given, mesh has %lld", (long long)n_cells, (long long)s->nc);
     int rc = upload_state(s, U4, turb, turb == nullptr);
@@ -416,6 +425,7 @@ int fvk_set_field(fvk_solver* s, const double* U4, const double* turb, int64_t n
 int fvk_get_field(const fvk_solver* sc, double* U4, double* turb, int64_t n_cells) {
     fvk_solver* s = const_cast<fvk_solver*>(sc);
     if (!s || !U4) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     if (n_cells != s->nc)
         return set_error(FVK_ERR_INVALID, "get_field: %lld cells given, mesh has %lld", (long long)n_cells, (long long)s->nc);
     const int64_t nc = s->nc, ld = s->ld;
@@ -445,6 +455,7 @@ int fvk_get_field(const fvk_solver* sc, double* U4, double* turb, int64_t n_cell
 
 int fvk_set_state(fvk_solver* s, const double* U_aos, int64_t n_cells) {
     if (!s || !U_aos) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     if (n_cells != s->nc)
         return set_error(FVK_ERR_INVALID, "set_state: %lld cells given, mesh has %lld", (long long)n_cells, (long long)s->nc);
     const int nv = s->nv;
@@ -460,6 +471,7 @@ int fvk_set_state(fvk_solver* s, const double* U_aos, int64_t n_cells) {
 
 int fvk_get_state(const fvk_solver* s, double* U_aos, int64_t n_cells) {
     if (!s || !U_aos) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     if (n_cells != s->nc)
         return set_error(FVK_ERR_INVALID, "get_state: %lld cells given, mesh has %lld", (long long)n_cells, (long long)s->nc);
     const int nv = s->nv;
@@ -485,6 +497,7 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
     int done = 0;
     while (done < nsteps) {
         int chunk = std::min(s->hist_cap, nsteps - done);
+        bool early = false;
         for (int i = 0; i < chunk; ++i) {
             int rc = FVK_OK;
             if (s->p.equation == FVK_EQ_DIFFUSION) {
@@ -495,6 +508,13 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
                     int64_t step = s->step_count;
                     if (s->p.equation == FVK_EQ_EULER) rc = launch_euler_stage(s, st, stages, 0, 1, step);
                     else rc = launch_visc_stage(s, st, stages, 1, step);
+                    // the chunk's last row is final after stage 1's launches, which reduce
+                    // stage 0's norms; the host need not wait for the later stages
+                    if (rc == FVK_OK && i == chunk - 1 && st == 1 && stages > 2 && !s->timing &&
+                        s->n_cases == 0 && s->fin_nblocks == 0) {
+                        rc = hip_check(hipEventRecord(s->row_ready, s->stream), "step event");
+                        early = rc == FVK_OK;
+                    }
                 }
             }
             if (rc != FVK_OK) return rc;
@@ -506,8 +526,9 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
             if (rc != FVK_OK) return rc;
         }
         // the finalize kernels wrote the rows into mapped host memory: one sync per chunk
-        hipError_t e = hipStreamSynchronize(s->stream);
+        hipError_t e = early ? hipEventSynchronize(s->row_ready) : hipStreamSynchronize(s->stream);
         if (e != hipSuccess) return hip_check(e, "step");
+        s->inflight = early;
         if (s->timing) timers_collect(s);
         // rows were written at index (global step % hist_cap)
         for (int i = 0; i < chunk; ++i) {
@@ -520,6 +541,11 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
         done += chunk;
     }
     return FVK_OK;
+}
+
+int fvk_synchronize(fvk_solver* s) {
+    if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
+    return fvk::settle(s);
 }
 
 int fvk_residual(const fvk_solver* s, double* norms) {
@@ -557,6 +583,7 @@ int64_t fvk_device_bytes(const fvk_solver* s) { return s ? s->device_bytes : 0; 
 
 int fvk_residual_eval(fvk_solver* s, double* R) {
     if (!s || !R) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     const int nv = s->nv;
     double* d = nullptr;
     int rc = hip_check(hipMalloc((void**)&d, sizeof(double) * nv * s->ld), "residual buffer");
@@ -580,6 +607,7 @@ int fvk_residual_eval(fvk_solver* s, double* R) {
 int fvk_gradient(fvk_solver* s, int32_t scheme, const double* phi, const double* bphi, double* grad) {
     if (!s || !phi || !bphi || !grad) return set_error(FVK_ERR_INVALID, "NULL argument");
     if (scheme < 0 || scheme > 1) return set_error(FVK_ERR_INVALID, "unknown gradient scheme %d", scheme);
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     const int64_t nc = s->nc, nf = s->nf;
     std::vector<double> hp(nc), hb(nf), hg(2 * nc);
     for (int64_t i = 0; i < nc; ++i) hp[i] = phi[s->cell_new2old[i]];
@@ -607,6 +635,7 @@ int fvk_gradient(fvk_solver* s, int32_t scheme, const double* phi, const double*
 
 int fvk_wall_distance(fvk_solver* s, double* dist) {
     if (!s || !dist) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     std::vector<double> h(s->nc);
     int rc = hip_check(hipMemcpy(h.data(), s->mesh.wdist, sizeof(double) * s->nc, hipMemcpyDeviceToHost), "wall distance");
     if (rc != FVK_OK) return rc;
@@ -647,12 +676,14 @@ int fvk_wall_distance_faces(const fvk_mesh_desc* mesh, const int32_t* wall_faces
 
 int fvk_residual_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar) {
     if (!s || !lambda || !U_bar) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     return adjoint_vjp(s, lambda, U_bar, X_bar);
 }
 
 int fvk_wall_forces(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref, double* cl,
                     double* cd) {
     if (!s || !cl || !cd) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     if (s->p.equation == FVK_EQ_DIFFUSION) return set_error(FVK_ERR_UNSUPPORTED, "no pressure for diffusion");
     // src/WallTraction.cpp:163-216, pressure part: A p_cell n per wall face
     std::vector<double> U(4 * (size_t)s->nc);
@@ -701,6 +732,7 @@ int fvk_wall_forces(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref,
 int fvk_wall_forces_vjp(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref, double w_cl,
                         double w_cd, double* U_bar, double* X_bar) {
     if (!s || (!U_bar && !X_bar)) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     if (s->p.equation == FVK_EQ_DIFFUSION) return set_error(FVK_ERR_UNSUPPORTED, "no pressure for diffusion");
     // J = w_cl Cl + w_cd Cd = sum over wall faces of p_cell (A n) . e, with
     // A n = (ey, -ex) for the edge e = node2 - node1 and
@@ -748,6 +780,7 @@ int fvk_wall_forces_vjp(fvk_solver* s, double rho_ref, double vx_ref, double vy_
 
 int fvk_set_kernel_timing(fvk_solver* s, int32_t enable) {
     if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     hipStreamSynchronize(s->stream);
     timers_collect(s);
     s->timing = enable != 0;
@@ -761,6 +794,7 @@ int fvk_kernel_times(const fvk_solver* sc, char* names, int64_t names_cap, doubl
                      int32_t cap, int32_t* n_kernels) {
     fvk_solver* s = const_cast<fvk_solver*>(sc);
     if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     hipStreamSynchronize(s->stream);
     timers_collect(s);
     std::string all;

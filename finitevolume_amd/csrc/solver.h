@@ -167,6 +167,13 @@ struct fvk_solver {
     bool periodic = false;
     hipStream_t stream = nullptr;
     int device = 0;
+    // With three or more RK stages the last step's residual row is final once
+    // stage 1's launches end (only stage 0 leaves norms pending), so fvk_step
+    // waits on row_ready, recorded there, and returns with the later stages
+    // still running (inflight); every other call that touches the device
+    // waits for the stream first (settle)
+    hipEvent_t row_ready = nullptr;
+    mutable bool inflight = false;
 
     // host copies
     std::vector<int> cell_new2old, cell_old2new;
@@ -249,6 +256,8 @@ int launch_residual_eval(fvk_solver* s, double* R_dev);
 int launch_euler_gradients(fvk_solver* s);
 int launch_case_finalize(fvk_solver* s);
 int flush_pending_finalize(fvk_solver* s);
+// waits for work that fvk_step left running on the solver's stream
+int settle(const fvk_solver* s);
 int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, int slot, double fixed_dt,
                          bool dt_row = false);
 int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);

@@ -43,6 +43,7 @@ def lib():
     L.fvk_destroy.restype = None
     L.fvk_step.argtypes = [vp, C.c_int32, dp, dp]
     L.fvk_residual.argtypes = [vp, dp]
+    L.fvk_synchronize.argtypes = [vp]
     L.fvk_set_field.argtypes = [vp, dp, dp, i64]
     L.fvk_get_field.argtypes = [vp, dp, dp, i64]
     L.fvk_set_cfl.argtypes = [vp, C.c_double]
@@ -118,6 +119,11 @@ class FVKSolver:
 
     def run(self, total_steps: int):
         self.step(total_steps)
+
+    def synchronize(self):
+        """Waits for the stages that step() left running (step() returns
+        once its residual rows are final)."""
+        _check(self.L.fvk_synchronize(self.h))
 
     def residual(self) -> np.ndarray:
         r = np.empty(self.nv)

@@ -199,8 +199,16 @@ void fvk_destroy(fvk_solver* s);
 /* Advance nsteps steps (one reference step() each).  residual_hist, if not
  * NULL, receives nsteps * fvk_num_vars rows of L2 residual norms; dt_hist,
  * if not NULL, receives the nsteps time steps (for local_dt: the minimum
- * local dt).  One device->host copy at the end of the call. */
+ * local dt).  One device->host copy at the end of the call.  The call
+ * returns once those rows are final: with three or more RK stages the last
+ * step's later stages may still be running.  Every other call on the solver
+ * waits for them before it reads or writes the device. */
 int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hist);
+
+/* Waits for the work fvk_step left running.  No reference counterpart (the
+ * reference's step() returns with the state updated); for callers that time
+ * the device or hand its stream's results to other code. */
+int fvk_synchronize(fvk_solver* s);
 
 /* Last step's residual norms [fvk_num_vars]: rho, rho_u, rho_v, E (, nut | k, omega);
  * all zero before the first step and after fvk_set_field. */
