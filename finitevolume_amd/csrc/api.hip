@@ -840,4 +840,19 @@ int fvk_copy_probe(int64_t n_doubles, int32_t reps, double* ms_per_copy) {
     return rc;
 }
 
+int fvk_divide_probe(int64_t n, const double* x, const double* y, double* q_fast, double* q_ieee) {
+    if (n < 0 || (n > 0 && (!x || !y || !q_fast || !q_ieee))) return set_error(FVK_ERR_INVALID, "bad probe arguments");
+    if (n == 0) return FVK_OK;
+    const size_t b = sizeof(double) * (size_t)n;
+    double* d = nullptr;
+    int rc = hip_check(hipMalloc(&d, 4 * b), "divide probe buffers");
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(d, x, b, hipMemcpyHostToDevice), "divide probe copy");
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(d + n, y, b, hipMemcpyHostToDevice), "divide probe copy");
+    if (rc == FVK_OK) rc = launch_div_probe(d, d + n, d + 2 * n, d + 3 * n, n, nullptr);
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(q_fast, d + 2 * n, b, hipMemcpyDeviceToHost), "divide probe copy");
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(q_ieee, d + 3 * n, b, hipMemcpyDeviceToHost), "divide probe copy");
+    if (d) hipFree(d);
+    return rc;
+}
+
 }  // extern "C"

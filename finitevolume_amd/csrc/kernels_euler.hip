@@ -1126,6 +1126,21 @@ __global__ __launch_bounds__(kCopyBlock) void k_copy(double2* __restrict__ dst, 
     for (; i < n2; i += stride) dst[i] = src[i];
 }
 
+// fdiv (the flux kernels' division) beside the compiler's x / y on the
+// caller's operand pairs, so both can be compared bit for bit.
+__global__ void k_div_probe(const double* x, const double* y, double* q_fast, double* q_ieee, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        q_fast[i] = fdiv(x[i], y[i]);
+        q_ieee[i] = x[i] / y[i];
+    }
+}
+
+int launch_div_probe(const double* x, const double* y, double* q_fast, double* q_ieee, int64_t n, hipStream_t st) {
+    const int64_t g = std::min<int64_t>((n + 255) / 256, 4096);
+    k_div_probe<<<(int)std::max<int64_t>(g, 1), 256, 0, st>>>(x, y, q_fast, q_ieee, n);
+    return hip_check(hipGetLastError(), "divide probe launch");
+}
+
 int launch_copy_probe(double* dst, const double* src, int64_t n, hipStream_t st) {
     int64_t n2 = n / 2;
     int grid = 256 * 16;

@@ -263,6 +263,7 @@ int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, i
 int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
 void adjoint_release(fvk_solver* s);
 int launch_copy_probe(double* dst, const double* src, int64_t n, hipStream_t st);
+int launch_div_probe(const double* x, const double* y, double* q_fast, double* q_ieee, int64_t n, hipStream_t st);
 
 // timers
 void timer_begin(fvk_solver* s, const char* name, hipEvent_t* ev_stop);

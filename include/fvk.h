@@ -298,6 +298,14 @@ int64_t fvk_device_bytes(const fvk_solver* s);
 /* A plain device copy of n doubles (bandwidth calibration for counters). */
 int fvk_copy_probe(int64_t n_doubles, int32_t reps, double* ms_per_copy);
 
+/* The device division the flux kernels use (a refined reciprocal without
+ * the range scaling and special-case fixup of the full IEEE sequence) and
+ * the compiler's IEEE x / y, on n caller pairs (host arrays).  Both agree
+ * bit for bit when x == 0 or when |x|, |y| and |x / y| lie in
+ * [2^-900, 2^900]; a zero divisor gives NaN instead of +-inf.  No reference
+ * counterpart: it checks that precondition. */
+int fvk_divide_probe(int64_t n, const double* x, const double* y, double* q_fast, double* q_ieee);
+
 /* Extension (no reference counterpart): independent cases stepped together.
  * The cases' meshes are the disconnected parts of one device mesh, each a
  * contiguous range of cells; every kernel launch covers all cases, the
