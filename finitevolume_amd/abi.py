@@ -86,6 +86,47 @@ class InitialConditionC(C.Structure):
                                   "initial_radius")]
 
 
+class WallReferenceC(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("rho_ref", "velocity_x_ref", "velocity_y_ref", "p_ref", "length_ref",
+                                          "moment_reference_x", "moment_reference_y")]
+
+
+WALL_SAMPLE_FIELDS = ("x_mid", "y_mid", "p", "rho", "effective_viscosity", "tau_wall", "tx", "ty", "y_wall_normal",
+                      "cf", "cp", "y_plus")
+WALL_REPORT_FIELDS = ("friction_drag", "pressure_drag", "total_drag", "cd_friction", "cd_pressure", "cd_total", "lift",
+                      "cl", "moment", "cm")
+
+
+class WallSampleC(C.Structure):
+    _fields_ = [("face_index", C.c_int32), ("patch_id", C.c_int32)] + [(n, C.c_double) for n in WALL_SAMPLE_FIELDS]
+
+
+class WallReportC(C.Structure):
+    _fields_ = [("patch_id", C.c_int32), ("reserved0", C.c_int32)] + [(n, C.c_double) for n in WALL_REPORT_FIELDS]
+
+
+# numpy views of arrays of the two structs (same layout)
+WALL_SAMPLE_DTYPE = np.dtype([("face_index", np.int32), ("patch_id", np.int32)] +
+                             [(n, np.float64) for n in WALL_SAMPLE_FIELDS])
+WALL_REPORT_DTYPE = np.dtype([("patch_id", np.int32), ("reserved0", np.int32)] +
+                             [(n, np.float64) for n in WALL_REPORT_FIELDS])
+
+
+def wall_reference(rho_ref=1.0, velocity_x_ref=1.0, velocity_y_ref=0.0, p_ref=0.0, length_ref=1.0,
+                   moment_reference_x=0.0, moment_reference_y=0.0) -> WallReferenceC:
+    """WallReferenceQuantities with the reference's defaults (include/WallTraction.h:99-106)."""
+    return WallReferenceC(rho_ref, velocity_x_ref, velocity_y_ref, p_ref, length_ref, moment_reference_x,
+                          moment_reference_y)
+
+
+def collect_wall_faces(mesh, bcs) -> np.ndarray:
+    """Boundary faces on wall patches, in face order (collect_wall_faces,
+    src/main.cpp:279-312)."""
+    fp = mesh.face_patch
+    b = np.nonzero(mesh.face_cells[:, 1] < 0)[0]
+    return np.ascontiguousarray([f for f in b if bcs[fp[f]].type == BC_WALL], dtype=np.int32)
+
+
 def default_params(**kw) -> ParamsC:
     """Reference defaults (mirrors fvk_params_default in the library)."""
     p = ParamsC()

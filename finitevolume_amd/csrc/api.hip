@@ -288,6 +288,16 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
         return set_error(FVK_ERR_UNSUPPORTED, "a cell has %d faces; the Euler kernels take at most %d", deg,
                          kMaxEulerDegree);
     }
+    // reference face id -> device face id, for entry points that take
+    // reference face lists in device memory
+    s->d_face_old2new = static_cast<int*>(dev_alloc(s, sizeof(int) * std::max<int64_t>(s->nf, 1)));
+    if (!s->d_face_old2new ||
+        hip_check(hipMemcpy(s->d_face_old2new, s->face_old2new.data(), sizeof(int) * s->nf, hipMemcpyHostToDevice),
+                  "face map upload") != FVK_OK) {
+        std::string msg = fvk::g_err.empty() ? std::string("device allocation failed") : fvk::g_err;
+        fvk_destroy(s);
+        return set_error(FVK_ERR_NOMEM, "%s", msg.c_str());
+    }
     const int64_t pad = 256;
     s->ld = (s->nc + pad - 1) / pad * pad;
     s->ldf = (s->nf + pad - 1) / pad * pad;
@@ -403,6 +413,79 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
 }
 
 }  // namespace fvk
+
+// The launch parameters of compute_wall_forces' reference quantities
+// (src/WallTraction.cpp:168-172), derived on the host as the reference does.
+void fvk::wall_launch_ref(WallLaunch& w, const fvk_wall_reference& ref) {
+    const double v_mag = std::hypot(ref.velocity_x_ref, ref.velocity_y_ref);
+    w.dir_x = ref.velocity_x_ref / v_mag;
+    w.dir_y = ref.velocity_y_ref / v_mag;
+    w.lift_x = -w.dir_y;
+    w.lift_y = w.dir_x;
+    w.q = 0.5 * ref.rho_ref * v_mag * v_mag * ref.length_ref;
+    w.q2 = w.q * ref.length_ref;
+    w.mx = ref.moment_reference_x;
+    w.my = ref.moment_reference_y;
+    w.rho_ref = ref.rho_ref;
+    w.v2 = ref.velocity_x_ref * ref.velocity_x_ref + ref.velocity_y_ref * ref.velocity_y_ref;
+    w.p_ref = ref.p_ref;
+}
+
+// Wall samples and force reports of host face lists, through a workspace
+// allocated for the call.
+int fvk::wall_reports(fvk_solver* s, const int32_t* wf, int64_t n, const fvk_wall_reference* ref, int pressure_only,
+                      fvk_wall_sample* samples, fvk_wall_force_report* reports, int32_t cap, int32_t* n_reports) {
+    if (s->p.equation == FVK_EQ_DIFFUSION) return set_error(FVK_ERR_UNSUPPORTED, "no wall forces for diffusion");
+    if (n < 0 || n > (int64_t)1 << 30 || cap < 0) return set_error(FVK_ERR_INVALID, "bad wall face count");
+    for (int64_t i = 0; i < n; ++i) {
+        if (wf[i] < 0 || wf[i] >= s->nf) return set_error(FVK_ERR_INVALID, "wall face %d out of range", wf[i]);
+        if (s->face_old2new[wf[i]] < s->nfi)
+            return set_error(FVK_ERR_INVALID, "face %d is an interior face, not a wall face", wf[i]);
+    }
+    if (int rc = settle(s); rc != FVK_OK) return rc;
+    const int np = s->n_bcs;
+    const size_t nn = (size_t)std::max<int64_t>(n, 1);
+    const size_t b_faces = sizeof(int) * nn, b_contrib = sizeof(double) * 5 * nn, b_patch = sizeof(int) * nn;
+    const size_t b_samples = samples ? sizeof(fvk_wall_sample) * nn : 0;
+    const size_t b_rep = sizeof(fvk_wall_force_report) * (size_t)(np + 1);
+    const size_t align = 256;
+    auto up = [&](size_t b) { return (b + align - 1) / align * align; };
+    const size_t total = up(b_faces) + up(b_contrib) + up(b_patch) + up(b_samples) + up(b_rep) + up(sizeof(int));
+    char* ws = nullptr;
+    int rc = hip_check(hipMalloc(&ws, total), "wall diagnostics workspace");
+    if (rc != FVK_OK) return rc;
+    char* p = ws;
+    auto take = [&](size_t b) { char* q = p; p += up(b); return q; };
+    WallLaunch w{};
+    int* d_faces = reinterpret_cast<int*>(take(b_faces));
+    w.contrib = reinterpret_cast<double*>(take(b_contrib));
+    w.patch = reinterpret_cast<int*>(take(b_patch));
+    w.samples = samples ? reinterpret_cast<fvk_wall_sample*>(take(b_samples)) : nullptr;
+    w.reports = reinterpret_cast<fvk_wall_force_report*>(take(b_rep));
+    w.n_reports = reinterpret_cast<int*>(take(sizeof(int)));
+    w.wall_faces = d_faces;
+    w.face_old2new = s->d_face_old2new;
+    w.n = (int)n;
+    w.n_patches = np;
+    w.pressure_only = pressure_only;
+    w.cap = np + 1;
+    wall_launch_ref(w, *ref);
+    if (n > 0) rc = hip_check(hipMemcpyAsync(d_faces, wf, sizeof(int) * n, hipMemcpyHostToDevice, s->stream), "copy");
+    if (rc == FVK_OK) rc = launch_wall_diagnostics(s, w);
+    int nrep = 0;
+    if (rc == FVK_OK)
+        rc = hip_check(hipMemcpyAsync(&nrep, w.n_reports, sizeof(int), hipMemcpyDeviceToHost, s->stream), "copy");
+    if (rc == FVK_OK) rc = hip_check(hipStreamSynchronize(s->stream), "wall diagnostics");
+    if (rc == FVK_OK && nrep > cap) rc = set_error(FVK_ERR_INVALID, "%d reports do not fit cap %d", nrep, cap);
+    if (rc == FVK_OK && nrep > 0)
+        rc = hip_check(hipMemcpy(reports, w.reports, sizeof(fvk_wall_force_report) * nrep, hipMemcpyDeviceToHost),
+                       "copy");
+    if (rc == FVK_OK && samples && n > 0)
+        rc = hip_check(hipMemcpy(samples, w.samples, sizeof(fvk_wall_sample) * n, hipMemcpyDeviceToHost), "copy");
+    if (rc == FVK_OK) *n_reports = nrep;
+    hipFree(ws);
+    return rc;
+}
 
 int fvk::settle(const fvk_solver* s) {
     if (!s->inflight) return FVK_OK;
@@ -683,50 +766,38 @@ int fvk_residual_vjp(fvk_solver* s, const double* lambda, double* U_bar, double*
 int fvk_wall_forces(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref, double* cl,
                     double* cd) {
     if (!s || !cl || !cd) return set_error(FVK_ERR_INVALID, "NULL argument");
-    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     if (s->p.equation == FVK_EQ_DIFFUSION) return set_error(FVK_ERR_UNSUPPORTED, "no pressure for diffusion");
-    // src/WallTraction.cpp:163-216, pressure part: A p_cell n per wall face
-    std::vector<double> U(4 * (size_t)s->nc);
-    int rc = fvk_get_field(s, U.data(), nullptr, s->nc);
-    if (rc != FVK_OK) return rc;
-    double vm = std::hypot(vx_ref, vy_ref);
-    double dx = vx_ref / vm, dy = vy_ref / vm, lx = -dy, ly = dx;
-    double q = 0.5 * rho_ref * vm * vm * length_ref;
-    double drag = 0.0, lift = 0.0;
-    const double g = s->p.gamma;
-    const std::vector<int>& f2o = s->face_new2old;
-    (void)f2o;
-    // walk the wall faces in reference face order for a deterministic sum
-    std::vector<double> nx(s->nf), ny(s->nf), ar(s->nf);
-    const double* X = s->node_xy.data();
-    for (int64_t i = 0; i < s->nf; ++i) {
-        int f = s->face_new2old[i];
-        const double* n1 = X + 2 * s->face_nodes_dev[2 * i];
-        const double* n2 = X + 2 * s->face_nodes_dev[2 * i + 1];
-        double ddx = n2[0] - n1[0], ddy = n2[1] - n1[1];
-        ar[f] = std::sqrt(ddx * ddx + ddy * ddy);
-        nx[f] = ddy / ar[f];
-        ny[f] = -ddx / ar[f];
-    }
-    std::vector<int> fl(s->nf), fr(s->nf);
-    {
-        std::vector<int> dl(s->nf), dr(s->nf);
-        hipMemcpy(dl.data(), s->mesh.fl, sizeof(int) * s->nf, hipMemcpyDeviceToHost);
-        hipMemcpy(dr.data(), s->mesh.fr, sizeof(int) * s->nf, hipMemcpyDeviceToHost);
-        for (int64_t i = 0; i < s->nf; ++i) { fl[s->face_new2old[i]] = dl[i]; fr[s->face_new2old[i]] = dr[i]; }
-    }
+    // src/WallTraction.cpp:163-216 over pressure-only samples (A p_cell n) of
+    // every wall face, in reference face order
+    std::vector<int32_t> wf;
     for (int64_t f = 0; f < s->nf; ++f) {
-        if (fr[f] >= 0 || s->bcs[-1 - fr[f]].type != FVK_BC_WALL) continue;
-        const double* u = &U[4 * (size_t)s->cell_new2old[fl[f]]];
-        double uu = u[1] / u[0], vv = u[2] / u[0];
-        double p = (g - 1.0) * (u[3] - 0.5 * u[0] * (uu * uu + vv * vv));
-        double px = ar[f] * p * nx[f], py = ar[f] * p * ny[f];
-        drag += px * dx + py * dy;
-        lift += px * lx + py * ly;
+        const int i = s->face_old2new[f];
+        if (i >= s->nfi && s->bcs[s->face_patch_old[f]].type == FVK_BC_WALL) wf.push_back((int32_t)f);
     }
-    *cd = drag / q;
-    *cl = lift / q;
+    fvk_wall_reference ref{rho_ref, vx_ref, vy_ref, 0.0, length_ref, 0.0, 0.0};
+    std::vector<fvk_wall_force_report> rep(s->n_bcs + 1);
+    int32_t nrep = 0;
+    int rc = wall_reports(s, wf.data(), (int64_t)wf.size(), &ref, 1, nullptr, rep.data(), (int32_t)rep.size(), &nrep);
+    if (rc != FVK_OK) return rc;
+    *cl = nrep > 0 ? rep[nrep - 1].cl : 0.0;
+    *cd = nrep > 0 ? rep[nrep - 1].cd_total : 0.0;
     return FVK_OK;
+}
+
+int fvk_wall_traction(fvk_solver* s, const int32_t* wall_faces, int64_t n_wall, const fvk_wall_reference* ref,
+                      fvk_wall_sample* samples) {
+    if (!s || !ref || (n_wall > 0 && (!wall_faces || !samples))) return set_error(FVK_ERR_INVALID, "NULL argument");
+    std::vector<fvk_wall_force_report> rep(s->n_bcs + 1);
+    int32_t nrep = 0;
+    return wall_reports(s, wall_faces, n_wall, ref, 0, samples, rep.data(), (int32_t)rep.size(), &nrep);
+}
+
+int fvk_wall_force_reports(fvk_solver* s, const int32_t* wall_faces, int64_t n_wall, const fvk_wall_reference* ref,
+                           fvk_wall_sample* samples, fvk_wall_force_report* reports, int32_t cap,
+                           int32_t* n_reports) {
+    if (!s || !ref || !n_reports || (n_wall > 0 && !wall_faces) || (cap > 0 && !reports))
+        return set_error(FVK_ERR_INVALID, "NULL argument");
+    return wall_reports(s, wall_faces, n_wall, ref, 0, samples, reports, cap, n_reports);
 }
 
 int fvk_wall_forces_vjp(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref, double w_cl,

@@ -1139,6 +1139,232 @@ int launch_gradient(fvk_solver* s, int scheme, const double* phi, const double* 
     return hip_check(hipGetLastError(), "gradient launch");
 }
 
+// ---------------------------------------------------------------------------
+// wall diagnostics: the samples of compute_wall_traction
+// (src/WallTraction.cpp:95-143) with the solvers' inputs
+// (src/NavierStokesFVMSolver.cpp:333-352, src/RANSTurbulenceSASolver.cpp:353-374,
+// src/RANSTurbulenceSSTSolver.cpp:435-464), one thread per wall face, then the
+// per-patch sums of compute_wall_forces (src/WallTraction.cpp:163-216) in the
+// caller's sample order by one workgroup
+// ---------------------------------------------------------------------------
+namespace {
+
+// SST F2 and eddy viscosity with the reference's guards
+// (src/SSTKOmega.cpp sst_F2, sst_eddy_viscosity)
+__device__ double sst_F2_ref(double k, double w, double nu, double d, const fvk_sst_constants& c) {
+    if (k <= 0.0 || w <= 0.0) return 0.0;
+    const double d2 = d * d;
+    const double arg2 = smax(2.0 * sqrt(k) / (c.beta_star * w * d), 500.0 * nu / (d2 * w));
+    return tanh(arg2 * arg2);
+}
+__device__ double sst_eddy_ref(double k, double w, double S, double Om, double F2, int variant,
+                               const fvk_sst_constants& c) {
+    if (k <= 0.0) return 0.0;
+    const double lim = (variant == FVK_SST_VORTICITY) ? Om : S;
+    return c.a1 * k / smax(c.a1 * w, lim * F2);
+}
+
+constexpr int kWallBlock = 256;
+
+template <int EQ>
+__global__ __launch_bounds__(kVB) void k_wall_samples(ViscArgs a, WallLaunch w) {
+    const int i = blockIdx.x * kVB + threadIdx.x;
+    if (i >= w.n) return;
+    const MeshView& m = a.m;
+    const int fref = w.wall_faces[i];
+    const bool valid = fref >= 0 && fref < m.nf && m.fr[w.face_old2new[fref]] < 0;
+    double* ct = w.contrib;
+    if (!valid) {
+        // not a boundary face: reported as face -1 with NaN values, left out of the sums
+        w.patch[i] = -1;
+        for (int q = 0; q < 5; ++q) ct[q * w.n + i] = 0.0;
+        if (w.samples) {
+            fvk_wall_sample o{};
+            o.face_index = -1;
+            o.patch_id = -1;
+            const double nan = __builtin_nan("");
+            o.x_mid = o.y_mid = o.p = o.rho = o.effective_viscosity = o.tau_wall = nan;
+            o.tx = o.ty = o.y_wall_normal = o.cf = o.cp = o.y_plus = nan;
+            w.samples[i] = o;
+        }
+        return;
+    }
+    const int f = w.face_old2new[fref];
+    const int cl = m.fl[f];
+    const int patch = -1 - m.fr[f];
+    const double g = a.gamma;
+    // the Euler solver keeps its state as one double4 per cell, the others as [nv][ld]
+    const State U = w.aos ? [&] {
+        const double4 q = reinterpret_cast<const double4*>(a.U)[cl];
+        return State{q.x, q.y, q.z, q.w};
+    }() : ldS(a.U, a.ld, cl);
+    const double u = U.ru / U.r, v = U.rv / U.r;
+    const double p = pressure(U, g);
+    const double nx = m.fnx[f], ny = m.fny[f];
+    double mu = 0.0, tau = 0.0;
+    if (EQ != 0 && !w.pressure_only) {
+        const double* gc = a.G + (int64_t)cl * w.ng;  // u (0, 1), v (2, 3) of the cell
+        double bv[7];
+        boundary_values<EQ>(a, m.bc[patch], f, U, u, v, temperature(U, g, a.Rgas),
+                            EQ == EQ_NS ? 0.0 : a.U[4 * a.ld + cl], EQ == EQ_SST ? a.U[5 * a.ld + cl] : 0.0, bv);
+        // face_gradient with the wall value, then the corrected vector
+        // (src/GradientReconstruction.cpp:153-200)
+        double gux, guy, gvx, gvy;
+        corrected(face_grad(m, f, true, u, bv[0], gc[0], gc[1], 0.0, 0.0), nx, ny, gux, guy);
+        corrected(face_grad(m, f, true, v, bv[1], gc[2], gc[3], 0.0, 0.0), nx, ny, gvx, gvy);
+        mu = a.mu;
+        if (EQ == EQ_SA) {
+            mu = a.mu + U.r * sa_eddy(a.U[4 * a.ld + cl], a.mu / U.r, a.sa);
+        } else if (EQ == EQ_SST) {
+            const double s11 = gc[0], s22 = gc[3], s12 = 0.5 * (gc[1] + gc[2]);
+            const double S = sqrt(2.0 * (s11 * s11 + s22 * s22 + 2.0 * s12 * s12));
+            const double Om = fabs(gc[2] - gc[1]);
+            const double k = a.U[4 * a.ld + cl], om = a.U[5 * a.ld + cl];
+            const double nul = a.mu / U.r;
+            const double F2 = sst_F2_ref(k, om, nul, m.wdist[cl], a.sst);
+            mu = a.mu + U.r * sst_eddy_ref(k, om, S, Om, F2, a.sst_variant, a.sst);
+        }
+        const double div_v = gux + gvy;
+        const double txx = mu * (2.0 * gux - (2.0 / 3.0) * div_v);
+        const double tyy = mu * (2.0 * gvy - (2.0 / 3.0) * div_v);
+        const double txy = mu * (guy + gvx);
+        const double trx = txx * nx + txy * ny;
+        const double try_ = txy * nx + tyy * ny;
+        tau = -(trx * (-ny) + try_ * nx);
+    }
+    const double tx = -ny, ty = nx;
+    const double xm = m.fmx[f], ym = m.fmy[f];
+    const double A = m.fa[f];
+    // compute_wall_forces, per sample (src/WallTraction.cpp:180-200)
+    const double fx = A * tau * tx, fy = A * tau * ty;
+    const double px = A * p * nx, py = A * p * ny;
+    const double rx = xm - w.mx, ry = ym - w.my;
+    const double dfx = fx + px, dfy = fy + py;
+    const double fd = fx * w.dir_x + fy * w.dir_y;
+    const double pd = px * w.dir_x + py * w.dir_y;
+    ct[i] = fd;
+    ct[w.n + i] = pd;
+    ct[2 * w.n + i] = fd + pd;
+    ct[3 * w.n + i] = (fx + px) * w.lift_x + (fy + py) * w.lift_y;
+    ct[4 * w.n + i] = rx * dfy - ry * dfx;
+    w.patch[i] = patch;
+    if (w.samples) {
+        fvk_wall_sample o;
+        o.face_index = fref;
+        o.patch_id = patch;
+        o.x_mid = xm;
+        o.y_mid = ym;
+        o.p = p;
+        o.rho = U.r;
+        o.effective_viscosity = mu;
+        o.tau_wall = tau;
+        o.tx = tx;
+        o.ty = ty;
+        o.y_wall_normal = m.fdc[f];
+        // skin_friction_coefficient, pressure_coefficient, wall_y_plus (src/WallTraction.cpp:145-161)
+        o.cf = tau / (0.5 * w.rho_ref * w.v2);
+        o.cp = (p - w.p_ref) / (0.5 * w.rho_ref * w.v2);
+        const double u_tau = sqrt(fabs(tau) / U.r);
+        o.y_plus = U.r * u_tau * o.y_wall_normal / mu;
+        w.samples[i] = o;
+    }
+}
+
+// one workgroup: per-patch and total sums in sample order (thread 0, from
+// LDS chunks the workgroup stages), then the reports by ascending patch id
+// and the total last (the reference's std::map order after its partition)
+__global__ __launch_bounds__(kWallBlock) void k_wall_reduce(WallLaunch w) {
+    extern __shared__ double wsm[];
+    const int np = w.n_patches;
+    double* acc = wsm;                        // [np][5]
+    double* chunk = wsm + 5 * np;             // [5][kWallBlock]
+    int* cp = reinterpret_cast<int*>(chunk + 5 * kWallBlock);  // [kWallBlock] patch of each staged sample
+    int* cnt = cp + kWallBlock;               // [np]
+    for (int q = threadIdx.x; q < 5 * np; q += kWallBlock) acc[q] = 0.0;
+    for (int q = threadIdx.x; q < np; q += kWallBlock) cnt[q] = 0;
+    double tot[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    int ntot = 0;
+    __syncthreads();
+    for (int base = 0; base < w.n; base += kWallBlock) {
+        const int i = base + threadIdx.x;
+        if (i < w.n) {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) chunk[q * kWallBlock + threadIdx.x] = w.contrib[q * w.n + i];
+            cp[threadIdx.x] = w.patch[i];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int e = min(kWallBlock, w.n - base);
+            for (int j = 0; j < e; ++j) {
+                const int pj = cp[j];
+                if (pj < 0 || pj >= np) continue;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    const double x = chunk[q * kWallBlock + j];
+                    acc[pj * 5 + q] += x;
+                    tot[q] += x;
+                }
+                cnt[pj] += 1;
+                ++ntot;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    int k = 0;
+    auto emit = [&](int pid, const double* s) {
+        fvk_wall_force_report r{};
+        r.patch_id = pid;
+        r.friction_drag = s[0];
+        r.pressure_drag = s[1];
+        r.total_drag = s[2];
+        r.lift = s[3];
+        r.moment = s[4];
+        r.cd_friction = r.friction_drag / w.q;
+        r.cd_pressure = r.pressure_drag / w.q;
+        r.cd_total = r.total_drag / w.q;
+        r.cl = r.lift / w.q;
+        r.cm = r.moment / w.q2;
+        if (k < w.cap) w.reports[k] = r;
+        ++k;
+    };
+    for (int pid = 0; pid < np; ++pid)
+        if (cnt[pid] > 0) emit(pid, acc + 5 * pid);
+    if (ntot > 0) emit(-1, tot);
+    *w.n_reports = k;
+}
+
+}  // namespace
+
+int launch_wall_diagnostics(fvk_solver* s, const WallLaunch& w0) {
+    WallLaunch w = w0;
+    const int eq = s->p.equation;
+    ViscArgs a = make_vargs(s);
+    a.U = s->U[s->cur];
+    w.ng = eq == FVK_EQ_RANS_SST ? 10 : (eq == FVK_EQ_RANS_SA ? 8 : 6);
+    w.aos = eq == FVK_EQ_EULER;
+    if (eq != FVK_EQ_EULER && !w.pressure_only) {
+        // cell gradients of the current state (the stage's gradient pass, its
+        // turbulence sources included, with no finalize folded in)
+        a.fin_nblocks = 0;
+        const int gc = grid_for(s->nc, kVB);
+        if (eq == FVK_EQ_NAVIER_STOKES) launch_grad_eq<EQ_NS>(s, gc, a);
+        else if (eq == FVK_EQ_RANS_SA) launch_grad_eq<EQ_SA>(s, gc, a);
+        else launch_grad_eq<EQ_SST>(s, gc, a);
+    }
+    if (w.n > 0) {
+        const int grid = (w.n + kVB - 1) / kVB;
+        if (eq == FVK_EQ_EULER || w.pressure_only) k_wall_samples<0><<<grid, kVB, 0, s->stream>>>(a, w);
+        else if (eq == FVK_EQ_NAVIER_STOKES) k_wall_samples<EQ_NS><<<grid, kVB, 0, s->stream>>>(a, w);
+        else if (eq == FVK_EQ_RANS_SA) k_wall_samples<EQ_SA><<<grid, kVB, 0, s->stream>>>(a, w);
+        else k_wall_samples<EQ_SST><<<grid, kVB, 0, s->stream>>>(a, w);
+    }
+    const size_t lds = sizeof(double) * (5 * (size_t)w.n_patches + 5 * kWallBlock) +
+                       sizeof(int) * ((size_t)kWallBlock + w.n_patches);
+    k_wall_reduce<<<1, kWallBlock, lds, s->stream>>>(w);
+    return hip_check(hipGetLastError(), "wall diagnostics launch");
+}
+
 int launch_wall_distance_segments(const double* cx, const double* cy, int nc, const double* seg, int nw, double* out,
                                   hipStream_t st) {
     int grid = grid_for(nc, kVB);

@@ -416,6 +416,7 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
     s->node_xy.assign(d->node_xy, d->node_xy + 2 * nn);
     s->cell_ptr_old.assign(d->cell_face_ptr, d->cell_face_ptr + nc + 1);
     s->cell_faces_old.assign(d->cell_faces, d->cell_faces + nnz);
+    s->face_patch_old.assign(d->face_patch, d->face_patch + d->n_faces);
 
     // ---- face geometry, reference order (UnstructuredMesh::compute_geometry) ----
     std::vector<double> mx(nf), my(nf), ar(nf), nx(nf), ny(nf);

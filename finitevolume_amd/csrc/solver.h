@@ -182,6 +182,8 @@ struct fvk_solver {
     std::vector<double> node_xy;                // for adjoint geometry
     std::vector<int> face_nodes_dev;            // [2*nf] device face order
     std::vector<int> cell_ptr_old, cell_faces_old;
+    std::vector<int> face_patch_old;            // patch of each reference face (-1 interior)
+    int* d_face_old2new = nullptr;              // device copy of face_old2new
 
     // device arrays (owned)
     std::vector<void*> allocations;
@@ -263,6 +265,26 @@ int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, i
 int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
 void adjoint_release(fvk_solver* s);
 int launch_copy_probe(double* dst, const double* src, int64_t n, hipStream_t st);
+// wall diagnostics (kernels_visc.hip): samples and force sums of the faces
+// wall_faces[n] (reference ids, device memory); contrib [5][n] and patch [n]
+// are scratch; dir, lift, q and q2 as compute_wall_forces derives them from
+// the reference quantities (host std::hypot)
+struct WallLaunch {
+    const int* wall_faces;
+    const int* face_old2new;
+    int n, n_patches, ng, pressure_only, aos;
+    double dir_x, dir_y, lift_x, lift_y, q, q2, mx, my, rho_ref, v2, p_ref;
+    fvk_wall_sample* samples;          // [n] or nullptr
+    double* contrib;
+    int* patch;
+    fvk_wall_force_report* reports;    // [cap]
+    int cap;
+    int* n_reports;
+};
+int launch_wall_diagnostics(fvk_solver* s, const WallLaunch& w);
+void wall_launch_ref(WallLaunch& w, const fvk_wall_reference& ref);
+int wall_reports(fvk_solver* s, const int32_t* wf, int64_t n, const fvk_wall_reference* ref, int pressure_only,
+                 fvk_wall_sample* samples, fvk_wall_force_report* reports, int32_t cap, int32_t* n_reports);
 int launch_div_probe(const double* x, const double* y, double* q_fast, double* q_ieee, int64_t n, hipStream_t st);
 
 // timers

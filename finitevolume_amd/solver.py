@@ -58,6 +58,9 @@ def lib():
     L.fvk_wall_forces.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
     L.fvk_wall_forces_vjp.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                       dp, dp]
+    L.fvk_wall_traction.argtypes = [vp, ip, i64, C.POINTER(abi.WallReferenceC), vp]
+    L.fvk_wall_force_reports.argtypes = [vp, ip, i64, C.POINTER(abi.WallReferenceC), vp, vp, C.c_int32,
+                                         C.POINTER(C.c_int32)]
     L.fvk_set_kernel_timing.argtypes = [vp, C.c_int32]
     L.fvk_kernel_times.argtypes = [vp, C.c_char_p, i64, dp, C.POINTER(i64), C.c_int32, C.POINTER(C.c_int32)]
     L.fvk_device_bytes.argtypes = [vp]
@@ -187,6 +190,29 @@ class FVKSolver:
         cl, cd = C.c_double(), C.c_double()
         _check(self.L.fvk_wall_forces(self.h, rho_ref, vx_ref, vy_ref, length_ref, C.byref(cl), C.byref(cd)))
         return cl.value, cd.value
+
+    def wall_traction(self, wall_faces, ref: abi.WallReferenceC) -> np.ndarray:
+        """compute_wall_traction_samples (src/NavierStokesFVMSolver.cpp:333-352)
+        with the per-sample Cf, Cp and y+; a structured array of
+        abi.WALL_SAMPLE_DTYPE."""
+        wf = np.ascontiguousarray(wall_faces, dtype=np.int32)
+        out = np.zeros(wf.shape[0], dtype=abi.WALL_SAMPLE_DTYPE)
+        _check(self.L.fvk_wall_traction(self.h, abi.as_iptr(wf), wf.shape[0], C.byref(ref),
+                                        out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def wall_force_reports(self, wall_faces, ref: abi.WallReferenceC, with_samples: bool = False):
+        """compute_wall_forces (src/WallTraction.cpp:163-216): per-patch reports
+        by ascending patch id, then the total (patch_id -1)."""
+        wf = np.ascontiguousarray(wall_faces, dtype=np.int32)
+        cap = len(self._bcs) + 1
+        rep = np.zeros(cap, dtype=abi.WALL_REPORT_DTYPE)
+        smp = np.zeros(wf.shape[0], dtype=abi.WALL_SAMPLE_DTYPE) if with_samples else None
+        n = C.c_int32()
+        _check(self.L.fvk_wall_force_reports(self.h, abi.as_iptr(wf), wf.shape[0], C.byref(ref),
+                                             smp.ctypes.data_as(C.c_void_p) if smp is not None else None,
+                                             rep.ctypes.data_as(C.c_void_p), cap, C.byref(n)))
+        return (rep[: n.value], smp) if with_samples else rep[: n.value]
 
     def wall_forces_vjp(self, rho_ref, vx_ref, vy_ref, length_ref=1.0, w_cl=1.0, w_cd=0.0, want_x: bool = True):
         """Gradient of w_cl Cl + w_cd Cd with respect to the state and (optionally) the node coordinates."""

@@ -279,6 +279,54 @@ int fvk_residual_vjp(fvk_solver* s, const double* lambda, double* U_bar, double*
 int fvk_wall_forces(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref,
                     double* cl, double* cd);
 
+/* ---- wall diagnostics on the device (src/WallTraction.cpp:95-216) ---- */
+
+/* WallReferenceQuantities, include/WallTraction.h:99-106 (same fields). */
+typedef struct fvk_wall_reference {
+    double rho_ref, velocity_x_ref, velocity_y_ref, p_ref, length_ref, moment_reference_x, moment_reference_y;
+} fvk_wall_reference;
+
+/* WallFaceSample, include/WallTraction.h:36-45, plus the three per-sample
+ * coefficients of skin_friction_coefficient, pressure_coefficient and
+ * wall_y_plus (src/WallTraction.cpp:145-161). */
+typedef struct fvk_wall_sample {
+    int32_t face_index;            /* reference face id */
+    int32_t patch_id;
+    double x_mid, y_mid, p, rho, effective_viscosity, tau_wall, tx, ty, y_wall_normal;
+    double cf, cp, y_plus;
+} fvk_wall_sample;
+
+/* WallForceReport, include/WallTraction.h:122-128 (patch_id -1: the total). */
+typedef struct fvk_wall_force_report {
+    int32_t patch_id;
+    int32_t reserved0;
+    double friction_drag, pressure_drag, total_drag, cd_friction, cd_pressure, cd_total, lift, cl, moment, cm;
+} fvk_wall_force_report;
+
+/* compute_wall_traction_samples (src/NavierStokesFVMSolver.cpp:333-352,
+ * RANSTurbulenceSASolver.cpp:353-374, RANSTurbulenceSSTSolver.cpp:435-464)
+ * followed by compute_wall_traction (src/WallTraction.cpp:95-146) for the
+ * boundary faces wall_faces[n_wall] (reference face ids, in the caller's
+ * order), on the device from the solver's current state: cell gradients of
+ * u and v with the solver's scheme and boundary fields, the Jasak-corrected
+ * wall gradient, the effective viscosity (mu, mu + rho nu_t for SA and SST),
+ * tau_wall, y_wall_normal and the coefficients against `ref`.  An Euler
+ * solver has no viscous stress: its samples carry the pressure with
+ * effective_viscosity = tau_wall = 0 (the reference wires wall forces only
+ * for no-slip walls).  An interior face in the list is an error. */
+int fvk_wall_traction(fvk_solver* s, const int32_t* wall_faces, int64_t n_wall, const fvk_wall_reference* ref,
+                      fvk_wall_sample* samples);
+
+/* compute_wall_forces (src/WallTraction.cpp:163-216) over the samples of
+ * fvk_wall_traction: one report per patch that has a sample, by ascending
+ * patch id, then the domain total (patch_id -1); each sum runs over the
+ * samples in the caller's order, as the reference's does.  reports has room
+ * for cap entries (n_patches + 1 always suffices); *n_reports receives the
+ * count.  samples may be NULL. */
+int fvk_wall_force_reports(fvk_solver* s, const int32_t* wall_faces, int64_t n_wall, const fvk_wall_reference* ref,
+                           fvk_wall_sample* samples, fvk_wall_force_report* reports, int32_t cap,
+                           int32_t* n_reports);
+
 /* Extension: gradient of J = w_cl Cl + w_cd Cd (the pressure forces of
  * fvk_wall_forces) with respect to the state, U_bar [4*n_cells], and the
  * node coordinates, X_bar [2*n_nodes]; either may be NULL.  Overwrites both. */

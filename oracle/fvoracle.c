@@ -1334,24 +1334,171 @@ int fvo_wall_distance_faces(const fvk_mesh_desc* d, const int32_t* wf, int64_t n
     return FVK_OK;
 }
 
-/* pressure force of the wall patches, src/WallTraction.cpp:163-216 (pressure
- * part; the wall pressure is the wall cell's own, WallTraction.h:59-66) */
-int fvo_wall_forces(fvo_solver* s, double rho_ref, double vx, double vy, double L, double* cl, double* cd) {
-    double vm = hypot(vx, vy);
-    double dx = vx / vm, dy = vy / vm;
-    double lx = -dy, ly = dx;
-    double q = 0.5 * rho_ref * vm * vm * L;
-    double drag = 0.0, lift = 0.0;
-    for (int64_t f = 0; f < s->nf; ++f) {
-        if (s->fcell[2 * f + 1] != -1 || s->bcs[s->fpatch[f]].type != FVK_BC_WALL) continue;
-        double p = pressure(ld4(s->U + 4 * s->fcell[2 * f]), s->p.gamma);
-        double px = s->farea[f] * p * s->fnx[f], py = s->farea[f] * p * s->fny[f];
-        drag += px * dx + py * dy;
-        lift += px * lx + py * ly;
+/* SST F2 and eddy viscosity with the guards of src/SSTKOmega.cpp (sst_F2,
+ * sst_eddy_viscosity), as compute_wall_traction_samples calls them */
+static double sst_F2_ref(double k, double w, double nu, double d, const fvk_sst_constants* c) {
+    if (k <= 0.0 || w <= 0.0) return 0.0;
+    double d2 = d * d;
+    double a2 = smax(2.0 * sqrt(k) / (c->beta_star * w * d), 500.0 * nu / (d2 * w));
+    return tanh(a2 * a2);
+}
+static double sst_eddy_ref(double k, double w, double S, double Om, double F2, int variant,
+                           const fvk_sst_constants* c) {
+    if (k <= 0.0) return 0.0;
+    double lim = (variant == FVK_SST_VORTICITY) ? Om : S;
+    return c->a1 * k / smax(c->a1 * w, lim * F2);
+}
+
+/* Wall samples and force reports.
+ * Samples: compute_wall_traction (src/WallTraction.cpp:95-143) with the
+ * inputs of compute_wall_traction_samples (src/NavierStokesFVMSolver.cpp:333-352,
+ * src/RANSTurbulenceSASolver.cpp:353-374, src/RANSTurbulenceSSTSolver.cpp:435-464);
+ * Euler, or pressure_only, gives pressure-only samples (mu_eff = tau = 0);
+ * coefficients: src/WallTraction.cpp:145-161.
+ * Reports: compute_wall_forces (src/WallTraction.cpp:163-216), patches in
+ * ascending id then the total. */
+int fvo_wall_reports(fvo_solver* s, const int32_t* wf, int64_t n, const fvk_wall_reference* ref, int32_t pressure_only,
+                     fvk_wall_sample* samples, fvk_wall_force_report* reports, int32_t cap, int32_t* n_reports) {
+    const fvk_params* p = &s->p;
+    const int64_t nc = s->nc, nf = s->nf;
+    const double g = p->gamma;
+    const int eq = p->equation;
+    if (eq == FVK_EQ_DIFFUSION) return FVK_ERR_UNSUPPORTED;
+    for (int64_t i = 0; i < n; ++i)
+        if (wf[i] < 0 || wf[i] >= nf || s->fcell[2 * wf[i] + 1] != -1) return FVK_ERR_INVALID;
+    const int visc = eq != FVK_EQ_EULER && !pressure_only;
+    double *u = NULL, *v = NULL, *bu = NULL, *bv = NULL, *gu = NULL, *gv = NULL;
+    if (visc) {
+        u = malloc(sizeof(double) * nc); v = malloc(sizeof(double) * nc);
+        bu = calloc(nf, sizeof(double)); bv = calloc(nf, sizeof(double));
+        gu = malloc(sizeof(double) * 2 * nc); gv = malloc(sizeof(double) * 2 * nc);
+        for (int64_t c = 0; c < nc; ++c) {
+            st4 Uc = ld4(s->U + 4 * c);
+            u[c] = Uc.ru / Uc.r;
+            v[c] = Uc.rv / Uc.r;
+        }
+        /* build_boundary_fields, src/NavierStokesFVMSolver.cpp:55-84 (u, v) */
+        for (int64_t f = 0; f < nf; ++f) {
+            if (s->fcell[2 * f + 1] != -1) continue;
+            const fvk_bc* bc = &s->bcs[s->fpatch[f]];
+            int cl = s->fcell[2 * f];
+            if (bc->type == FVK_BC_WALL) { bu[f] = bc->wall_u; bv[f] = bc->wall_v; }
+            else { bu[f] = u[cl]; bv[f] = v[cl]; }
+        }
+        gradient(s, p->gradient_scheme, u, 1, bu, gu);
+        gradient(s, p->gradient_scheme, v, 1, bv, gv);
     }
-    *cd = drag / q;
-    *cl = lift / q;
-    return FVK_OK;
+    /* compute_wall_forces' reference directions, :168-172 */
+    double v_mag = hypot(ref->velocity_x_ref, ref->velocity_y_ref);
+    double dir_x = ref->velocity_x_ref / v_mag, dir_y = ref->velocity_y_ref / v_mag;
+    double lift_x = -dir_y, lift_y = dir_x;
+    double q = 0.5 * ref->rho_ref * v_mag * v_mag * ref->length_ref;
+    double q2 = q * ref->length_ref;
+    double v2 = ref->velocity_x_ref * ref->velocity_x_ref + ref->velocity_y_ref * ref->velocity_y_ref;
+    int np = s->n_bcs;
+    double* acc = calloc((size_t)(np + 1) * 5, sizeof(double));
+    int* cnt = calloc((size_t)(np + 1), sizeof(int));
+    const double zero2[2] = {0.0, 0.0};
+    for (int64_t i = 0; i < n; ++i) {
+        int f = wf[i], cl = s->fcell[2 * f], patch = s->fpatch[f];
+        st4 Uc = ld4(s->U + 4 * cl);
+        double pc = pressure(Uc, g);
+        double nx = s->fnx[f], ny = s->fny[f];
+        double mu = 0.0, tau = 0.0;
+        if (visc) {
+            double gux, guy, gvx, gvy;
+            corrected_vector(face_gradient(s, f, u[cl], 0.0, gu + 2 * cl, zero2, bu[f]), nx, ny, &gux, &guy);
+            corrected_vector(face_gradient(s, f, v[cl], 0.0, gv + 2 * cl, zero2, bv[f]), nx, ny, &gvx, &gvy);
+            mu = p->mu;
+            if (eq == FVK_EQ_RANS_SA) {
+                mu = p->mu + Uc.r * sa_eddy(s->turb[cl], p->mu / Uc.r, &p->sa);
+            } else if (eq == FVK_EQ_RANS_SST) {
+                const double* a = gu + 2 * cl;
+                const double* b = gv + 2 * cl;
+                double s11 = a[0], s22 = b[1], s12 = 0.5 * (a[1] + b[0]);
+                double S = sqrt(2.0 * (s11 * s11 + s22 * s22 + 2.0 * s12 * s12));
+                double Om = fabs(b[0] - a[1]);
+                double k = s->turb[2 * cl], om = s->turb[2 * cl + 1];
+                double F2 = sst_F2_ref(k, om, p->mu / Uc.r, s->wall_dist[cl], &p->sst);
+                mu = p->mu + Uc.r * sst_eddy_ref(k, om, S, Om, F2, p->sst_variant, &p->sst);
+            }
+            double div_v = gux + gvy;
+            double txx = mu * (2.0 * gux - (2.0 / 3.0) * div_v);
+            double tyy = mu * (2.0 * gvy - (2.0 / 3.0) * div_v);
+            double txy = mu * (guy + gvx);
+            double trx = txx * nx + txy * ny;
+            double try_ = txy * nx + tyy * ny;
+            tau = -(trx * (-ny) + try_ * nx);
+        }
+        double tx = -ny, ty = nx;
+        double A = s->farea[f];
+        double xm = s->fmid[2 * f], ym = s->fmid[2 * f + 1];
+        if (samples) {
+            fvk_wall_sample* o = samples + i;
+            o->face_index = f;
+            o->patch_id = patch;
+            o->x_mid = xm; o->y_mid = ym;
+            o->p = pc; o->rho = Uc.r;
+            o->effective_viscosity = mu;
+            o->tau_wall = tau;
+            o->tx = tx; o->ty = ty;
+            o->y_wall_normal = face_normal_distance(s, f);
+            o->cf = tau / (0.5 * ref->rho_ref * v2);
+            o->cp = (pc - ref->p_ref) / (0.5 * ref->rho_ref * v2);
+            double u_tau = sqrt(fabs(tau) / Uc.r);
+            o->y_plus = Uc.r * u_tau * o->y_wall_normal / mu;
+        }
+        double fx = A * tau * tx, fy = A * tau * ty;
+        double px = A * pc * nx, py = A * pc * ny;
+        double rx = xm - ref->moment_reference_x, ry = ym - ref->moment_reference_y;
+        double dfx = fx + px, dfy = fy + py;
+        double mom = rx * dfy - ry * dfx;
+        double fd = fx * dir_x + fy * dir_y;
+        double pd = px * dir_x + py * dir_y;
+        double lift = (fx + px) * lift_x + (fy + py) * lift_y;
+        int keys[2] = {patch, np};
+        for (int k = 0; k < 2; ++k) {
+            double* a = acc + 5 * keys[k];
+            a[0] += fd; a[1] += pd; a[2] += fd + pd; a[3] += lift; a[4] += mom;
+            cnt[keys[k]] += 1;
+        }
+    }
+    int k = 0;
+    for (int pid = 0; pid <= np; ++pid) {
+        if (cnt[pid] == 0) continue;
+        if (k < cap) {
+            fvk_wall_force_report* r = reports + k;
+            const double* a = acc + 5 * pid;
+            memset(r, 0, sizeof *r);
+            r->patch_id = pid == np ? -1 : pid;
+            r->friction_drag = a[0]; r->pressure_drag = a[1]; r->total_drag = a[2]; r->lift = a[3]; r->moment = a[4];
+            r->cd_friction = a[0] / q; r->cd_pressure = a[1] / q; r->cd_total = a[2] / q;
+            r->cl = a[3] / q; r->cm = a[4] / q2;
+        }
+        ++k;
+    }
+    *n_reports = k;
+    free(acc); free(cnt);
+    free(u); free(v); free(bu); free(bv); free(gu); free(gv);
+    return k > cap ? FVK_ERR_INVALID : FVK_OK;
+}
+
+/* pressure lift and drag over every wall face in face order: the total of
+ * fvo_wall_reports' pressure-only samples */
+int fvo_wall_forces(fvo_solver* s, double rho_ref, double vx, double vy, double L, double* cl, double* cd) {
+    int32_t* wf = malloc(sizeof(int32_t) * (size_t)(s->nf > 0 ? s->nf : 1));
+    int64_t n = 0;
+    for (int64_t f = 0; f < s->nf; ++f)
+        if (s->fcell[2 * f + 1] == -1 && s->bcs[s->fpatch[f]].type == FVK_BC_WALL) wf[n++] = (int32_t)f;
+    fvk_wall_reference ref = {rho_ref, vx, vy, 0.0, L, 0.0, 0.0};
+    fvk_wall_force_report* rep = malloc(sizeof(fvk_wall_force_report) * (size_t)(s->n_bcs + 1));
+    int32_t nrep = 0;
+    int rc = fvo_wall_reports(s, wf, n, &ref, 1, NULL, rep, s->n_bcs + 1, &nrep);
+    *cl = nrep > 0 ? rep[nrep - 1].cl : 0.0;
+    *cd = nrep > 0 ? rep[nrep - 1].cd_total : 0.0;
+    free(wf);
+    free(rep);
+    return rc;
 }
 
 int fvo_last_gradients(const fvo_solver* s, double* grad, double* phi) {

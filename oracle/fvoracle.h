@@ -38,6 +38,9 @@ int fvo_wall_distance(fvo_solver* s, double* dist);
 int fvo_wall_distance_faces(const fvk_mesh_desc* mesh, const int32_t* wall_faces, int64_t n_wall, double* dist);
 int fvo_wall_forces(fvo_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref, double* cl,
                     double* cd);
+int fvo_wall_reports(fvo_solver* s, const int32_t* wall_faces, int64_t n, const fvk_wall_reference* ref,
+                     int32_t pressure_only, fvk_wall_sample* samples, fvk_wall_force_report* reports, int32_t cap,
+                     int32_t* n_reports);
 /* limited primitive gradients of the last residual evaluation (order 2):
  * [8*n_cells] per cell (drho/dx, drho/dy, du/dx, du/dy, dv/dx, dv/dy, dp/dx, dp/dy) and
  * the limiter values [4*n_cells] */

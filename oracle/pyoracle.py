@@ -43,6 +43,8 @@ def lib():
     L.fvo_wall_distance_faces.argtypes = [C.POINTER(abi.MeshDescC), ip, C.c_int64, dp]
     L.fvo_wall_forces.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
     L.fvo_last_gradients.argtypes = [vp, dp, dp]
+    L.fvo_wall_reports.argtypes = [vp, ip, C.c_int64, C.POINTER(abi.WallReferenceC), C.c_int32, vp, vp, C.c_int32,
+                                   C.POINTER(C.c_int32)]
     L.fvo_flux.argtypes = [C.c_int32, C.c_int64, dp, dp, dp, C.c_double, C.c_double, C.c_int32, dp]
     L.fvo_sa_source.argtypes = [C.c_int64, dp, dp, dp, dp, dp, C.POINTER(abi.SAConstantsC), dp, dp]
     L.fvo_sst_source.argtypes = [C.c_int64, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_int32,
@@ -128,6 +130,20 @@ class OracleSolver:
         d = np.empty(self.n)
         self.L.fvo_wall_distance(self.h, abi.as_dptr(d))
         return d
+
+    def wall_reports(self, wall_faces, ref, pressure_only: bool = False):
+        """Wall samples and force reports (structured arrays of
+        abi.WALL_SAMPLE_DTYPE and abi.WALL_REPORT_DTYPE)."""
+        wf = np.ascontiguousarray(wall_faces, dtype=np.int32)
+        smp = np.zeros(wf.shape[0], dtype=abi.WALL_SAMPLE_DTYPE)
+        cap = len(self._bcs) + 1
+        rep = np.zeros(cap, dtype=abi.WALL_REPORT_DTYPE)
+        n = C.c_int32()
+        rc = self.L.fvo_wall_reports(self.h, abi.as_iptr(wf), wf.shape[0], C.byref(ref), int(pressure_only),
+                         smp.ctypes.data_as(C.c_void_p), rep.ctypes.data_as(C.c_void_p), cap, C.byref(n))
+        if rc != 0:
+            raise RuntimeError(f"fvo_wall_reports failed: {rc}")
+        return smp, rep[: n.value]
 
     def wall_forces(self, rho_ref, vx, vy, length=1.0):
         cl, cd = C.c_double(), C.c_double()

@@ -47,6 +47,8 @@ def lib(flavour: str = "strict"):
     L.fvr_next_cfl.argtypes = [C.c_double, C.c_double, dp, C.c_int32, C.c_double, C.c_double, C.c_double,
                                C.c_int32, C.c_double]
     L.fvr_next_cfl.restype = C.c_double
+    L.fvr_wall_reports.argtypes = [vp, ip, C.c_int64, C.POINTER(abi.WallReferenceC), C.c_int32, vp, vp, C.c_int32,
+                                   C.POINTER(C.c_int32)]
     _libs[flavour] = L
     return L
 
@@ -102,6 +104,20 @@ class RefSolver:
 
     def set_cfl(self, cfl: float):
         self.L.fvr_set_cfl(self.h, cfl)
+
+    def wall_reports(self, wall_faces, ref, pressure_only: bool = False):
+        """Wall samples and force reports (structured arrays of
+        abi.WALL_SAMPLE_DTYPE and abi.WALL_REPORT_DTYPE)."""
+        wf = np.ascontiguousarray(wall_faces, dtype=np.int32)
+        smp = np.zeros(wf.shape[0], dtype=abi.WALL_SAMPLE_DTYPE)
+        cap = len(self._bcs) + 1
+        rep = np.zeros(cap, dtype=abi.WALL_REPORT_DTYPE)
+        n = C.c_int32()
+        rc = self.L.fvr_wall_reports(self.h, abi.as_iptr(wf), wf.shape[0], C.byref(ref), int(pressure_only),
+                         smp.ctypes.data_as(C.c_void_p), rep.ctypes.data_as(C.c_void_p), cap, C.byref(n))
+        if rc != 0:
+            raise RuntimeError(f"fvr_wall_reports failed: {rc}")
+        return smp, rep[: n.value]
 
 
 def flux(scheme: int, UL, UR, nxy, gamma=1.4, tol=1e-6, max_iter=20, flavour="strict"):

@@ -29,6 +29,7 @@
 #include "SpalartAllmaras.h"
 #include "UnstructuredFVMSolver.h"
 #include "WallDistance.h"
+#include "WallTraction.h"
 
 namespace {
 
@@ -112,6 +113,8 @@ SSTModelConstants sst_of(const fvk_sst_constants& c) {
 struct RefSolver {
     int eq = 0;
     int64_t n = 0;
+    double gamma = 1.4;
+    UnstructuredMesh mesh;  // the solver's mesh with compute_geometry() done (wall diagnostics)
     std::unique_ptr<EulerFVMSolver> euler;
     std::unique_ptr<NavierStokesFVMSolver> ns;
     std::unique_ptr<RANSTurbulenceSASolver> sa;
@@ -131,6 +134,9 @@ int fvr_create(const fvk_mesh_desc* d, const fvk_bc* bcs, int32_t n_bcs, const f
     auto* r = new RefSolver;
     r->eq = p->equation;
     r->n = d->n_cells;
+    r->gamma = p->gamma;
+    r->mesh = mesh;
+    r->mesh.compute_geometry();
     EulerInitialCondition e = ic_of(ic);
     switch (p->equation) {
         case FVK_EQ_EULER: {
@@ -293,6 +299,77 @@ void fvr_set_cfl(void* h, double cfl) {
         case FVK_EQ_RANS_SA: r->sa->set_cfl(cfl); break;
         case FVK_EQ_RANS_SST: r->sst->set_cfl(cfl); break;
     }
+}
+
+// Wall samples and force reports: the solvers' compute_wall_traction_samples
+// (NS, SA, SST) or, for Euler and pressure_only, pressure samples built as
+// compute_wall_traction fills them with zero viscosity (p of the wall cell,
+// tau = 0); then the coefficient functions and compute_wall_forces.
+int fvr_wall_reports(void* h, const int32_t* wf, int64_t n, const fvk_wall_reference* rq, int32_t pressure_only,
+                     fvk_wall_sample* samples, fvk_wall_force_report* reports, int32_t cap, int32_t* n_reports) {
+    auto* r = static_cast<RefSolver*>(h);
+    std::vector<int> faces(wf, wf + n);
+    std::vector<WallFaceSample> ws;
+    if (!pressure_only && r->eq == FVK_EQ_NAVIER_STOKES) ws = r->ns->compute_wall_traction_samples(faces);
+    else if (!pressure_only && r->eq == FVK_EQ_RANS_SA) ws = r->sa->compute_wall_traction_samples(faces);
+    else if (!pressure_only && r->eq == FVK_EQ_RANS_SST) ws = r->sst->compute_wall_traction_samples(faces);
+    else {
+        const std::vector<EulerState>* U = nullptr;
+        switch (r->eq) {
+            case FVK_EQ_EULER: U = &r->euler->field(); break;
+            case FVK_EQ_NAVIER_STOKES: U = &r->ns->field(); break;
+            case FVK_EQ_RANS_SA: U = &r->sa->field(); break;
+            case FVK_EQ_RANS_SST: U = &r->sst->field(); break;
+            default: return FVK_ERR_UNSUPPORTED;
+        }
+        for (int f : faces) {
+            const Face& face = r->mesh.faces[f];
+            WallFaceSample smp;
+            smp.face_index = f;
+            smp.x_mid = face.x_mid;
+            smp.y_mid = face.y_mid;
+            smp.p = pressure((*U)[face.cell_left], r->gamma);
+            smp.rho = (*U)[face.cell_left].rho;
+            smp.effective_viscosity = 0.0;
+            smp.tau_wall = 0.0;
+            smp.tx = -face.ny;
+            smp.ty = face.nx;
+            smp.y_wall_normal = face_normal_distance(r->mesh, face);
+            ws.push_back(smp);
+        }
+    }
+    WallReferenceQuantities ref;
+    ref.rho_ref = rq->rho_ref;
+    ref.velocity_x_ref = rq->velocity_x_ref;
+    ref.velocity_y_ref = rq->velocity_y_ref;
+    ref.p_ref = rq->p_ref;
+    ref.length_ref = rq->length_ref;
+    ref.moment_reference_x = rq->moment_reference_x;
+    ref.moment_reference_y = rq->moment_reference_y;
+    for (size_t i = 0; samples && i < ws.size(); ++i) {
+        const WallFaceSample& w = ws[i];
+        fvk_wall_sample& o = samples[i];
+        o.face_index = w.face_index;
+        o.patch_id = r->mesh.faces[w.face_index].patch_id;
+        o.x_mid = w.x_mid; o.y_mid = w.y_mid; o.p = w.p; o.rho = w.rho;
+        o.effective_viscosity = w.effective_viscosity; o.tau_wall = w.tau_wall;
+        o.tx = w.tx; o.ty = w.ty; o.y_wall_normal = w.y_wall_normal;
+        o.cf = skin_friction_coefficient(w, ref);
+        o.cp = pressure_coefficient(w, ref);
+        o.y_plus = wall_y_plus(w);
+    }
+    std::vector<WallForceReport> rep = compute_wall_forces(r->mesh, ws, ref);
+    for (size_t i = 0; i < rep.size() && (int32_t)i < cap; ++i) {
+        const WallForceReport& w = rep[i];
+        fvk_wall_force_report& o = reports[i];
+        o.patch_id = w.patch_id;
+        o.reserved0 = 0;
+        o.friction_drag = w.friction_drag; o.pressure_drag = w.pressure_drag; o.total_drag = w.total_drag;
+        o.cd_friction = w.cd_friction; o.cd_pressure = w.cd_pressure; o.cd_total = w.cd_total;
+        o.lift = w.lift; o.cl = w.cl; o.moment = w.moment; o.cm = w.cm;
+    }
+    *n_reports = (int32_t)rep.size();
+    return (int32_t)rep.size() > cap ? FVK_ERR_INVALID : FVK_OK;
 }
 
 // ---- standalone functions -------------------------------------------------

@@ -86,9 +86,54 @@ def run_case(case, nsteps, snapshots=()):
     return out
 
 
-def main():
+WALL_REF = dict(rho_ref=1.0, velocity_x_ref=0.3, velocity_y_ref=0.1, p_ref=0.7, length_ref=1.0,
+                moment_reference_x=0.25, moment_reference_y=0.0)
+
+
+def wall_cases():
+    """The wall-diagnostics fixture cases: (key, case, steps).  The NACA
+    Euler case gives pressure samples; the NS/RANS cases the full traction."""
+    return [
+        ("euler_naca", cases.case_naca(64, 32), 20),
+        ("ns_ls", cases.case_ns_wavy(), 20),
+        ("ns_gg", cases.case_ns_wavy(gradient=abi.GRAD_GREEN_GAUSS), 20),
+        ("sa_ls", cases.case_rans_wavy(abi.EQ_RANS_SA), 20),
+        ("sst_v1k1", cases.case_rans_wavy(abi.EQ_RANS_SST, variant=1, kato=1), 20),
+        ("sst_couette", cases.case_couette(abi.EQ_RANS_SST), 50),
+    ]
+
+
+def write_wall_fixture(w):
+    """compute_wall_traction_samples + compute_wall_forces of the reference
+    (src/WallTraction.cpp:95-216) after a few steps of each wall case: the
+    state they are taken from, the samples and the reports."""
+    out = {}
+    ref = abi.wall_reference(**WALL_REF)
+    for key, case, steps in wall_cases():
+        m, bcs, p, ic, U0 = case
+        r = pyref.RefSolver(m, bcs, p, ic)
+        if U0 is not None:
+            r.set_field(U0)
+        r.step(steps)
+        f = r.field()
+        out[f"{key}_U"] = f[0] if isinstance(f, tuple) else f
+        if isinstance(f, tuple):
+            out[f"{key}_T"] = f[1]
+        wf = abi.collect_wall_faces(m, bcs)
+        for po in (0, 1):
+            smp, rep = r.wall_reports(wf, ref, bool(po))
+            out[f"{key}_p{po}_samples"] = smp
+            out[f"{key}_p{po}_reports"] = rep
+    w("wall_reports.npz", **out)
+
+
+def main(only=None):
     assert pyref.available("strict"), "oracle/_ref/libfvref_strict.so missing: make -C oracle ref"
     w = lambda name, **arrs: np.savez_compressed(os.path.join(HERE, name), **arrs)
+    if only == "wall":
+        write_wall_fixture(w)
+        print("wall fixture written to", HERE)
+        return
 
     # ---- numerical fluxes ----
     UL, UR, nxy = flux_inputs()
@@ -173,8 +218,10 @@ def main():
     }
     outs = {k: np.array([pyref.next_cfl(cur, 10.0, s) for cur in (0.05, 1.0, 9.0)]) for k, s in seqs.items()}
     w("cfl_ramp.npz", **{f"seq_{k}": v for k, v in seqs.items()}, **{f"out_{k}": v for k, v in outs.items()})
+    write_wall_fixture(w)
     print("fixtures written to", HERE)
 
 
 if __name__ == "__main__":
-    main()
+    # `make_golden.py wall` rewrites only the wall-diagnostics fixture
+    main(sys.argv[1] if len(sys.argv) > 1 else None)

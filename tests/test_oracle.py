@@ -437,3 +437,62 @@ def test_vortex_convergence_order():
     assert e2[1] < e1[1]
     assert o2 > 1.6, (o1, o2)
     assert 0.5 < o1 < 1.4, (o1, o2)
+
+
+# ---------------------------------------------------------------------------
+# wall diagnostics (src/WallTraction.cpp:95-216)
+# ---------------------------------------------------------------------------
+
+def _wall_fixture_cases():
+    import sys
+    sys.path.insert(0, GOLD)
+    from make_golden import WALL_REF, wall_cases
+    return WALL_REF, wall_cases()
+
+
+def test_wall_reports_match_reference_fixture():
+    """Samples (traction, Cf, Cp, y+) and per-patch force reports of the
+    oracle against the reference's compute_wall_traction_samples and
+    compute_wall_forces, from the same states, bit for bit."""
+    g = gold("wall_reports.npz")
+    wref, wcases = _wall_fixture_cases()
+    ref = abi.wall_reference(**wref)
+    for key, case, _ in wcases:
+        m, bcs, p, ic, _ = case
+        o = pyoracle.OracleSolver(m, bcs, p, ic)
+        o.set_field(g[f"{key}_U"], g[f"{key}_T"] if f"{key}_T" in g.files else None)
+        wf = abi.collect_wall_faces(m, bcs)
+        for po in (0, 1):
+            smp, rep = o.wall_reports(wf, ref, bool(po))
+            gs, gr = g[f"{key}_p{po}_samples"], g[f"{key}_p{po}_reports"]
+            for k in smp.dtype.names:
+                assert np.array_equal(smp[k], gs[k], equal_nan=True), (key, po, k)
+            assert len(rep) == len(gr), key
+            for k in rep.dtype.names:
+                assert np.array_equal(rep[k], gr[k]), (key, po, k)
+        if p.equation != abi.EQ_EULER:
+            assert np.any(g[f"{key}_p0_samples"]["tau_wall"] != 0.0), key
+
+
+def test_wall_forces_vanish_on_closed_body_under_uniform_pressure():
+    """A uniform pressure on a closed section gives zero net force and zero
+    moment: the sum of A n over a closed polygon is zero, and so is the sum
+    of r x (A n)."""
+    m, bcs, p, ic, _ = cases.case_naca(64, 32)
+    o = pyoracle.OracleSolver(m, bcs, p, ic)
+    U = np.tile(M.from_primitive(1.3, 0.0, 0.0, 2.5, 1.4), (m.n_cells, 1))
+    o.set_field(U)
+    wf = abi.collect_wall_faces(m, bcs)
+    ref = abi.wall_reference(1.0, 1.0, 0.2, 0.0, 1.0, 0.25, 0.05)
+    smp, rep = o.wall_reports(wf, ref)
+    tot = rep[-1]
+    assert tot["patch_id"] == -1
+    scale = 2.5 * len(wf)  # pressure times the number of terms (chord 1, every |r| <= 1)
+    for k in ("total_drag", "lift", "moment"):
+        assert abs(tot[k]) <= 1e-14 * scale, (k, tot[k])
+    cl, cd = o.wall_forces(1.0, 1.0, 0.2)
+    assert abs(cl) <= 1e-13 and abs(cd) <= 1e-13
+    # and a non-uniform pressure does not vanish
+    U[:, 3] *= 1.0 + 0.1 * m.cell_xy[:, 0]
+    o.set_field(U)
+    assert abs(o.wall_forces(1.0, 1.0, 0.2)[1]) > 1e-3
