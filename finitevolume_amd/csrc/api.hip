@@ -273,6 +273,7 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
         }
     }
     int rc = hip_check(hipGetDevice(&s->device), "hipGetDevice");
+    if (rc == FVK_OK) rc = noalloc_setup(s->device);
     if (rc == FVK_OK) rc = hip_check(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), "hipStreamCreate");
     if (rc == FVK_OK) rc = hip_check(hipEventCreateWithFlags(&s->row_ready, hipEventDisableTiming), "hipEventCreate");
     if (rc == FVK_OK) rc = build_layout(s, mesh);
@@ -628,7 +629,8 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
 
 int fvk_synchronize(fvk_solver* s) {
     if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
-    return fvk::settle(s);
+    s->inflight = false;
+    return hip_check(hipStreamSynchronize(s->stream), "synchronize");
 }
 
 int fvk_residual(const fvk_solver* s, double* norms) {

@@ -282,6 +282,8 @@ struct WallLaunch {
     int* n_reports;
 };
 int launch_wall_diagnostics(fvk_solver* s, const WallLaunch& w);
+// per-device address of the no-alloc entry point's scratch (noalloc.hip), once per device
+int noalloc_setup(int device);
 void wall_launch_ref(WallLaunch& w, const fvk_wall_reference& ref);
 int wall_reports(fvk_solver* s, const int32_t* wf, int64_t n, const fvk_wall_reference* ref, int pressure_only,
                  fvk_wall_sample* samples, fvk_wall_force_report* reports, int32_t cap, int32_t* n_reports);

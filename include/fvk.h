@@ -205,9 +205,10 @@ void fvk_destroy(fvk_solver* s);
  * waits for them before it reads or writes the device. */
 int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hist);
 
-/* Waits for the work fvk_step left running.  No reference counterpart (the
- * reference's step() returns with the state updated); for callers that time
- * the device or hand its stream's results to other code. */
+/* Waits for all work queued on the solver's stream (what fvk_step or
+ * fvk_wall_force_reports_noalloc left running).  No reference counterpart
+ * (the reference's step() returns with the state updated); for callers
+ * that time the device or hand its stream's results to other code. */
 int fvk_synchronize(fvk_solver* s);
 
 /* Last step's residual norms [fvk_num_vars]: rho, rho_u, rho_v, E (, nut | k, omega);
@@ -326,6 +327,25 @@ int fvk_wall_traction(fvk_solver* s, const int32_t* wall_faces, int64_t n_wall, 
 int fvk_wall_force_reports(fvk_solver* s, const int32_t* wall_faces, int64_t n_wall, const fvk_wall_reference* ref,
                            fvk_wall_sample* samples, fvk_wall_force_report* reports, int32_t cap,
                            int32_t* n_reports);
+
+/* No counterpart in the reference: fvk_wall_force_reports for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It allocates nothing: wall_faces
+ * (reference face ids), samples (may be NULL), reports [cap >= n_bcs + 1]
+ * and n_reports are the caller's DEVICE buffers, and the kernels' scratch
+ * is a fixed device array of the library, one per GPU, which holds at most
+ * FVK_NOALLOC_MAX_WALL_FACES wall faces (larger n_wall is an error).  The
+ * work is queued on the solver's stream and the call returns without
+ * waiting: the outputs are final after fvk_synchronize.  Every entry of the
+ * scratch a call reads it wrote first, so nothing carries over between
+ * calls; but calls that overlap on one GPU (two solvers on two host threads)
+ * share the scratch and race.  An entry of wall_faces that is not a boundary
+ * face gives a sample with face_index -1 and NaN values and is left out of
+ * the sums. */
+#define FVK_NOALLOC_MAX_WALL_FACES 65536
+int fvk_wall_force_reports_noalloc(fvk_solver* s, const int32_t* wall_faces, int64_t n_wall,
+                                   const fvk_wall_reference* ref, fvk_wall_sample* samples,
+                                   fvk_wall_force_report* reports, int32_t cap, int32_t* n_reports);
 
 /* Extension: gradient of J = w_cl Cl + w_cd Cd (the pressure forces of
  * fvk_wall_forces) with respect to the state, U_bar [4*n_cells], and the
