@@ -127,6 +127,47 @@ def collect_wall_faces(mesh, bcs) -> np.ndarray:
     return np.ascontiguousarray([f for f in b if bcs[fp[f]].type == BC_WALL], dtype=np.int32)
 
 
+CKPT_DIFFUSION, CKPT_EULER, CKPT_ADVECTION_DIFFUSION, CKPT_NAVIER_STOKES, CKPT_RANS_SA, CKPT_RANS_SST = range(6)
+RUN_COMPLETED, RUN_CONVERGED, RUN_DIVERGED = 0, 1, 2
+
+
+class RunConfigC(C.Structure):
+    _fields_ = [
+        ("nsteps", C.c_int64), ("cfl_ramp", C.c_int32), ("ramp_window", C.c_int32), ("cfl", C.c_double),
+        ("ramp_cfl_min", C.c_double), ("ramp_growth", C.c_double), ("ramp_shrink", C.c_double),
+        ("ramp_divergence_threshold", C.c_double), ("tolerance", C.c_double * 6),
+        ("residual_file", C.c_char_p), ("residual_interval", C.c_int32), ("output_precision", C.c_int32),
+        ("checkpoint_file", C.c_char_p), ("build_number", C.c_uint64),
+    ]
+
+
+class RunResultC(C.Structure):
+    _fields_ = [
+        ("resume_start", C.c_int64), ("last_step", C.c_int64), ("status", C.c_int32),
+        ("resumed_equation", C.c_int32), ("residual", C.c_double * 6), ("cfl", C.c_double),
+    ]
+
+
+def run_config(nsteps, cfl=0.5, cfl_ramp=False, tolerance=None, residual_file=None, residual_interval=1,
+               output_precision=6, checkpoint_file=None, build_number=0, **ramp) -> RunConfigC:
+    """A run configuration with the reference's CaseInput defaults."""
+    c = RunConfigC()
+    c.nsteps, c.cfl, c.cfl_ramp = nsteps, cfl, int(cfl_ramp)
+    c.ramp_window = ramp.get("window", 15)
+    c.ramp_cfl_min = ramp.get("cfl_min", 0.05)
+    c.ramp_growth = ramp.get("growth", 1.5)
+    c.ramp_shrink = ramp.get("shrink", 0.5)
+    c.ramp_divergence_threshold = ramp.get("divergence_threshold", 2.0)
+    tol = list(tolerance or []) + [-1.0] * 6
+    for i in range(6):
+        c.tolerance[i] = tol[i]
+    c.residual_file = residual_file.encode() if residual_file else None
+    c.residual_interval, c.output_precision = residual_interval, output_precision
+    c.checkpoint_file = checkpoint_file.encode() if checkpoint_file else None
+    c.build_number = build_number
+    return c
+
+
 def default_params(**kw) -> ParamsC:
     """Reference defaults (mirrors fvk_params_default in the library)."""
     p = ParamsC()

@@ -61,6 +61,12 @@ def lib():
     L.fvk_wall_traction.argtypes = [vp, ip, i64, C.POINTER(abi.WallReferenceC), vp]
     L.fvk_wall_force_reports.argtypes = [vp, ip, i64, C.POINTER(abi.WallReferenceC), vp, vp, C.c_int32,
                                          C.POINTER(C.c_int32)]
+    L.fvk_next_cfl.argtypes = [C.c_double, C.c_double, dp, C.c_int32, C.c_double, C.c_double, C.c_double,
+                               C.c_int32, C.c_double]
+    L.fvk_next_cfl.restype = C.c_double
+    L.fvk_checkpoint_write.argtypes = [vp, C.c_char_p, i64, C.c_uint64]
+    L.fvk_checkpoint_read.argtypes = [vp, C.c_char_p, C.POINTER(i64), C.POINTER(C.c_uint64), ip]
+    L.fvk_run.argtypes = [vp, C.POINTER(abi.RunConfigC), C.POINTER(abi.RunResultC)]
     L.fvk_set_kernel_timing.argtypes = [vp, C.c_int32]
     L.fvk_kernel_times.argtypes = [vp, C.c_char_p, i64, dp, C.POINTER(i64), C.c_int32, C.POINTER(C.c_int32)]
     L.fvk_device_bytes.argtypes = [vp]
@@ -152,6 +158,24 @@ class FVKSolver:
 
     def set_cfl(self, cfl: float):
         _check(self.L.fvk_set_cfl(self.h, cfl))
+
+    # ---- driver contract ----
+    def checkpoint_write(self, path: str, step_index: int, build_number: int = 0):
+        """Checkpoint::write of the current field (src/Checkpoint.cpp)."""
+        _check(self.L.fvk_checkpoint_write(self.h, os.fsencode(path), step_index, build_number))
+
+    def checkpoint_read(self, path: str):
+        """Loads a checkpoint with the reference driver's resume rules;
+        returns (step_index, build_number, source equation tag)."""
+        st, bd, eq = C.c_int64(), C.c_uint64(), C.c_int32()
+        _check(self.L.fvk_checkpoint_read(self.h, os.fsencode(path), C.byref(st), C.byref(bd), C.byref(eq)))
+        return st.value, bd.value, eq.value
+
+    def run_case(self, cfg: abi.RunConfigC) -> abi.RunResultC:
+        """The reference driver's step loop (src/main.cpp:3491-3557) on this solver."""
+        res = abi.RunResultC()
+        _check(self.L.fvk_run(self.h, C.byref(cfg), C.byref(res)))
+        return res
 
     def gas_gamma(self) -> float:
         return self.L.fvk_gas_gamma(self.h)
@@ -353,6 +377,14 @@ def wall_distance(mesh: UnstructuredMesh, wall_faces) -> np.ndarray:
     out = np.empty(mesh.n_cells)
     _check(L.fvk_wall_distance_faces(md.ref, abi.as_iptr(wf), wf.shape[0], abi.as_dptr(out)))
     return out
+
+
+def next_cfl(current, cfl_max, history, cfl_min=0.05, growth=1.5, shrink=0.5, window=15,
+             divergence_threshold=2.0) -> float:
+    """next_cfl (src/CflRamp.cpp:7-27), the library's host implementation."""
+    h = np.ascontiguousarray(history, dtype=np.float64)
+    return lib().fvk_next_cfl(current, cfl_max, abi.as_dptr(h), h.shape[0], cfl_min, growth, shrink, window,
+                              divergence_threshold)
 
 
 def copy_probe(n_doubles: int, reps: int = 20) -> float:

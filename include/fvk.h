@@ -231,6 +231,71 @@ int fvk_get_state(const fvk_solver* s, double* U_aos, int64_t n_cells);
 int fvk_set_cfl(fvk_solver* s, double cfl);
 double fvk_gas_gamma(const fvk_solver* s);
 
+/* ---- the driver contract (src/main.cpp:3400-3564 and its NS/SA/SST and
+ * diffusion twins) ---- */
+
+/* next_cfl, src/CflRamp.cpp:7-27 (CflRampParams fields as arguments). */
+double fvk_next_cfl(double current_cfl, double cfl_max, const double* history, int32_t n_history, double cfl_min,
+                    double growth_factor, double shrink_factor, int32_t window, double divergence_threshold);
+
+/* CheckpointEquation, include/Checkpoint.h (same tags). */
+enum fvk_checkpoint_equation {
+    FVK_CKPT_DIFFUSION = 0, FVK_CKPT_EULER = 1, FVK_CKPT_ADVECTION_DIFFUSION = 2, FVK_CKPT_NAVIER_STOKES = 3,
+    FVK_CKPT_RANS_SA = 4, FVK_CKPT_RANS_SST = 5
+};
+
+/* Checkpoint::write, src/Checkpoint.cpp: the 40-byte header ("FVMCKPT\0",
+ * version 2, the solver's equation tag, cell count, step_index,
+ * build_number), then the field in reference cell order: EulerState AoS
+ * (4 doubles per cell; diffusion: phi), then nut (SA) or k then omega
+ * (SST), one array each.  Byte for byte the reference's file. */
+int fvk_checkpoint_write(fvk_solver* s, const char* path, int64_t step_index, uint64_t build_number);
+
+/* Reads a checkpoint into the solver with the reference driver's resume
+ * rules: diffusion needs a diffusion checkpoint (Checkpoint::read); a flow
+ * solver takes any flow-field checkpoint's EulerState payload
+ * (Checkpoint::read_flow_field) and the turbulence fields only from one of
+ * its own model, else it keeps its own (set_field(U)).  Header, equation
+ * and cell-count mismatches are errors with the reference's messages. */
+int fvk_checkpoint_read(fvk_solver* s, const char* path, int64_t* step_index, uint64_t* build_number,
+                        int32_t* source_equation);
+
+/* The run loop of the reference driver (CaseInput fields it reads). */
+typedef struct fvk_run_config {
+    int64_t nsteps;                /* last step index to run ("nsteps") */
+    int32_t cfl_ramp;              /* 1: cfl_mode = ramp, from ramp_cfl_min up to cfl */
+    int32_t ramp_window;           /* CflRampParams::window (15) */
+    double cfl;                    /* "cfl": the fixed CFL, or the ramp's ceiling */
+    double ramp_cfl_min, ramp_growth, ramp_shrink, ramp_divergence_threshold; /* 0.05, 1.5, 0.5, 2.0 */
+    double tolerance[6];           /* residual_tolerance_{rho, rho_u, rho_v, E, nut | k, omega}
+                                      (diffusion: [0] = residual_tolerance); < 0: not checked */
+    const char* residual_file;     /* NULL or "": no residual CSV */
+    int32_t residual_interval;     /* a CSV row every N steps (1) */
+    int32_t output_precision;      /* significant digits of every CSV double (6) */
+    const char* checkpoint_file;   /* NULL or "": none; resumed from when it exists, written at the end */
+    uint64_t build_number;         /* written into the checkpoint header */
+} fvk_run_config;
+
+typedef struct fvk_run_result {
+    int64_t resume_start;          /* first step index run: 1, or the checkpoint's step + 1 */
+    int64_t last_step;             /* last completed step index */
+    int32_t status;                /* 0: ran to nsteps, 1: converged, 2: diverged (NaN/Inf residual) */
+    int32_t resumed_equation;      /* the checkpoint's equation tag, -1 when nothing was resumed */
+    double residual[6];            /* residual norms of the last step */
+    double cfl;                    /* CFL after the last step */
+} fvk_run_result;
+
+void fvk_run_config_default(fvk_run_config* c);
+
+/* The reference driver's loop on the device solver (src/main.cpp:3491-3557;
+ * NS :3739-3830, SA :4018-4110, SST :4343-4440, diffusion :3139-3190):
+ * auto-resume from checkpoint_file, then per step: one step, the NaN/Inf
+ * bit test on every residual component, the CFL ramp on the density
+ * residual, the CSV row (std::setprecision(output_precision), flushed),
+ * the convergence test on every set tolerance; at the end a checkpoint
+ * unless the run diverged.  Divergence is a status, not an error. */
+int fvk_run(fvk_solver* s, const fvk_run_config* cfg, fvk_run_result* result);
+
 /* Launch shape the solver runs its Euler stage with (no counterpart in the
  * reference: it reports what the defaults or a tuner chose, and the tile
  * layout of variant 3). */

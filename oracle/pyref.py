@@ -47,6 +47,9 @@ def lib(flavour: str = "strict"):
     L.fvr_next_cfl.argtypes = [C.c_double, C.c_double, dp, C.c_int32, C.c_double, C.c_double, C.c_double,
                                C.c_int32, C.c_double]
     L.fvr_next_cfl.restype = C.c_double
+    L.fvr_checkpoint_write.argtypes = [vp, C.c_char_p, C.c_int64, C.c_uint64]
+    L.fvr_checkpoint_read.argtypes = [C.c_char_p, C.c_int32, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
+                                      dp, dp]
     L.fvr_wall_reports.argtypes = [vp, ip, C.c_int64, C.POINTER(abi.WallReferenceC), C.c_int32, vp, vp, C.c_int32,
                                    C.POINTER(C.c_int32)]
     _libs[flavour] = L
@@ -104,6 +107,12 @@ class RefSolver:
 
     def set_cfl(self, cfl: float):
         self.L.fvr_set_cfl(self.h, cfl)
+
+    def checkpoint_write(self, path, step, build=0):
+        """Checkpoint::write with the solver's own equation tag."""
+        rc = self.L.fvr_checkpoint_write(self.h, os.fsencode(path), step, build)
+        if rc != 0:
+            raise RuntimeError(f"Checkpoint::write failed for {path}")
 
     def wall_reports(self, wall_faces, ref, pressure_only: bool = False):
         """Wall samples and force reports (structured arrays of
@@ -199,3 +208,17 @@ def next_cfl(current, cfl_max, hist, cfl_min=0.05, growth=1.5, shrink=0.5, windo
     h = _f64(hist)
     return L.fvr_next_cfl(current, cfl_max, abi.as_dptr(h), h.shape[0], cfl_min, growth, shrink, window,
                           divergence_threshold)
+
+
+def checkpoint_read(path, tag, n_cells):
+    """Checkpoint::read of a file with equation tag `tag`: (step, build, U, turb)."""
+    L = lib()
+    nt = {4: 1, 5: 2}.get(tag, 0)
+    U = np.empty(n_cells if tag in (0, 2) else (n_cells, 4))
+    T = np.empty((n_cells, nt)) if nt else None
+    st, bd = C.c_int64(), C.c_uint64()
+    rc = L.fvr_checkpoint_read(os.fsencode(path), tag, n_cells, C.byref(st), C.byref(bd), abi.as_dptr(U),
+                               abi.as_dptr(T))
+    if rc != 0:
+        raise RuntimeError(f"Checkpoint::read failed for {path}")
+    return st.value, bd.value, U, T

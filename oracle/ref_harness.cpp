@@ -17,6 +17,7 @@
 #include "fvk.h"
 
 #include "CflRamp.h"
+#include "Checkpoint.h"
 #include "EulerFVMSolver.h"
 #include "ExactRiemannFlux.h"
 #include "GradientReconstruction.h"
@@ -370,6 +371,70 @@ int fvr_wall_reports(void* h, const int32_t* wf, int64_t n, const fvk_wall_refer
     }
     *n_reports = (int32_t)rep.size();
     return (int32_t)rep.size() > cap ? FVK_ERR_INVALID : FVK_OK;
+}
+
+// Checkpoint::write of the solver's field with the solver's own tag (the
+// reference driver's closing write, src/main.cpp:3553 and twins).
+int fvr_checkpoint_write(void* h, const char* path, int64_t step, uint64_t build) {
+    auto* r = static_cast<RefSolver*>(h);
+    bool ok = false;
+    switch (r->eq) {
+        case FVK_EQ_DIFFUSION:
+            ok = Checkpoint::write(path, CheckpointEquation::Diffusion, step, build, r->diff->field());
+            break;
+        case FVK_EQ_EULER: ok = Checkpoint::write(path, CheckpointEquation::Euler, step, build, r->euler->field()); break;
+        case FVK_EQ_NAVIER_STOKES:
+            ok = Checkpoint::write(path, CheckpointEquation::NavierStokes, step, build, r->ns->field());
+            break;
+        case FVK_EQ_RANS_SA:
+            ok = Checkpoint::write(path, CheckpointEquation::RANS_SA, step, build, r->sa->field(), r->sa->nut_field());
+            break;
+        case FVK_EQ_RANS_SST:
+            ok = Checkpoint::write(path, CheckpointEquation::RANS_SST, step, build, r->sst->field(), r->sst->k_field(),
+                                   r->sst->omega_field());
+            break;
+    }
+    return ok ? FVK_OK : FVK_ERR_INVALID;
+}
+
+// Checkpoint::read of a file with the given tag and cell count: U [4n]
+// (diffusion: phi [n]) and turb [nt n] (nut, or k then omega, AoS per cell).
+int fvr_checkpoint_read(const char* path, int32_t tag, int64_t n, int64_t* step, uint64_t* build, double* U,
+                        double* turb) {
+    long long st = 0;
+    unsigned long long bd = 0;
+    bool ok = false;
+    auto cp = [&](const std::vector<EulerState>& v) {
+        for (int64_t c = 0; c < n; ++c) {
+            U[4 * c] = v[c].rho; U[4 * c + 1] = v[c].rho_u; U[4 * c + 2] = v[c].rho_v; U[4 * c + 3] = v[c].E;
+        }
+    };
+    CheckpointEquation eq = static_cast<CheckpointEquation>(tag);
+    if (eq == CheckpointEquation::Diffusion || eq == CheckpointEquation::AdvectionDiffusion) {
+        std::vector<double> phi;
+        ok = Checkpoint::read(path, eq, (size_t)n, st, bd, phi);
+        if (ok) std::memcpy(U, phi.data(), sizeof(double) * n);
+    } else if (eq == CheckpointEquation::RANS_SA) {
+        std::vector<EulerState> v;
+        std::vector<double> nut;
+        ok = Checkpoint::read(path, eq, (size_t)n, st, bd, v, nut);
+        if (ok) { cp(v); std::memcpy(turb, nut.data(), sizeof(double) * n); }
+    } else if (eq == CheckpointEquation::RANS_SST) {
+        std::vector<EulerState> v;
+        std::vector<double> k, w;
+        ok = Checkpoint::read(path, eq, (size_t)n, st, bd, v, k, w);
+        if (ok) {
+            cp(v);
+            for (int64_t c = 0; c < n; ++c) { turb[2 * c] = k[c]; turb[2 * c + 1] = w[c]; }
+        }
+    } else {
+        std::vector<EulerState> v;
+        ok = Checkpoint::read(path, eq, (size_t)n, st, bd, v);
+        if (ok) cp(v);
+    }
+    *step = st;
+    *build = bd;
+    return ok ? FVK_OK : FVK_ERR_INVALID;
 }
 
 // ---- standalone functions -------------------------------------------------

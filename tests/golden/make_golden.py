@@ -127,12 +127,40 @@ def write_wall_fixture(w):
     w("wall_reports.npz", **out)
 
 
+def checkpoint_cases():
+    """The checkpoint fixture cases: (file name, case, steps, step index,
+    build number)."""
+    return [
+        ("ckpt_euler_naca.bin", cases.case_naca(32, 16), 20, 20, 7),
+        ("ckpt_ns_wavy.bin", cases.case_ns_wavy(), 20, 20, 7),
+        ("ckpt_sa_wavy.bin", cases.case_rans_wavy(abi.EQ_RANS_SA), 20, 1020, 8),
+        ("ckpt_sst_wavy.bin", cases.case_rans_wavy(abi.EQ_RANS_SST, variant=1, kato=1), 20, 20, 9),
+        ("ckpt_diffusion.bin", cases.case_diffusion(), 30, 30, 10),
+    ]
+
+
+def write_checkpoint_fixtures():
+    """Checkpoint::write of the reference solvers after a few steps
+    (src/Checkpoint.cpp): the files themselves are the fixtures."""
+    for name, case, steps, step_index, build in checkpoint_cases():
+        m, bcs, p, ic, U0 = case
+        r = pyref.RefSolver(m, bcs, p, ic)
+        if U0 is not None:
+            r.set_field(U0)
+        r.step(steps)
+        r.checkpoint_write(os.path.join(HERE, name), step_index, build)
+
+
 def main(only=None):
     assert pyref.available("strict"), "oracle/_ref/libfvref_strict.so missing: make -C oracle ref"
     w = lambda name, **arrs: np.savez_compressed(os.path.join(HERE, name), **arrs)
     if only == "wall":
         write_wall_fixture(w)
         print("wall fixture written to", HERE)
+        return
+    if only == "checkpoint":
+        write_checkpoint_fixtures()
+        print("checkpoint fixtures written to", HERE)
         return
 
     # ---- numerical fluxes ----
@@ -219,9 +247,10 @@ def main(only=None):
     outs = {k: np.array([pyref.next_cfl(cur, 10.0, s) for cur in (0.05, 1.0, 9.0)]) for k, s in seqs.items()}
     w("cfl_ramp.npz", **{f"seq_{k}": v for k, v in seqs.items()}, **{f"out_{k}": v for k, v in outs.items()})
     write_wall_fixture(w)
+    write_checkpoint_fixtures()
     print("fixtures written to", HERE)
 
 
 if __name__ == "__main__":
-    # `make_golden.py wall` rewrites only the wall-diagnostics fixture
+    # `make_golden.py wall` (or `checkpoint`) rewrites only those fixtures
     main(sys.argv[1] if len(sys.argv) > 1 else None)
