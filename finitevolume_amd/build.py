@@ -20,7 +20,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libfvk.so")
-SOURCES = ["api.hip", "layout.cpp", "kernels_euler.hip", "kernels_visc.hip", "adjoint.hip", "batch.hip", "meshgen.cpp", "noalloc.hip", "driver.cpp"]
+SOURCES = ["api.hip", "layout.cpp", "kernels_euler.hip", "kernels_visc.hip", "adjoint.hip", "batch.hip", "meshgen.cpp", "noalloc.hip", "driver.cpp", "meshio.cpp"]
 ARCH = os.environ.get("FVK_OFFLOAD_ARCH", "gfx950")
 
 

@@ -1,6 +1,6 @@
 """Unstructured 2D polygon meshes in the reference's layout, plus generators.
 
-The arrays mirror ``UnstructuredMesh`` (include/UnstructuredMesh.h:246-300):
+The arrays mirror ``UnstructuredMesh`` (include/UnstructuredMesh.h:20-109):
 nodes, faces (node1, node2, cell_left, cell_right, patch_id) and cells
 (volume, centroid, ordered face list, ordered node list).  Face creation
 order, face orientation and each cell's face order follow the rules of the

@@ -91,7 +91,7 @@ enum fvk_limiter { FVK_LIMITER_NONE = 0, FVK_LIMITER_VENKATAKRISHNAN = 1, FVK_LI
 
 /*
  * Mesh, in the reference's UnstructuredMesh layout (include/UnstructuredMesh.h:
- * Node :246-248, Face :254-262, Cell :265-271) flattened to arrays.  Face
+ * Node :20, Face :28, Cell :39) flattened to arrays.  Face
  * geometry (normal, length, midpoint) is computed by the library from the
  * node coordinates, as the reference solver constructors do
  * (UnstructuredMesh::compute_geometry, include/UnstructuredMesh.h:89-105);
@@ -463,6 +463,27 @@ int fvk_batch_set_graph(fvk_batch* b, int32_t enable);
 int fvk_batch_get_field(fvk_batch* b, int32_t k, double* U);
 /* The solver over the union mesh (kernel timing, device bytes). */
 fvk_solver* fvk_batch_solver(fvk_batch* b);
+
+/* ---- mesh ingest (src/MeshReader.cpp, docs/fvmesh-format.md) ---- */
+
+/* A mesh read from a file; owns the arrays fvk_mesh_file_desc points into. */
+typedef struct fvk_mesh_file fvk_mesh_file;
+
+/* MeshReader::read (src/MeshReader.cpp:963-988): ".msh" (Gmsh ASCII 2.2 or
+ * 4.x, by $MeshFormat) or ".fvmesh" (FVMESH 1.0).  Faces, their orientation
+ * and order, Cell::faces, patch numbering and names, and cell volumes and
+ * centroids are those the reference's reader builds
+ * (build_cells_faces_patches, :317-456); a mesh it rejects (untagged
+ * boundary edges, inverted, degenerate or self-intersecting cells, zero-
+ * length faces, malformed sections) is an error here with its message. */
+int fvk_mesh_read(const char* path, fvk_mesh_file** out);
+void fvk_mesh_file_destroy(fvk_mesh_file* m);
+/* The mesh as fvk_create takes it (pointers into the handle's arrays). */
+int fvk_mesh_file_desc(const fvk_mesh_file* m, fvk_mesh_desc* desc);
+/* Cell::node_ids as CSR: ptr [n_cells + 1], nodes [ptr[n_cells]]. */
+int fvk_mesh_file_cell_nodes(const fvk_mesh_file* m, const int32_t** ptr, const int32_t** nodes);
+/* BoundaryPatch::name of patch id `patch`, NULL out of range. */
+const char* fvk_mesh_file_patch_name(const fvk_mesh_file* m, int32_t patch);
 
 /* Mesh-generation helper (host only, no reference counterpart): Bridson
  * Poisson-disk samples of radius `radius` in [x0,x1) x [y0,y1), outside a

@@ -50,6 +50,10 @@ def lib(flavour: str = "strict"):
     L.fvr_checkpoint_write.argtypes = [vp, C.c_char_p, C.c_int64, C.c_uint64]
     L.fvr_checkpoint_read.argtypes = [C.c_char_p, C.c_int32, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
                                       dp, dp]
+    L.fvr_mesh_read.argtypes = [C.c_char_p, C.POINTER(vp), C.POINTER(C.c_int64)]
+    L.fvr_mesh_arrays.argtypes = [vp, dp, ip, ip, ip, ip, ip, ip, dp, dp, C.c_char_p, C.c_int64]
+    L.fvr_mesh_free.argtypes = [vp]
+    L.fvr_mesh_free.restype = None
     L.fvr_wall_reports.argtypes = [vp, ip, C.c_int64, C.POINTER(abi.WallReferenceC), C.c_int32, vp, vp, C.c_int32,
                                    C.POINTER(C.c_int32)]
     _libs[flavour] = L
@@ -222,3 +226,30 @@ def checkpoint_read(path, tag, n_cells):
     if rc != 0:
         raise RuntimeError(f"Checkpoint::read failed for {path}")
     return st.value, bd.value, U, T
+
+
+def read_mesh(path):
+    """MeshReader::read of the reference: a dict of its mesh arrays, or
+    None when the reference rejects the file."""
+    L = lib()
+    h = C.c_void_p()
+    cnt = (C.c_int64 * 5)()
+    if L.fvr_mesh_read(os.fsencode(path), C.byref(h), cnt) != 0:
+        return None
+    nn, nc, nf, npch, nnz = [int(x) for x in cnt]
+    out = dict(node_xy=np.empty((nn, 2)), face_nodes=np.empty((nf, 2), np.int32),
+               face_cells=np.empty((nf, 2), np.int32), face_patch=np.empty(nf, np.int32),
+               cell_face_ptr=np.empty(nc + 1, np.int32), cell_faces=np.empty(nnz, np.int32),
+               cell_nodes=np.empty(nnz, np.int32), cell_volume=np.empty(nc), cell_xy=np.empty((nc, 2)))
+    names = C.create_string_buffer(1 << 16)
+    try:
+        rc = L.fvr_mesh_arrays(h, abi.as_dptr(out["node_xy"]), abi.as_iptr(out["face_nodes"]),
+                               abi.as_iptr(out["face_cells"]), abi.as_iptr(out["face_patch"]),
+                               abi.as_iptr(out["cell_face_ptr"]), abi.as_iptr(out["cell_faces"]),
+                               abi.as_iptr(out["cell_nodes"]), abi.as_dptr(out["cell_volume"]),
+                               abi.as_dptr(out["cell_xy"]), names, len(names))
+    finally:
+        L.fvr_mesh_free(h)
+    assert rc == 0
+    out["patches"] = names.value.decode().split("\n")[:npch]
+    return out

@@ -6,7 +6,7 @@
 //
 // Every entry point takes the same fvk_mesh_desc / fvk_bc / fvk_params
 // structs as the product's C ABI (include/fvk.h) and builds the reference's
-// UnstructuredMesh (include/UnstructuredMesh.h:246-300) from them.
+// UnstructuredMesh (include/UnstructuredMesh.h:20-109) from them.
 
 #include <cmath>
 #include <cstring>
@@ -21,6 +21,7 @@
 #include "EulerFVMSolver.h"
 #include "ExactRiemannFlux.h"
 #include "GradientReconstruction.h"
+#include "MeshReader.h"
 #include "HllcFlux.h"
 #include "NavierStokesFVMSolver.h"
 #include "RANSTurbulenceSASolver.h"
@@ -436,6 +437,58 @@ int fvr_checkpoint_read(const char* path, int32_t tag, int64_t n, int64_t* step,
     *build = bd;
     return ok ? FVK_OK : FVK_ERR_INVALID;
 }
+
+// MeshReader::read into a heap mesh: counts [nodes, cells, faces, patches,
+// sum of cell face counts]; fvr_mesh_arrays copies its arrays out.
+int fvr_mesh_read(const char* path, void** out, int64_t* counts) {
+    auto* m = new UnstructuredMesh;
+    if (!MeshReader::read(path, *m)) {
+        delete m;
+        return FVK_ERR_INVALID;
+    }
+    int64_t nnz = 0;
+    for (const Cell& c : m->cells) nnz += (int64_t)c.faces.size();
+    counts[0] = (int64_t)m->nodes.size();
+    counts[1] = (int64_t)m->cells.size();
+    counts[2] = (int64_t)m->faces.size();
+    counts[3] = (int64_t)m->patches.size();
+    counts[4] = nnz;
+    *out = m;
+    return FVK_OK;
+}
+
+int fvr_mesh_arrays(void* h, double* node_xy, int32_t* face_nodes, int32_t* face_cells, int32_t* face_patch,
+                    int32_t* cell_ptr, int32_t* cell_faces, int32_t* cell_nodes, double* cell_volume, double* cell_xy,
+                    char* names, int64_t names_cap) {
+    auto* m = static_cast<UnstructuredMesh*>(h);
+    for (size_t i = 0; i < m->nodes.size(); ++i) { node_xy[2 * i] = m->nodes[i].x; node_xy[2 * i + 1] = m->nodes[i].y; }
+    for (size_t f = 0; f < m->faces.size(); ++f) {
+        const Face& fc = m->faces[f];
+        face_nodes[2 * f] = fc.node1; face_nodes[2 * f + 1] = fc.node2;
+        face_cells[2 * f] = fc.cell_left; face_cells[2 * f + 1] = fc.cell_right;
+        face_patch[f] = fc.patch_id;
+    }
+    int64_t k = 0;
+    cell_ptr[0] = 0;
+    for (size_t c = 0; c < m->cells.size(); ++c) {
+        const Cell& cl = m->cells[c];
+        for (size_t j = 0; j < cl.faces.size(); ++j) {
+            cell_faces[k] = cl.faces[j];
+            cell_nodes[k] = cl.node_ids[j];
+            ++k;
+        }
+        cell_ptr[c + 1] = (int32_t)k;
+        cell_volume[c] = cl.volume;
+        cell_xy[2 * c] = cl.x_centroid; cell_xy[2 * c + 1] = cl.y_centroid;
+    }
+    std::string all;
+    for (const BoundaryPatch& p : m->patches) all += p.name + "\n";
+    if ((int64_t)all.size() + 1 > names_cap) return FVK_ERR_INVALID;
+    std::memcpy(names, all.c_str(), all.size() + 1);
+    return FVK_OK;
+}
+
+void fvr_mesh_free(void* h) { delete static_cast<UnstructuredMesh*>(h); }
 
 // ---- standalone functions -------------------------------------------------
 
