@@ -11,7 +11,7 @@ from typing import Optional, Sequence
 import numpy as np
 
 # enums (include/fvk.h)
-EQ_EULER, EQ_NAVIER_STOKES, EQ_RANS_SA, EQ_RANS_SST, EQ_DIFFUSION = 0, 1, 2, 3, 4
+EQ_EULER, EQ_NAVIER_STOKES, EQ_RANS_SA, EQ_RANS_SST, EQ_DIFFUSION, EQ_ADVECTION_DIFFUSION = 0, 1, 2, 3, 4, 5
 FLUX_RUSANOV, FLUX_HLLC, FLUX_EXACT = 0, 1, 2
 GRAD_GREEN_GAUSS, GRAD_LEAST_SQUARES = 0, 1
 BC_WALL, BC_FARFIELD, BC_OUTFLOW = 0, 1, 2
@@ -67,7 +67,7 @@ class ParamsC(C.Structure):
         ("order", C.c_int32), ("limiter", C.c_int32), ("venkat_k", C.c_double),
         ("rk_stages", C.c_int32), ("local_dt", C.c_int32),
         ("kernel_variant", C.c_int32), ("block_size", C.c_int32), ("tile_cells", C.c_int32),
-        ("reserved1", C.c_int32),
+        ("reserved1", C.c_int32), ("advection_u", C.c_double), ("advection_v", C.c_double),
     ]
 
 
@@ -269,7 +269,8 @@ class MeshDesc:
 
 
 def nvars(equation: int) -> int:
-    return {EQ_EULER: 4, EQ_NAVIER_STOKES: 4, EQ_RANS_SA: 5, EQ_RANS_SST: 6, EQ_DIFFUSION: 1}[equation]
+    return {EQ_EULER: 4, EQ_NAVIER_STOKES: 4, EQ_RANS_SA: 5, EQ_RANS_SST: 6, EQ_DIFFUSION: 1,
+            EQ_ADVECTION_DIFFUSION: 1}[equation]
 
 
 def as_dptr(a: Optional[np.ndarray]):

@@ -108,8 +108,11 @@ static void timers_collect(fvk_solver* s) {
 
 using namespace fvk;
 
+// the two scalar-field solvers (UnstructuredFVMSolver, AdvectionDiffusionFVMSolver)
+static bool scalar_eq(int eq) { return eq == FVK_EQ_DIFFUSION || eq == FVK_EQ_ADVECTION_DIFFUSION; }
+
 static int nvars_eq(int eq) {
-    return eq == FVK_EQ_RANS_SA ? 5 : (eq == FVK_EQ_RANS_SST ? 6 : (eq == FVK_EQ_DIFFUSION ? 1 : 4));
+    return eq == FVK_EQ_RANS_SA ? 5 : (eq == FVK_EQ_RANS_SST ? 6 : (scalar_eq(eq) ? 1 : 4));
 }
 
 extern "C" {
@@ -168,7 +171,7 @@ int fvk::upload_state(fvk_solver* s, const double* U4, const double* turb, bool 
     const int nv = s->nv;
     std::vector<double> h((size_t)nv * ld, 0.0);
     if (!U4) return set_error(FVK_ERR_INVALID, "state pointer is NULL");
-    if (s->p.equation == FVK_EQ_DIFFUSION) {
+    if (scalar_eq(s->p.equation)) {
         for (int64_t i = 0; i < nc; ++i) h[i] = U4[s->cell_new2old[i]];
     } else if (s->p.equation == FVK_EQ_EULER) {
         // Euler: one double4 per cell
@@ -215,7 +218,7 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
                   const fvk_initial_condition* ic, const int32_t* cell_case, int32_t n_cases, fvk_solver** out) {
     if (!mesh || !params || !out) return set_error(FVK_ERR_INVALID, "NULL argument to fvk_create");
     *out = nullptr;
-    if (params->equation < FVK_EQ_EULER || params->equation > FVK_EQ_DIFFUSION)
+    if (params->equation < FVK_EQ_EULER || params->equation > FVK_EQ_ADVECTION_DIFFUSION)
         return set_error(FVK_ERR_INVALID, "unknown equation %d", params->equation);
     if (params->flux_scheme < 0 || params->flux_scheme > 2)
         return set_error(FVK_ERR_INVALID, "unknown flux scheme %d", params->flux_scheme);
@@ -227,14 +230,14 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
         return set_error(FVK_ERR_UNSUPPORTED, "second-order reconstruction is built for the Euler equations only");
     if (params->rk_stages < 1 || params->rk_stages > 3)
         return set_error(FVK_ERR_INVALID, "rk_stages must be 1, 2 or 3 (got %d)", params->rk_stages);
-    if (params->equation == FVK_EQ_DIFFUSION && (params->rk_stages != 1 || params->local_dt))
-        return set_error(FVK_ERR_UNSUPPORTED, "the diffusion solver is forward Euler with a fixed dt");
+    if (scalar_eq(params->equation) && (params->rk_stages != 1 || params->local_dt))
+        return set_error(FVK_ERR_UNSUPPORTED, "the scalar solvers are forward Euler with a fixed dt");
     if (n_bcs < 0 || n_bcs > kMaxPatches) return set_error(FVK_ERR_INVALID, "n_bcs out of range (%d)", n_bcs);
     if (n_bcs > 0 && !bcs) return set_error(FVK_ERR_INVALID, "bcs is NULL");
 
     if (params->kernel_variant < 0 || params->kernel_variant > 3)
         return set_error(FVK_ERR_INVALID, "kernel_variant must be 0, 1, 2 or 3 (got %d)", params->kernel_variant);
-    if (params->kernel_variant == 3 && (params->equation == FVK_EQ_DIFFUSION || cell_case))
+    if (params->kernel_variant == 3 && (scalar_eq(params->equation) || cell_case))
         return set_error(FVK_ERR_UNSUPPORTED,
                          "the fused tile kernels (variant 3) are built for single Euler, Navier-Stokes and RANS cases");
     {
@@ -315,10 +318,13 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
         s->dtf = static_cast<double*>(dev_alloc(s, sizeof(double) * s->ldf));
     }
     s->dtc = static_cast<double*>(dev_alloc(s, sizeof(double) * s->ld));
-    int ng = euler ? (params->order == 2 ? 8 : 0) : (params->equation == FVK_EQ_DIFFUSION ? 0 : 10);
+    const bool advdiff = params->equation == FVK_EQ_ADVECTION_DIFFUSION;
+    int ng = euler ? (params->order == 2 ? 8 : 0)
+                   : (params->equation == FVK_EQ_DIFFUSION ? 0 : (advdiff ? 2 : 10));
     if (ng) s->grad = static_cast<double*>(dev_alloc(s, sizeof(double) * ng * s->ld));
     if (params->equation == FVK_EQ_RANS_SA || params->equation == FVK_EQ_RANS_SST)
         s->aux = static_cast<double*>(dev_alloc(s, sizeof(double) * 4 * s->ld));
+    if (advdiff) s->aux = static_cast<double*>(dev_alloc(s, sizeof(double) * s->ldf));  // boundary phi
     // launch grids (workgroups of 64 threads or more, or one per tile plus a
     // spare) are rounded up to a multiple of the 8 XCDs
     int64_t max_blocks = std::max((std::max(s->nc, s->nf) + 63) / 64, (int64_t)s->mesh.ntiles + 1) + 8;
@@ -377,7 +383,7 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
         return rc;
     }
     // initial condition (solver constructors, e.g. src/EulerFVMSolver.cpp:24-37)
-    std::vector<double> U4((size_t)(s->p.equation == FVK_EQ_DIFFUSION ? 1 : 4) * s->nc, 0.0);
+    std::vector<double> U4((size_t)(scalar_eq(s->p.equation) ? 1 : 4) * s->nc, 0.0);
     std::vector<double> T((size_t)(nv > 4 ? nv - 4 : 1) * s->nc, 0.0);
     if (ic) {
         double g = params->gamma;
@@ -386,9 +392,10 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
         };
         for (int64_t c = 0; c < s->nc; ++c) {
             double x = mesh->cell_xy[2 * c], y = mesh->cell_xy[2 * c + 1];
-            if (params->equation == FVK_EQ_DIFFUSION) {
-                // src/UnstructuredFVMSolver.cpp:13-21
-                if (ic->mode != FVK_IC_NONE && std::sqrt(x * x + y * y) < ic->initial_radius) U4[c] = ic->initial_value;
+            if (scalar_eq(params->equation)) {
+                // src/UnstructuredFVMSolver.cpp:13-21 (sqrt), src/AdvectionDiffusionFVMSolver.cpp:22-31 (hypot)
+                const double r = params->equation == FVK_EQ_DIFFUSION ? std::sqrt(x * x + y * y) : std::hypot(x, y);
+                if (ic->mode != FVK_IC_NONE && r < ic->initial_radius) U4[c] = ic->initial_value;
                 continue;
             }
             if (ic->mode == FVK_IC_FREESTREAM) prim(&U4[4 * c], ic->rho, ic->u, ic->v, ic->p);
@@ -436,7 +443,7 @@ void fvk::wall_launch_ref(WallLaunch& w, const fvk_wall_reference& ref) {
 // allocated for the call.
 int fvk::wall_reports(fvk_solver* s, const int32_t* wf, int64_t n, const fvk_wall_reference* ref, int pressure_only,
                       fvk_wall_sample* samples, fvk_wall_force_report* reports, int32_t cap, int32_t* n_reports) {
-    if (s->p.equation == FVK_EQ_DIFFUSION) return set_error(FVK_ERR_UNSUPPORTED, "no wall forces for diffusion");
+    if (scalar_eq(s->p.equation)) return set_error(FVK_ERR_UNSUPPORTED, "no wall forces for a scalar field");
     if (n < 0 || n > (int64_t)1 << 30 || cap < 0) return set_error(FVK_ERR_INVALID, "bad wall face count");
     for (int64_t i = 0; i < n; ++i) {
         if (wf[i] < 0 || wf[i] >= s->nf) return set_error(FVK_ERR_INVALID, "wall face %d out of range", wf[i]);
@@ -518,7 +525,7 @@ int fvk_get_field(const fvk_solver* sc, double* U4, double* turb, int64_t n_cell
     hipError_t e = hipMemcpyAsync(h.data(), s->U[s->cur], sizeof(double) * nv * ld, hipMemcpyDeviceToHost, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return hip_check(e, "download state");
-    if (s->p.equation == FVK_EQ_DIFFUSION) {
+    if (scalar_eq(s->p.equation)) {
         for (int64_t i = 0; i < nc; ++i) U4[s->cell_new2old[i]] = h[i];
         return FVK_OK;
     }
@@ -575,7 +582,7 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
     if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
     if (nsteps < 0) return set_error(FVK_ERR_INVALID, "nsteps < 0");
     const int nv = s->nv;
-    const int stages = s->p.equation == FVK_EQ_DIFFUSION ? 1 : s->p.rk_stages;
+    const int stages = scalar_eq(s->p.equation) ? 1 : s->p.rk_stages;
     const double* hr = s->res_hist_host;
     const double* hd = s->dt_hist_host;
     int done = 0;
@@ -586,6 +593,8 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
             int rc = FVK_OK;
             if (s->p.equation == FVK_EQ_DIFFUSION) {
                 rc = launch_diffusion_step(s, 1, i);
+            } else if (s->p.equation == FVK_EQ_ADVECTION_DIFFUSION) {
+                rc = launch_advdiff_step(s, 1, i);
             } else {
                 for (int st = 0; st < stages && rc == FVK_OK; ++st) {
                     // the history index is the step within this chunk; the dt slot follows the global count
@@ -617,7 +626,7 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
         // rows were written at index (global step % hist_cap)
         for (int i = 0; i < chunk; ++i) {
             int64_t gstep = s->step_count - chunk + i;
-            int64_t idx = s->p.equation == FVK_EQ_DIFFUSION ? i : gstep % s->hist_cap;
+            int64_t idx = scalar_eq(s->p.equation) ? i : gstep % s->hist_cap;
             if (residual_hist) std::memcpy(residual_hist + (int64_t)(done + i) * nv, &hr[idx * nv], sizeof(double) * nv);
             if (dt_hist) dt_hist[done + i] = hd[idx];
             if (i == chunk - 1) std::memcpy(s->last_res, &hr[idx * nv], sizeof(double) * nv);
@@ -675,6 +684,7 @@ int fvk_residual_eval(fvk_solver* s, double* R) {
     if (rc != FVK_OK) return rc;
     if (s->p.equation == FVK_EQ_EULER) rc = launch_euler_residual_eval(s, d);
     else if (s->p.equation == FVK_EQ_DIFFUSION) rc = launch_diffusion_residual_eval(s, d);
+    else if (s->p.equation == FVK_EQ_ADVECTION_DIFFUSION) rc = launch_advdiff_residual_eval(s, d);
     else rc = launch_visc_residual_eval(s, d);
     std::vector<double> h((size_t)nv * s->ld);
     if (rc == FVK_OK) rc = hip_check(hipMemcpyAsync(h.data(), d, sizeof(double) * nv * s->ld, hipMemcpyDeviceToHost, s->stream), "residual copy");
@@ -768,7 +778,7 @@ int fvk_residual_vjp(fvk_solver* s, const double* lambda, double* U_bar, double*
 int fvk_wall_forces(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref, double* cl,
                     double* cd) {
     if (!s || !cl || !cd) return set_error(FVK_ERR_INVALID, "NULL argument");
-    if (s->p.equation == FVK_EQ_DIFFUSION) return set_error(FVK_ERR_UNSUPPORTED, "no pressure for diffusion");
+    if (scalar_eq(s->p.equation)) return set_error(FVK_ERR_UNSUPPORTED, "no pressure for a scalar field");
     // src/WallTraction.cpp:163-216 over pressure-only samples (A p_cell n) of
     // every wall face, in reference face order
     std::vector<int32_t> wf;
@@ -806,7 +816,7 @@ int fvk_wall_forces_vjp(fvk_solver* s, double rho_ref, double vx_ref, double vy_
                         double w_cd, double* U_bar, double* X_bar) {
     if (!s || (!U_bar && !X_bar)) return set_error(FVK_ERR_INVALID, "NULL argument");
     if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
-    if (s->p.equation == FVK_EQ_DIFFUSION) return set_error(FVK_ERR_UNSUPPORTED, "no pressure for diffusion");
+    if (scalar_eq(s->p.equation)) return set_error(FVK_ERR_UNSUPPORTED, "no pressure for a scalar field");
     // J = w_cl Cl + w_cd Cd = sum over wall faces of p_cell (A n) . e, with
     // A n = (ey, -ex) for the edge e = node2 - node1 and
     // e = (w_cl lift_dir + w_cd drag_dir) / (0.5 rho_ref |V|^2 L_ref)

@@ -35,12 +35,15 @@ constexpr uint32_t kVersion = 2;
 int ckpt_tag(int eq) {
     switch (eq) {
         case FVK_EQ_DIFFUSION: return FVK_CKPT_DIFFUSION;
+        case FVK_EQ_ADVECTION_DIFFUSION: return FVK_CKPT_ADVECTION_DIFFUSION;
         case FVK_EQ_EULER: return FVK_CKPT_EULER;
         case FVK_EQ_NAVIER_STOKES: return FVK_CKPT_NAVIER_STOKES;
         case FVK_EQ_RANS_SA: return FVK_CKPT_RANS_SA;
         default: return FVK_CKPT_RANS_SST;
     }
 }
+
+bool scalar(int eq) { return eq == FVK_EQ_DIFFUSION || eq == FVK_EQ_ADVECTION_DIFFUSION; }
 
 bool is_flow_field(uint32_t tag) {
     return tag == FVK_CKPT_EULER || tag == FVK_CKPT_NAVIER_STOKES || tag == FVK_CKPT_RANS_SA ||
@@ -88,7 +91,7 @@ int fvk_checkpoint_write(fvk_solver* s, const char* path, int64_t step_index, ui
     const int64_t n = s->nc;
     const int eq = s->p.equation;
     const int nt = s->nv > 4 ? s->nv - 4 : 0;
-    std::vector<double> U((eq == FVK_EQ_DIFFUSION ? 1 : 4) * (size_t)n), T((size_t)nt * n);
+    std::vector<double> U((scalar(eq) ? 1 : 4) * (size_t)n), T((size_t)nt * n);
     int rc = fvk_get_field(s, U.data(), nt ? T.data() : nullptr, n);
     if (rc != FVK_OK) return rc;
     std::ofstream out(path, std::ios::binary);
@@ -128,17 +131,17 @@ int fvk_checkpoint_read(fvk_solver* s, const char* path, int64_t* step_index, ui
                          h.version, kVersion);
     const int eq = s->p.equation;
     const int64_t n = s->nc;
-    // the resume rules of the reference driver: diffusion needs its own
-    // tag (src/main.cpp:3117); the flow solvers take any flow field
+    // the resume rules of the reference driver: the scalar solvers need
+    // their own tag (src/main.cpp:3117, 3242); the flow solvers take any flow field
     // (read_flow_field, :3461, 3662, 3940, 4260) and the turbulence fields
     // only from a checkpoint of their own model (:3944-3960, 4264-4281)
-    if (eq == FVK_EQ_DIFFUSION ? h.equation != FVK_CKPT_DIFFUSION : !is_flow_field(h.equation))
+    if (scalar(eq) ? h.equation != (uint32_t)ckpt_tag(eq) : !is_flow_field(h.equation))
         return set_error(FVK_ERR_INVALID, "checkpoint file '%s' holds equation tag %u, which this solver cannot resume",
                          path, h.equation);
     if (h.cell_count != (uint64_t)n)
         return set_error(FVK_ERR_INVALID, "checkpoint file '%s' has %llu cells but the mesh has %lld", path,
                          (unsigned long long)h.cell_count, (long long)n);
-    const int nw = eq == FVK_EQ_DIFFUSION ? 1 : 4;
+    const int nw = scalar(eq) ? 1 : 4;
     std::vector<double> U((size_t)nw * n);
     in.read(reinterpret_cast<char*>(U.data()), (std::streamsize)(sizeof(double) * U.size()));
     if (!in) return set_error(FVK_ERR_INVALID, "checkpoint file '%s' is truncated or corrupt (short payload)", path);
@@ -187,9 +190,9 @@ int fvk_run(fvk_solver* s, const fvk_run_config* cfg, fvk_run_result* res) {
     res->resumed_equation = -1;
     // ramp start (src/main.cpp:3438-3442): from cfl_min, not the ceiling
     double cfl = cfg->cfl;
-    const bool ramp = cfg->cfl_ramp && eq != FVK_EQ_DIFFUSION;
+    const bool ramp = cfg->cfl_ramp && !scalar(eq);
     if (ramp) cfl = cfg->ramp_cfl_min;
-    if (eq != FVK_EQ_DIFFUSION) {
+    if (!scalar(eq)) {
         if (int rc = fvk_set_cfl(s, cfl); rc != FVK_OK) return rc;
     }
     // auto-resume (src/main.cpp:3449-3472)
@@ -216,7 +219,8 @@ int fvk_run(fvk_solver* s, const fvk_run_config* cfg, fvk_run_result* res) {
         if (!csv.is_open()) return set_error(FVK_ERR_INVALID, "cannot open residual file '%s'", cfg->residual_file);
         csv << std::setprecision(cfg->output_precision);
         switch (eq) {
-            case FVK_EQ_DIFFUSION: csv << "step,residual\n"; break;
+            case FVK_EQ_DIFFUSION:
+            case FVK_EQ_ADVECTION_DIFFUSION: csv << "step,residual\n"; break;
             case FVK_EQ_RANS_SA: csv << "step,residual_rho,residual_rho_u,residual_rho_v,residual_E,residual_nut\n"; break;
             case FVK_EQ_RANS_SST:
                 csv << "step,residual_rho,residual_rho_u,residual_rho_v,residual_E,residual_k,residual_omega\n";

@@ -854,6 +854,56 @@ __global__ __launch_bounds__(kVB) void k_diff_cell(ViscArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// advection-diffusion (AdvectionDiffusionFVMSolver::step,
+// src/AdvectionDiffusionFVMSolver.cpp:42-104): boundary values, the cell
+// gradients of phi (k_scalar_grad), the face fluxes, then the diffusion
+// solver's cell pass (k_diff_cell)
+// ---------------------------------------------------------------------------
+
+// build_boundary_phi (src/AdvectionDiffusionFVMSolver.cpp:33-40): the
+// Dirichlet value, else the cell's own phi; a.aux holds [ldf] face values
+__global__ __launch_bounds__(kVB) void k_advdiff_bphi(ViscArgs a) {
+    const MeshView& m = a.m;
+    const int f = m.nfi + xcd_index();
+    if (f >= m.nf) return;
+    const DevBC& bc = m.bc[-1 - m.fr[f]];
+    a.aux[f] = bc.type == 0 ? bc.value : a.U[m.fl[f]];
+}
+
+// the face pass (:50-72) and boundary_flux (:117-138): upwind advection
+// with the uniform velocity plus Jasak-corrected diffusion; a.G holds the
+// cell gradients [2][nc] (x then y, per cell pairs)
+__global__ __launch_bounds__(kVB) void k_advdiff_face(ViscArgs a, double u_adv, double v_adv) {
+    const MeshView& m = a.m;
+    const int f = xcd_index();
+    if (f >= m.nf) return;
+    const int cl = m.fl[f], cr = m.fr[f];
+    const double phiL = a.U[cl];
+    const double A = m.fa[f];
+    const double Vn = u_adv * m.fnx[f] + v_adv * m.fny[f];
+    const double* g = a.G;
+    double F;
+    if (cr >= 0) {
+        const double phiR = a.U[cr];
+        const FG fg = face_grad(m, f, false, phiL, phiR, g[2 * cl], g[2 * cl + 1], g[2 * cr], g[2 * cr + 1]);
+        const double diffusive = -a.alpha * fg.dn * A;
+        const double up = (Vn >= 0.0) ? phiL : phiR;
+        F = Vn * up * A + diffusive;
+    } else {
+        const DevBC& bc = m.bc[-1 - cr];
+        if (bc.type == 1) {
+            F = Vn * phiL * A + bc.value * A;
+        } else {
+            const FG fg = face_grad(m, f, true, phiL, bc.value, g[2 * cl], g[2 * cl + 1], 0.0, 0.0);
+            const double diffusive = -a.alpha * fg.dn * A;
+            const double up = (Vn >= 0.0) ? phiL : bc.value;
+            F = Vn * up * A + diffusive;
+        }
+    }
+    a.flux[f] = F;
+}
+
+// ---------------------------------------------------------------------------
 // standalone gradient (fvk_gradient): one scalar field with caller-given
 // boundary values (device face order)
 // ---------------------------------------------------------------------------
@@ -1128,6 +1178,47 @@ int launch_diffusion_residual_eval(fvk_solver* s, double* R_dev) {
     k_diff_face<<<gf, kVB, 0, s->stream>>>(a);
     k_diff_cell<<<gc, kVB, 0, s->stream>>>(a);
     return hip_check(hipGetLastError(), "diffusion residual launch");
+}
+
+namespace {
+void advdiff_pass(fvk_solver* s, ViscArgs& a) {
+    const int gc = grid_for(s->nc, kVB);
+    const int gf = grid_for(s->nf, kVB);
+    const int gb = grid_for(std::max<int64_t>(s->nf - s->nfi, 1), kVB);
+    hipEvent_t ev = nullptr;
+    timer_begin(s, "advdiff_grad", &ev);
+    k_advdiff_bphi<<<gb, kVB, 0, s->stream>>>(a);
+    if (s->p.gradient_scheme == FVK_GRAD_LEAST_SQUARES)
+        k_scalar_grad<FVK_GRAD_LEAST_SQUARES><<<gc, kVB, 0, s->stream>>>(s->mesh, a.U, a.aux, a.G, s->periodic);
+    else
+        k_scalar_grad<FVK_GRAD_GREEN_GAUSS><<<gc, kVB, 0, s->stream>>>(s->mesh, a.U, a.aux, a.G, s->periodic);
+    timer_end(s, ev);
+    timer_begin(s, "advdiff_face", &ev);
+    k_advdiff_face<<<gf, kVB, 0, s->stream>>>(a, s->p.advection_u, s->p.advection_v);
+    timer_end(s, ev);
+    timer_begin(s, "advdiff_cell", &ev);
+    k_diff_cell<<<gc, kVB, 0, s->stream>>>(a);
+    timer_end(s, ev);
+}
+}  // namespace
+
+int launch_advdiff_step(fvk_solver* s, int want_res, int64_t step) {
+    ViscArgs a = make_vargs(s);
+    a.U = s->U[s->cur];
+    a.Unew = s->U[s->cur];
+    a.want_res = want_res;
+    a.hist_idx = step % s->hist_cap;
+    advdiff_pass(s, a);
+    if (want_res) launch_norm_finalize(s, grid_for(s->nc, kVB), 1, a.hist_idx, 0, s->p.fixed_dt);
+    return hip_check(hipGetLastError(), "advection-diffusion launch");
+}
+
+int launch_advdiff_residual_eval(fvk_solver* s, double* R_dev) {
+    ViscArgs a = make_vargs(s);
+    a.U = s->U[s->cur];
+    a.R = R_dev;
+    advdiff_pass(s, a);
+    return hip_check(hipGetLastError(), "advection-diffusion residual launch");
 }
 
 int launch_gradient(fvk_solver* s, int scheme, const double* phi, const double* bphi_dev, double* grad) {

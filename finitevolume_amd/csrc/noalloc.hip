@@ -45,7 +45,8 @@ extern "C" int fvk_wall_force_reports_noalloc(fvk_solver* s, const int32_t* wall
     using namespace fvk;
     if (!s || !ref || !reports || !n_reports || (n_wall > 0 && !wall_faces))
         return set_error(FVK_ERR_INVALID, "NULL argument");
-    if (s->p.equation == FVK_EQ_DIFFUSION) return set_error(FVK_ERR_UNSUPPORTED, "no wall forces for diffusion");
+    if (s->p.equation == FVK_EQ_DIFFUSION || s->p.equation == FVK_EQ_ADVECTION_DIFFUSION)
+        return set_error(FVK_ERR_UNSUPPORTED, "no wall forces for a scalar field");
     if (n_wall < 0 || n_wall > kNoallocMaxFaces)
         return set_error(FVK_ERR_INVALID, "n_wall %lld outside 0..%d, the scratch capacity of the no-alloc entry point",
                          (long long)n_wall, kNoallocMaxFaces);

@@ -251,6 +251,8 @@ void* dev_alloc(fvk_solver* s, size_t bytes);
 int launch_euler_stage(fvk_solver* s, int stage, int stages, int dt_mode_first, int want_res, int64_t step);
 int launch_visc_stage(fvk_solver* s, int stage, int stages, int want_res, int64_t step);
 int launch_diffusion_step(fvk_solver* s, int want_res, int64_t step);
+int launch_advdiff_step(fvk_solver* s, int want_res, int64_t step);
+int launch_advdiff_residual_eval(fvk_solver* s, double* R_dev);
 int launch_gradient(fvk_solver* s, int scheme, const double* phi, const double* bphi_dev, double* grad);
 int launch_wall_distance(fvk_solver* s, const int* wall_faces, int nw, const double* node_xy_dev,
                          const int* fnodes_dev, double* out);

@@ -140,7 +140,7 @@ class FVKSolver:
         return r
 
     def field(self):
-        if self.eq == abi.EQ_DIFFUSION:
+        if self.eq in (abi.EQ_DIFFUSION, abi.EQ_ADVECTION_DIFFUSION):
             U = np.empty(self.n)
             _check(self.L.fvk_get_field(self.h, abi.as_dptr(U), abi.as_dptr(None), self.n))
             return U
@@ -348,6 +348,23 @@ class UnstructuredFVMSolver(FVKSolver):
 
     def __init__(self, mesh, patch_bcs, alpha, dt, initial_value, initial_radius):
         p = abi.default_params(equation=abi.EQ_DIFFUSION, alpha=alpha, fixed_dt=dt)
+        ic = abi.initial_condition(abi.IC_FREESTREAM, initial_value=initial_value, initial_radius=initial_radius)
+        super().__init__(mesh, patch_bcs, p, ic)
+
+    def residual_norm(self) -> float:
+        return float(self.residual()[0])
+
+
+class AdvectionDiffusionFVMSolver(FVKSolver):
+    """AdvectionDiffusionFVMSolver(mesh, alpha, u_adv, v_adv, dt,
+    gradient_scheme, initial_value, initial_radius),
+    src/AdvectionDiffusionFVMSolver.cpp:16-31; the patches' BoundaryType /
+    value come as fvk_bc entries (type 0 Dirichlet, 1 Neumann)."""
+
+    def __init__(self, mesh, patch_bcs, alpha, u_adv, v_adv, dt, gradient_scheme=abi.GRAD_LEAST_SQUARES,
+                 initial_value=0.0, initial_radius=0.0):
+        p = abi.default_params(equation=abi.EQ_ADVECTION_DIFFUSION, alpha=alpha, fixed_dt=dt, advection_u=u_adv,
+                               advection_v=v_adv, gradient_scheme=gradient_scheme)
         ic = abi.initial_condition(abi.IC_FREESTREAM, initial_value=initial_value, initial_radius=initial_radius)
         super().__init__(mesh, patch_bcs, p, ic)
 

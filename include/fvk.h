@@ -10,6 +10,7 @@
  *                        RANSTurbulenceSASolver::...           include/RANSTurbulenceSASolver.h:107-111
  *                        RANSTurbulenceSSTSolver::...          src/RANSTurbulenceSSTSolver.cpp:40-46
  *                        UnstructuredFVMSolver::...            include/UnstructuredFVMSolver.h:34-35
+ *                        AdvectionDiffusionFVMSolver::...      src/AdvectionDiffusionFVMSolver.cpp:16-31
  *   fvk_step          <- *::step() / *::run(int)               src/EulerFVMSolver.cpp:58-121
  *   fvk_residual      <- *::residual(), nut_residual(),
  *                        k_residual(), omega_residual(),
@@ -61,7 +62,8 @@ enum fvk_equation {
     FVK_EQ_NAVIER_STOKES = 1,  /* NavierStokesFVMSolver */
     FVK_EQ_RANS_SA = 2,        /* RANSTurbulenceSASolver */
     FVK_EQ_RANS_SST = 3,       /* RANSTurbulenceSSTSolver */
-    FVK_EQ_DIFFUSION = 4       /* UnstructuredFVMSolver (scalar diffusion) */
+    FVK_EQ_DIFFUSION = 4,      /* UnstructuredFVMSolver (scalar diffusion) */
+    FVK_EQ_ADVECTION_DIFFUSION = 5 /* AdvectionDiffusionFVMSolver (scalar, uniform velocity) */
 };
 
 /* NumericalFluxScheme, include/EulerFVMSolver.h:40-44 (same order). */
@@ -164,6 +166,10 @@ typedef struct fvk_params {
                                       for variant 3 any multiple of 64 up to 512 (0: 512; variant 3: 256) */
     int32_t tile_cells;            /* variant 3: most cells of one tile, at most block_size (0: block_size) */
     int32_t reserved1;
+    /* FVK_EQ_ADVECTION_DIFFUSION: the uniform advection velocity (u_adv,
+     * v_adv of src/AdvectionDiffusionFVMSolver.cpp:16-20); alpha, fixed_dt
+     * and gradient_scheme as above */
+    double advection_u, advection_v;
 } fvk_params;
 
 /* Initial condition (EulerInitialCondition, include/EulerFVMSolver.h:59-67,
@@ -189,7 +195,7 @@ void fvk_params_default(fvk_params* p);
 void fvk_initial_condition_default(fvk_initial_condition* ic);
 
 /* Number of conserved variables per cell for an equation: 4 (Euler, NS),
- * 5 (SA: + nut), 6 (SST: + k, omega), 1 (diffusion). */
+ * 5 (SA: + nut), 6 (SST: + k, omega), 1 (diffusion, advection-diffusion). */
 int fvk_num_vars(int32_t equation);
 
 int fvk_create(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, const fvk_params* params,

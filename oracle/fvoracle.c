@@ -57,8 +57,11 @@ struct fvo_solver {
 static inline double smin(double a, double b) { return (b < a) ? b : a; }
 static inline double smax(double a, double b) { return (a < b) ? b : a; }
 
+/* the two scalar-field solvers (UnstructuredFVMSolver, AdvectionDiffusionFVMSolver) */
+static int scalar_of(int eq) { return eq == FVK_EQ_DIFFUSION || eq == FVK_EQ_ADVECTION_DIFFUSION; }
+
 static int nvars_of(int eq) {
-    return eq == FVK_EQ_RANS_SA ? 5 : (eq == FVK_EQ_RANS_SST ? 6 : (eq == FVK_EQ_DIFFUSION ? 1 : 4));
+    return eq == FVK_EQ_RANS_SA ? 5 : (eq == FVK_EQ_RANS_SST ? 6 : (scalar_of(eq) ? 1 : 4));
 }
 
 /* ------------------------------------------------------------------------
@@ -852,6 +855,45 @@ static void residual(fvo_solver* s, const double* U, const double* turb, double*
                 }
             }
         }
+    } else if (p->equation == FVK_EQ_ADVECTION_DIFFUSION) {
+        /* AdvectionDiffusionFVMSolver::step face pass + boundary_flux,
+         * src/AdvectionDiffusionFVMSolver.cpp:42-72, 117-138, with the
+         * boundary values of build_boundary_phi (:33-40) */
+        double* bphi = calloc(nf, sizeof(double));
+        double* gr = malloc(sizeof(double) * 2 * nc);
+        for (int64_t f = 0; f < nf; ++f) {
+            if (s->fcell[2 * f + 1] != -1) continue;
+            const fvk_bc* bc = &s->bcs[s->fpatch[f]];
+            bphi[f] = bc->type == FVK_BC_DIRICHLET ? bc->value : U[s->fcell[2 * f]];
+        }
+        gradient(s, p->gradient_scheme, U, 1, bphi, gr);
+        const double zero2[2] = {0.0, 0.0};
+#pragma omp parallel for schedule(static)
+        for (int64_t f = 0; f < nf; ++f) {
+            int cl = s->fcell[2 * f], cr = s->fcell[2 * f + 1];
+            double phiL = U[cl];
+            double Vn = p->advection_u * s->fnx[f] + p->advection_v * s->fny[f];
+            double A = s->farea[f];
+            if (cr != -1) {
+                double phiR = U[cr];
+                facegrad fg = face_gradient(s, f, phiL, phiR, gr + 2 * cl, gr + 2 * cr, 0.0);
+                double diffusive = -p->alpha * fg.dphidn * A;
+                double up = (Vn >= 0.0) ? phiL : phiR;
+                ff[f] = Vn * up * A + diffusive;
+            } else {
+                const fvk_bc* bc = &s->bcs[s->fpatch[f]];
+                if (bc->type == FVK_BC_NEUMANN) {
+                    ff[f] = Vn * phiL * A + bc->value * A;
+                } else {
+                    facegrad fg = face_gradient(s, f, phiL, 0.0, gr + 2 * cl, zero2, bc->value);
+                    double diffusive = -p->alpha * fg.dphidn * A;
+                    double up = (Vn >= 0.0) ? phiL : bc->value;
+                    ff[f] = Vn * up * A + diffusive;
+                }
+            }
+        }
+        free(bphi);
+        free(gr);
     } else if (p->equation == FVK_EQ_EULER) {
         if (p->order == 2) limited_gradients(s, U);
 #pragma omp parallel for schedule(static)
@@ -1112,7 +1154,7 @@ static int one_step(fvo_solver* s, double* res_out, double* dt_out) {
     const int nt = nv > 4 ? nv - 4 : 0;
     const int64_t nc = s->nc, nf = s->nf;
     int stages = p->rk_stages < 1 ? 1 : (p->rk_stages > 3 ? 3 : p->rk_stages);
-    int diff = p->equation == FVK_EQ_DIFFUSION;
+    int diff = scalar_of(p->equation);
     int sstm = p->equation == FVK_EQ_RANS_SST;
     int su = diff ? 1 : 4; /* doubles per cell in U */
 
@@ -1219,7 +1261,7 @@ int fvo_create(const fvk_mesh_desc* d, const fvk_bc* bcs, int32_t n_bcs, const f
     s->grad8 = calloc(8 * s->nc, sizeof(double));
     s->lim4 = calloc(4 * s->nc, sizeof(double));
     int nv = nvars_of(params->equation);
-    int su = params->equation == FVK_EQ_DIFFUSION ? 1 : 4;
+    int su = scalar_of(params->equation) ? 1 : 4;
     s->U = calloc(su * s->nc, sizeof(double));
     if (nv > 4) s->turb = calloc((nv - 4) * s->nc, sizeof(double));
     /* wall distance to no-slip walls (RANS constructors) */
@@ -1246,6 +1288,11 @@ int fvo_create(const fvk_mesh_desc* d, const fvk_bc* bcs, int32_t n_bcs, const f
                 s->U[c] = (sqrt(x * x + y * y) < ic->initial_radius) ? ic->initial_value : 0.0;
                 continue;
             }
+            if (params->equation == FVK_EQ_ADVECTION_DIFFUSION) {
+                /* src/AdvectionDiffusionFVMSolver.cpp:22-31 (hypot) */
+                s->U[c] = (hypot(s->cxy[2 * c], s->cxy[2 * c + 1]) < ic->initial_radius) ? ic->initial_value : 0.0;
+                continue;
+            }
             st4 v;
             if (ic->mode == FVK_IC_FREESTREAM) v = from_primitive(ic->rho, ic->u, ic->v, ic->p, g);
             else if (s->cxy[2 * c] < ic->x0) v = from_primitive(ic->rho_l, ic->u_l, ic->v_l, ic->p_l, g);
@@ -1254,7 +1301,7 @@ int fvo_create(const fvk_mesh_desc* d, const fvk_bc* bcs, int32_t n_bcs, const f
             if (nv == 5) s->turb[c] = ic->initial_nut;
             if (nv == 6) { s->turb[2 * c] = ic->initial_k; s->turb[2 * c + 1] = ic->initial_omega; }
         }
-    } else if (ic && params->equation != FVK_EQ_DIFFUSION) {
+    } else if (ic && !scalar_of(params->equation)) {
         for (int64_t c = 0; c < s->nc; ++c) {
             if (nv == 5) s->turb[c] = ic->initial_nut;
             if (nv == 6) { s->turb[2 * c] = ic->initial_k; s->turb[2 * c + 1] = ic->initial_omega; }
@@ -1290,7 +1337,7 @@ int fvo_residual(const fvo_solver* s, double* norms) {
 
 int fvo_set_field(fvo_solver* s, const double* U4, const double* turb) {
     int nv = nvars_of(s->p.equation);
-    int su = s->p.equation == FVK_EQ_DIFFUSION ? 1 : 4;
+    int su = scalar_of(s->p.equation) ? 1 : 4;
     memcpy(s->U, U4, sizeof(double) * su * s->nc);
     if (turb && nv > 4) memcpy(s->turb, turb, sizeof(double) * (nv - 4) * s->nc);
     memset(s->res, 0, sizeof s->res);
@@ -1299,7 +1346,7 @@ int fvo_set_field(fvo_solver* s, const double* U4, const double* turb) {
 
 int fvo_get_field(const fvo_solver* s, double* U4, double* turb) {
     int nv = nvars_of(s->p.equation);
-    int su = s->p.equation == FVK_EQ_DIFFUSION ? 1 : 4;
+    int su = scalar_of(s->p.equation) ? 1 : 4;
     memcpy(U4, s->U, sizeof(double) * su * s->nc);
     if (turb && nv > 4) memcpy(turb, s->turb, sizeof(double) * (nv - 4) * s->nc);
     return FVK_OK;
@@ -1363,7 +1410,7 @@ int fvo_wall_reports(fvo_solver* s, const int32_t* wf, int64_t n, const fvk_wall
     const int64_t nc = s->nc, nf = s->nf;
     const double g = p->gamma;
     const int eq = p->equation;
-    if (eq == FVK_EQ_DIFFUSION) return FVK_ERR_UNSUPPORTED;
+    if (scalar_of(eq)) return FVK_ERR_UNSUPPORTED;
     for (int64_t i = 0; i < n; ++i)
         if (wf[i] < 0 || wf[i] >= nf || s->fcell[2 * wf[i] + 1] != -1) return FVK_ERR_INVALID;
     const int visc = eq != FVK_EQ_EULER && !pressure_only;

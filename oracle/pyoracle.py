@@ -92,7 +92,7 @@ class OracleSolver:
         self.L.fvo_set_field(self.h, abi.as_dptr(U), abi.as_dptr(t))
 
     def field(self):
-        if self.eq == abi.EQ_DIFFUSION:
+        if self.eq in (abi.EQ_DIFFUSION, abi.EQ_ADVECTION_DIFFUSION):
             U = np.empty(self.n)
             self.L.fvo_get_field(self.h, abi.as_dptr(U), abi.as_dptr(None))
             return U

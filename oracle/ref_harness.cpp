@@ -16,6 +16,7 @@
 
 #include "fvk.h"
 
+#include "AdvectionDiffusionFVMSolver.h"
 #include "CflRamp.h"
 #include "Checkpoint.h"
 #include "EulerFVMSolver.h"
@@ -122,9 +123,12 @@ struct RefSolver {
     std::unique_ptr<RANSTurbulenceSASolver> sa;
     std::unique_ptr<RANSTurbulenceSSTSolver> sst;
     std::unique_ptr<UnstructuredFVMSolver> diff;
+    std::unique_ptr<AdvectionDiffusionFVMSolver> adv;
 };
 
-int nvars(int eq) { return eq == FVK_EQ_RANS_SA ? 5 : (eq == FVK_EQ_RANS_SST ? 6 : (eq == FVK_EQ_DIFFUSION ? 1 : 4)); }
+int nvars(int eq) {
+    return eq == FVK_EQ_RANS_SA ? 5 : (eq == FVK_EQ_RANS_SST ? 6 : (eq >= FVK_EQ_DIFFUSION ? 1 : 4));
+}
 
 }  // namespace
 
@@ -191,6 +195,11 @@ int fvr_create(const fvk_mesh_desc* d, const fvk_bc* bcs, int32_t n_bcs, const f
             r->diff.reset(new UnstructuredFVMSolver(mesh, p->alpha, p->fixed_dt, ic ? ic->initial_value : 0.0,
                                                     ic ? ic->initial_radius : 0.0));
             break;
+        case FVK_EQ_ADVECTION_DIFFUSION:
+            r->adv.reset(new AdvectionDiffusionFVMSolver(mesh, p->alpha, p->advection_u, p->advection_v, p->fixed_dt,
+                                                         grad_of(p->gradient_scheme), ic ? ic->initial_value : 0.0,
+                                                         ic ? ic->initial_radius : 0.0));
+            break;
         default:
             delete r;
             return FVK_ERR_INVALID;
@@ -205,6 +214,10 @@ int fvr_set_field(void* h, const double* U4, const double* turb) {
     auto* r = static_cast<RefSolver*>(h);
     if (r->eq == FVK_EQ_DIFFUSION) {
         r->diff->set_field(std::vector<double>(U4, U4 + r->n));
+        return FVK_OK;
+    }
+    if (r->eq == FVK_EQ_ADVECTION_DIFFUSION) {
+        r->adv->set_field(std::vector<double>(U4, U4 + r->n));
         return FVK_OK;
     }
     std::vector<EulerState> U((size_t)r->n);
@@ -231,8 +244,9 @@ int fvr_set_field(void* h, const double* U4, const double* turb) {
 
 int fvr_get_field(void* h, double* U4, double* turb) {
     auto* r = static_cast<RefSolver*>(h);
-    if (r->eq == FVK_EQ_DIFFUSION) {
-        std::memcpy(U4, r->diff->field().data(), sizeof(double) * r->n);
+    if (r->eq == FVK_EQ_DIFFUSION || r->eq == FVK_EQ_ADVECTION_DIFFUSION) {
+        const std::vector<double>& phi = r->eq == FVK_EQ_DIFFUSION ? r->diff->field() : r->adv->field();
+        std::memcpy(U4, phi.data(), sizeof(double) * r->n);
         return FVK_OK;
     }
     const std::vector<EulerState>* U = nullptr;
@@ -259,8 +273,8 @@ int fvr_get_field(void* h, double* U4, double* turb) {
 }
 
 static void residual_of(RefSolver* r, double* out) {
-    if (r->eq == FVK_EQ_DIFFUSION) {
-        out[0] = r->diff->residual_norm();
+    if (r->eq == FVK_EQ_DIFFUSION || r->eq == FVK_EQ_ADVECTION_DIFFUSION) {
+        out[0] = r->eq == FVK_EQ_DIFFUSION ? r->diff->residual_norm() : r->adv->residual_norm();
         return;
     }
     const EulerResidualNorms* n = nullptr;
@@ -287,6 +301,7 @@ int fvr_step(void* h, int32_t nsteps, double* res_hist) {
             case FVK_EQ_RANS_SA: r->sa->step(); break;
             case FVK_EQ_RANS_SST: r->sst->step(); break;
             case FVK_EQ_DIFFUSION: r->diff->step(); break;
+            case FVK_EQ_ADVECTION_DIFFUSION: r->adv->step(); break;
         }
         if (res_hist) residual_of(r, res_hist + (int64_t)s * nv);
     }
@@ -382,6 +397,9 @@ int fvr_checkpoint_write(void* h, const char* path, int64_t step, uint64_t build
     switch (r->eq) {
         case FVK_EQ_DIFFUSION:
             ok = Checkpoint::write(path, CheckpointEquation::Diffusion, step, build, r->diff->field());
+            break;
+        case FVK_EQ_ADVECTION_DIFFUSION:
+            ok = Checkpoint::write(path, CheckpointEquation::AdvectionDiffusion, step, build, r->adv->field());
             break;
         case FVK_EQ_EULER: ok = Checkpoint::write(path, CheckpointEquation::Euler, step, build, r->euler->field()); break;
         case FVK_EQ_NAVIER_STOKES:

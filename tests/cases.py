@@ -105,6 +105,19 @@ def case_diffusion(n=16):
     return m, bcs, p, ic, None
 
 
+def case_advdiff(gradient=abi.GRAD_LEAST_SQUARES, n=16, u=0.8, v=-0.3):
+    """AdvectionDiffusionFVMSolver on the wavy mesh: Dirichlet bottom/top,
+    Neumann sides, a blob of radius 0.5 around the origin, a uniform
+    velocity that leaves through two sides and enters through the others."""
+    m = M.skewed(n, 0.15)
+    bcs = [abi.make_bc(abi.BC_DIRICHLET, value=1.0), abi.make_bc(abi.BC_NEUMANN, value=0.3),
+           abi.make_bc(abi.BC_DIRICHLET, value=0.0), abi.make_bc(abi.BC_NEUMANN, value=0.0)]
+    p = abi.default_params(equation=abi.EQ_ADVECTION_DIFFUSION, alpha=0.05, fixed_dt=2e-3, advection_u=u,
+                           advection_v=v, gradient_scheme=gradient)
+    ic = abi.initial_condition(abi.IC_FREESTREAM, initial_value=2.0, initial_radius=0.5)
+    return m, bcs, p, ic, None
+
+
 def rel_state_diff(a, b, floor=1e-300):
     """Per-cell norm-wise relative difference max_k |a_k - b_k| / max_k |b_k|,
     maximised over cells (a component that is zero by symmetry is judged

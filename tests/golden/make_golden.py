@@ -136,6 +136,7 @@ def checkpoint_cases():
         ("ckpt_sa_wavy.bin", cases.case_rans_wavy(abi.EQ_RANS_SA), 20, 1020, 8),
         ("ckpt_sst_wavy.bin", cases.case_rans_wavy(abi.EQ_RANS_SST, variant=1, kato=1), 20, 20, 9),
         ("ckpt_diffusion.bin", cases.case_diffusion(), 30, 30, 10),
+        ("ckpt_advdiff.bin", cases.case_advdiff(), 30, 30, 11),
     ]
 
 
@@ -182,12 +183,22 @@ def write_mesh_fixtures(w):
     w("mesh_reader.npz", **out)
 
 
+def write_advdiff_fixtures(w):
+    """AdvectionDiffusionFVMSolver trajectories (LSQ and Green-Gauss)."""
+    for gs, name in ((abi.GRAD_LEAST_SQUARES, "ls"), (abi.GRAD_GREEN_GAUSS, "gg")):
+        w(f"advdiff_wavy_{name}.npz", **run_case(cases.case_advdiff(gradient=gs), 300))
+
+
 def main(only=None):
     assert pyref.available("strict"), "oracle/_ref/libfvref_strict.so missing: make -C oracle ref"
     w = lambda name, **arrs: np.savez_compressed(os.path.join(HERE, name), **arrs)
     if only == "wall":
         write_wall_fixture(w)
         print("wall fixture written to", HERE)
+        return
+    if only == "advdiff":
+        write_advdiff_fixtures(w)
+        print("advection-diffusion fixtures written to", HERE)
         return
     if only == "mesh":
         write_mesh_fixtures(w)
@@ -220,6 +231,7 @@ def main(only=None):
     w("sa_couette.npz", **run_case(cases.case_couette(abi.EQ_RANS_SA), 2000))
     w("sst_couette.npz", **run_case(cases.case_couette(abi.EQ_RANS_SST), 2000))
     w("diffusion_wavy.npz", **run_case(cases.case_diffusion(), 300))
+    write_advdiff_fixtures(w)
 
     # ---- gradients on the wavy 8x8 mesh of --verify-gradient (src/main.cpp:1443-1527) ----
     m = M.skewed(8, 0.15)
@@ -288,5 +300,5 @@ def main(only=None):
 
 
 if __name__ == "__main__":
-    # `make_golden.py wall` (or `checkpoint`, `mesh`) rewrites only those fixtures
+    # `make_golden.py wall` (or `checkpoint`, `mesh`, `advdiff`) rewrites only those fixtures
     main(sys.argv[1] if len(sys.argv) > 1 else None)

@@ -45,7 +45,7 @@ def test_next_cfl_branches(solver_mod):
 
 @pytest.mark.parametrize("name,tag,nt,ncell", [("ckpt_euler_naca.bin", 1, 4, 1024), ("ckpt_ns_wavy.bin", 3, 4, 256),
                                                ("ckpt_sa_wavy.bin", 4, 5, 256), ("ckpt_sst_wavy.bin", 5, 6, 256),
-                                               ("ckpt_diffusion.bin", 0, 1, 256)])
+                                               ("ckpt_diffusion.bin", 0, 1, 256), ("ckpt_advdiff.bin", 2, 1, 256)])
 def test_checkpoint_fixture_layout(name, tag, nt, ncell):
     h, payload = read_checkpoint(os.path.join(GOLD, name))
     assert h["magic"] == b"FVMCKPT" and h["version"] == 2

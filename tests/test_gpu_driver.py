@@ -40,7 +40,7 @@ def _fresh(name):
 def _expected_field(name, m, eq):
     h, pay = read_checkpoint(os.path.join(GOLD, name))
     n = m.n_cells
-    if eq == abi.EQ_DIFFUSION:
+    if eq in (abi.EQ_DIFFUSION, abi.EQ_ADVECTION_DIFFUSION):
         return pay[:n], None
     U = pay[: 4 * n].reshape(n, 4)
     nt = {abi.EQ_RANS_SA: 1, abi.EQ_RANS_SST: 2}.get(eq, 0)
@@ -78,7 +78,8 @@ def test_resume_reference_checkpoint_and_write_it_back(name, tmp_path):
 
 @pytest.mark.skipif(not pyref.available(), reason="reference build (oracle/_ref) not present")
 def test_reference_reads_gpu_checkpoint(tmp_path):
-    for name in ("ckpt_euler_naca.bin", "ckpt_sa_wavy.bin", "ckpt_sst_wavy.bin", "ckpt_diffusion.bin"):
+    for name in ("ckpt_euler_naca.bin", "ckpt_sa_wavy.bin", "ckpt_sst_wavy.bin", "ckpt_diffusion.bin",
+                 "ckpt_advdiff.bin"):
         m, bcs, p, ic, U0, _, _ = _fresh(name)
         g = FVKSolver(m, bcs, p, ic)
         if U0 is not None:

@@ -71,6 +71,8 @@ TRAJ = [
     ("sa_couette.npz", lambda: cases.case_couette(abi.EQ_RANS_SA), 2000, ()),
     ("sst_couette.npz", lambda: cases.case_couette(abi.EQ_RANS_SST), 2000, ()),
     ("diffusion_wavy.npz", lambda: cases.case_diffusion(), 300, ()),
+    ("advdiff_wavy_ls.npz", lambda: cases.case_advdiff(gradient=abi.GRAD_LEAST_SQUARES), 300, ()),
+    ("advdiff_wavy_gg.npz", lambda: cases.case_advdiff(gradient=abi.GRAD_GREEN_GAUSS), 300, ()),
 ]
 
 
