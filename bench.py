@@ -282,11 +282,18 @@ def batch_leg(n_cases, max_steps, drop=1e-6):
     del b
     print(f"[bench] batch: all cases done in {rb['wall']:.1f}s ({rb['total_steps']} steps)", file=sys.stderr,
           flush=True)
+    # the sequential runs take the single-case launch shape the autotuner
+    # picks on the median-size case (the fused tile kernel where it wins)
+    med = int(np.argsort(sizes)[n_cases // 2])
+    seq_params = abi.ParamsC.from_buffer_copy(params)
+    seq_tuned = autotune.tune(meshes[med], bcs[med], seq_params, ics[med], use_cache=False)
+    print(f"[bench] batch: sequential shape {seq_tuned['variant']}/{seq_tuned['block']}/{seq_tuned['tile_cells']}",
+          file=sys.stderr, flush=True)
     seq_steps = np.full(n_cases, -1)
     seq_time = np.zeros(n_cases)
     t_seq0 = time.perf_counter()
     for k in range(n_cases):
-        g = FVKSolver(meshes[k], bcs[k], params, ics[k])
+        g = FVKSolver(meshes[k], bcs[k], seq_params, ics[k])
         t1 = time.perf_counter()
         first, done = None, 0
         while done < max_steps:
@@ -330,6 +337,7 @@ def batch_leg(n_cases, max_steps, drop=1e-6):
         "mesh_generation_s": round(t_gen, 1),
         "hip_graph": True,
         "autotune": tuned,
+        "sequential_launch_shape": {k: seq_tuned[k] for k in ("variant", "block", "tile_cells")},
     }
 
 

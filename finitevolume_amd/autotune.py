@@ -28,7 +28,8 @@ BLOCKS = (64, 128, 256, 512)
 # 1.2 times that and its faces about 1.6 times, so these pairs cover one
 # pass per phase (staging, gradients, faces) up to two passes.
 TILE_SHAPES = ((128, 112), (192, 160), (256, 192), (256, 200), (256, 208), (256, 216), (256, 224), (256, 240), (256, 256),
-               (320, 224), (320, 256), (384, 256), (384, 320))
+               (320, 224), (320, 256), (384, 256), (384, 320), (512, 400), (512, 432), (512, 448), (512, 464),
+               (512, 480))
 
 
 def _cache_path() -> str:
@@ -115,11 +116,16 @@ def tune(mesh, bcs, params: abi.ParamsC, ic, steps: int = 20, warmup: int = 3,
     return best
 
 
+BATCH_TILE_SHAPES = ((128, 112), (256, 192), (256, 216), (256, 240), (512, 448))
+
+
 def tune_batch(meshes: Sequence, bcs: Sequence, params: abi.ParamsC, ics: Sequence, steps: int = 10,
                warmup: int = 2, blocks: Iterable[int] = (128, 256, 512), use_cache: bool = True) -> Dict:
-    """Fastest workgroup size for a batch (the batch runs the face sweep +
-    gather variant); sets it on `params` too.  Workgroups larger than the
-    smallest case are skipped."""
+    """Fastest launch shape for a batch: the persistent fused tile kernel
+    (variant 3, SSP-RK stages only) at several tile shapes, and the face
+    sweep + gather (variant 1) at several workgroup sizes; sets it on
+    `params` too.  Variant-1 workgroups larger than the smallest case are
+    skipped."""
     from .batch import FVKBatch
     n = sum(m.n_cells for m in meshes)
     nf = sum(m.n_faces for m in meshes)
@@ -130,18 +136,31 @@ def tune_batch(meshes: Sequence, bcs: Sequence, params: abi.ParamsC, ics: Sequen
         best = db[key]
     else:
         smallest = min(m.n_cells for m in meshes)
-        table = {}
+        table, shape = {}, {}
         for b in blocks:
             if b > smallest:
                 continue
             p = abi.ParamsC.from_buffer_copy(params)
-            p.kernel_variant, p.block_size = 1, b
-            table[f"b{b}"] = _time_solver(lambda: FVKBatch(meshes, bcs, p, ics), steps, warmup) * 1e3
+            p.kernel_variant, p.block_size, p.tile_cells = 1, b, 0
+            table[f"v1_b{b}"] = _time_solver(lambda: FVKBatch(meshes, bcs, p, ics), steps, warmup) * 1e3
+            shape[f"v1_b{b}"] = (1, b, 0)
+        if params.rk_stages >= 2:
+            for b, tc in BATCH_TILE_SHAPES:
+                p = abi.ParamsC.from_buffer_copy(params)
+                p.kernel_variant, p.block_size, p.tile_cells = 3, b, tc
+                try:
+                    table[f"v3_b{b}_t{tc}"] = _time_solver(lambda: FVKBatch(meshes, bcs, p, ics), steps,
+                                                           warmup) * 1e3
+                except Exception:
+                    continue  # not built for these meshes (LDS)
+                shape[f"v3_b{b}_t{tc}"] = (3, b, tc)
         name = min(table, key=table.get)
-        best = {"variant": 1, "block": int(name[1:]), "ms_per_step": table[name],
+        v, b, tc = shape[name]
+        best = {"variant": v, "block": b, "tile_cells": tc, "ms_per_step": table[name],
                 "table_ms_per_step": {k: round(x, 4) for k, x in table.items()}}
         if use_cache:
             db[key] = best
             _store(db)
     params.kernel_variant, params.block_size = best["variant"], best["block"]
+    params.tile_cells = best.get("tile_cells", 0)
     return best

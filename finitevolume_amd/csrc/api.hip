@@ -237,9 +237,10 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
 
     if (params->kernel_variant < 0 || params->kernel_variant > 3)
         return set_error(FVK_ERR_INVALID, "kernel_variant must be 0, 1, 2 or 3 (got %d)", params->kernel_variant);
-    if (params->kernel_variant == 3 && (scalar_eq(params->equation) || cell_case))
+    if (params->kernel_variant == 3 && (scalar_eq(params->equation) || (cell_case && params->equation != FVK_EQ_EULER)))
         return set_error(FVK_ERR_UNSUPPORTED,
-                         "the fused tile kernels (variant 3) are built for single Euler, Navier-Stokes and RANS cases");
+                         "the fused tile kernels (variant 3) are built for Euler, Navier-Stokes and RANS cases "
+                         "and for batched Euler cases");
     {
         const int b = params->block_size;
         const bool pow2 = b == 64 || b == 128 || b == 256 || b == 512;

@@ -1,15 +1,18 @@
 // Batched independent cases: one solver over the union of the cases' meshes.
 //
 // The cases are disconnected components of one mesh, laid out so that each
-// case is a contiguous range of the device cell order (RCM within it).  A
-// stage is then one launch per kernel for all cases: small cases that
-// would leave most of the 256 CUs idle on their own share the launches, and
-// the per-launch cost is paid once per stage instead of once per case.  The
-// residual norms are reduced per case (block sums split at case
-// boundaries, then one block per case).  A converged case is frozen: its
-// threads skip the work and keep the state.  With the graph on, one step
-// (two for forward Euler, whose buffers alternate) is captured in a
-// hipGraph and replayed.
+// case is a contiguous range of the device cell order (RCM within it), its
+// cell tiles a contiguous range of the tiles.  A stage is one launch of the
+// fused tile kernel for all cases over the list of the active cases' tiles,
+// so small cases that would leave most of the 256 CUs idle on their own
+// share the launch, and the per-launch cost is paid once per stage instead
+// of once per case.
+// The residual norms are reduced per case from per-tile sums in a fixed
+// order.  A converged case is frozen: its tiles leave the list and its state
+// stays.  With the graph on, one step is captured in a hipGraph and
+// replayed; the list lives in device memory, so freezing needs no new
+// capture.  Variant 1 (face sweep + gather, block sums split at case
+// boundaries) remains for forward Euler.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -72,6 +75,21 @@ int build_graph(fvk_batch* b) {
     return rc;
 }
 
+// the tiles of the active cases, in tile order, and their count: what the
+// tile launch works through (a captured graph reads both from
+// device memory, so freezing cases needs no new capture)
+int upload_tile_list(fvk_solver* s, const int32_t* active) {
+    std::vector<int> list;
+    for (int k = 0; k < s->n_cases; ++k)
+        if (active[k])
+            for (int t = s->case_tiles[k]; t < s->case_tiles[k + 1]; ++t) list.push_back(t);
+    const int n = (int)list.size();
+    int rc = FVK_OK;
+    if (n > 0) rc = hip_check(hipMemcpy(s->d_tile_list, list.data(), sizeof(int) * n, hipMemcpyHostToDevice), "tile list");
+    if (rc == FVK_OK) rc = hip_check(hipMemcpy(s->d_n_tile_list, &n, sizeof(int), hipMemcpyHostToDevice), "tile list");
+    return rc;
+}
+
 }  // namespace
 }  // namespace fvk
 
@@ -87,7 +105,13 @@ int fvk_batch_create(int32_t n_cases, const fvk_mesh_desc* meshes, const fvk_bc*
     if (!params->local_dt)
         return set_error(FVK_ERR_UNSUPPORTED, "batched cases step with local time steps (local_dt = 1)");
     if (params->kernel_variant == 2)
-        return set_error(FVK_ERR_UNSUPPORTED, "batched cases use the face sweep + gather kernels (variant 1)");
+        return set_error(FVK_ERR_UNSUPPORTED,
+                         "batched cases use the fused tile kernel (variant 3) or the face sweep + gather (variant 1)");
+    // the fused tile kernel by default; it leaves a frozen case's state in
+    // place, which needs SSP-RK stages (forward Euler alternates buffers)
+    const int variant = params->kernel_variant != 0 ? params->kernel_variant : (params->rk_stages >= 2 ? 3 : 1);
+    if (variant == 3 && params->rk_stages < 2)
+        return set_error(FVK_ERR_UNSUPPORTED, "batched cases on the tile kernel need rk_stages >= 2");
     // ---- union of the meshes ----
     int64_t nn = 0, nc = 0, nf = 0, nnz = 0, nb = 0;
     for (int k = 0; k < n_cases; ++k) {
@@ -152,7 +176,7 @@ int fvk_batch_create(int32_t n_cases, const fvk_mesh_desc* meshes, const fvk_bc*
     u.cell_xy = cxy.data();
     u.face_shift = nullptr;
     fvk_params p = *params;
-    p.kernel_variant = 1;
+    p.kernel_variant = variant;
     int rc = create_solver(&u, allbc.data(), (int32_t)nb, &p, &ics[0], ccase.data(), n_cases, &b->s);
     if (rc != FVK_OK) {
         delete b;
@@ -180,7 +204,7 @@ int fvk_batch_create(int32_t n_cases, const fvk_mesh_desc* meshes, const fvk_bc*
     const int bs = s->block;
     const int64_t nblocks_raw = (nc + bs - 1) / bs;
     const int64_t nblocks = (nblocks_raw + 7) / 8 * 8;  // the launch grid (a multiple of the 8 XCDs)
-    for (int k = 0; k < n_cases && rc == FVK_OK; ++k)
+    for (int k = 0; k < n_cases && rc == FVK_OK && variant == 1; ++k)
         if (s->case_cell_start[k + 1] - s->case_cell_start[k] < bs)
             rc = set_error(FVK_ERR_INVALID, "case %d has %d cells, fewer than the %d of a workgroup", k,
                            s->case_cell_start[k + 1] - s->case_cell_start[k], bs);
@@ -206,6 +230,33 @@ int fvk_batch_create(int32_t n_cases, const fvk_mesh_desc* meshes, const fvk_bc*
             !s->case_res || !s->d_step)
             rc = set_error(FVK_ERR_NOMEM, "batch: device allocation failed");
     }
+    if (rc == FVK_OK && variant == 3) {
+        // each tile lies in one case (the cases are disconnected and tiles
+        // grow over faces): the cases' tile ranges
+        const int nt = s->mesh.ntiles;
+        s->case_tiles.assign(n_cases + 1, nt);
+        s->case_tiles[0] = 0;
+        int t = 0;
+        for (int k = 0; k < n_cases && rc == FVK_OK; ++k) {
+            s->case_tiles[k] = t;
+            while (t < nt && s->tile_start[t + 1] <= s->case_cell_start[k + 1]) ++t;
+            if (t < nt && s->tile_start[t] < s->case_cell_start[k + 1])
+                rc = set_error(FVK_ERR_INVALID, "batch: tile %d straddles cases %d and %d", t, k, k + 1);
+        }
+        s->case_tiles[n_cases] = t;
+        if (rc == FVK_OK && t != nt) rc = set_error(FVK_ERR_INVALID, "batch: %d tiles are in no case", nt - t);
+        if (rc == FVK_OK) {
+            s->d_tile_list = static_cast<int*>(dev_alloc(s, sizeof(int) * std::max(nt, 1)));
+            s->d_n_tile_list = static_cast<int*>(dev_alloc(s, sizeof(int)));
+            s->d_case_tiles = static_cast<int*>(dev_alloc(s, sizeof(int) * (n_cases + 1)));
+            s->tile_partials = static_cast<double*>(dev_alloc(s, sizeof(double) * 4 * std::max(nt, 1)));
+            if (!s->d_tile_list || !s->d_n_tile_list || !s->d_case_tiles || !s->tile_partials)
+                rc = set_error(FVK_ERR_NOMEM, "batch: device allocation failed");
+        }
+        if (rc == FVK_OK)
+            rc = hip_check(hipMemcpy(s->d_case_tiles, s->case_tiles.data(), sizeof(int) * (n_cases + 1),
+                                     hipMemcpyHostToDevice), "batch maps");
+    }
     if (rc == FVK_OK) {
         std::vector<int> ones(n_cases, 1);
         rc = hip_check(hipMemcpy(s->d_case_start, s->case_cell_start.data(), sizeof(int) * (n_cases + 1),
@@ -217,6 +268,7 @@ int fvk_batch_create(int32_t n_cases, const fvk_mesh_desc* meshes, const fvk_bc*
         if (rc == FVK_OK) rc = hip_check(hipMemset(s->d_step, 0, 256), "batch maps");
         if (rc == FVK_OK) rc = hip_check(hipDeviceSynchronize(), "batch set-up");
     }
+    if (rc == FVK_OK && variant == 3) rc = upload_tile_list(s, std::vector<int>(n_cases, 1).data());
     if (rc != FVK_OK) {
         fvk_destroy(s);
         delete b;
@@ -246,6 +298,7 @@ int fvk_batch_set_active(fvk_batch* b, const int32_t* active) {
     int rc = hip_check(hipStreamSynchronize(s->stream), "batch");
     if (rc == FVK_OK)
         rc = hip_check(hipMemcpy(s->d_active, active, sizeof(int) * b->n_cases, hipMemcpyHostToDevice), "batch active");
+    if (rc == FVK_OK && s->d_tile_list) rc = upload_tile_list(s, active);
     return rc;
 }
 

@@ -26,6 +26,8 @@
 // L2) sweeps one contiguous range of the RCM-ordered cells or faces.
 #include <algorithm>
 #include <cfloat>
+#include <mutex>
+#include <unordered_map>
 
 #include "physics.h"
 #include "solver.h"
@@ -74,6 +76,12 @@ struct EulerArgs {
     int fin_nblocks, fin_slot, fin_rearm, fin_dt_row;
     int64_t fin_hist_idx;
     int rearm_slot;       // >= 0: the gather re-arms this dt slot (forward Euler with a folded finalize)
+    // batched cases on the tile kernel: the tiles of the active cases
+    // (tile_list[0 .. *n_tile_list)), a workgroup per entry; each
+    // tile's residual sums go to tile_partials [4][ntiles]
+    const int* tile_list;
+    const int* n_tile_list;
+    double* tile_partials;
 };
 
 
@@ -628,22 +636,9 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
 // thread's gradients are done, before the barrier that ends them; triangles
 // with the HLLC and Rusanov fluxes fit so without spills.
 template <int D, int SCHEME, int ORDER, int WPE>
-__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_e_tile(EulerArgs a) {
+__device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t) {
     constexpr bool PF = WPE < 4;
-    extern __shared__ double lds[];
     const MeshView& m = a.m;
-    // a pending finalize (the previous stage's norms) takes the first logical
-    // workgroup, dispatched first so that its one-workgroup reduction overlaps
-    // the tiles; the tiles follow it
-    int t = logical_block();
-    if (a.fin_nblocks > 0) {
-        if (t == 0) {
-            finalize_norms(a.partials, a.fin_nblocks, 4, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin,
-                           a.fin_slot, -1.0, lds, a.fin_rearm != 0, a.fin_dt_row != 0);
-            return;
-        }
-        --t;
-    }
     const double g = a.gamma;
     const int bs = blockDim.x, tid = threadIdx.x;
     TileLds L;
@@ -807,10 +802,13 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
         }
         if (tid == 0) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) a.partials[q * gridDim.x + blockIdx.x] = lds[q * bs];
+            for (int q = 0; q < 4; ++q) {
+                if (a.tile_list) a.tile_partials[(int64_t)q * m.ntiles + t] = lds[q * bs];
+                else a.partials[q * gridDim.x + blockIdx.x] = lds[q * bs];
+            }
         }
     }
-    if (a.face_dt) {
+    if (a.face_dt && !a.tile_list) {  // batched cases keep no step dt
         __syncthreads();
         lds[tid] = cand_min;
         __syncthreads();
@@ -826,7 +824,78 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
     }
 }
 
+// batched cases: one launch per stage over the list of the active cases'
+// tiles, a workgroup per list entry; the grid covers every tile of the
+// batch, so the same launch (captured once in a graph) serves any set of
+// active cases, and the workgroups past the list's end return at once.
+// (A grid-stride loop over the list keeps the kernel arguments live across
+// its iterations and spills at this register budget.)
+template <int D, int SCHEME, int ORDER, int WPE>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_e_tile_cases(EulerArgs a) {
+    extern __shared__ double lds[];
+    const int it = logical_block();
+    if (it >= *a.n_tile_list) return;
+    tile_body<D, SCHEME, ORDER, WPE>(a, lds, a.tile_list[it]);
+}
+
+template <int D, int SCHEME, int ORDER, int WPE>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_e_tile(EulerArgs a) {
+    extern __shared__ double lds[];
+    int t = logical_block();
+    // a pending finalize (the previous stage's norms) takes the first logical
+    // workgroup, dispatched first so that its one-workgroup reduction overlaps
+    // the tiles; the tiles follow it
+    if (a.fin_nblocks > 0) {
+        if (t == 0) {
+            finalize_norms(a.partials, a.fin_nblocks, 4, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin,
+                           a.fin_slot, -1.0, lds, a.fin_rearm != 0, a.fin_dt_row != 0);
+            return;
+        }
+        --t;
+    }
+    tile_body<D, SCHEME, ORDER, WPE>(a, lds, t);
+}
+
+// batched cases on tiles: one block per case, the case's residual norms from
+// the per-tile sums of its tile range in a fixed tree order (a frozen case
+// gets zero rows), into row (*step mod cap) of the per-case history
+__global__ __launch_bounds__(kCaseFinBlock) void k_case_finalize_tiles(const double* tile_partials, int ntiles,
+                                                                       const int* case_tiles, const int* active,
+                                                                       double* case_res, const unsigned int* step,
+                                                                       int cap, int n_cases) {
+    __shared__ double sm[kCaseFinBlock];
+    const int k = blockIdx.x;
+    const int t0 = case_tiles[k], t1 = case_tiles[k + 1];
+    const bool on = active[k] != 0;
+    const int64_t row = (int64_t)(*step % (unsigned int)cap);
+    for (int q = 0; q < 4; ++q) {
+        double acc = 0.0;
+        if (on)
+            for (int t = t0 + (int)threadIdx.x; t < t1; t += kCaseFinBlock) acc += tile_partials[(int64_t)q * ntiles + t];
+        sm[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = kCaseFinBlock / 2; s > 0; s >>= 1) {
+            if (threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) case_res[(row * n_cases + k) * 4 + q] = sqrt(sm[0]);
+        __syncthreads();
+    }
+}
+
 constexpr int kCuLds = 160 * 1024;  // LDS per CU
+
+// Raises a kernel's dynamic LDS limit to `lds` when it exceeds the default
+// 64 KB (a gfx950 workgroup may use the CU's whole 160 KB), once per kernel.
+void allow_lds(const void* kernel, size_t lds) {
+    if (lds <= 65536) return;
+    static std::mutex mu;
+    static std::unordered_map<const void*, size_t> raised;
+    std::lock_guard<std::mutex> lock(mu);
+    size_t& cur = raised[kernel];
+    if (cur >= lds) return;
+    if (hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess) cur = lds;
+}
 
 // ---------------------------------------------------------------------------
 // host launchers
@@ -893,15 +962,22 @@ struct Launch {
         // four wavefronts per SIMD when the CU's LDS holds enough workgroups for them
         const bool four = D == 3 && ORDER == 2 && s->p.flux_scheme != FLUX_EXACT &&
                           (int64_t)(kCuLds / std::max<size_t>(lds, 1)) * s->block >= 4 * 4 * 64;
+        // batched cases take the tile-list instance
+        auto go = [&](auto single, auto cases) {
+            const void* k = a.tile_list ? reinterpret_cast<const void*>(cases) : reinterpret_cast<const void*>(single);
+            allow_lds(k, lds);
+            if (a.tile_list) cases<<<grid, s->block, lds, s->stream>>>(a);
+            else single<<<grid, s->block, lds, s->stream>>>(a);
+        };
         if (four) {
-            if (s->p.flux_scheme == FLUX_HLLC) k_e_tile<3, FLUX_HLLC, 2, 4><<<grid, s->block, lds, s->stream>>>(a);
-            else k_e_tile<3, FLUX_RUSANOV, 2, 4><<<grid, s->block, lds, s->stream>>>(a);
+            if (s->p.flux_scheme == FLUX_HLLC) go(k_e_tile<3, FLUX_HLLC, 2, 4>, k_e_tile_cases<3, FLUX_HLLC, 2, 4>);
+            else go(k_e_tile<3, FLUX_RUSANOV, 2, 4>, k_e_tile_cases<3, FLUX_RUSANOV, 2, 4>);
             return;
         }
         switch (s->p.flux_scheme) {
-            case FLUX_HLLC: k_e_tile<D, FLUX_HLLC, ORDER, 1><<<grid, s->block, lds, s->stream>>>(a); break;
-            case FLUX_EXACT: k_e_tile<D, FLUX_EXACT, ORDER, 1><<<grid, s->block, lds, s->stream>>>(a); break;
-            default: k_e_tile<D, FLUX_RUSANOV, ORDER, 1><<<grid, s->block, lds, s->stream>>>(a); break;
+            case FLUX_HLLC: go(k_e_tile<D, FLUX_HLLC, ORDER, 1>, k_e_tile_cases<D, FLUX_HLLC, ORDER, 1>); break;
+            case FLUX_EXACT: go(k_e_tile<D, FLUX_EXACT, ORDER, 1>, k_e_tile_cases<D, FLUX_EXACT, ORDER, 1>); break;
+            default: go(k_e_tile<D, FLUX_RUSANOV, ORDER, 1>, k_e_tile_cases<D, FLUX_RUSANOV, ORDER, 1>); break;
         }
     }
     static void tile(fvk_solver* s, const EulerArgs& a, int grid) {
@@ -924,6 +1000,26 @@ void face_o(fvk_solver* s, const EulerArgs& a) {
 template <int D>
 void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
     hipEvent_t ev = nullptr;
+    if (s->variant == 3 && s->n_cases > 0) {
+        // batched cases: one launch over the active cases' tiles, then the
+        // per-case norms
+        a.tile_list = s->d_tile_list;
+        a.n_tile_list = s->d_n_tile_list;
+        a.tile_partials = s->tile_partials;
+        a.fin_nblocks = 0;
+        timer_begin(s, "euler_tile", &ev);
+        Launch<D>::tile(s, a, (s->mesh.ntiles + kXcds - 1) / kXcds * kXcds);
+        timer_end(s, ev);
+        if (a.want_res) {
+            timer_begin(s, "euler_finalize", &ev);
+            k_case_finalize_tiles<<<s->n_cases, kCaseFinBlock, 0, s->stream>>>(
+                s->tile_partials, s->mesh.ntiles, s->d_case_tiles, s->d_active, s->case_res, s->d_step, s->hist_cap,
+                s->n_cases);
+            k_step_tick<<<1, 1, 0, s->stream>>>(s->d_step);
+            timer_end(s, ev);
+        }
+        return;
+    }
     if (s->variant == 3) {
         if (dt_pass) {  // its block minimum needs a power-of-two workgroup
             timer_begin(s, "euler_dt", &ev);

@@ -1041,9 +1041,13 @@ void launch_tile_eq(fvk_solver* s, int grid, const ViscArgs& a0) {
     ViscArgs a = a0;
     a.fin_nblocks = 0;  // a pending finalize went to this stage's gradient launch
     const size_t lds = s->tile_lds;
-    if (s->p.flux_scheme == FLUX_HLLC) k_visc_tile<EQ, FLUX_HLLC><<<grid, s->block, lds, s->stream>>>(a);
-    else if (s->p.flux_scheme == FLUX_EXACT) k_visc_tile<EQ, FLUX_EXACT><<<grid, s->block, lds, s->stream>>>(a);
-    else k_visc_tile<EQ, FLUX_RUSANOV><<<grid, s->block, lds, s->stream>>>(a);
+    auto go = [&](auto kernel) {
+        allow_lds(reinterpret_cast<const void*>(kernel), lds);
+        kernel<<<grid, s->block, lds, s->stream>>>(a);
+    };
+    if (s->p.flux_scheme == FLUX_HLLC) go(k_visc_tile<EQ, FLUX_HLLC>);
+    else if (s->p.flux_scheme == FLUX_EXACT) go(k_visc_tile<EQ, FLUX_EXACT>);
+    else go(k_visc_tile<EQ, FLUX_RUSANOV>);
 }
 
 template <int EQ>

@@ -15,7 +15,7 @@ namespace fvk {
 
 constexpr int kMaxPatches = 1024;
 constexpr int kMaxEulerDegree = 16;  // most faces of one cell the per-cell Euler kernels take
-constexpr int kMaxTileLds = 65536;   // LDS bytes one tile of the fused Euler stage may use
+constexpr int kMaxTileLds = 163840;  // LDS bytes one tile may use: a gfx950 workgroup may take the CU's 160 KB
 constexpr int kXcds = 8;             // accelerator complex dies of an MI355X, one L2 each
 
 // Workgroups are dispatched round-robin over the XCDs; the remap below gives
@@ -227,6 +227,13 @@ struct fvk_solver {
     double* case_res = nullptr;                  // [hist_cap][n_cases][4] per-case residual norms
     double* case_partials = nullptr;             // [4][2][nblocks] block sums split at a case boundary
     unsigned int* d_step = nullptr;              // device step counter (row of case_res)
+    // batched cases on the tile kernel: the tiles of the active cases, their
+    // count, each case's tile range and per-tile residual sums
+    int* d_tile_list = nullptr;
+    int* d_n_tile_list = nullptr;
+    int* d_case_tiles = nullptr;                 // [n_cases + 1]
+    std::vector<int> case_tiles;                 // host copy
+    double* tile_partials = nullptr;             // [4][ntiles]
 
     // timing
     bool timing = false;
@@ -267,6 +274,8 @@ int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, i
 int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
 void adjoint_release(fvk_solver* s);
 int launch_copy_probe(double* dst, const double* src, int64_t n, hipStream_t st);
+// raises a kernel's dynamic LDS limit above the default 64 KB (once per kernel)
+void allow_lds(const void* kernel, size_t lds);
 // wall diagnostics (kernels_visc.hip): samples and force sums of the faces
 // wall_faces[n] (reference ids, device memory); contrib [5][n] and patch [n]
 // are scratch; dir, lift, q and q2 as compute_wall_forces derives them from

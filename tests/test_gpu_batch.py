@@ -38,9 +38,19 @@ def three():
     return small_cases()
 
 
+def with_variant(params, variant, block=0, tile_cells=0):
+    p = abi.ParamsC.from_buffer_copy(params)
+    p.kernel_variant, p.block_size, p.tile_cells = variant, block, tile_cells
+    return p
+
+
+@pytest.mark.parametrize("variant,block,tc", [(1, 0, 0), (3, 0, 0), (3, 256, 200), (3, 512, 448)])
 @pytest.mark.parametrize("graph", [False, True])
-def test_batch_matches_single_cases(three, graph):
-    meshes, bcs, params, ics = three
+def test_batch_matches_single_cases(three, graph, variant, block, tc):
+    """The persistent tile launch (variant 3, the default for SSP-RK) and the
+    face sweep + gather (variant 1) against each case alone."""
+    meshes, bcs, params0, ics = three
+    params = with_variant(params0, variant, block, tc)
     b = FVKBatch(meshes, bcs, params, ics)
     b.set_graph(graph)
     rb = b.step(25)
@@ -61,9 +71,12 @@ def test_batch_case_matches_oracle(three):
     assert cases.rel_state_diff(b.field(1), o.field(), floor=1e-12) <= 1e-12
 
 
-def test_frozen_case_keeps_state(three):
-    meshes, bcs, params, ics = three
+@pytest.mark.parametrize("variant", [1, 3])
+def test_frozen_case_keeps_state(three, variant):
+    meshes, bcs, params0, ics = three
+    params = with_variant(params0, variant)
     b = FVKBatch(meshes, bcs, params, ics)
+    b.set_graph(True)  # freezing changes the device tile list, not the captured launches
     b.step(10)
     U1 = b.field(1).copy()
     b.set_active([1, 0, 1])
@@ -106,4 +119,5 @@ def test_autotuner_picks_a_valid_shape(three, tmp_path, monkeypatch):
     assert np.array_equal(g.field(), ref.field())
     pb = abi.ParamsC.from_buffer_copy(params)
     bb = autotune.tune_batch(meshes, bcs, pb, ics, steps=3, warmup=1)
-    assert pb.block_size == bb["block"] and pb.kernel_variant == 1
+    assert pb.block_size == bb["block"] and pb.kernel_variant == bb["variant"] in (1, 3)
+    assert pb.tile_cells == bb["tile_cells"]

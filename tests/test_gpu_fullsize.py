@@ -46,7 +46,7 @@ def test_config2_full_size_against_oracle(config2):
     ref = fields(o)
     # tiles of 224 cells run three workgroups per CU, tiles of 208 four (the
     # 128-VGPR tile kernel)
-    for variant, block, tc in ((1, 512, 0), (3, 256, 224), (3, 256, 208)):
+    for variant, block, tc in ((1, 512, 0), (3, 256, 224), (3, 256, 208), (3, 512, 448)):
         g = FVKSolver(m, bcs, with_shape(p, variant, block, tc), ic)
         rg = g.step(3)
         for a, b in zip(fields(g), ref):

@@ -180,11 +180,11 @@ def test_kernel_variants_bit_identical(order):
 def test_tile_variant_bit_identical(case):
     """The fused tile kernels against the face sweep + gather on periodic,
     exact-Riemann and first-order cases, several tile sizes."""
-    for bs3 in (64, 256, 320):
+    for bs3 in (64, 256, 320, 512):
         out = []
         for variant in (1, 3):
             bs = bs3 if variant == 3 else min(bs3, 256)  # the tile kernels also take 5 wavefronts
-            tc = 224 if variant == 3 and bs3 == 320 else 0
+            tc = {320: 224, 512: 448}.get(bs3, 0) if variant == 3 else 0
             if case == "1r_o2_periodic":
                 c = cases.case_1r(flux=abi.FLUX_HLLC, order=2, limiter=abi.LIMITER_VENKATAKRISHNAN, rk_stages=2,
                                   kernel_variant=variant, block_size=bs, tile_cells=tc)
