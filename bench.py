@@ -91,16 +91,18 @@ def timed_steps(solver, seconds):
             return n, t
 
 
-def adjoint_leg(s, m, bcs, params, ic, steps_done, reps=5, n_check=32, h=1e-7):
+def adjoint_leg(s, m, bcs, params, ic, steps_done, reps=5, n_check=32):
     """Discrete adjoint of the config-2 residual at its 1000-step state:
     device time of one VJP (state and node coordinates, lambda ~ N(0, 1) from
     default_rng(0)) against one residual evaluation with the same gradient
-    and limiter recompute; and central differences (step 1e-7) of
-    lambda . R at 32 cells and 32 nodes drawn by default_rng(1), each
-    evaluated on the six-ring sub-mesh around the perturbed entity (the
-    residual there is the whole mesh's up to an additive constant that the
-    difference cancels)."""
+    and limiter recompute; and finite differences of lambda . R at 32 state
+    entries and 32 node coordinates drawn by default_rng(1) among those whose
+    adjoint is at least 1e-4 of the largest, each evaluated on the six-ring
+    sub-mesh around the perturbed entity (the residual there is the whole
+    mesh's up to entries the difference cancels), relative to the entry
+    itself at the best of a sweep of steps (finitevolume_amd.fdcheck)."""
     from finitevolume_amd import abi
+    from finitevolume_amd.fdcheck import fd_sweep
     from finitevolume_amd.solver import FVKSolver
     if steps_done < 1000:
         s.step(1000 - steps_done)
@@ -129,83 +131,7 @@ def adjoint_leg(s, m, bcs, params, ic, steps_done, reps=5, n_check=32, h=1e-7):
     dev_p = sum(tot for tot, _ in tp.values()) / reps
     dev_a = sum(tot for tot, _ in ta.values()) / reps
     dev_u = sum(tot for tot, _ in tu.values()) / reps
-
-    cut_bc = next((b for b in bcs if b.type == abi.BC_FARFIELD), abi.make_bc(abi.BC_OUTFLOW))
-    sub_bcs = list(bcs) + [cut_bc]
-    cut = len(bcs)
-
-    def J(mesh, Us, ls):
-        g = FVKSolver(mesh, sub_bcs, params, ic)
-        g.set_field(Us)
-        return float(np.sum(ls * g.residual_eval()))
-
-    if n_check <= 0:
-        return {
-            "workload": "config4: VJP of the config-2 residual (state + node coordinates) at the 1000-step state",
-            "vjp_device_ms": round(dev_a, 4), "primal_residual_device_ms": round(dev_p, 4),
-            "adjoint_over_primal": round(dev_a / dev_p, 3), "vjp_state_only_device_ms": round(dev_u, 4),
-            "adjoint_state_only_over_primal": round(dev_u / dev_p, 3),
-            "vjp_kernel_avg_ms": {k: round(tot / cnt, 5) for k, (tot, cnt) in sorted(ta.items()) if cnt},
-            "fd_check": None}
-    rng = np.random.default_rng(1)
-    cells = rng.choice(nc, n_check, replace=False)
-    nodes = rng.choice(m.n_nodes, n_check, replace=False)
-    scale_u, scale_x = float(np.abs(Ub).max()), float(np.abs(Xb).max())
-    # [max error where the difference is smooth, number refined to a smaller step, unresolved]
-    stats = {"state": [0.0, 0, 0], "nodes": [0.0, 0, 0]}
-
-    def check(kind, evalj, j0, h0, ad, scale):
-        """The limiter's min/max make lambda . R piecewise smooth, and in the
-        thin wall layers a step of 1e-7 can exceed the local variation of the
-        state.  Where the one-sided differences disagree the step is refined
-        (down to 1e-3 of it); points that never become smooth are counted."""
-        st = stats[kind]
-        den = max(abs(ad), 1e-6 * scale)
-        for r in range(4):
-            hh = h0 * 10.0 ** (-r)
-            jp, jm = evalj(hh)
-            f_fd, b_fd = (jp - j0) / hh, (j0 - jm) / hh
-            if abs(f_fd - b_fd) <= 1e-4 * max(abs(f_fd), abs(b_fd), 1e-6 * scale):
-                st[0] = max(st[0], abs(ad - (jp - jm) / (2 * hh)) / den)
-                st[1] += r > 0
-                return
-        st[2] += 1
-
-    for c in cells:
-        q = int(rng.integers(0, 4))
-        sel = m.cell_rings([c], 6)
-        sub, _ = m.submesh(sel, cut)
-        loc = int(np.searchsorted(sel, c))
-        ls, Us = lam[sel], U[sel]
-
-        def ev(hh):
-            Up, Um = Us.copy(), Us.copy()
-            Up[loc, q] += hh
-            Um[loc, q] -= hh
-            return J(sub, Up, ls), J(sub, Um, ls)
-        check("state", ev, J(sub, Us, ls), h, Ub[c, q], scale_u)
-    owner = np.repeat(np.arange(nc), np.diff(m.cell_node_ptr))
-    e = m.node_xy[m.face_nodes[:, 1]] - m.node_xy[m.face_nodes[:, 0]]
-    edge = np.hypot(e[:, 0], e[:, 1])
-    h_nodes = []
-    for n in nodes:
-        # the wall layers are far thinner than 1e-7: the step is capped at
-        # 1e-3 of the node's shortest edge so that no cell folds
-        hn = min(h, 1e-3 * float(edge[(m.face_nodes == n).any(axis=1)].min()))
-        h_nodes.append(hn)
-        sel = m.cell_rings(owner[m.cell_nodes == n], 6)
-        sub, used = m.submesh(sel, cut)
-        loc = int(np.searchsorted(used, n))
-        ls, Us = lam[sel], U[sel]
-        j0 = J(sub, Us, ls)
-        for d in range(2):
-            def ev(hh):
-                Xp, Xm = sub.node_xy.copy(), sub.node_xy.copy()
-                Xp[loc, d] += hh
-                Xm[loc, d] -= hh
-                return J(sub.with_nodes(Xp), Us, ls), J(sub.with_nodes(Xm), Us, ls)
-            check("nodes", ev, j0, hn, Xb[n, d], scale_x)
-    return {
+    out = {
         "workload": "config4: VJP of the config-2 residual (state + node coordinates) at the 1000-step state",
         "vjp_device_ms": round(dev_a, 4),
         "primal_residual_device_ms": round(dev_p, 4),
@@ -216,15 +142,84 @@ def adjoint_leg(s, m, bcs, params, ic, steps_done, reps=5, n_check=32, h=1e-7):
         "vjp_wall_ms": round(wall_a * 1e3, 3),
         "primal_residual_wall_ms": round(wall_p * 1e3, 3),
         "wall_note": "wall times include the host copies and reordering of lambda, U_bar, X_bar and R",
-        "fd_check": {"cells": n_check, "nodes": n_check, "step": h,
-                     "node_steps_below_1e-7": int(sum(x < h for x in h_nodes)),
-                     "min_node_step": float(f"{min(h_nodes):.3e}"),
-                     "max_rel_err_state": float(f"{stats['state'][0]:.3e}"),
-                     "max_rel_err_nodes": float(f"{stats['nodes'][0]:.3e}"),
-                     "refined_state": stats["state"][1], "unresolved_state": stats["state"][2],
-                     "refined_nodes": stats["nodes"][1], "unresolved_nodes": stats["nodes"][2],
-                     "method": "central differences of lambda . R on six-ring sub-meshes (GPU residual)"},
+        "fd_check": None,
     }
+    if n_check <= 0:
+        return out
+
+    cut_bc = next((b for b in bcs if b.type == abi.BC_FARFIELD), abi.make_bc(abi.BC_OUTFLOW))
+    sub_bcs = list(bcs) + [cut_bc]
+    cut = len(bcs)
+    steps = [1e-3, 1e-4, 1e-5, 1e-6, 1e-7]
+    rng = np.random.default_rng(1)
+
+    def draw(vals, n):
+        idx = np.nonzero(np.abs(vals) >= 1e-4 * np.abs(vals).max())[0]
+        return rng.choice(idx, min(n, idx.size), replace=False)
+
+    def summary(errs, kinks):
+        e = np.array(errs) if errs else np.array([np.nan])
+        return {"entries": len(errs) + kinks, "kinks": kinks, "median_rel_err": float(f"{np.median(e):.3e}"),
+                "p90_rel_err": float(f"{np.quantile(e, 0.9):.3e}"), "max_rel_err": float(f"{e.max():.3e}"),
+                "within_1e-7": int(np.sum(e <= 1e-7))}
+
+    errs, kinks = [], 0
+    for k in draw(Ub.ravel(), n_check):
+        c, q = divmod(int(k), 4)
+        sel = m.cell_rings([c], 6)
+        sub, _ = m.submesh(sel, cut)
+        loc = int(np.searchsorted(sel, c))
+        ls, Us = lam[sel], U[sel]
+        g = FVKSolver(sub, sub_bcs, params, ic)
+        g.set_field(Us)
+        r0 = g.residual_eval()
+        sc = max(abs(U[c, q]), 1e-3)
+
+        def gs(h):
+            Up = Us.copy()
+            Up[loc, q] += h * sc
+            g.set_field(Up)
+            return float(np.sum(ls * (g.residual_eval() - r0))) / sc
+        err, _, _, kink = fd_sweep(gs, Ub[c, q], steps)
+        if kink:
+            kinks += 1
+        else:
+            errs.append(err / abs(Ub[c, q]))
+    fd = {"state": summary(errs, kinks)}
+    owner = np.repeat(np.arange(nc), np.diff(m.cell_node_ptr))
+    e = m.node_xy[m.face_nodes[:, 1]] - m.node_xy[m.face_nodes[:, 0]]
+    edge = np.hypot(e[:, 0], e[:, 1])
+    errs, kinks = [], 0
+    for k in draw(Xb.ravel(), n_check):
+        n, d = divmod(int(k), 2)
+        sel = m.cell_rings(owner[m.cell_nodes == n], 6)
+        sub, used = m.submesh(sel, cut)
+        loc = int(np.searchsorted(used, n))
+        ls, Us = lam[sel], U[sel]
+        g = FVKSolver(sub, sub_bcs, params, ic)
+        g.set_field(Us)
+        r0 = g.residual_eval()
+        # steps relative to the node's shortest edge: the wall layers are far
+        # thinner than any absolute step
+        sc = float(edge[(m.face_nodes == n).any(axis=1)].min())
+
+        def gx(h):
+            X = sub.node_xy.copy()
+            X[loc, d] += h * sc
+            gg = FVKSolver(sub.with_nodes(X), sub_bcs, params, ic)
+            gg.set_field(Us)
+            return float(np.sum(ls * (gg.residual_eval() - r0))) / sc
+        err, _, _, kink = fd_sweep(gx, Xb[n, d], steps)
+        if kink:
+            kinks += 1
+        else:
+            errs.append(err / abs(Xb[n, d]))
+    fd["nodes"] = summary(errs, kinks)
+    fd["method"] = ("per entry, relative to the entry: best of central, Richardson and one-sided differences of "
+                    "lambda . R over steps 1e-3..1e-7 (x |U| or x shortest edge) on six-ring sub-meshes (GPU "
+                    "residual); kinks = one-sided slopes disagree (limiter/HLLC ties), not scored")
+    out["fd_check"] = fd
+    return out
 
 
 def _gen_batch_case(prm):

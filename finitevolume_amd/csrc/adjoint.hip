@@ -47,6 +47,8 @@ struct AdjointWork {
     double2 *mb = nullptr, *pb = nullptr, *xcv = nullptr, *xbar = nullptr, *node_xy = nullptr;
     int *poly = nullptr, *pslot = nullptr;
     int *cvptr = nullptr, *cvent = nullptr;
+    double4* lsqm = nullptr;  // per cell: LSQ normal matrix m00, m01, m11 and its determinant
+    double2* lsqd = nullptr;  // per ELL slot: offset (dx, dy) to the LSQ neighbour point
 };
 
 void adjoint_release(fvk_solver* s) {
@@ -65,6 +67,8 @@ struct AdjArgs {
     const int *poly, *pslot;
     const int *cvptr, *cvent;
     const double2* node_xy;
+    const double4* lsqm;
+    const double2* lsqd;
     double gamma, venk_k;
     int limiter, order, grad, nn;
 };
@@ -353,19 +357,24 @@ __device__ __forceinline__ void adj_cell_body(const AdjArgs& a, int c, const int
         }
         prim_rev(Uc, Prim{wcb[0], wcb[1], wcb[2], wcb[3]}, Ub, g);
         if (WX) {
-            // P = M^-1 (stored), Mb = -P Pb P with Pb the symmetric adjoint of P
-            const double p0 = iv.x, p1 = iv.y, p2 = iv.z, h = 0.5 * i1b;
-            const double t00 = i0b * p0 + h * p1, t01 = i0b * p1 + h * p2;
-            const double t10 = h * p0 + i2b * p1, t11 = h * p1 + i2b * p2;
-            const double M00 = -(p0 * t00 + p1 * t10), M11 = -(p1 * t01 + p2 * t11);
-            const double Ms = -0.5 * ((p0 * t01 + p1 * t11) + (p1 * t00 + p2 * t10));
+            // The stored inverse is P = (m11, -m01, m00) / det with det =
+            // m00 m11 - m01^2 (the set-up's formula): reversed operation by
+            // operation.  (-P Pb P would be the derivative of the exact
+            // inverse; for the thin, nearly collinear stencils of stretched
+            // triangles det keeps few digits and P is not that inverse.)
+            const double4 mm = a.lsqm[c];
+            const double idet = fdiv(1.0, mm.w);
+            const double detb = -(i0b * iv.x + i1b * iv.y + i2b * iv.z) * idet;
+            const double M00 = i2b * idet + detb * mm.z;
+            const double M11 = i0b * idet + detb * mm.x;
+            const double Ms = 0.5 * (-i1b * idet - 2.0 * detb * mm.y);
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 if (en[k].x < 0) continue;
-                // offset d to the neighbour point, weight w = 1/|d|^2; (w dx, w dy) is stored
-                const double w = eg[k].x * eg[k].x + eg[k].y * eg[k].y;
-                const double iw = fdiv(1.0, w);
-                const double dx = eg[k].x * iw, dy = eg[k].y * iw;
+                // offset d to the neighbour point and weight w = 1/|d|^2, as the set-up formed them
+                const double2 dd = a.lsqd[k * ld + c];
+                const double dx = dd.x, dy = dd.y;
+                const double w = fdiv(1.0, dx * dx + dy * dy);
                 const double sq = M00 * dx * dx + 2.0 * Ms * dx * dy + M11 * dy * dy;
                 const double wq = wdxb[k] * dx + wdyb[k] * dy;
                 const double w2 = w * w;
@@ -552,6 +561,41 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_gather(AdjArgs a) {
         if (k < nv) a.xcv[k * ld + c] = double2{xb[k], yb[k]};
 }
 
+// set-up (node coordinates): per cell the LSQ normal matrix and its
+// determinant, per slot the offset to the LSQ point, formed exactly as the
+// host set-up of the stored inverse forms them (same operands, same order,
+// IEEE division), so that the reverse of the inverse's formula sees its
+// operands' own bits
+template <int D>
+__global__ __launch_bounds__(kAdjBlock) void k_adj_lsq_setup(MeshView m, double4* lsqm, double2* lsqd) {
+    const int c = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (c >= m.nc) return;
+    const int64_t ld = m.ld;
+    const double4 cc = m.cgeo[c];
+    double m00 = 0.0, m01 = 0.0, m11 = 0.0;
+    for (int k = 0; k < D; ++k) {
+        const int2 e = m.ell[k * ld + c];
+        if (e.x < 0) continue;
+        double qx, qy;
+        if (e.y < 0) {
+            const double2 mid = m.fmid2[e.x >> 1];
+            qx = mid.x;
+            qy = mid.y;
+        } else {
+            const double4 o = m.cgeo[e.y];
+            qx = o.x;
+            qy = o.y;
+        }
+        const double dx = qx - cc.x, dy = qy - cc.y;
+        const double w = 1.0 / (dx * dx + dy * dy);
+        m00 += w * dx * dx;
+        m01 += w * dx * dy;
+        m11 += w * dy * dy;
+        lsqd[k * ld + c] = double2{dx, dy};
+    }
+    lsqm[c] = double4{m00, m01, m11, m00 * m11 - m01 * m01};
+}
+
 // per node: the vertex contributions of the cells around it
 __global__ __launch_bounds__(kAdjBlock) void k_adj_nodes(AdjArgs a) {
     const int n = blockIdx.x * kAdjBlock + threadIdx.x;
@@ -682,14 +726,27 @@ int build_geo_maps(fvk_solver* s, AdjointWork* w, const std::vector<int2>& ell) 
     w->pslot = alloc_n<int>(s, (int64_t)D * ld);
     w->cvptr = alloc_n<int>(s, nn + 1);
     w->cvent = alloc_n<int>(s, (int64_t)cvent.size());
+    w->lsqm = alloc_n<double4>(s, ld);
+    w->lsqd = alloc_n<double2>(s, (int64_t)D * ld);
     if (!w->mb || !w->pb || !w->cb || !w->xcv || !w->xbar || !w->node_xy || !w->poly || !w->pslot || !w->cvptr ||
-        !w->cvent)
+        !w->cvent || !w->lsqm || !w->lsqd)
         return set_error(FVK_ERR_NOMEM, "adjoint: device allocation failed");
     int rc = upload_vec(s, w->poly, poly);
     if (rc == FVK_OK) rc = upload_vec(s, w->pslot, pslot);
     if (rc == FVK_OK) rc = upload_vec(s, w->cvptr, cvptr);
     if (rc == FVK_OK) rc = upload_vec(s, w->cvent, cvent);
     if (rc == FVK_OK) rc = upload_vec(s, w->node_xy, xy);
+    if (rc == FVK_OK && s->p.order == 2) {
+        const int gc = grid_of(nc);
+        switch (D) {
+            case 3: k_adj_lsq_setup<3><<<gc, kAdjBlock, 0, s->stream>>>(s->mesh, w->lsqm, w->lsqd); break;
+            case 4: k_adj_lsq_setup<4><<<gc, kAdjBlock, 0, s->stream>>>(s->mesh, w->lsqm, w->lsqd); break;
+            case 9: k_adj_lsq_setup<9><<<gc, kAdjBlock, 0, s->stream>>>(s->mesh, w->lsqm, w->lsqd); break;
+            default: k_adj_lsq_setup<kMaxEulerDegree><<<gc, kAdjBlock, 0, s->stream>>>(s->mesh, w->lsqm, w->lsqd);
+        }
+        rc = hip_check(hipGetLastError(), "adjoint LSQ set-up launch");
+        if (rc == FVK_OK) rc = hip_check(hipStreamSynchronize(s->stream), "adjoint LSQ set-up");
+    }
     if (rc == FVK_OK) w->geo = true;
     return rc;
 }
@@ -802,6 +859,7 @@ int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_ba
     a.mb = w->mb; a.pb = w->pb; a.cb = w->cb; a.xcv = w->xcv; a.xbar = w->xbar;
     a.poly = w->poly; a.pslot = w->pslot; a.cvptr = w->cvptr;
     a.cvent = w->cvent; a.node_xy = w->node_xy;
+    a.lsqm = w->lsqm; a.lsqd = w->lsqd;
     a.gamma = s->p.gamma;
     a.venk_k = s->p.venkat_k;
     a.limiter = s->p.limiter;

@@ -14,7 +14,9 @@ from finitevolume_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libfvoracle.so")
+JVP_PATH = os.path.join(HERE, "_build", "libfvjvp.so")
 _lib = None
+_jvp = None
 
 
 def available() -> bool:
@@ -60,6 +62,42 @@ def lib():
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def residual_jvp(mesh, bcs, params: abi.ParamsC, U, vU=None, vX=None, long_double: bool = False):
+    """Euler residual R(U, X) and its forward-mode derivative J v along the
+    state direction vU [n_cells, 4] and the node-coordinate direction vX
+    [n_nodes, 2] (oracle/euler_jvp.cpp, dual numbers).  Returns (R, JV),
+    both [n_cells, 4].  long_double=True evaluates in long double: the
+    difference from the double evaluation estimates the latter's rounding."""
+    global _jvp
+    if _jvp is None:
+        if not os.path.exists(JVP_PATH):
+            raise FileNotFoundError(f"{JVP_PATH} not built (make -C oracle restatement)")
+        L = C.CDLL(JVP_PATH)
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.fvj_residual_jvp.argtypes = [C.POINTER(abi.MeshDescC), ip, ip, C.POINTER(abi.BCC), C.POINTER(abi.ParamsC),
+                                       dp, dp, dp, dp, dp, dp]
+        L.fvj_residual_jvp_ld.argtypes = L.fvj_residual_jvp.argtypes
+        _jvp = L
+    md = abi.MeshDesc(mesh)
+    arr, _ = abi.bc_array(bcs)
+    ptr = np.ascontiguousarray(mesh.cell_node_ptr, dtype=np.int32)
+    nodes = np.ascontiguousarray(mesh.cell_nodes, dtype=np.int32)
+    U = _f64(U)
+    X = _f64(mesh.node_xy)
+    vU = None if vU is None else _f64(vU)
+    vX = None if vX is None else _f64(vX)
+    R = np.empty((mesh.n_cells, 4))
+    JV = np.empty((mesh.n_cells, 4))
+    ipp = C.POINTER(C.c_int32)
+    fn = _jvp.fvj_residual_jvp_ld if long_double else _jvp.fvj_residual_jvp
+    rc = fn(md.ref, ptr.ctypes.data_as(ipp), nodes.ctypes.data_as(ipp), arr, C.byref(params),
+                               abi.as_dptr(U), abi.as_dptr(X), abi.as_dptr(vU), abi.as_dptr(vX), abi.as_dptr(R),
+                               abi.as_dptr(JV))
+    if rc != 0:
+        raise RuntimeError(f"fvj_residual_jvp failed: {rc}")
+    return R, JV
 
 
 class OracleSolver:

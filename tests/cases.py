@@ -118,6 +118,75 @@ def case_advdiff(gradient=abi.GRAD_LEAST_SQUARES, n=16, u=0.8, v=-0.3):
     return m, bcs, p, ic, None
 
 
+def adjoint_naca(flux=abi.FLUX_HLLC, order=2, **launch):
+    """Config-2 numerics on a small NACA 0012 O-grid (40 x 20 triangle pairs,
+    farfield at 6 chords, growth 1.2: cells of moderate aspect ratio).
+    Returns (mesh, bcs, params, ic, initial state or None)."""
+    m = M.naca_ogrid(40, 20, r_far=6.0, ratio=1.2)
+    rho, u, v, pinf = freestream(0.8, 1.25)
+    ff = M.from_primitive(rho, u, v, pinf, GAMMA)
+    bcs = [abi.make_bc(abi.BC_WALL), abi.make_bc(abi.BC_FARFIELD, ff)]
+    ext = dict(order=2, limiter=abi.LIMITER_VENKATAKRISHNAN, rk_stages=3, local_dt=1) if order == 2 else {}
+    p = abi.default_params(equation=abi.EQ_EULER, flux_scheme=flux, cfl=0.8, **ext, **launch)
+    ic = abi.initial_condition(abi.IC_FREESTREAM, rho=rho, u=u, v=v, p=pinf)
+    return m, bcs, p, ic, None
+
+
+def adjoint_wavy(order=2, flux=abi.FLUX_HLLC, **launch):
+    """Skewed quads, slip wall below, farfield left, outflow top and right,
+    with a smooth non-uniform state (every ghost type, four-face cells)."""
+    m = M.skewed(12, 0.15)
+    ff = M.from_primitive(1.0, 0.4, 0.1, 1.0, GAMMA)
+    bcs = [abi.make_bc(abi.BC_WALL), abi.make_bc(abi.BC_FARFIELD, ff), abi.make_bc(abi.BC_OUTFLOW),
+           abi.make_bc(abi.BC_OUTFLOW)]
+    ext = dict(order=2, limiter=abi.LIMITER_VENKATAKRISHNAN) if order == 2 else {}
+    p = abi.default_params(equation=abi.EQ_EULER, flux_scheme=flux, cfl=0.5, **ext, **launch)
+    ic = abi.initial_condition(abi.IC_FREESTREAM, rho=1.0, u=0.4, v=0.1, p=1.0)
+    x, y = m.cell_xy[:, 0], m.cell_xy[:, 1]
+    U0 = M.from_primitive(1.0 + 0.2 * np.sin(3 * x) * np.cos(2 * y), 0.4 + 0.1 * np.cos(4 * y), 0.1 * np.sin(5 * x),
+                          1.0 + 0.15 * np.cos(3 * x + y), GAMMA)
+    return m, bcs, p, ic, U0
+
+
+ADJOINT_CASES = {
+    "naca_hllc_o2": lambda: adjoint_naca(abi.FLUX_HLLC, 2),
+    "naca_rusanov_o2": lambda: adjoint_naca(abi.FLUX_RUSANOV, 2),
+    "naca_hllc_o1": lambda: adjoint_naca(abi.FLUX_HLLC, 1),
+    "naca_rusanov_o1": lambda: adjoint_naca(abi.FLUX_RUSANOV, 1),
+    "wavy_outflow_o2": lambda: adjoint_wavy(2),
+    "wavy_outflow_rusanov_o2": lambda: adjoint_wavy(2, abi.FLUX_RUSANOV),
+    # a solver with cell tiles (the adjoint itself runs the same passes)
+    "naca_hllc_o2_tiles": lambda: adjoint_naca(abi.FLUX_HLLC, 2, kernel_variant=3, block_size=128),
+    # the free stream with a density bump at the leading edge: away from it
+    # the state is exactly uniform (zero gradients, zero MUSCL increments)
+    "naca_hllc_o2_bump": lambda: adjoint_naca_bump(),
+}
+# steps of the oracle from the case's start to the state the adjoint is taken at
+ADJOINT_STEPS = {"naca_hllc_o2_bump": 0}
+
+
+def adjoint_naca_bump():
+    m, bcs, p, ic, _ = adjoint_naca(abi.FLUX_HLLC, 2)
+    rho, u, v, pinf = freestream(0.8, 1.25)
+    r2 = (m.cell_xy[:, 0] - 0.0) ** 2 + m.cell_xy[:, 1] ** 2
+    bump = np.where(r2 < 0.25, 0.2 * np.exp(-8.0 * r2), 0.0)
+    U0 = M.from_primitive(rho * (1.0 + bump), np.full(m.n_cells, u), np.full(m.n_cells, v), np.full(m.n_cells, pinf),
+                          GAMMA)
+    return m, bcs, p, ic, U0
+
+
+def developed_state(case, steps=40):
+    """The oracle's state after `steps` steps from the case's start."""
+    from oracle.pyoracle import OracleSolver
+    m, bcs, p, ic, U0 = case
+    o = OracleSolver(m, bcs, p, ic)
+    if U0 is not None:
+        o.set_field(U0)
+    if steps:
+        o.step(steps)
+    return o.field()
+
+
 def rel_state_diff(a, b, floor=1e-300):
     """Per-cell norm-wise relative difference max_k |a_k - b_k| / max_k |b_k|,
     maximised over cells (a component that is zero by symmetry is judged

@@ -1,67 +1,40 @@
-"""Discrete adjoint of the Euler residual on the GPU against central finite
-differences of the oracle's residual (absolute step 1e-7 on the state and on
-node coordinates, as the adjoint check specifies).  The oracle's residual is
-itself pinned to the reference (tests/test_oracle.py); the adjoint has no
-reference, so its parity is this derivative check: for J = lambda . R(U, X),
-  (J(U + h V) - J(U - h V)) / 2h  ==  U_bar . V
-  (J(X + h e_n) - J(X - h e_n)) / 2h  ==  X_bar[n]
-and the same for the pressure lift/drag objective."""
+"""Discrete adjoint of the Euler residual on the GPU (fvk_residual_vjp).
+The reference has no adjoint; the adjoint's parity is with the derivative of
+the oracle's residual, which is pinned to the reference
+(tests/test_oracle.py):
+
+  * exactly, by the dot-product identity <lambda, J v> = <J^T lambda, v>
+    against the oracle's forward-mode derivative J v (dual numbers,
+    oracle/euler_jvp.cpp, itself pinned to finite differences in
+    tests/test_oracle_jvp.py): along random state and node directions at
+    1e-10 relative, and entry by entry (v a unit vector) at 1e-10 of the
+    entry plus 1e-13 of the sum of |lambda_i (J v)_i| (the rounding of a dot
+    product that cancels);
+  * by finite differences of lambda . R(U, X) with the oracle's residual,
+    entry by entry relative to the entry itself, at the best of a sweep of
+    steps (central, Richardson and one-sided estimates);
+and the pressure lift objective's gradient (fvk_wall_forces_vjp) the same
+way against finite differences of fvk_wall_forces."""
 import numpy as np
 import pytest
 
 import cases
 from finitevolume_amd import abi
-from finitevolume_amd import mesh as M
+from finitevolume_amd.fdcheck import best_fd, fd_sweep
 from finitevolume_amd.solver import FVKSolver
-from oracle.pyoracle import OracleSolver
+from oracle.pyoracle import OracleSolver, residual_jvp
 
 pytestmark = pytest.mark.gpu
 
-H = 1e-7
+STEPS = [1e-2, 1e-3, 1e-4, 1e-5, 1e-6, 1e-7, 1e-8]
+naca_case = cases.adjoint_naca
+developed_state = cases.developed_state
 
 
-def naca_case(flux=abi.FLUX_HLLC, order=2, **launch):
-    """Config-2 numerics on a small NACA 0012 O-grid (40 x 20 triangles pairs,
-    farfield at 6 chords, growth 1.2: cells of moderate aspect ratio)."""
-    m = M.naca_ogrid(40, 20, r_far=6.0, ratio=1.2)
-    rho, u, v, pinf = cases.freestream(0.8, 1.25)
-    ff = M.from_primitive(rho, u, v, pinf, cases.GAMMA)
-    bcs = [abi.make_bc(abi.BC_WALL), abi.make_bc(abi.BC_FARFIELD, ff)]
-    ext = dict(order=2, limiter=abi.LIMITER_VENKATAKRISHNAN, rk_stages=3, local_dt=1) if order == 2 else {}
-    p = abi.default_params(equation=abi.EQ_EULER, flux_scheme=flux, cfl=0.8, **ext, **launch)
-    ic = abi.initial_condition(abi.IC_FREESTREAM, rho=rho, u=u, v=v, p=pinf)
-    return m, bcs, p, ic, None
-
-
-def wavy_outflow_case(order=2, **launch):
-    """Skewed quads, slip wall below, farfield left, outflow top and right,
-    with a smooth non-uniform state (every ghost type, four-face cells)."""
-    m = M.skewed(12, 0.15)
-    ff = M.from_primitive(1.0, 0.4, 0.1, 1.0, cases.GAMMA)
-    bcs = [abi.make_bc(abi.BC_WALL), abi.make_bc(abi.BC_FARFIELD, ff), abi.make_bc(abi.BC_OUTFLOW),
-           abi.make_bc(abi.BC_OUTFLOW)]
-    ext = dict(order=2, limiter=abi.LIMITER_VENKATAKRISHNAN) if order == 2 else {}
-    p = abi.default_params(equation=abi.EQ_EULER, flux_scheme=abi.FLUX_HLLC, cfl=0.5, **ext, **launch)
-    ic = abi.initial_condition(abi.IC_FREESTREAM, rho=1.0, u=0.4, v=0.1, p=1.0)
-    x, y = m.cell_xy[:, 0], m.cell_xy[:, 1]
-    U0 = M.from_primitive(1.0 + 0.2 * np.sin(3 * x) * np.cos(2 * y), 0.4 + 0.1 * np.cos(4 * y), 0.1 * np.sin(5 * x),
-                          1.0 + 0.15 * np.cos(3 * x + y), cases.GAMMA)
-    return m, bcs, p, ic, U0
-
-
-def developed_state(case, steps=40):
-    m, bcs, p, ic, U0 = case
-    o = OracleSolver(m, bcs, p, ic)
-    if U0 is not None:
-        o.set_field(U0)
-    o.step(steps)
-    return o.field()
-
-
-def objective(m, bcs, p, ic, U, lam):
+def residual(m, bcs, p, ic, U):
     o = OracleSolver(m, bcs, p, ic)
     o.set_field(U)
-    return float(np.sum(lam * o.residual_eval()))
+    return o.residual_eval()
 
 
 def gpu_vjp(case, U, lam, want_x=True):
@@ -71,68 +44,120 @@ def gpu_vjp(case, U, lam, want_x=True):
     return g.residual_vjp(lam, want_x=want_x)
 
 
-CASES = {
-    "naca_hllc_o2": lambda: naca_case(abi.FLUX_HLLC, 2),
-    "naca_rusanov_o2": lambda: naca_case(abi.FLUX_RUSANOV, 2),
-    "naca_hllc_o1": lambda: naca_case(abi.FLUX_HLLC, 1),
-    "wavy_outflow_o2": lambda: wavy_outflow_case(2),
-    # a solver with cell tiles (the adjoint itself runs the same passes)
-    "naca_hllc_o2_tiles": lambda: naca_case(abi.FLUX_HLLC, 2, kernel_variant=3, block_size=128),
-}
+CASES = cases.ADJOINT_CASES
 
 
 @pytest.fixture(scope="module", params=sorted(CASES))
 def setup(request):
     case = CASES[request.param]()
-    U = developed_state(case)
+    U = developed_state(case, cases.ADJOINT_STEPS.get(request.param, 40))
     rng = np.random.default_rng(0)
     lam = rng.standard_normal(U.shape)
     Ub, Xb = gpu_vjp(case, U, lam)
     return request.param, case, U, lam, Ub, Xb
 
 
-def test_state_directional(setup):
-    name, case, U, lam, Ub, _ = setup
+def _dot_ok(exact, got, terms, big, rel=1e-10, cancel=1e-13):
+    """Within 1e-10 of the entry, or within rounding: 1e-13 of the sum of
+    |lambda_i (J e)_i| plus 1e-13 of the largest adjoint entry (a column
+    whose face terms cancel within each cell, as on the uniform far field,
+    leaves an entry of pure rounding of terms of that size)."""
+    return abs(got - exact) <= rel * abs(exact) + cancel * (terms + big)
+
+
+def test_dot_product_random_directions(setup):
+    name, case, U, lam, Ub, Xb = setup
     m, bcs, p, ic, _ = case
     rng = np.random.default_rng(2)
-    V = rng.standard_normal(U.shape) * np.abs(U).mean(axis=0)
-    fd = (objective(m, bcs, p, ic, U + H * V, lam) - objective(m, bcs, p, ic, U - H * V, lam)) / (2 * H)
-    ad = float(np.sum(Ub * V))
-    scale = float(np.sum(np.abs(Ub * V)))
-    assert abs(fd - ad) <= 1e-6 * scale, (name, fd, ad)
+    vU = rng.standard_normal(U.shape) * np.abs(U).mean(axis=0)
+    vX = rng.standard_normal(m.node_xy.shape)
+    _, JU = residual_jvp(m, bcs, p, U, vU)
+    _, JX = residual_jvp(m, bcs, p, U, None, vX)
+    for lhs, rhs in [(np.sum(lam * JU), np.sum(Ub * vU)), (np.sum(lam * JX), np.sum(Xb * vX))]:
+        assert abs(lhs - rhs) <= 1e-10 * abs(lhs), (name, lhs, rhs, abs(lhs - rhs) / abs(lhs))
 
 
-def test_state_cells(setup):
-    """32 cells drawn without replacement, one component each."""
-    name, case, U, lam, Ub, _ = setup
+def test_dot_product_entries(setup):
+    """48 state entries and 48 node coordinates (the wall's nodes among
+    them), each against the column J e of the forward-mode derivative."""
+    name, case, U, lam, Ub, Xb = setup
     m, bcs, p, ic, _ = case
-    rng = np.random.default_rng(1)
-    cells = rng.choice(U.shape[0], size=32, replace=False)
-    comps = rng.integers(0, 4, size=32)
-    scale = np.abs(Ub).max()
-    for c, q in zip(cells, comps):
-        Up, Um = U.copy(), U.copy()
-        Up[c, q] += H
-        Um[c, q] -= H
-        fd = (objective(m, bcs, p, ic, Up, lam) - objective(m, bcs, p, ic, Um, lam)) / (2 * H)
-        assert abs(fd - Ub[c, q]) <= 1e-5 * abs(Ub[c, q]) + 1e-6 * scale, (name, c, q, fd, Ub[c, q])
-
-
-def test_node_coordinates(setup):
-    """32 nodes drawn without replacement, both coordinates."""
-    name, case, U, lam, _, Xb = setup
-    m, bcs, p, ic, _ = case
-    rng = np.random.default_rng(1)
-    nodes = rng.choice(m.n_nodes, size=32, replace=False)
-    scale = np.abs(Xb).max()
+    bu, bx = float(np.abs(Ub).max()), float(np.abs(Xb).max())
+    rng = np.random.default_rng(3)
+    cells = rng.choice(m.n_cells, 48, replace=False)
+    for c, q in zip(cells, rng.integers(0, 4, 48)):
+        e = np.zeros_like(U)
+        e[c, q] = 1.0
+        t = lam * residual_jvp(m, bcs, p, U, e)[1]
+        assert _dot_ok(float(np.sum(t)), Ub[c, q], float(np.abs(t).sum()), bu), (name, c, q, np.sum(t), Ub[c, q])
+    wall = np.unique(m.face_nodes[m.face_patch == 0])
+    nodes = np.concatenate([rng.choice(wall, min(12, wall.size), replace=False), rng.choice(m.n_nodes, 12)])
     for n in nodes:
         for d in range(2):
-            Xp, Xm = m.node_xy.copy(), m.node_xy.copy()
-            Xp[n, d] += H
-            Xm[n, d] -= H
-            fd = (objective(m.with_nodes(Xp), bcs, p, ic, U, lam) -
-                  objective(m.with_nodes(Xm), bcs, p, ic, U, lam)) / (2 * H)
-            assert abs(fd - Xb[n, d]) <= 1e-5 * abs(Xb[n, d]) + 1e-6 * scale, (name, n, d, fd, Xb[n, d])
+            e = np.zeros_like(m.node_xy)
+            e[n, d] = 1.0
+            t = lam * residual_jvp(m, bcs, p, U, None, e)[1]
+            assert _dot_ok(float(np.sum(t)), Xb[n, d], float(np.abs(t).sum()), bx), (name, n, d, np.sum(t), Xb[n, d])
+
+
+def _large(rng, vals, n):
+    """n entries drawn among those whose adjoint is at least 1e-4 of the
+    largest (finite differences of a derivative that cancels to rounding
+    check nothing; the dot-product tests cover every entry)."""
+    idx = np.nonzero(np.abs(vals) >= 1e-4 * np.abs(vals).max())[0]
+    return rng.choice(idx, min(n, idx.size), replace=False)
+
+
+def test_state_entries_fd(setup):
+    name, case, U, lam, Ub, _ = setup
+    m, bcs, p, ic, _ = case
+    R0 = residual(m, bcs, p, ic, U)
+    rng = np.random.default_rng(1)
+    errs, kinks = [], 0
+    for k in _large(rng, Ub.ravel(), 24):
+        c, q = divmod(int(k), 4)
+        sc = max(abs(U[c, q]), 1e-3)
+        e = np.zeros_like(U)
+        e[c, q] = sc
+
+        def g(s):
+            return float(np.sum(lam * (residual(m, bcs, p, ic, U + s * e) - R0))) / sc
+
+        err, h, kind, kink = fd_sweep(g, Ub[c, q], STEPS)
+        if kink:
+            kinks += 1
+            continue
+        errs.append(err / abs(Ub[c, q]))
+        assert err <= 5e-6 * abs(Ub[c, q]), (name, c, q, Ub[c, q], err, h, kind)
+    assert kinks <= len(errs) // 3, (name, kinks, len(errs))
+    assert np.median(errs) <= 1e-8, (name, np.median(errs))
+
+
+def test_node_entries_fd(setup):
+    name, case, U, lam, _, Xb = setup
+    m, bcs, p, ic, _ = case
+    R0 = residual(m, bcs, p, ic, U)
+    fe = m.node_xy[m.face_nodes[:, 1]] - m.node_xy[m.face_nodes[:, 0]]
+    elen = np.hypot(fe[:, 0], fe[:, 1])
+    rng = np.random.default_rng(1)
+    errs, kinks = [], 0
+    for k in _large(rng, Xb.ravel(), 24):
+        n, d = divmod(int(k), 2)
+        sc = float(elen[(m.face_nodes == n).any(axis=1)].min())
+        e = np.zeros_like(m.node_xy)
+        e[n, d] = sc
+
+        def g(s):
+            return float(np.sum(lam * (residual(m.with_nodes(m.node_xy + s * e), bcs, p, ic, U) - R0))) / sc
+
+        err, h, kind, kink = fd_sweep(g, Xb[n, d], STEPS)
+        if kink:
+            kinks += 1
+            continue
+        errs.append(err / abs(Xb[n, d]))
+        assert err <= 5e-6 * abs(Xb[n, d]), (name, n, d, Xb[n, d], err, h, kind)
+    assert kinks <= len(errs) // 3, (name, kinks, len(errs))
+    assert np.median(errs) <= 1e-8, (name, np.median(errs))
 
 
 def test_linear_in_lambda(setup):
@@ -156,23 +181,26 @@ def test_wall_forces_gradient():
     g = FVKSolver(m, bcs, p, ic)
     g.set_field(U)
     Ub, Xb = g.wall_forces_vjp(rho, u, v, 1.0, w_cl=1.0, w_cd=0.0)
+    c0 = cl(m, U)
     wall_cells = np.unique(m.face_cells[m.face_patch == 0, 0])
     wall_nodes = np.unique(m.face_nodes[m.face_patch == 0])
+    fe = m.node_xy[m.face_nodes[:, 1]] - m.node_xy[m.face_nodes[:, 0]]
+    elen = np.hypot(fe[:, 0], fe[:, 1])
     rng = np.random.default_rng(3)
     for c in rng.choice(wall_cells, 8, replace=False):
         q = int(rng.integers(0, 4))
-        Up, Um = U.copy(), U.copy()
-        Up[c, q] += H
-        Um[c, q] -= H
-        fd = (cl(m, Up) - cl(m, Um)) / (2 * H)
-        assert abs(fd - Ub[c, q]) <= 1e-6 * max(1.0, abs(Ub[c, q])), (c, q, fd, Ub[c, q])
+        sc = abs(U[c, q])
+        e = np.zeros_like(U)
+        e[c, q] = sc
+        err, h, kind = best_fd(lambda s: (cl(m, U + s * e) - c0) / sc, Ub[c, q], STEPS)
+        assert err <= 1e-7 * abs(Ub[c, q]), (c, q, Ub[c, q], err, h, kind)
     for n in rng.choice(wall_nodes, 8, replace=False):
+        sc = float(elen[(m.face_nodes == n).any(axis=1)].min())
         for d in range(2):
-            Xp, Xm = m.node_xy.copy(), m.node_xy.copy()
-            Xp[n, d] += H
-            Xm[n, d] -= H
-            fd = (cl(m.with_nodes(Xp), U) - cl(m.with_nodes(Xm), U)) / (2 * H)
-            assert abs(fd - Xb[n, d]) <= 1e-6 * max(1.0, abs(Xb[n, d])), (n, d, fd, Xb[n, d])
+            e = np.zeros_like(m.node_xy)
+            e[n, d] = sc
+            err, h, kind = best_fd(lambda s: (cl(m.with_nodes(m.node_xy + s * e), U) - c0) / sc, Xb[n, d], STEPS)
+            assert err <= 1e-7 * abs(Xb[n, d]), (n, d, Xb[n, d], err, h, kind)
 
 
 def test_exact_flux_rejected():

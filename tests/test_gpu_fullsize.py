@@ -90,3 +90,74 @@ def test_config2_variants_identical_after_100_steps(config2):
         assert np.array_equal(f, out[0][0])
         assert np.allclose(r, out[0][1], rtol=1e-12, atol=0.0)
         assert np.array_equal(dt, out[0][2])
+
+
+def lsq_condition(m):
+    """Condition number of each cell's LSQ normal matrix sum_k w_k d_k d_k^T
+    (w = 1/|d|^2, d to the neighbour's centroid or the boundary face's
+    midpoint)."""
+    mid, _, _, _ = m.face_geometry()
+    fl, fr = m.face_cells[:, 0], m.face_cells[:, 1]
+    inner = fr >= 0
+    cells = np.concatenate([fl, fr[inner]])
+    q = np.concatenate([np.where(inner[:, None], m.cell_xy[np.maximum(fr, 0)], mid), m.cell_xy[fl[inner]]])
+    d = q - m.cell_xy[cells]
+    w = 1.0 / np.einsum("ij,ij->i", d, d)
+    M = np.zeros((m.n_cells, 3))
+    np.add.at(M, cells, np.stack([w * d[:, 0] ** 2, w * d[:, 0] * d[:, 1], w * d[:, 1] ** 2], axis=1))
+    tr, det = M[:, 0] + M[:, 2], M[:, 0] * M[:, 2] - M[:, 1] ** 2
+    disc = np.sqrt(np.maximum(tr * tr / 4 - det, 0.0))
+    return (tr / 2 + disc) / np.maximum(tr / 2 - disc, 1e-300)
+
+
+def test_config2_adjoint_dot_product_full_size(config2):
+    """The config-4 adjoint (VJP of the config-2 residual, state and node
+    coordinates) on the full 1,048,576-cell mesh at a state 200 steps from
+    the free stream, against the oracle's forward-mode derivative by the
+    dot-product identity <lambda, J v> = <J^T lambda, v>.
+
+    The O-grid's wall layers are split quads of aspect ratio up to 1e8: the
+    three LSQ points of such a triangle are nearly collinear, its normal
+    matrix has a condition number up to 1e13, and the residual's double
+    evaluation keeps few digits there (the reference's formula, kept for
+    parity).  The exact side is therefore the forward derivative evaluated
+    in long double (JVP_ld), and the identity is asserted
+      * over the whole mesh to the double evaluation's own rounding:
+        |VJP - JVP_ld| <= 1e-10 |JVP_ld| + 10 |JVP - JVP_ld|;
+      * at 1e-10 relative for directions supported three rings clear of the
+        cells whose LSQ condition number exceeds 1e4."""
+    from oracle.pyoracle import residual_jvp
+    m, bcs, p, ic = config2
+    g = FVKSolver(m, bcs, p, ic)
+    g.step(200)
+    U = g.field()
+    rng = np.random.default_rng(0)
+    lam = rng.standard_normal(U.shape)
+    Ub, Xb = g.residual_vjp(lam, want_x=True)
+    vU = rng.standard_normal(U.shape) * np.abs(U).mean(axis=0)
+    vX = rng.standard_normal(m.node_xy.shape)
+    R = residual_jvp(m, bcs, p, U)[0]
+    assert np.abs(R - g.residual_eval()).max() <= 1e-10 * np.abs(R).max()
+
+    def check(label, vu, vx, slack):
+        jv = residual_jvp(m, bcs, p, U, vu, vx)[1]
+        jl = residual_jvp(m, bcs, p, U, vu, vx, long_double=True)[1]
+        exact, exact_ld = float(np.sum(lam * jv)), float(np.sum(lam * jl))
+        got = float(np.sum(Ub * vu)) if vu is not None else float(np.sum(Xb * vx))
+        noise = abs(exact - exact_ld)
+        print(f"{label}: VJP {got:.12g}  JVP {exact:.12g}  JVP_ld {exact_ld:.12g}  "
+              f"|VJP-JVP_ld|/|JVP_ld| {abs(got - exact_ld) / abs(exact_ld):.2e}  "
+              f"|JVP-JVP_ld|/|JVP_ld| {noise / abs(exact_ld):.2e}")
+        assert abs(got - exact_ld) <= 1e-10 * abs(exact_ld) + slack * noise, label
+
+    check("state, whole mesh", vU, None, 10.0)
+    check("nodes, whole mesh", None, vX, 10.0)
+    near = np.zeros(m.n_cells, dtype=bool)
+    near[m.cell_rings(np.nonzero(lsq_condition(m) > 1e4)[0], 3)] = True
+    keep_nodes = np.ones(m.n_nodes, dtype=bool)
+    owner = np.repeat(np.arange(m.n_cells), np.diff(m.cell_node_ptr))
+    keep_nodes[m.cell_nodes[near[owner]]] = False
+    print(f"well-conditioned: {(~near).sum()} of {m.n_cells} cells, {keep_nodes.sum()} of {m.n_nodes} nodes")
+    assert (~near).sum() > m.n_cells // 4
+    check("state, well-conditioned", vU * (~near)[:, None], None, 0.0)
+    check("nodes, well-conditioned", None, vX * keep_nodes[:, None], 0.0)

@@ -7,15 +7,18 @@ torch = pytest.importorskip("torch")  # imported at collection, outside any test
 
 from finitevolume_amd import abi  # noqa: E402
 from finitevolume_amd.torch_adjoint import ResidualFunction  # noqa: E402
-from test_gpu_adjoint import H, developed_state, gpu_vjp, naca_case  # noqa: E402
+from cases import adjoint_naca as naca_case, developed_state  # noqa: E402
+from oracle.pyoracle import residual_jvp  # noqa: E402
+from test_gpu_adjoint import gpu_vjp  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
 
 def test_torch_autograd_function():
-    """backward() of the wrapper returns the library's VJP, and central
-    differences (step 1e-7) of forward() along joint state and node
-    directions agree with it."""
+    """backward() of the wrapper returns the library's VJP; forward()
+    returns the residual, and along joint state and node directions the
+    backward product agrees with the oracle's forward-mode derivative
+    (dot-product identity, 1e-10 relative)."""
     case = naca_case(abi.FLUX_HLLC, 2)
     m, bcs, p, ic, _ = case
     U = developed_state(case)
@@ -27,16 +30,14 @@ def test_torch_autograd_function():
     Ub, Xb = gpu_vjp(case, U, lam)
     assert np.array_equal(Ut.grad.numpy(), Ub)
     assert np.array_equal(Xt.grad.numpy(), Xb)
+    with torch.no_grad():
+        r = ResidualFunction.apply(torch.from_numpy(U), torch.from_numpy(m.node_xy), (m, bcs, p, ic)).numpy()
     rng = np.random.default_rng(4)
     for _ in range(3):
         V = rng.standard_normal(U.shape) * np.abs(U).mean(axis=0)
         W = rng.standard_normal(m.node_xy.shape) * 1e-2
-        def J(h):
-            with torch.no_grad():
-                r = ResidualFunction.apply(torch.from_numpy(U + h * V), torch.from_numpy(m.node_xy + h * W),
-                                           (m, bcs, p, ic))
-            return float((r * torch.from_numpy(lam)).sum())
-        fd = (J(H) - J(-H)) / (2 * H)
+        R0, JV = residual_jvp(m, bcs, p, U, V, W)
+        assert np.abs(r - R0).max() <= 1e-10 * np.abs(R0).max()
+        exact = float(np.sum(lam * JV))
         ad = float(np.sum(Ub * V) + np.sum(Xb * W))
-        scale = float(np.sum(np.abs(Ub * V)) + np.sum(np.abs(Xb * W)))
-        assert abs(fd - ad) <= 1e-6 * scale, (fd, ad)
+        assert abs(exact - ad) <= 1e-10 * abs(exact), (exact, ad)
