@@ -141,6 +141,11 @@ class RunConfigC(C.Structure):
     ]
 
 
+class AdjointReportC(C.Structure):
+    """fvk_adjoint_report (include/fvk.h)."""
+    _fields_ = [("iterations", C.c_int32), ("residual_first", C.c_double), ("residual_final", C.c_double)]
+
+
 class RunResultC(C.Structure):
     _fields_ = [
         ("resume_start", C.c_int64), ("last_step", C.c_int64), ("status", C.c_int32),

@@ -27,6 +27,8 @@
 //            adjoints onto its vertices
 //   nodes    (node coordinates) per node: gathers its cells' vertex adjoints
 #include <algorithm>
+#include <cfloat>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -47,6 +49,9 @@ struct AdjointWork {
     double2 *mb = nullptr, *pb = nullptr, *xcv = nullptr, *xbar = nullptr, *node_xy = nullptr;
     int *poly = nullptr, *pslot = nullptr;
     int *cvptr = nullptr, *cvent = nullptr;
+    // adjoint solve: dJ/dU, the RK stage base, the pseudo-time scaling, residual partials
+    double4 *sg = nullptr, *slam0 = nullptr;
+    double *sP = nullptr, *spart = nullptr;
     double4* lsqm = nullptr;  // per cell: LSQ normal matrix m00, m01, m11 and its determinant
     double2* lsqd = nullptr;  // per ELL slot: offset (dx, dy) to the LSQ neighbour point
 };
@@ -826,29 +831,20 @@ void launch_faces(fvk_solver* s, const AdjArgs& a, bool wx) {
 
 }  // namespace
 
-int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar) {
+namespace {
+
+int check_supported(fvk_solver* s, bool wx) {
     if (s->p.equation != FVK_EQ_EULER)
         return set_error(FVK_ERR_UNSUPPORTED, "the residual VJP is built for the Euler equations");
     if (s->p.flux_scheme == FLUX_EXACT)
         return set_error(FVK_ERR_UNSUPPORTED, "no adjoint of the exact Riemann flux (use Rusanov or HLLC)");
-    const bool wx = X_bar != nullptr;
     if (wx && s->periodic)
         return set_error(FVK_ERR_UNSUPPORTED, "node-coordinate adjoint of a periodic mesh (its cells are not closed polygons)");
-    int rc = ensure_work(s, wx);
-    if (rc != FVK_OK) return rc;
+    return FVK_OK;
+}
+
+AdjArgs make_args(fvk_solver* s) {
     AdjointWork* w = s->adj;
-    const int64_t nc = s->nc, ld = s->mesh.ld;
-    {
-        std::vector<double4> h(ld, double4{0, 0, 0, 0});
-        for (int64_t i = 0; i < nc; ++i) {
-            const double* l = lambda + 4 * (int64_t)s->cell_new2old[i];
-            h[i] = double4{l[0], l[1], l[2], l[3]};
-        }
-        rc = hip_check(hipMemcpyAsync(w->lam, h.data(), sizeof(double4) * ld, hipMemcpyHostToDevice, s->stream),
-                       "adjoint lambda upload");
-        if (rc == FVK_OK) rc = hip_check(hipStreamSynchronize(s->stream), "adjoint lambda upload");
-        if (rc != FVK_OK) return rc;
-    }
     AdjArgs a{};
     a.m = s->mesh;
     a.U = reinterpret_cast<const double4*>(s->U[s->cur]);
@@ -866,11 +862,16 @@ int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_ba
     a.order = s->p.order == 2 ? 2 : 1;
     a.grad = a.order == 2 && s->p.limiter != FVK_LIMITER_FIRST_ORDER;
     a.nn = (int)s->nn;
+    return a;
+}
+
+// the VJP's passes over lambda already in w->lam (device order): U_bar into
+// w->ubar and, with wx, X_bar into w->xbar.  The forward gradients the
+// passes read (s->grad) must be those of the current state.
+int run_passes(fvk_solver* s, bool wx) {
+    AdjointWork* w = s->adj;
+    const AdjArgs a = make_args(s);
     hipEvent_t ev = nullptr;
-    if (a.order == 2) {
-        rc = launch_euler_gradients(s);
-        if (rc != FVK_OK) return rc;
-    }
     timer_begin(s, "adj_face", &ev);
     if (a.order == 2) launch_faces<2>(s, a, wx);
     else launch_faces<1>(s, a, wx);
@@ -881,27 +882,217 @@ int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_ba
         case 9: launch_cells<9>(s, a, wx); break;
         default: launch_cells<kMaxEulerDegree>(s, a, wx); break;
     }
-    rc = hip_check(hipGetLastError(), "adjoint launch");
-    if (rc != FVK_OK) return rc;
-    std::vector<double4> hu(ld);
-    rc = hip_check(hipMemcpyAsync(hu.data(), w->ubar, sizeof(double4) * ld, hipMemcpyDeviceToHost, s->stream),
-                   "adjoint copy");
-    std::vector<double2> hx;
-    if (rc == FVK_OK && wx) {
-        hx.resize(s->nn);
-        rc = hip_check(hipMemcpyAsync(hx.data(), w->xbar, sizeof(double2) * s->nn, hipMemcpyDeviceToHost, s->stream),
-                       "adjoint copy");
+    return hip_check(hipGetLastError(), "adjoint launch");
+}
+
+int upload_cells(fvk_solver* s, double4* dst, const double* src) {
+    const int64_t nc = s->nc, ld = s->mesh.ld;
+    std::vector<double4> h(ld, double4{0, 0, 0, 0});
+    for (int64_t i = 0; i < nc; ++i) {
+        const double* l = src + 4 * (int64_t)s->cell_new2old[i];
+        h[i] = double4{l[0], l[1], l[2], l[3]};
     }
-    int rs = hip_check(hipStreamSynchronize(s->stream), "adjoint");
+    int rc = hip_check(hipMemcpyAsync(dst, h.data(), sizeof(double4) * ld, hipMemcpyHostToDevice, s->stream),
+                       "adjoint upload");
+    if (rc == FVK_OK) rc = hip_check(hipStreamSynchronize(s->stream), "adjoint upload");
+    return rc;
+}
+
+int download_cells(fvk_solver* s, double* dst, const double4* src) {
+    const int64_t nc = s->nc, ld = s->mesh.ld;
+    std::vector<double4> h(ld);
+    int rc = hip_check(hipMemcpyAsync(h.data(), src, sizeof(double4) * ld, hipMemcpyDeviceToHost, s->stream),
+                       "adjoint copy");
+    int rs = hip_check(hipStreamSynchronize(s->stream), "adjoint copy");
     if (rc == FVK_OK) rc = rs;
     if (rc != FVK_OK) return rc;
     for (int64_t i = 0; i < nc; ++i) {
-        double* o = U_bar + 4 * (int64_t)s->cell_new2old[i];
-        o[0] = hu[i].x; o[1] = hu[i].y; o[2] = hu[i].z; o[3] = hu[i].w;
+        double* o = dst + 4 * (int64_t)s->cell_new2old[i];
+        o[0] = h[i].x; o[1] = h[i].y; o[2] = h[i].z; o[3] = h[i].w;
     }
-    if (wx)
-        for (int64_t n = 0; n < s->nn; ++n) { X_bar[2 * n] = hx[n].x; X_bar[2 * n + 1] = hx[n].y; }
     return FVK_OK;
+}
+
+int download_nodes(fvk_solver* s, double* dst) {
+    std::vector<double2> hx(s->nn);
+    int rc = hip_check(hipMemcpyAsync(hx.data(), s->adj->xbar, sizeof(double2) * s->nn, hipMemcpyDeviceToHost,
+                                      s->stream), "adjoint copy");
+    int rs = hip_check(hipStreamSynchronize(s->stream), "adjoint copy");
+    if (rc == FVK_OK) rc = rs;
+    if (rc != FVK_OK) return rc;
+    for (int64_t n = 0; n < s->nn; ++n) { dst[2 * n] = hx[n].x; dst[2 * n + 1] = hx[n].y; }
+    return FVK_OK;
+}
+
+// ---- adjoint solve: pseudo-time iteration of  dlambda/dtau = P (J^T lambda + g) ----
+
+// P_c = dt_c / V_c with dt_c the primal's local time step at the current
+// state (the minimum over the cell's faces of CFL len / (|V.n| + c)): the
+// transpose system is iterated with the primal's own preconditioner
+template <int D>
+__global__ __launch_bounds__(kAdjBlock) void k_adj_precond(MeshView m, const double4* U, double gamma, double cfl,
+                                                          double* P) {
+    const int c = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (c >= m.nc) return;
+    const int64_t ld = m.ld;
+    double d = DBL_MAX;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        const int2 e = m.ell[k * ld + c];
+        if (e.x < 0) continue;
+        const int f = e.x >> 1;
+        const double nx = m.fnx[f], ny = m.fny[f];
+        auto wave = [&](int i) {
+            const State u = st4(U[i]);
+            const double vn = fdiv(u.ru, u.r) * nx + fdiv(u.rv, u.r) * ny;
+            return fabs(vn) + sound_speed(u, gamma);
+        };
+        const int l = m.fl[f], r = m.fr[f];
+        double smax = wave(l), len;
+        if (r >= 0) {
+            smax = fmax(smax, wave(r));
+            len = fdiv(fmin(m.vol[l], m.vol[r]), m.fa[f]);
+        } else {
+            len = fdiv(m.vol[l], m.fa[f]);
+        }
+        d = fmin(d, fdiv(cfl * len, smax));
+    }
+    P[c] = fdiv(d, m.vol[c]);
+}
+
+// one SSP-RK stage: lam <- a lam0 + b (lam + P (ubar + g)); with partials,
+// per-block sums of |ubar + g|^2 (the adjoint residual of the stage's lam)
+__global__ __launch_bounds__(kAdjBlock) void k_adj_rk(int nc, const double4* lam0, double4* lam, const double4* ubar,
+                                                     const double4* g, const double* P, double a, double b,
+                                                     double* partials) {
+    __shared__ double red[kAdjBlock];
+    const int c = blockIdx.x * kAdjBlock + threadIdx.x;
+    double sq = 0.0;
+    if (c < nc) {
+        const double4 u = ubar[c], gg = g[c], l = lam[c], l0 = lam0[c];
+        const double4 r{u.x + gg.x, u.y + gg.y, u.z + gg.z, u.w + gg.w};
+        const double pc = P[c];
+        lam[c] = double4{a * l0.x + b * (l.x + pc * r.x), a * l0.y + b * (l.y + pc * r.y),
+                         a * l0.z + b * (l.z + pc * r.z), a * l0.w + b * (l.w + pc * r.w)};
+        sq = r.x * r.x + r.y * r.y + r.z * r.z + r.w * r.w;
+    }
+    if (!partials) return;
+    red[threadIdx.x] = sq;
+    __syncthreads();
+    for (int s2 = kAdjBlock / 2; s2 > 0; s2 >>= 1) {
+        if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+}  // namespace
+
+int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar) {
+    const bool wx = X_bar != nullptr;
+    int rc = check_supported(s, wx);
+    if (rc != FVK_OK) return rc;
+    rc = ensure_work(s, wx);
+    if (rc != FVK_OK) return rc;
+    AdjointWork* w = s->adj;
+    rc = upload_cells(s, w->lam, lambda);
+    if (rc != FVK_OK) return rc;
+    if (s->p.order == 2) {
+        rc = launch_euler_gradients(s);
+        if (rc != FVK_OK) return rc;
+    }
+    rc = run_passes(s, wx);
+    if (rc != FVK_OK) return rc;
+    rc = download_cells(s, U_bar, w->ubar);
+    if (rc == FVK_OK && wx) rc = download_nodes(s, X_bar);
+    return rc;
+}
+
+int adjoint_solve(fvk_solver* s, const double* g_host, double cfl, int max_iter, double tol, int check_every,
+                  double* lambda_out, double* X_bar, fvk_adjoint_report* rep) {
+    const bool wx = X_bar != nullptr;
+    int rc = check_supported(s, wx);
+    if (rc != FVK_OK) return rc;
+    if (max_iter < 0 || !(cfl > 0.0) || check_every < 1) return set_error(FVK_ERR_INVALID, "adjoint solve arguments");
+    rc = ensure_work(s, wx);
+    if (rc != FVK_OK) return rc;
+    AdjointWork* w = s->adj;
+    const int64_t nc = s->nc, ld = s->mesh.ld;
+    const int gc = grid_of(nc);
+    if (!w->sg) {
+        w->sg = alloc_n<double4>(s, ld);
+        w->slam0 = alloc_n<double4>(s, ld);
+        w->sP = alloc_n<double>(s, ld);
+        w->spart = alloc_n<double>(s, gc);
+        if (!w->sg || !w->slam0 || !w->sP || !w->spart)
+            return set_error(FVK_ERR_NOMEM, "adjoint solve: device allocation failed");
+    }
+    double4 *g = w->sg, *lam0 = w->slam0;
+    double *P = w->sP, *part = w->spart;
+    rc = upload_cells(s, g, g_host);
+    if (rc == FVK_OK) rc = hip_check(hipMemsetAsync(w->lam, 0, sizeof(double4) * ld, s->stream), "adjoint init");
+    if (rc == FVK_OK && s->p.order == 2) rc = launch_euler_gradients(s);
+    if (rc != FVK_OK) return rc;
+    const double4* U = reinterpret_cast<const double4*>(s->U[s->cur]);
+    switch (w->D) {
+        case 3: k_adj_precond<3><<<gc, kAdjBlock, 0, s->stream>>>(s->mesh, U, s->p.gamma, cfl, P); break;
+        case 4: k_adj_precond<4><<<gc, kAdjBlock, 0, s->stream>>>(s->mesh, U, s->p.gamma, cfl, P); break;
+        case 9: k_adj_precond<9><<<gc, kAdjBlock, 0, s->stream>>>(s->mesh, U, s->p.gamma, cfl, P); break;
+        default: k_adj_precond<kMaxEulerDegree><<<gc, kAdjBlock, 0, s->stream>>>(s->mesh, U, s->p.gamma, cfl, P);
+    }
+    rc = hip_check(hipGetLastError(), "adjoint solve launch");
+    if (rc != FVK_OK) return rc;
+    // SSP-RK3 in pseudo time (the primal's integrator and local step)
+    const double A[3] = {0.0, 0.75, 1.0 / 3.0}, B[3] = {1.0, 0.25, 2.0 / 3.0};
+    std::vector<double> hp(gc);
+    double r0 = -1.0, rn = -1.0;
+    int it = 0;
+    for (; it < max_iter; ++it) {
+        const bool check = it % check_every == 0 || it + 1 == max_iter;
+        rc = hip_check(hipMemcpyAsync(lam0, w->lam, sizeof(double4) * ld, hipMemcpyDeviceToDevice, s->stream),
+                       "adjoint solve copy");
+        for (int st = 0; st < 3 && rc == FVK_OK; ++st) {
+            rc = run_passes(s, false);
+            if (rc != FVK_OK) break;
+            k_adj_rk<<<gc, kAdjBlock, 0, s->stream>>>((int)nc, lam0, w->lam, w->ubar, g, P, A[st], B[st],
+                                                     st == 0 && check ? part : nullptr);
+            rc = hip_check(hipGetLastError(), "adjoint solve launch");
+        }
+        if (rc != FVK_OK) return rc;
+        if (check) {
+            rc = hip_check(hipMemcpyAsync(hp.data(), part, sizeof(double) * gc, hipMemcpyDeviceToHost, s->stream),
+                           "adjoint solve residual");
+            if (rc == FVK_OK) rc = hip_check(hipStreamSynchronize(s->stream), "adjoint solve");
+            if (rc != FVK_OK) return rc;
+            double sq = 0.0;
+            for (double v : hp) sq += v;
+            rn = std::sqrt(sq);
+            if (r0 < 0.0) r0 = rn;
+            if (!std::isfinite(rn)) return set_error(FVK_ERR_DIVERGED, "adjoint solve diverged (non-finite residual)");
+            if (rn <= tol * r0) { ++it; break; }
+        }
+    }
+    // the residual of the final lambda, and its X adjoint
+    rc = run_passes(s, wx);
+    if (rc != FVK_OK) return rc;
+    k_adj_rk<<<gc, kAdjBlock, 0, s->stream>>>((int)nc, w->lam, lam0, w->ubar, g, P, 1.0, 0.0, part);
+    rc = hip_check(hipGetLastError(), "adjoint solve launch");
+    if (rc == FVK_OK)
+        rc = hip_check(hipMemcpyAsync(hp.data(), part, sizeof(double) * gc, hipMemcpyDeviceToHost, s->stream),
+                       "adjoint solve residual");
+    if (rc == FVK_OK) rc = hip_check(hipStreamSynchronize(s->stream), "adjoint solve");
+    if (rc != FVK_OK) return rc;
+    double sq = 0.0;
+    for (double v : hp) sq += v;
+    rn = std::sqrt(sq);
+    if (rep) {
+        rep->iterations = it;
+        rep->residual_first = r0 < 0.0 ? rn : r0;
+        rep->residual_final = rn;
+    }
+    if (lambda_out) rc = download_cells(s, lambda_out, w->lam);
+    if (rc == FVK_OK && wx) rc = download_nodes(s, X_bar);
+    return rc;
 }
 
 }  // namespace fvk

@@ -862,6 +862,20 @@ int fvk_wall_forces_vjp(fvk_solver* s, double rho_ref, double vx_ref, double vy_
     return FVK_OK;
 }
 
+int fvk_adjoint_solve_forces(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref,
+                             double w_cl, double w_cd, double cfl, int32_t max_iter, double tol, int32_t check_every,
+                             double* dJ_dX, double* lambda_out, fvk_adjoint_report* report) {
+    if (!s || !dJ_dX) return set_error(FVK_ERR_INVALID, "NULL argument");
+    if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
+    std::vector<double> g(4 * (size_t)s->nc), jx(2 * (size_t)s->nn);
+    int rc = fvk_wall_forces_vjp(s, rho_ref, vx_ref, vy_ref, length_ref, w_cl, w_cd, g.data(), jx.data());
+    if (rc != FVK_OK) return rc;
+    rc = fvk::adjoint_solve(s, g.data(), cfl, max_iter, tol, check_every, lambda_out, dJ_dX, report);
+    if (rc != FVK_OK) return rc;
+    for (size_t i = 0; i < jx.size(); ++i) dJ_dX[i] += jx[i];
+    return FVK_OK;
+}
+
 int fvk_set_kernel_timing(fvk_solver* s, int32_t enable) {
     if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
     if (int rc = fvk::settle(s); rc != FVK_OK) return rc;

@@ -635,7 +635,7 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
 // small enough that four workgroups share a CU's LDS) it is loaded once the
 // thread's gradients are done, before the barrier that ends them; triangles
 // with the HLLC and Rusanov fluxes fit so without spills.
-template <int D, int SCHEME, int ORDER, int WPE>
+template <int D, int SCHEME, int ORDER, int WPE, bool BATCH>
 __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t) {
     constexpr bool PF = WPE < 4;
     const MeshView& m = a.m;
@@ -675,17 +675,22 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
     // overlaps it: the local ELL of cell tid (its gradient, and its gather
     // when owned), the gradient mesh data of cell tid and (PF) the mesh data
     // of the first two faces of this thread
+    // (LEAN: cells wider than triangles read their slots where they use them
+    // instead, which keeps nine-slot polygons within half the register file)
+    constexpr bool LEAN = D > 3;
     int pen[D];
-    double4 peg[D], piv{0, 0, 0, 0};
-    if (tid < (ORDER == 2 ? ng : n)) {
+    double4 peg[LEAN ? 1 : D], piv{0, 0, 0, 0};
+    if (!LEAN && tid < (ORDER == 2 ? ng : n)) {
 #pragma unroll
         for (int k = 0; k < D; ++k) pen[k] = te[k * MG + tid];
     }
     // tile-ordered geometry (second order only): no index load first
     const double4* tg = ORDER == 2 ? m.tegeo + (int64_t)t * D * MG : nullptr;
     if (ORDER == 2 && tid < ng) {
+        if (!LEAN) {
 #pragma unroll
-        for (int k = 0; k < D; ++k) peg[k] = tg[k * MG + tid];
+            for (int k = 0; k < D; ++k) peg[k] = tg[k * MG + tid];
+        }
         piv = m.tclsq[(int64_t)t * MG + tid];
     }
     int pfd[2] = {0, 0};
@@ -720,8 +725,14 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
     // increments of the tile faces on their sides
     if (ORDER == 2) {
         const double2* rec = m.trec ? m.trec + (int64_t)t * D * MG : nullptr;
-        if (tid < ng) tile_grad<D>(m, g, a.limiter, L, tid, pen, peg, piv, f0, rec ? rec + tid : nullptr);
-        for (int j = tid + bs; j < ng; j += bs) {
+        if constexpr (LEAN) {
+            for (int j = tid; j < ng; j += bs)
+                tile_grad_lean<D>(m, g, a.limiter, L, j, te + j, tg + j, j == tid ? piv : m.tclsq[(int64_t)t * MG + j],
+                                  f0, rec ? rec + j : nullptr);
+        } else {
+            if (tid < ng) tile_grad<D>(m, g, a.limiter, L, tid, pen, peg, piv, f0, rec ? rec + tid : nullptr);
+        }
+        for (int j = tid + bs; !LEAN && j < ng; j += bs) {
             int en[D];
             double4 eg[D];
 #pragma unroll
@@ -740,9 +751,9 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
             }
         }
     }
-    // the update's inputs of owned cell tid
+    // the update's inputs of owned cell tid (LEAN: after the face pass)
     UpdateInputs upd{};
-    if (tid < n) upd = load_update_inputs(a, c0 + tid);
+    if (!LEAN && tid < n) upd = load_update_inputs(a, c0 + tid);
     __syncthreads();
 
     // 3. fluxes of the tile's faces
@@ -760,6 +771,11 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
 
     // 4. gather in Cell::faces order, owner minus, and the update (as k_e_gather)
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    if (LEAN && tid < n) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) pen[k] = te[k * MG + tid];
+        upd = load_update_inputs(a, c0 + tid);
+    }
     if (tid < n) {
         double dmin = DBL_MAX;
 #pragma unroll
@@ -803,12 +819,12 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
         if (tid == 0) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                if (a.tile_list) a.tile_partials[(int64_t)q * m.ntiles + t] = lds[q * bs];
+                if (BATCH) a.tile_partials[(int64_t)q * m.ntiles + t] = lds[q * bs];
                 else a.partials[q * gridDim.x + blockIdx.x] = lds[q * bs];
             }
         }
     }
-    if (a.face_dt && !a.tile_list) {  // batched cases keep no step dt
+    if (!BATCH && a.face_dt) {  // batched cases keep no step dt
         __syncthreads();
         lds[tid] = cand_min;
         __syncthreads();
@@ -835,7 +851,8 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
     extern __shared__ double lds[];
     const int it = logical_block();
     if (it >= *a.n_tile_list) return;
-    tile_body<D, SCHEME, ORDER, WPE>(a, lds, a.tile_list[it]);
+    // the entry is the workgroup's own: keep it (and the tile's ranges) scalar
+    tile_body<D, SCHEME, ORDER, WPE, true>(a, lds, __builtin_amdgcn_readfirstlane(a.tile_list[it]));
 }
 
 template <int D, int SCHEME, int ORDER, int WPE>
@@ -853,7 +870,7 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
         }
         --t;
     }
-    tile_body<D, SCHEME, ORDER, WPE>(a, lds, t);
+    tile_body<D, SCHEME, ORDER, WPE, false>(a, lds, t);
 }
 
 // batched cases on tiles: one block per case, the case's residual norms from
@@ -969,10 +986,12 @@ struct Launch {
             if (a.tile_list) cases<<<grid, s->block, lds, s->stream>>>(a);
             else single<<<grid, s->block, lds, s->stream>>>(a);
         };
-        if (four) {
-            if (s->p.flux_scheme == FLUX_HLLC) go(k_e_tile<3, FLUX_HLLC, 2, 4>, k_e_tile_cases<3, FLUX_HLLC, 2, 4>);
-            else go(k_e_tile<3, FLUX_RUSANOV, 2, 4>, k_e_tile_cases<3, FLUX_RUSANOV, 2, 4>);
-            return;
+        if constexpr (D == 3) {
+            if (four) {
+                if (s->p.flux_scheme == FLUX_HLLC) go(k_e_tile<3, FLUX_HLLC, 2, 4>, k_e_tile_cases<3, FLUX_HLLC, 2, 4>);
+                else go(k_e_tile<3, FLUX_RUSANOV, 2, 4>, k_e_tile_cases<3, FLUX_RUSANOV, 2, 4>);
+                return;
+            }
         }
         switch (s->p.flux_scheme) {
             case FLUX_HLLC: go(k_e_tile<D, FLUX_HLLC, ORDER, 1>, k_e_tile_cases<D, FLUX_HLLC, ORDER, 1>); break;

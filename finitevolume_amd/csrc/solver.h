@@ -272,6 +272,8 @@ int settle(const fvk_solver* s);
 int launch_norm_finalize(fvk_solver* s, int nblocks, int nv, int64_t hist_idx, int slot, double fixed_dt,
                          bool dt_row = false);
 int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar);
+int adjoint_solve(fvk_solver* s, const double* g_host, double cfl, int max_iter, double tol, int check_every,
+                  double* lambda_out, double* X_bar, fvk_adjoint_report* rep);
 void adjoint_release(fvk_solver* s);
 int launch_copy_probe(double* dst, const double* src, int64_t n, hipStream_t st);
 // raises a kernel's dynamic LDS limit above the default 64 KB (once per kernel)

@@ -30,10 +30,18 @@ struct TileLds {
 // reconstruct(): r the offset to the face midpoint, frec's value; rec,
 // stride MG, holds it on a periodic mesh, else egeo's equal midpoint
 // offset).  f0: the tile's first face in the tile face arrays.
-template <int D>
-__device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limiter, const TileLds& L, int j,
-                                          const int (&en)[D], const double4 (&eg)[D], const double4& iv, int f0,
-                                          const double2* rec) {
+//
+// EN(k) and EG(k) give slot k's local ELL entry and tile geometry; EG is
+// called again in each of the three slot loops.  With arrays held in
+// registers (tile_grad) they are register reads; tile_grad_lean reads them
+// from memory in each loop instead, so wide cells (Voronoi polygons, up to
+// nine slots) do not hold nine double4 across the function.
+template <int D, bool LEAN, class EnF, class EgF>
+__device__ __forceinline__ void tile_grad_with(const MeshView& m, double g, int limiter, const TileLds& L, int j,
+                                               EnF EN, EgF EG, const double4& iv, int f0, const double2* rec) {
+    int en[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) en[k] = EN(k);
     const int MG = L.MG;
     const State Uc{L.U[j], L.U[MG + j], L.U[2 * MG + j], L.U[3 * MG + j]};
     const double wc[4] = {L.U[j], L.W[j], L.W[MG + j], L.W[2 * MG + j]};
@@ -44,6 +52,7 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         if (en[k] == -1) continue;
+        const double4 egk = EG(k);
         double wn[4];
         if (tell_is_boundary(en[k])) {  // the ghost state; the face from the tile's list or, ring 1, the mesh
             const int x = tell_index(en[k]);
@@ -75,8 +84,8 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const double dphi = wn[q] - wc[q];
-            bx[q] += eg[k].x * dphi;
-            by[q] += eg[k].y * dphi;
+            bx[q] += egk.x * dphi;
+            by[q] += egk.y * dphi;
             wmin[q] = smin(wmin[q], wn[q]);
             wmax[q] = smax(wmax[q], wn[q]);
         }
@@ -93,10 +102,12 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
         VenkatMin vm[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) vm[q] = VenkatMin{1.0, 1.0};
+        if (LEAN) __asm__ volatile("" ::: "memory");  // reload the slot geometry below
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             if (en[k] == -1) continue;
-            const double rx = eg[k].z, ry = eg[k].w;
+            const double4 egk = EG(k);
+            const double rx = egk.z, ry = egk.w;
 #pragma unroll
             for (int q = 0; q < 4; ++q) venkat_pick(vm[q], wmin[q], wmax[q], wc[q], gx[q] * rx + gy[q] * ry, eps2);
         }
@@ -113,15 +124,20 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
         G[2 * q + 1] = lim[q] * gy[q];
     }
     const int MF = L.MF;
+    if (LEAN) __asm__ volatile("" ::: "memory");
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         const int lf = tell_lf(en[k]);
         if (en[k] == -1 || lf < 0) continue;
-        double rx = eg[k].z, ry = eg[k].w;
+        double rx, ry;
         if (rec) {
             const double2 r = rec[k * MG];
             rx = r.x;
             ry = r.y;
+        } else {
+            const double4 egk = EG(k);
+            rx = egk.z;
+            ry = egk.w;
         }
         double* I = L.I + (tell_owner(en[k]) ? 0 : 4 * MF) + lf;
         I[0] = G[0] * rx + G[1] * ry;
@@ -129,6 +145,24 @@ __device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limit
         I[2 * MF] = G[4] * rx + G[5] * ry;
         I[3 * MF] = G[6] * rx + G[7] * ry;
     }
+}
+
+template <int D>
+__device__ __forceinline__ void tile_grad(const MeshView& m, double g, int limiter, const TileLds& L, int j,
+                                          const int (&en)[D], const double4 (&eg)[D], const double4& iv, int f0,
+                                          const double2* rec) {
+    tile_grad_with<D, false>(
+        m, g, limiter, L, j, [&](int k) { return en[k]; }, [&](int k) { return eg[k]; }, iv, f0, rec);
+}
+
+// te, tg: the cell's first slot in the tile ELL and tile geometry (stride MG)
+template <int D>
+__device__ __forceinline__ void tile_grad_lean(const MeshView& m, double g, int limiter, const TileLds& L, int j,
+                                               const int* te, const double4* tg, const double4& iv, int f0,
+                                               const double2* rec) {
+    const int MG = L.MG;
+    tile_grad_with<D, true>(
+        m, g, limiter, L, j, [&](int k) { return te[k * MG]; }, [&](int k) { return tg[k * MG]; }, iv, f0, rec);
 }
 
 // stage local cell j: conserved state (owned and ring-1 cells) and, for

@@ -58,6 +58,9 @@ def lib():
     L.fvk_wall_forces.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
     L.fvk_wall_forces_vjp.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                       dp, dp]
+    L.fvk_adjoint_solve_forces.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, C.c_double, C.c_int32, C.c_double, C.c_int32, dp, dp,
+                                           C.POINTER(abi.AdjointReportC)]
     L.fvk_wall_traction.argtypes = [vp, ip, i64, C.POINTER(abi.WallReferenceC), vp]
     L.fvk_wall_force_reports.argtypes = [vp, ip, i64, C.POINTER(abi.WallReferenceC), vp, vp, C.c_int32,
                                          C.POINTER(C.c_int32)]
@@ -245,6 +248,21 @@ class FVKSolver:
         _check(self.L.fvk_wall_forces_vjp(self.h, rho_ref, vx_ref, vy_ref, length_ref, w_cl, w_cd, abi.as_dptr(Ub),
                                           abi.as_dptr(Xb)))
         return (Ub, Xb) if want_x else Ub
+
+    def adjoint_solve_forces(self, rho_ref, vx_ref, vy_ref, length_ref=1.0, w_cl=1.0, w_cd=0.0, cfl=None,
+                             max_iter=20000, tol=1e-10, check_every=20, want_lambda: bool = False):
+        """dJ/dX [n_nodes, 2] of J = w_cl Cl + w_cd Cd through the flow, from one
+        adjoint solve at the current (converged) state; returns (dJ_dX, report)
+        or (dJ_dX, lambda, report).  cfl defaults to the solver's."""
+        dX = np.empty((self.mesh.n_nodes, 2))
+        lam = np.empty((self.n, 4)) if want_lambda else None
+        rep = abi.AdjointReportC()
+        c = self.params.cfl if cfl is None else cfl
+        _check(self.L.fvk_adjoint_solve_forces(self.h, rho_ref, vx_ref, vy_ref, length_ref, w_cl, w_cd, c, max_iter,
+                                               tol, check_every, abi.as_dptr(dX), abi.as_dptr(lam), C.byref(rep)))
+        r = {"iterations": rep.iterations, "residual_first": rep.residual_first,
+             "residual_final": rep.residual_final}
+        return (dX, lam, r) if want_lambda else (dX, r)
 
     def set_kernel_timing(self, on: bool):
         _check(self.L.fvk_set_kernel_timing(self.h, 1 if on else 0))

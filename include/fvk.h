@@ -26,7 +26,7 @@
  * Entry points with no counterpart in the reference (extensions that
  * reduce to the reference path when switched off, see fvk_params):
  *   fvk_residual_eval, fvk_residual_vjp, fvk_wall_forces, fvk_wall_forces_vjp,
- *   fvk_batch_*.
+ *   fvk_adjoint_solve_forces, fvk_batch_*.
  *
  * Conventions (same as the reference):
  *  - cell and face indices are the caller's (reference) order; the library
@@ -53,7 +53,8 @@ enum fvk_status {
     FVK_ERR_INVALID = 1,     /* bad argument, size mismatch, unknown enum value */
     FVK_ERR_HIP = 2,         /* a HIP runtime call failed */
     FVK_ERR_NOMEM = 3,       /* device or host allocation failed */
-    FVK_ERR_UNSUPPORTED = 4  /* option combination not built */
+    FVK_ERR_UNSUPPORTED = 4, /* option combination not built */
+    FVK_ERR_DIVERGED = 5     /* an iterative solve left the finite numbers */
 };
 
 /* Which solver class the handle stands in for. */
@@ -423,6 +424,28 @@ int fvk_wall_force_reports_noalloc(fvk_solver* s, const int32_t* wall_faces, int
  * node coordinates, X_bar [2*n_nodes]; either may be NULL.  Overwrites both. */
 int fvk_wall_forces_vjp(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref, double w_cl,
                         double w_cd, double* U_bar, double* X_bar);
+
+/* Extension: the shape sensitivity of J = w_cl Cl + w_cd Cd through the
+ * flow, by one adjoint solve at the current state (a converged steady
+ * state of the solver's residual R):
+ *   (dR/dU)^T lambda = -dJ/dU,     dJ/dX = dJ/dX|_U + lambda^T dR/dX.
+ * lambda is iterated on the device in pseudo time, d lambda/d tau =
+ * P ((dR/dU)^T lambda + dJ/dU), with SSP-RK3 and P = dt_c / V_c the
+ * primal's local time step at CFL `cfl` (the transposed operator has the
+ * primal's spectrum, so the primal's stable CFL serves), from lambda = 0,
+ * for at most max_iter iterations, stopping once the adjoint residual
+ * |(dR/dU)^T lambda + dJ/dU| has dropped by `tol` (checked every
+ * check_every iterations).  dJ_dX [2*n_nodes]; lambda_out [4*n_cells]
+ * (optional); report (optional) the iterations and the first and final
+ * residual.  Euler, Rusanov or HLLC, no periodic faces.  A non-finite
+ * residual returns FVK_ERR_DIVERGED. */
+typedef struct fvk_adjoint_report {
+    int32_t iterations;
+    double residual_first, residual_final;
+} fvk_adjoint_report;
+int fvk_adjoint_solve_forces(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref,
+                             double w_cl, double w_cd, double cfl, int32_t max_iter, double tol, int32_t check_every,
+                             double* dJ_dX, double* lambda_out, fvk_adjoint_report* report);
 
 /* Per-kernel device time accumulated while timing is on, measured with HIP
  * events on the solver's stream around each launch.  names is a

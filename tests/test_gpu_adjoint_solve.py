@@ -1,0 +1,107 @@
+"""Shape sensitivity of the pressure lift from one adjoint solve
+(fvk_adjoint_solve_forces) at a converged steady state: NACA 0012 O-grid,
+Mach 0.5, 2 degrees, first-order HLLC (whose steady state converges to
+round-off, so that finite differences of the converged lift are clean).
+
+  * lambda against the exact solution of (dR/dU)^T lambda = -dCl/dU with
+    the Jacobian assembled column by column from the oracle's forward-mode
+    derivative and solved densely;
+  * dCl/dX against dCl/dX|_U + lambda_ref^T dR/dX (the VJP already pinned
+    by the dot-product tests);
+  * dCl/dX at wall nodes against central differences of the lift of the
+    re-converged flow on the moved mesh (step sweep)."""
+import numpy as np
+import pytest
+
+import cases
+from finitevolume_amd import abi
+from finitevolume_amd import mesh as M
+from finitevolume_amd.solver import FVKSolver
+from oracle.pyoracle import residual_jvp
+
+pytestmark = pytest.mark.gpu
+
+MACH, AOA = 0.5, 2.0
+
+
+def setup_case():
+    m = M.naca_ogrid(40, 20, r_far=8.0, ratio=1.2)
+    rho, u, v, p = cases.freestream(MACH, AOA)
+    ff = M.from_primitive(rho, u, v, p, cases.GAMMA)
+    bcs = [abi.make_bc(abi.BC_WALL), abi.make_bc(abi.BC_FARFIELD, ff)]
+    prm = abi.default_params(equation=abi.EQ_EULER, flux_scheme=abi.FLUX_HLLC, cfl=0.8, rk_stages=3, local_dt=1)
+    ic = abi.initial_condition(abi.IC_FREESTREAM, rho=rho, u=u, v=v, p=p)
+    return m, bcs, prm, ic, (rho, u, v)
+
+
+def converge(m, bcs, prm, ic, start=None, steps=9000):
+    g = FVKSolver(m, bcs, prm, ic)
+    if start is not None:
+        g.set_field(start)
+    r = g.step(steps)
+    return g, r
+
+
+@pytest.fixture(scope="module")
+def solved():
+    m, bcs, prm, ic, ref = setup_case()
+    g, r = converge(m, bcs, prm, ic)
+    assert r[-1, 0] <= 1e-12 * r[0, 0], r[-1, 0] / r[0, 0]
+    rho, u, v = ref
+    dX, lam, rep = g.adjoint_solve_forces(rho, u, v, 1.0, w_cl=1.0, w_cd=0.0, max_iter=20000, tol=1e-12,
+                                          want_lambda=True)
+    return m, bcs, prm, ic, ref, g, dX, lam, rep, r[0, 0]
+
+
+def test_adjoint_residual_converged(solved):
+    rep = solved[8]
+    assert rep["residual_final"] <= 1e-11 * rep["residual_first"], rep
+    assert rep["iterations"] < 20000
+
+
+def test_lambda_matches_dense_solve(solved):
+    m, bcs, prm, ic, ref, g, dX, lam, rep, _ = solved
+    U = g.field()
+    n = 4 * m.n_cells
+    Jm = np.empty((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        Jm[:, j] = residual_jvp(m, bcs, prm, U, e.reshape(-1, 4))[1].ravel()
+    rho, u, v = ref
+    gU, gX = g.wall_forces_vjp(rho, u, v, 1.0, w_cl=1.0, w_cd=0.0)
+    lam_ref = np.linalg.solve(Jm.T, -gU.ravel()).reshape(-1, 4)
+    assert np.abs(lam - lam_ref).max() <= 1e-8 * np.abs(lam_ref).max()
+    _, Xb = g.residual_vjp(lam_ref, want_x=True)
+    dX_ref = gX + Xb
+    assert np.abs(dX - dX_ref).max() <= 1e-8 * np.abs(dX_ref).max()
+
+
+def test_shape_sensitivity_matches_reconverged_fd(solved):
+    m, bcs, prm, ic, ref, g, dX, lam, rep, r_start = solved
+    rho, u, v = ref
+    U0 = g.field()
+    wall = np.unique(m.face_nodes[m.face_patch == 0])
+    fe = m.node_xy[m.face_nodes[:, 1]] - m.node_xy[m.face_nodes[:, 0]]
+    elen = np.hypot(fe[:, 0], fe[:, 1])
+    rng = np.random.default_rng(7)
+    # the wall nodes with the largest sensitivities, and a random pair
+    mag = np.hypot(dX[wall, 0], dX[wall, 1])
+    picks = np.concatenate([wall[np.argsort(-mag)[:3]], rng.choice(wall, 2, replace=False)])
+
+    def lift(X):
+        gg, r = converge(m.with_nodes(X), bcs, prm, ic, start=U0, steps=6000)
+        assert r[-1, 0] <= 1e-12 * r_start  # as converged as the unmoved flow
+        return gg.wall_forces(rho, u, v, 1.0)[0]
+
+    for n in picks:
+        sc = float(elen[(m.face_nodes == n).any(axis=1)].min())
+        for d in range(2):
+            best = np.inf
+            for h in (1e-3, 1e-4):
+                Xp, Xm = m.node_xy.copy(), m.node_xy.copy()
+                Xp[n, d] += h * sc
+                Xm[n, d] -= h * sc
+                fd = (lift(Xp) - lift(Xm)) / (2 * h * sc)
+                best = min(best, abs(fd - dX[n, d]))
+            assert best <= 1e-6 * np.abs(dX[wall]).max() + 1e-5 * abs(dX[n, d]), (n, d, dX[n, d], best)
