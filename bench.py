@@ -22,6 +22,11 @@ import time
 
 import numpy as np
 
+# the OpenMP CPU baselines: one thread per core, pinned close (set before any
+# OpenMP runtime is loaded; an explicit setting in the environment wins)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
@@ -417,6 +422,7 @@ def rans_leg(steps, warmup, cpu_seconds, no_cpu, args=None):
                 ref = nc * n2 / t2
                 out["cpu_baseline_reference_3R"] = {
                     "value": round(ref / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores, "kind": "reference",
+                    "threading": cpu_threading(),
                     "sample": f"reference RANSTurbulenceSASolver built from its sources with its Release flags "
                               f"({cores} threads), mode 3R on the same mesh, {n2} steps ({t2:.2f} s)"}
                 out["gpu_3R_over_reference_3R"] = round(out["gpu_3R"]["value"] * 1e9 / ref, 1)
@@ -433,6 +439,27 @@ def host_cores():
         n = os.cpu_count() or 1
     omp = os.environ.get("OMP_NUM_THREADS")
     return int(omp) if omp and omp.isdigit() else n
+
+
+def host_sockets():
+    """Distinct physical packages among the CPUs this process may run on."""
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = list(range(os.cpu_count() or 1))
+    pk = set()
+    for c in cpus:
+        try:
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/physical_package_id") as f:
+                pk.add(f.read().strip())
+        except OSError:
+            pass
+    return len(pk) or None
+
+
+def cpu_threading():
+    return {"cores": host_cores(), "sockets": host_sockets(), "cpu_model": cpu_model(),
+            "OMP_PROC_BIND": os.environ.get("OMP_PROC_BIND"), "OMP_PLACES": os.environ.get("OMP_PLACES")}
 
 
 def cpu_model():
@@ -540,6 +567,7 @@ def main():
     b_alg = b_alg_config2(fpc)
     achieved = b_alg * nc / (stage_ms * 1e-3) / 1e9
     dominant = max(per_launch, key=per_launch.get)
+    s_shape = s.launch_shape()
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
@@ -555,12 +583,17 @@ def main():
         "wall_with_events_s": round(wall_ev, 3),
     }
     prof = os.path.join(REPO, "profiles", "traffic_config2.json")
+    shape = s_shape if s_shape else {}
+    key = f"v{shape.get('kernel_variant', 1)}_b{shape.get('block_size', 0)}_t{shape.get('tile_cells', 0)}"
+    roofline["traffic_shape"] = key
     if os.path.exists(prof):
         with open(prof) as f:
-            tr = json.load(f).get("variants", {}).get(str(params.kernel_variant or 1))
-        if tr:  # measured for this kernel variant on this workload (rocprofv3 FETCH_SIZE, WRITE_SIZE)
+            tr = json.load(f).get("shapes", {}).get(key)
+        if tr:  # measured for this launch shape on this workload (rocprofv3 FETCH_SIZE, WRITE_SIZE)
             roofline["traffic"] = tr.get("stage_bytes")
             roofline["traffic_per_cell_update"] = tr.get("bytes_per_cell_update")
+        else:
+            roofline["traffic_note"] = "no counter measurement for this launch shape in profiles/traffic_config2.json"
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -623,14 +656,15 @@ def main():
         cores = host_cores()
         try:
             from oracle import pyoracle
-            o = pyoracle.OracleSolver(m, bcs, params, ic)
+            o = pyoracle.OracleSolver(m, bcs, params, ic, flavour="fast")
             o.step(1)
             n_cpu, tc = timed_steps(o, args.cpu_seconds)
             cpu_cups = nc * stages * n_cpu / tc
             out["cpu_baseline"] = {"value": round(cpu_cups / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores,
-                                   "kind": "port",
-                                   "sample": f"oracle C restatement (OpenMP, {cores} threads, {cpu_model()}) on the "
-                                             f"same config-2 mesh and numerics, {n_cpu} RK3 steps after "
+                                   "kind": "port", "threading": cpu_threading(),
+                                   "sample": f"oracle C restatement built with the reference's Release flags "
+                                             f"(-O2 -ffast-math -flto -fopenmp; {cores} threads, {cpu_model()}) on "
+                                             f"the same config-2 mesh and numerics, {n_cpu} RK3 steps after "
                                              f"1 warm-up step ({tc:.2f} s)"}
             out["gpu_over_cpu"] = round(cups / cpu_cups, 1)
             del o
@@ -647,6 +681,7 @@ def main():
                 ref_cups = nc * n2 / tr2
                 out["cpu_baseline_reference_2R"] = {
                     "value": round(ref_cups / 1e9, 6), "unit": "Gcell-updates/s", "cores": cores, "kind": "reference",
+                    "threading": cpu_threading(),
                     "sample": f"reference solver built from its own sources with its Release flags "
                               f"(-O2 -ffast-math -flto -fopenmp, {cores} threads), reference mode 2R "
                               f"(first-order HLLC, forward Euler, global dt) on the same mesh, {n2} steps ({tr2:.2f} s)"}

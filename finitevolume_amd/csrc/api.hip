@@ -207,6 +207,7 @@ extern "C" {
 
 int fvk_create(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, const fvk_params* params,
                const fvk_initial_condition* ic, fvk_solver** out) {
+    fvk::TraceRange tr("fvk_create");
     return fvk::create_solver(mesh, bcs, n_bcs, params, ic, nullptr, 0, out);
 }
 
@@ -580,6 +581,7 @@ int fvk_get_state(const fvk_solver* s, double* U_aos, int64_t n_cells) {
 }
 
 int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hist) {
+    fvk::TraceRange tr("fvk_step");
     if (!s) return set_error(FVK_ERR_INVALID, "NULL solver");
     if (nsteps < 0) return set_error(FVK_ERR_INVALID, "nsteps < 0");
     const int nv = s->nv;
@@ -677,6 +679,7 @@ int fvk_get_launch_shape(const fvk_solver* s, fvk_launch_shape* shape) {
 int64_t fvk_device_bytes(const fvk_solver* s) { return s ? s->device_bytes : 0; }
 
 int fvk_residual_eval(fvk_solver* s, double* R) {
+    fvk::TraceRange tr("fvk_residual_eval");
     if (!s || !R) return set_error(FVK_ERR_INVALID, "NULL argument");
     if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     const int nv = s->nv;
@@ -771,6 +774,7 @@ int fvk_wall_distance_faces(const fvk_mesh_desc* mesh, const int32_t* wall_faces
 }
 
 int fvk_residual_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar) {
+    fvk::TraceRange tr("fvk_residual_vjp");
     if (!s || !lambda || !U_bar) return set_error(FVK_ERR_INVALID, "NULL argument");
     if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     return adjoint_vjp(s, lambda, U_bar, X_bar);
@@ -865,6 +869,7 @@ int fvk_wall_forces_vjp(fvk_solver* s, double rho_ref, double vx_ref, double vy_
 int fvk_adjoint_solve_forces(fvk_solver* s, double rho_ref, double vx_ref, double vy_ref, double length_ref,
                              double w_cl, double w_cd, double cfl, int32_t max_iter, double tol, int32_t check_every,
                              double* dJ_dX, double* lambda_out, fvk_adjoint_report* report) {
+    fvk::TraceRange tr("fvk_adjoint_solve_forces");
     if (!s || !dJ_dX) return set_error(FVK_ERR_INVALID, "NULL argument");
     if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     std::vector<double> g(4 * (size_t)s->nc), jx(2 * (size_t)s->nn);

@@ -303,6 +303,7 @@ int fvk_batch_set_active(fvk_batch* b, const int32_t* active) {
 }
 
 int fvk_batch_step(fvk_batch* b, int32_t nsteps, double* case_residuals) {
+    fvk::TraceRange tr("fvk_batch_step");
     if (!b) return set_error(FVK_ERR_INVALID, "NULL batch");
     if (nsteps < 0) return set_error(FVK_ERR_INVALID, "nsteps < 0");
     fvk_solver* s = b->s;

@@ -180,6 +180,7 @@ void fvk_run_config_default(fvk_run_config* c) {
 }
 
 int fvk_run(fvk_solver* s, const fvk_run_config* cfg, fvk_run_result* res) {
+    fvk::TraceRange tr("fvk_run");
     if (!s || !cfg || !res) return set_error(FVK_ERR_INVALID, "NULL argument");
     if (cfg->residual_interval <= 0 || cfg->output_precision < 1 || cfg->output_precision > 17 ||
         (cfg->cfl_ramp && cfg->ramp_window <= 0))

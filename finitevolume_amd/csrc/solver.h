@@ -4,6 +4,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdint>
 #include <string>
@@ -12,6 +13,16 @@
 #include "fvk.h"
 
 namespace fvk {
+
+// A named range on the ROCTx marker stream for the lifetime of the object
+// (rocprofv3 --marker-trace shows the library's entry points around their
+// kernels; without a profiler attached the calls return at once).
+struct TraceRange {
+    explicit TraceRange(const char* name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+    TraceRange(const TraceRange&) = delete;
+    TraceRange& operator=(const TraceRange&) = delete;
+};
 
 constexpr int kMaxPatches = 1024;
 constexpr int kMaxEulerDegree = 16;  // most faces of one cell the per-cell Euler kernels take

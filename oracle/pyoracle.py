@@ -14,8 +14,12 @@ from finitevolume_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libfvoracle.so")
+# the same sources with the reference's Release flags (-O2 -ffast-math
+# -flto): timed as the CPU baseline only, never used as the checker
+FAST_PATH = os.path.join(HERE, "_build", "libfvoracle_fast.so")
 JVP_PATH = os.path.join(HERE, "_build", "libfvjvp.so")
 _lib = None
+_fast = None
 _jvp = None
 
 
@@ -23,13 +27,23 @@ def available() -> bool:
     return os.path.exists(LIB_PATH)
 
 
-def lib():
-    global _lib
+def lib(flavour: str = "strict"):
+    global _lib, _fast
+    if flavour == "fast":
+        if _fast is None:
+            if not os.path.exists(FAST_PATH):
+                raise FileNotFoundError(f"{FAST_PATH} not built (make -C oracle restatement)")
+            _fast = _bind(C.CDLL(FAST_PATH))
+        return _fast
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise FileNotFoundError(f"{LIB_PATH} not built (make -C oracle restatement)")
-    L = C.CDLL(LIB_PATH)
+    _lib = _bind(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def _bind(L):
     dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.c_void_p
     L.fvo_create.argtypes = [C.POINTER(abi.MeshDescC), C.POINTER(abi.BCC), C.c_int32, C.POINTER(abi.ParamsC),
                              C.POINTER(abi.InitialConditionC), C.POINTER(vp)]
@@ -56,7 +70,6 @@ def lib():
     L.fvo_next_cfl.restype = C.c_double
     L.fvo_venkatakrishnan.argtypes = [C.c_double, C.c_double, C.c_double]
     L.fvo_venkatakrishnan.restype = C.c_double
-    _lib = L
     return L
 
 
@@ -104,8 +117,9 @@ class OracleSolver:
     """The restatement's solver object, same construction arguments as the
     product's fvk_create."""
 
-    def __init__(self, mesh, bcs, params: abi.ParamsC, ic: Optional[abi.InitialConditionC] = None):
-        self.L = lib()
+    def __init__(self, mesh, bcs, params: abi.ParamsC, ic: Optional[abi.InitialConditionC] = None,
+                 flavour: str = "strict"):
+        self.L = lib(flavour)
         self.md = abi.MeshDesc(mesh)
         self.mesh = mesh
         self.n = mesh.n_cells
