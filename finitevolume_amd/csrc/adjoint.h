@@ -16,13 +16,6 @@
 
 namespace fvk {
 
-// primitive variables of a state: rho, u, v, p (u = rho_u / rho, p from pressure())
-struct Prim {
-    double r, u, v, p;
-};
-
-FVK_HD Prim to_prim(const State& s, double g) { return Prim{s.r, fdiv(s.ru, s.r), fdiv(s.rv, s.r), pressure(s, g)}; }
-
 // adjoint of to_prim: sb += (dW/dU)^T wb
 // (reverse-only arithmetic: one reciprocal of rho instead of five divisions)
 FVK_HD void prim_rev(const State& s, const Prim& wb, State& sb, double g) {
@@ -243,28 +236,243 @@ FVK_HD void euler_ghost_rev(int type, const State& L, double nx, double ny, doub
     ghost_slip_wall_rev(L, nx, ny, g, Gb, Lb, nxb, nyb);
 }
 
-// adjoint of the MUSCL face state: out = from_primitive(W(U) + d) with the
-// increments d_q = G_q . r, or U itself when the reconstructed density or
-// pressure is not positive.  Returns db (the adjoint of the four increments)
-// and adds dU.  Where the increments are all zero the forward returns U
-// exactly, which is the same function, so the general derivative applies.
-FVK_HD void reconstruct_rev(const State& Uc, const double4& g0, const double4& g1, double rx, double ry, double gam,
-                            const State& Ob, State& Ucb, double (&db)[4]) {
-    const double d0 = g0.x * rx + g0.y * ry;
-    const double d1 = g0.z * rx + g0.w * ry;
-    const double d2 = g1.x * rx + g1.y * ry;
-    const double d3 = g1.z * rx + g1.w * ry;
-    const double u = fdiv(Uc.ru, Uc.r), v = fdiv(Uc.rv, Uc.r), p = pressure(Uc, gam);
-    const double rho = Uc.r + d0, pp = p + d3;
-    if (!(rho > 0.0) || !(pp > 0.0)) {
-        Ucb.r += Ob.r; Ucb.ru += Ob.ru; Ucb.rv += Ob.rv; Ucb.E += Ob.E;
-        db[0] = db[1] = db[2] = db[3] = 0.0;
+// ---- second order: adjoints of the fluxes on primitive face states ----
+
+FVK_HD void add_to(Prim& a, const Prim& b) {
+    a.r += b.r; a.u += b.u; a.v += b.v; a.p += b.p;
+}
+
+// adjoint of e = specific_energy(w) = p / ((g - 1) rho) + (u^2 + v^2) / 2
+FVK_HD void specific_energy_rev(const Prim& w, double eb, Prim& wb, double g) {
+    const double t = fdiv(eb, (g - 1.0) * w.r);  // de/dp
+    wb.p += t;
+    wb.r -= t * fdiv(w.p, w.r);
+    wb.u += eb * w.u;
+    wb.v += eb * w.v;
+}
+
+// adjoint of phys_flux_w(w, E, Vn, nx, ny): adds wb, Eb, Vnb, nxb, nyb
+FVK_HD void phys_flux_w_rev(const Prim& w, double E, double Vn, double nx, double ny, const State& Fb, Prim& wb,
+                            double& Eb, double& Vnb, double& nxb, double& nyb) {
+    const double m = w.r * Vn;
+    const double mb = Fb.r + Fb.ru * w.u + Fb.rv * w.v;
+    wb.u += Fb.ru * m;
+    wb.v += Fb.rv * m;
+    wb.p += Fb.ru * nx + Fb.rv * ny + Fb.E * Vn;
+    nxb += Fb.ru * w.p;
+    nyb += Fb.rv * w.p;
+    Eb += Fb.E * Vn;
+    Vnb += Fb.E * (E + w.p) + mb * w.r;
+    wb.r += mb * Vn;
+}
+
+// adjoint of Vn = u nx + v ny
+FVK_HD void normal_velocity_rev(const Prim& w, double nx, double ny, double Vnb, Prim& wb, double& nxb, double& nyb) {
+    wb.u += Vnb * nx;
+    wb.v += Vnb * ny;
+    nxb += Vnb * w.u;
+    nyb += Vnb * w.v;
+}
+
+// adjoint of the flux of one primitive state (an HLLC supersonic early out)
+FVK_HD void phys_flux_state_w_rev(const Prim& w, double nx, double ny, double g, const State& Ob, Prim& wb, double& nxb,
+                                  double& nyb) {
+    const double e = specific_energy(w, g), Vn = w.u * nx + w.v * ny;
+    double Eb = 0.0, Vnb = 0.0;
+    phys_flux_w_rev(w, w.r * e, Vn, nx, ny, Ob, wb, Eb, Vnb, nxb, nyb);
+    wb.r += Eb * e;
+    specific_energy_rev(w, Eb * w.r, wb, g);
+    normal_velocity_rev(w, nx, ny, Vnb, wb, nxb, nyb);
+}
+
+// adjoint of rusanov_w(L, R, nx, ny)
+FVK_HD void rusanov_w_rev(const Prim& L, const Prim& R, double nx, double ny, double g, const State& Ob, Prim& Lb,
+                          Prim& Rb, double& nxb, double& nyb) {
+    const double VnL = L.u * nx + L.v * ny, VnR = R.u * nx + R.v * ny;
+    const double cL = sqrt(fdiv(g * L.p, L.r)), cR = sqrt(fdiv(g * R.p, R.r));
+    const double a = fabs(VnL) + cL, b = fabs(VnR) + cR;
+    const bool takeR = a < b;  // smax(a, b)
+    const double h = 0.5 * (takeR ? b : a);
+    const double eL = specific_energy(L, g), eR = specific_energy(R, g);
+    const double EL = L.r * eL, ER = R.r * eR;
+    // out = (F_L + F_R) / 2 - h (U_R - U_L)
+    const State Fh{0.5 * Ob.r, 0.5 * Ob.ru, 0.5 * Ob.rv, 0.5 * Ob.E};
+    double ELb = 0.0, ERb = 0.0, VnLb = 0.0, VnRb = 0.0;
+    phys_flux_w_rev(L, EL, VnL, nx, ny, Fh, Lb, ELb, VnLb, nxb, nyb);
+    phys_flux_w_rev(R, ER, VnR, nx, ny, Fh, Rb, ERb, VnRb, nxb, nyb);
+    const double hb = -(Ob.r * (R.r - L.r) + Ob.ru * (R.r * R.u - L.r * L.u) + Ob.rv * (R.r * R.v - L.r * L.v) +
+                        Ob.E * (ER - EL));
+    // U = (rho, rho u, rho v, E) of each side, weighted by -h (right) and +h (left)
+    Rb.r -= h * (Ob.r + Ob.ru * R.u + Ob.rv * R.v);
+    Rb.u -= h * Ob.ru * R.r;
+    Rb.v -= h * Ob.rv * R.r;
+    ERb -= h * Ob.E;
+    Lb.r += h * (Ob.r + Ob.ru * L.u + Ob.rv * L.v);
+    Lb.u += h * Ob.ru * L.r;
+    Lb.v += h * Ob.rv * L.r;
+    ELb += h * Ob.E;
+    // E = rho e
+    Lb.r += ELb * eL;
+    specific_energy_rev(L, ELb * L.r, Lb, g);
+    Rb.r += ERb * eR;
+    specific_energy_rev(R, ERb * R.r, Rb, g);
+    // Smax = |Vn_K| + c_K of the larger side
+    const double Sb = 0.5 * hb;
+    if (takeR) {
+        VnRb += VnR >= 0.0 ? Sb : -Sb;
+        sound_speed_rev(R.r, cR, Sb, Rb, g);
+    } else {
+        VnLb += VnL >= 0.0 ? Sb : -Sb;
+        sound_speed_rev(L.r, cL, Sb, Lb, g);
+    }
+    normal_velocity_rev(L, nx, ny, VnLb, Lb, nxb, nyb);
+    normal_velocity_rev(R, nx, ny, VnRb, Rb, nxb, nyb);
+}
+
+// adjoint of hllc_w(L, R, nx, ny)
+FVK_HD void hllc_w_rev(const Prim& L, const Prim& R, double nx, double ny, double g, const State& Ob, Prim& Lb,
+                       Prim& Rb, double& nxb, double& nyb) {
+    const double VnL = L.u * nx + L.v * ny, VnR = R.u * nx + R.v * ny;
+    const double cL = sqrt(fdiv(g * L.p, L.r)), cR = sqrt(fdiv(g * R.p, R.r));
+    const double a1 = VnL - cL, a2 = VnR - cR;
+    const bool slR = a2 < a1;  // smin
+    const double SL = slR ? a2 : a1;
+    const double b1 = VnL + cL, b2 = VnR + cR;
+    const bool srR = b1 < b2;  // smax
+    const double SR = srR ? b2 : b1;
+    if (SL >= 0.0) { phys_flux_state_w_rev(L, nx, ny, g, Ob, Lb, nxb, nyb); return; }
+    if (SR <= 0.0) { phys_flux_state_w_rev(R, nx, ny, g, Ob, Rb, nxb, nyb); return; }
+    const double mL = L.r * (SL - VnL), mR = R.r * (SR - VnR);
+    const double den = mL - mR;
+    const double Ss = fdiv(R.p - L.p + mL * VnL - mR * VnR, den);
+    const bool left = Ss >= 0.0;
+    const Prim& K = left ? L : R;
+    const double SK = left ? SL : SR, VnK = left ? VnL : VnR, mK = left ? mL : mR;
+    const double VtK = K.v * nx - K.u * ny;
+    const double eK = specific_energy(K, g), EK = K.r * eK;
+    const double E2 = SK - Ss;
+    const double factor = fdiv(mK, E2);
+    const double mn = factor * Ss, mt = factor * VtK;
+    const double imK = fdiv(1.0, mK);
+    const double T = Ss + K.p * imK;
+    const double Q = eK + (Ss - VnK) * T;
+    const double Es = factor * Q;
+
+    // out = F(K) + SK (U* - U_K)
+    Prim Kb{0.0, 0.0, 0.0, 0.0};
+    double EKb = 0.0, VnKb = 0.0;
+    phys_flux_w_rev(K, EK, VnK, nx, ny, Ob, Kb, EKb, VnKb, nxb, nyb);
+    double SKb = Ob.r * (factor - K.r) + Ob.ru * ((mn * nx - mt * ny) - K.r * K.u) +
+                 Ob.rv * ((mn * ny + mt * nx) - K.r * K.v) + Ob.E * (Es - EK);
+    double factorb = Ob.r * SK;
+    Kb.r -= SK * (Ob.r + Ob.ru * K.u + Ob.rv * K.v);
+    Kb.u -= SK * Ob.ru * K.r;
+    Kb.v -= SK * Ob.rv * K.r;
+    EKb -= SK * Ob.E;
+    const double mnb = SK * (Ob.ru * nx + Ob.rv * ny);
+    const double mtb = SK * (-Ob.ru * ny + Ob.rv * nx);
+    nxb += SK * (Ob.ru * mn + Ob.rv * mt);
+    nyb += SK * (-Ob.ru * mt + Ob.rv * mn);
+    const double Esb = SK * Ob.E;
+    // Es = factor Q, Q = e_K + (Ss - Vn_K) T, T = Ss + p_K / m_K
+    factorb += Esb * Q;
+    const double Qb = Esb * factor;
+    double eKb = Qb;
+    double Ssb = Qb * T;
+    VnKb -= Qb * T;
+    const double Tb = Qb * (Ss - VnK);
+    Ssb += Tb;
+    Kb.p += Tb * imK;
+    double mKb = -Tb * K.p * imK * imK;
+    // mn = factor Ss, mt = factor Vt_K
+    factorb += mnb * Ss + mtb * VtK;
+    Ssb += mnb * factor;
+    const double VtKb = mtb * factor;
+    // factor = m_K / (SK - Ss)
+    const double iE2 = fdiv(1.0, E2);
+    mKb += factorb * iE2;
+    const double E2b = -factorb * factor * iE2;
+    SKb += E2b;
+    Ssb -= E2b;
+    // E_K = rho_K e_K, e_K, Vt_K = v_K nx - u_K ny
+    Kb.r += EKb * eK;
+    eKb += EKb * K.r;
+    specific_energy_rev(K, eKb, Kb, g);
+    Kb.v += VtKb * nx;
+    Kb.u -= VtKb * ny;
+    nxb += VtKb * K.v;
+    nyb -= VtKb * K.u;
+    // Ss = num / den, num = p_R - p_L + m_L Vn_L - m_R Vn_R, den = m_L - m_R
+    const double iden = fdiv(1.0, den);
+    const double numb = Ssb * iden;
+    const double denb = -Ssb * Ss * iden;
+    Prim wLb{0.0, 0.0, 0.0, -numb}, wRb{0.0, 0.0, 0.0, numb};
+    double mLb = numb * VnL + denb, mRb = -numb * VnR - denb;
+    double VnLb = numb * mL, VnRb = -numb * mR;
+    double SLb = 0.0, SRb = 0.0;
+    if (left) { SLb += SKb; VnLb += VnKb; mLb += mKb; add_to(wLb, Kb); }
+    else { SRb += SKb; VnRb += VnKb; mRb += mKb; add_to(wRb, Kb); }
+    // m_L = rho_L (S_L - Vn_L), m_R = rho_R (S_R - Vn_R)
+    wLb.r += mLb * (SL - VnL);
+    SLb += mLb * L.r;
+    VnLb -= mLb * L.r;
+    wRb.r += mRb * (SR - VnR);
+    SRb += mRb * R.r;
+    VnRb -= mRb * R.r;
+    // Davis speeds
+    double cLb = 0.0, cRb = 0.0;
+    if (slR) { VnRb += SLb; cRb -= SLb; } else { VnLb += SLb; cLb -= SLb; }
+    if (srR) { VnRb += SRb; cRb += SRb; } else { VnLb += SRb; cLb += SRb; }
+    sound_speed_rev(L.r, cL, cLb, wLb, g);
+    sound_speed_rev(R.r, cR, cRb, wRb, g);
+    normal_velocity_rev(L, nx, ny, VnLb, wLb, nxb, nyb);
+    normal_velocity_rev(R, nx, ny, VnRb, wRb, nxb, nyb);
+    add_to(Lb, wLb);
+    add_to(Rb, wRb);
+}
+
+template <int SCHEME>
+FVK_HD void num_flux_w_rev(const Prim& L, const Prim& R, double nx, double ny, double g, const State& Ob, Prim& Lb,
+                           Prim& Rb, double& nxb, double& nyb) {
+    static_assert(SCHEME != FLUX_EXACT, "no adjoint of the exact Riemann flux");
+    if constexpr (SCHEME == FLUX_HLLC) hllc_w_rev(L, R, nx, ny, g, Ob, Lb, Rb, nxb, nyb);
+    else rusanov_w_rev(L, R, nx, ny, g, Ob, Lb, Rb, nxb, nyb);
+}
+
+// adjoint of euler_ghost_w (farfield: constant; outflow: copy; wall: the
+// normal velocity reflected)
+FVK_HD void euler_ghost_w_rev(int type, const Prim& L, double nx, double ny, const Prim& Gb, Prim& Lb, double& nxb,
+                              double& nyb) {
+    if (type == BC_FARFIELD) return;
+    if (type == BC_OUTFLOW) {
+        add_to(Lb, Gb);
         return;
     }
-    Prim wb{0.0, 0.0, 0.0, 0.0};
-    from_prim_rev(Prim{rho, u + d1, v + d2, pp}, Ob, wb, gam);
-    db[0] = wb.r; db[1] = wb.u; db[2] = wb.v; db[3] = wb.p;
-    prim_rev(Uc, wb, Ucb, gam);
+    const double Vn = L.u * nx + L.v * ny;
+    const double Vnb = -2.0 * (Gb.u * nx + Gb.v * ny);
+    nxb += -2.0 * Vn * Gb.u;
+    nyb += -2.0 * Vn * Gb.v;
+    Lb.r += Gb.r;
+    Lb.u += Gb.u;
+    Lb.v += Gb.v;
+    Lb.p += Gb.p;
+    normal_velocity_rev(L, nx, ny, Vnb, Lb, nxb, nyb);
+}
+
+// adjoint of face_state_w: W_f = W(U) + d with the increments d_q = G_q . r,
+// or W(U) itself where the reconstructed density or pressure is not
+// positive.  Returns db (the adjoint of the four increments) and adds dU.
+FVK_HD void face_state_w_rev(const State& Uc, const double4& g0, const double4& g1, double rx, double ry, double gam,
+                             const Prim& Wfb, State& Ucb, double (&db)[4]) {
+    const double d0 = g0.x * rx + g0.y * ry;
+    const double d3 = g1.z * rx + g1.w * ry;
+    const Prim w = to_prim(Uc, gam);
+    const bool rec = (w.r + d0 > 0.0) && (w.p + d3 > 0.0);
+    db[0] = rec ? Wfb.r : 0.0;
+    db[1] = rec ? Wfb.u : 0.0;
+    db[2] = rec ? Wfb.v : 0.0;
+    db[3] = rec ? Wfb.p : 0.0;
+    prim_rev(Uc, Wfb, Ucb, gam);
 }
 
 // partial derivatives of venkatakrishnan(d1, d2, eps2)

@@ -99,42 +99,43 @@ __device__ __forceinline__ void adj_face_body(const AdjArgs& a, int f, int cr, c
     const MeshView& m = a.m;
     const double g = a.gamma;
     const double nx = fg.x, ny = fg.y, A = fg.z;
-    const State Lf = ORDER == 2 ? reconstruct(L, gl0, gl1, ro.x, ro.y, g) : L;
-    State Rf;
-    int type = -1;
-    State Fb;
-    if (cr >= 0) {
-        Rf = ORDER == 2 ? reconstruct(R, gr0, gr1, ro.z, ro.w, g) : R;
-        Fb = State{lr.x - ll.x, lr.y - ll.y, lr.z - ll.z, lr.w - ll.w};
-    } else {
-        const DevBC& bc = m.bc[-1 - cr];
-        type = bc.type;
-        Rf = euler_ghost_state(bc.type, bc.ff, Lf, nx, ny, g);
-        Fb = State{-ll.x, -ll.y, -ll.z, -ll.w};
-    }
-    const State H = num_flux<SCHEME>(Lf, Rf, nx, ny, g, 0.0, 0);
-    const double Ab = Fb.r * H.r + Fb.ru * H.ru + Fb.rv * H.rv + Fb.E * H.E;
+    const bool interior = cr >= 0;
+    const State Fb = interior ? State{lr.x - ll.x, lr.y - ll.y, lr.z - ll.z, lr.w - ll.w}
+                              : State{-ll.x, -ll.y, -ll.z, -ll.w};
     const State Hb{A * Fb.r, A * Fb.ru, A * Fb.rv, A * Fb.E};
-    State Lfb{0, 0, 0, 0}, Rfb{0, 0, 0, 0};
-    double nxb = 0.0, nyb = 0.0;
-    num_flux_rev<SCHEME>(Lf, Rf, nx, ny, g, Hb, Lfb, Rfb, nxb, nyb);
-    if (cr < 0) euler_ghost_rev(type, Lf, nx, ny, g, Rfb, Lfb, nxb, nyb);
+    double nxb = 0.0, nyb = 0.0, Ab;
+    const DevBC* bc = interior ? nullptr : &m.bc[-1 - cr];
     if (ORDER == 2) {
+        // primitive face states, as the forward face pass forms them
+        const Prim Lf = face_state_w(L, gl0, gl1, ro.x, ro.y, g);
+        const Prim Rf = interior ? face_state_w(R, gr0, gr1, ro.z, ro.w, g)
+                                 : euler_ghost_w(bc->type, bc->ff, Lf, nx, ny, g);
+        const State H = num_flux_w<SCHEME>(Lf, Rf, nx, ny, g, 0.0, 0);
+        Ab = Fb.r * H.r + Fb.ru * H.ru + Fb.rv * H.rv + Fb.E * H.E;
+        Prim Lfb{0, 0, 0, 0}, Rfb{0, 0, 0, 0};
+        num_flux_w_rev<SCHEME>(Lf, Rf, nx, ny, g, Hb, Lfb, Rfb, nxb, nyb);
+        if (!interior) euler_ghost_w_rev(bc->type, Lf, nx, ny, Rfb, Lfb, nxb, nyb);
         State ULb{0, 0, 0, 0};
         double dl[4];
-        reconstruct_rev(L, gl0, gl1, ro.x, ro.y, g, Lfb, ULb, dl);
+        face_state_w_rev(L, gl0, gl1, ro.x, ro.y, g, Lfb, ULb, dl);
         a.ful[f] = d4(ULb);
         a.fdl[f] = double4{dl[0], dl[1], dl[2], dl[3]};
-        if (cr >= 0) {
+        if (interior) {
             State URb{0, 0, 0, 0};
             double dr[4];
-            reconstruct_rev(R, gr0, gr1, ro.z, ro.w, g, Rfb, URb, dr);
+            face_state_w_rev(R, gr0, gr1, ro.z, ro.w, g, Rfb, URb, dr);
             a.fur[f] = d4(URb);
             a.fdr[f] = double4{dr[0], dr[1], dr[2], dr[3]};
         }
     } else {
+        const State Rf = interior ? R : euler_ghost_state(bc->type, bc->ff, L, nx, ny, g);
+        const State H = num_flux<SCHEME>(L, Rf, nx, ny, g, 0.0, 0);
+        Ab = Fb.r * H.r + Fb.ru * H.ru + Fb.rv * H.rv + Fb.E * H.E;
+        State Lfb{0, 0, 0, 0}, Rfb{0, 0, 0, 0};
+        num_flux_rev<SCHEME>(L, Rf, nx, ny, g, Hb, Lfb, Rfb, nxb, nyb);
+        if (!interior) euler_ghost_rev(bc->type, L, nx, ny, g, Rfb, Lfb, nxb, nyb);
         a.ful[f] = d4(Lfb);
-        if (cr >= 0) a.fur[f] = d4(Rfb);
+        if (interior) a.fur[f] = d4(Rfb);
     }
     if (WX) {
         double exb = 0.0, eyb = 0.0;

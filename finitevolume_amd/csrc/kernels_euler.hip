@@ -124,6 +124,15 @@ __device__ __forceinline__ double dt_candidate(const State& L, const State* R, d
     return fdiv(cfl * len, Smax);
 }
 
+// dt_candidate() from the cells' primitives: the same bits, since W holds
+// the results of the same divisions and pressure()
+__device__ __forceinline__ double dt_candidate_w(const Prim& L, const Prim* R, double nx, double ny, double len,
+                                                 double g, double cfl) {
+    double Smax = fabs(L.u * nx + L.v * ny) + sqrt(fdiv(g * L.p, L.r));
+    if (R) Smax = smax(Smax, fabs(R->u * nx + R->v * ny) + sqrt(fdiv(g * R->p, R->r)));
+    return fdiv(cfl * len, Smax);
+}
+
 // block minimum, then one atomic per block.  Positive doubles order like
 // their bit patterns and NaN bits exceed +inf and DBL_MAX, so a NaN
 // candidate never wins (std::min semantics of the reference's reduction).
@@ -327,9 +336,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_grad(EulerArgs a) {
     const State Uc = st(a.U[c]);
     const double wc[4] = {Uc.r, fdiv(Uc.ru, Uc.r), fdiv(Uc.rv, Uc.r), pressure(Uc, g)};
     double bx[4] = {0.0, 0.0, 0.0, 0.0}, by[4] = {0.0, 0.0, 0.0, 0.0};
-    double wmin[4], wmax[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) wmin[q] = wmax[q] = wc[q];
+    double dlo[4] = {0.0, 0.0, 0.0, 0.0}, dhi[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         if (en[k].x < 0) continue;  // ELL padding (always after the cell's faces)
@@ -354,8 +361,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_grad(EulerArgs a) {
             const double dphi = wn[q] - wc[q];
             bx[q] += eg[k].x * dphi;
             by[q] += eg[k].y * dphi;
-            wmin[q] = smin(wmin[q], wn[q]);
-            wmax[q] = smax(wmax[q], wn[q]);
+            venkat_extend(dlo[q], dhi[q], dphi);
         }
     }
     const double4 iv = m.clsq[c];  // LSQ inverse and eps^2
@@ -376,7 +382,7 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_grad(EulerArgs a) {
             if (en[k].x < 0) continue;
             const double rx = eg[k].z, ry = eg[k].w;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) venkat_pick(vm[q], wmin[q], wmax[q], wc[q], gx[q] * rx + gy[q] * ry, eps2);
+            for (int q = 0; q < 4; ++q) venkat_pick_d(vm[q], dlo[q], dhi[q], gx[q] * rx + gy[q] * ry, eps2);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) lim[q] = venkat_value(vm[q]);
@@ -422,26 +428,30 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_cell(EulerArgs a) {
             double cand = DBL_MAX;
             if (en[k].y < 0) {
                 // boundary face: this cell is the owner
-                State Lf = Uc;
                 if (ORDER == 2) {
                     const double4 ro = m.frec[f];
-                    Lf = reconstruct(Uc, gc0, gc1, ro.x, ro.y, g);
+                    const Prim Lf = face_state_w(Uc, gc0, gc1, ro.x, ro.y, g);
+                    const DevBC& bc = m.bc[-1 - en[k].y];
+                    F = num_flux_w<SCHEME>(Lf, euler_ghost_w(bc.type, bc.ff, Lf, fg.x, fg.y, g), fg.x, fg.y, g,
+                                           a.tol, a.maxit);
+                } else {
+                    const State Rf = euler_ghost(m.bc[-1 - en[k].y], Uc, fg.x, fg.y, g);
+                    F = num_flux<SCHEME>(Uc, Rf, fg.x, fg.y, g, a.tol, a.maxit);
                 }
-                const State Rf = euler_ghost(m.bc[-1 - en[k].y], Lf, fg.x, fg.y, g);
-                F = num_flux<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
                 if (a.face_dt) cand = dt_candidate(Uc, nullptr, fg.x, fg.y, fg.w, g, a.cfl);
             } else {
                 const State Uo = st(a.U[en[k].y]);
                 const State& L = owner ? Uc : Uo;
                 const State& R = owner ? Uo : Uc;
-                State Lf = L, Rf = R;
                 if (ORDER == 2) {
                     const double4 go0 = a.G[2 * en[k].y], go1 = a.G[2 * en[k].y + 1];
                     const double4 ro = m.frec[f];
-                    Lf = reconstruct(L, owner ? gc0 : go0, owner ? gc1 : go1, ro.x, ro.y, g);
-                    Rf = reconstruct(R, owner ? go0 : gc0, owner ? go1 : gc1, ro.z, ro.w, g);
+                    const Prim Lf = face_state_w(L, owner ? gc0 : go0, owner ? gc1 : go1, ro.x, ro.y, g);
+                    const Prim Rf = face_state_w(R, owner ? go0 : gc0, owner ? go1 : gc1, ro.z, ro.w, g);
+                    F = num_flux_w<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
+                } else {
+                    F = num_flux<SCHEME>(L, R, fg.x, fg.y, g, a.tol, a.maxit);
                 }
-                F = num_flux<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
                 if (a.face_dt) cand = dt_candidate(L, &R, fg.x, fg.y, fg.w, g, a.cfl);
             }
             const double A = fg.z;
@@ -488,21 +498,24 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_face(EulerArgs a) {
         const int cl = m.fl[f], cr = m.fr[f];
         const double4 fg = m.fgeo[f];
         const State L = st(a.U[cl]);
-        State R = L, Lf = L, Rf;
+        State R = L;
+        if (cr >= 0) R = st(a.U[cr]);
+        State F;
         if (ORDER == 2) {
             const double4 ro = m.frec[f];  // midpoint - centroid, left and right (precomputed, same bits)
-            Lf = reconstruct(L, a.G[2 * cl], a.G[2 * cl + 1], ro.x, ro.y, g);
+            const Prim Lf = face_state_w(L, a.G[2 * cl], a.G[2 * cl + 1], ro.x, ro.y, g);
+            Prim Rf;
             if (cr >= 0) {
-                R = st(a.U[cr]);
-                Rf = reconstruct(R, a.G[2 * cr], a.G[2 * cr + 1], ro.z, ro.w, g);
+                Rf = face_state_w(R, a.G[2 * cr], a.G[2 * cr + 1], ro.z, ro.w, g);
             } else {
-                Rf = euler_ghost(m.bc[-1 - cr], Lf, fg.x, fg.y, g);
+                const DevBC& bc = m.bc[-1 - cr];
+                Rf = euler_ghost_w(bc.type, bc.ff, Lf, fg.x, fg.y, g);
             }
+            F = num_flux_w<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
         } else {
-            if (cr >= 0) { R = st(a.U[cr]); Rf = R; }
-            else Rf = euler_ghost(m.bc[-1 - cr], L, fg.x, fg.y, g);
+            F = num_flux<SCHEME>(L, cr >= 0 ? R : euler_ghost(m.bc[-1 - cr], L, fg.x, fg.y, g), fg.x, fg.y, g, a.tol,
+                                 a.maxit);
         }
-        const State F = num_flux<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
         const double A = fg.z;
         a.flux[f] = double4{A * F.r, A * F.ru, A * F.rv, A * F.E};
         if (a.face_dt) {
@@ -579,40 +592,41 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
     const double g = a.gamma;
     const int MG = L.MG, MF = L.MF;
     const int l = tface_left(fd), r = tface_right(fd);
-    const State Ls{L.U[l], L.U[MG + l], L.U[2 * MG + l], L.U[3 * MG + l]};
-    State R = Ls, Lf = Ls, Rf;
+    State F;
+    double cand = DBL_MAX;
     if (ORDER == 2) {
+        // primitive face states: the cells' primitives from LDS plus their increments
         const double* in = L.I + e;
-        const double* wl = L.W + l;
-        Lf = reconstruct_inc(Ls, double4{Ls.r, wl[0], wl[MG], wl[2 * MG]}, in[0], in[MF], in[2 * MF], in[3 * MF],
-                             g);
+        const Prim Wl{L.U[l], L.W[l], L.W[MG + l], L.W[2 * MG + l]};
+        const Prim Lf = reconstruct_w(Wl, in[0], in[MF], in[2 * MF], in[3 * MF]);
+        Prim Wr, Rf;
         if (r >= 0) {
-            R = State{L.U[r], L.U[MG + r], L.U[2 * MG + r], L.U[3 * MG + r]};
-            const double* wr = L.W + r;
-            Rf = reconstruct_inc(R, double4{R.r, wr[0], wr[MG], wr[2 * MG]}, in[4 * MF], in[5 * MF], in[6 * MF],
-                                 in[7 * MF], g);
+            Wr = Prim{L.U[r], L.W[r], L.W[MG + r], L.W[2 * MG + r]};
+            Rf = reconstruct_w(Wr, in[4 * MF], in[5 * MF], in[6 * MF], in[7 * MF]);
         } else {
-            Rf = euler_ghost(m.bc[-1 - r], Lf, fg.x, fg.y, g);
+            const DevBC& bc = m.bc[-1 - r];
+            Rf = euler_ghost_w(bc.type, bc.ff, Lf, fg.x, fg.y, g);
         }
+        F = num_flux_w<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
+        if (a.face_dt) cand = dt_candidate_w(Wl, r >= 0 ? &Wr : nullptr, fg.x, fg.y, fg.w, g, a.cfl);
     } else {
+        const State Ls{L.U[l], L.U[MG + l], L.U[2 * MG + l], L.U[3 * MG + l]};
+        State R = Ls, Rf;
         if (r >= 0) {
             R = State{L.U[r], L.U[MG + r], L.U[2 * MG + r], L.U[3 * MG + r]};
             Rf = R;
         } else {
             Rf = euler_ghost(m.bc[-1 - r], Ls, fg.x, fg.y, g);
         }
+        F = num_flux<SCHEME>(Ls, Rf, fg.x, fg.y, g, a.tol, a.maxit);
+        if (a.face_dt) cand = dt_candidate(Ls, r >= 0 ? &R : nullptr, fg.x, fg.y, fg.w, g, a.cfl);
     }
-    const State F = num_flux<SCHEME>(Lf, Rf, fg.x, fg.y, g, a.tol, a.maxit);
     const double A = fg.z;
     L.F[e] = A * F.r;
     L.F[MF + e] = A * F.ru;
     L.F[2 * MF + e] = A * F.rv;
     L.F[3 * MF + e] = A * F.E;
-    double cand = DBL_MAX;
-    if (a.face_dt) {
-        cand = dt_candidate(Ls, r >= 0 ? &R : nullptr, fg.x, fg.y, fg.w, g, a.cfl);
-        L.Dt[e] = cand;
-    }
+    if (a.face_dt) L.Dt[e] = cand;
     return cand;
 }
 

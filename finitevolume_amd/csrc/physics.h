@@ -5,7 +5,9 @@
 // the reference (cited per function) so that, with FMA contraction off
 // (-ffp-contract=off) and correctly rounded divide/sqrt, first-order
 // results match the reference bit for bit.  min/max follow std::min /
-// std::max (NaN in the second argument is ignored), not fmin/fmax.
+// std::max (NaN in the second argument is ignored), not fmin/fmax, except
+// for the limiter's extremes (venkat_extend), where the two give the same
+// limiter.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -67,6 +69,14 @@ FVK_HD double sound_speed(const State& s, double g) { return sqrt(fdiv(g * press
 
 // temperature, include/EulerState.h:73-75
 FVK_HD double temperature(const State& s, double g, double R) { return fdiv(pressure(s, g), s.r * R); }
+
+// primitive variables of a state: rho, u, v, p (u = rho_u / rho, p from
+// pressure(); the bits of the reference's own conversions)
+struct Prim {
+    double r, u, v, p;
+};
+
+FVK_HD Prim to_prim(const State& s, double g) { return Prim{s.r, fdiv(s.ru, s.r), fdiv(s.rv, s.r), pressure(s, g)}; }
 
 // flux, include/EulerState.h:89-101
 FVK_HD State phys_flux(const State& s, double nx, double ny, double g) {
@@ -249,32 +259,95 @@ FVK_HD State euler_ghost_state(int type, const double* ff, const State& L, doubl
     return ghost_slip_wall(L, nx, ny, g);
 }
 
-// MUSCL face state (extension): primitive increments d_q = G_q . r from the
-// limited gradients g0 = (rho_x, rho_y, u_x, u_y), g1 = (v_x, v_y, p_x, p_y);
-// the cell's own state when every increment is exactly zero or the
-// reconstructed density or pressure is not positive.
-FVK_HD State reconstruct(const State& Uc, const double4& g0, const double4& g1, double rx, double ry, double gam) {
+// ---- second-order faces on primitive face states (extension) ----
+// A MUSCL face state is primitive, W = (rho, u, v, p).  The fluxes below
+// take it as it is and form conserved quantities only where the flux uses
+// them, instead of converting W to conserved variables and dividing them
+// back.  Each equals its conserved counterpart of from_primitive(W) in exact
+// arithmetic; the rounding is this formulation's own.
+
+// specific total energy E / rho of a primitive state
+FVK_HD double specific_energy(const Prim& w, double g) {
+    return fdiv(w.p, (g - 1.0) * w.r) + 0.5 * (w.u * w.u + w.v * w.v);
+}
+
+// physical flux of a primitive state with total energy E and normal velocity Vn
+FVK_HD State phys_flux_w(const Prim& w, double E, double Vn, double nx, double ny) {
+    const double m = w.r * Vn;
+    return State{m, m * w.u + w.p * nx, m * w.v + w.p * ny, (E + w.p) * Vn};
+}
+
+// rusanov() on primitive states
+FVK_HD State rusanov_w(const Prim& L, const Prim& R, double nx, double ny, double g) {
+    const double VnL = L.u * nx + L.v * ny, VnR = R.u * nx + R.v * ny;
+    const double Smax = smax(fabs(VnL) + sqrt(fdiv(g * L.p, L.r)), fabs(VnR) + sqrt(fdiv(g * R.p, R.r)));
+    const double EL = L.r * specific_energy(L, g), ER = R.r * specific_energy(R, g);
+    const State FL = phys_flux_w(L, EL, VnL, nx, ny), FR = phys_flux_w(R, ER, VnR, nx, ny);
+    const double h = 0.5 * Smax;
+    return State{0.5 * (FL.r + FR.r) - h * (R.r - L.r), 0.5 * (FL.ru + FR.ru) - h * (R.r * R.u - L.r * L.u),
+                 0.5 * (FL.rv + FR.rv) - h * (R.r * R.v - L.r * L.v), 0.5 * (FL.E + FR.E) - h * (ER - EL)};
+}
+
+// hllc() on primitive states: Davis wave speeds, supersonic early-outs,
+// contact speed (Toro 10.37), star state (10.39), F_K + S_K (U*_K - U_K);
+// m_K = rho_K (S_K - Vn_K) is the mass flux through the outer wave
+FVK_HD State hllc_w(const Prim& L, const Prim& R, double nx, double ny, double g) {
+    const double VnL = L.u * nx + L.v * ny, VnR = R.u * nx + R.v * ny;
+    const double cL = sqrt(fdiv(g * L.p, L.r)), cR = sqrt(fdiv(g * R.p, R.r));
+    const double SL = smin(VnL - cL, VnR - cR);
+    const double SR = smax(VnL + cL, VnR + cR);
+    if (SL >= 0.0) return phys_flux_w(L, L.r * specific_energy(L, g), VnL, nx, ny);
+    if (SR <= 0.0) return phys_flux_w(R, R.r * specific_energy(R, g), VnR, nx, ny);
+    const double mL = L.r * (SL - VnL), mR = R.r * (SR - VnR);
+    const double Ss = fdiv(R.p - L.p + mL * VnL - mR * VnR, mL - mR);
+    const bool left = Ss >= 0.0;
+    const Prim K = left ? L : R;
+    const double SK = left ? SL : SR, VnK = left ? VnL : VnR, mK = left ? mL : mR;
+    const double VtK = K.v * nx - K.u * ny;
+    const double eK = specific_energy(K, g), EK = K.r * eK;
+    const double factor = fdiv(mK, SK - Ss);
+    const double mn = factor * Ss, mt = factor * VtK;
+    const double Es = factor * (eK + (Ss - VnK) * (Ss + fdiv(K.p, mK)));
+    const State F = phys_flux_w(K, EK, VnK, nx, ny);
+    return State{F.r + SK * (factor - K.r), F.ru + SK * ((mn * nx - mt * ny) - K.r * K.u),
+                 F.rv + SK * ((mn * ny + mt * nx) - K.r * K.v), F.E + SK * (Es - EK)};
+}
+
+template <int SCHEME>
+FVK_HD State num_flux_w(const Prim& L, const Prim& R, double nx, double ny, double g, double tol, int maxit) {
+    if constexpr (SCHEME == FLUX_HLLC) return hllc_w(L, R, nx, ny, g);
+    else if constexpr (SCHEME == FLUX_EXACT)
+        return exact_flux(from_primitive(L.r, L.u, L.v, L.p, g), from_primitive(R.r, R.u, R.v, R.p, g), nx, ny, g,
+                          tol, maxit);
+    else return rusanov_w(L, R, nx, ny, g);
+}
+
+// euler_ghost_state() of a primitive face state: farfield -> the prescribed
+// state, outflow -> copy, wall -> the normal velocity reflected
+FVK_HD Prim euler_ghost_w(int type, const double* ff, const Prim& L, double nx, double ny, double g) {
+    if (type == BC_FARFIELD) return to_prim(State{ff[0], ff[1], ff[2], ff[3]}, g);
+    if (type == BC_OUTFLOW) return L;
+    const double Vn = L.u * nx + L.v * ny;
+    return Prim{L.r, L.u - 2.0 * Vn * nx, L.v - 2.0 * Vn * ny, L.p};
+}
+
+// MUSCL face state from the cell's primitives w and the increments d_q =
+// G_q . r: w + d, or w itself where the reconstructed density or pressure
+// is not positive
+FVK_HD Prim reconstruct_w(const Prim& w, double d0, double d1, double d2, double d3) {
+    const double rho = w.r + d0, p = w.p + d3;
+    if (!(rho > 0.0) || !(p > 0.0)) return w;
+    return Prim{rho, w.u + d1, w.v + d2, p};
+}
+
+// reconstruct_w() of a cell with conserved state Uc and limited gradients
+// g0 = (rho_x, rho_y, u_x, u_y), g1 = (v_x, v_y, p_x, p_y) at offset r
+FVK_HD Prim face_state_w(const State& Uc, const double4& g0, const double4& g1, double rx, double ry, double gam) {
     const double d0 = g0.x * rx + g0.y * ry;
     const double d1 = g0.z * rx + g0.w * ry;
     const double d2 = g1.x * rx + g1.y * ry;
     const double d3 = g1.z * rx + g1.w * ry;
-    if (d0 == 0.0 && d1 == 0.0 && d2 == 0.0 && d3 == 0.0) return Uc;
-    const double u = fdiv(Uc.ru, Uc.r), v = fdiv(Uc.rv, Uc.r), p = pressure(Uc, gam);
-    const double rho = Uc.r + d0, pp = p + d3;
-    if (!(rho > 0.0) || !(pp > 0.0)) return Uc;
-    return from_primitive(rho, u + d1, v + d2, pp, gam);
-}
-
-// reconstruct() from its increments d_q = G_q . r, with the cell's
-// primitives w = (rho, u, v, p) given, as computed from Uc by the same
-// operations (u = ru / rho, v = rv / rho, p = pressure(Uc)): the same result
-// bits without the divisions.
-FVK_HD State reconstruct_inc(const State& Uc, const double4& w, double d0, double d1, double d2, double d3,
-                             double gam) {
-    if (d0 == 0.0 && d1 == 0.0 && d2 == 0.0 && d3 == 0.0) return Uc;
-    const double rho = Uc.r + d0, pp = w.w + d3;
-    if (!(rho > 0.0) || !(pp > 0.0)) return Uc;
-    return from_primitive(rho, w.y + d1, w.z + d2, pp, gam);
+    return reconstruct_w(to_prim(Uc, gam), d0, d1, d2, d3);
 }
 
 // Venkatakrishnan limiter function (extension, no reference code):
@@ -304,5 +377,25 @@ FVK_HD bool venkat_pick(VenkatMin& b, double wlo, double whi, double wc, double 
     return take;
 }
 FVK_HD double venkat_value(const VenkatMin& b) { return fdiv(b.num, b.den); }
+
+// The same pick with the allowed increments kept as extremes of the rounded
+// differences dphi = wn - wc: dlo = min(0, min dphi), dhi = max(0, max dphi).
+// Rounding is monotone, so the extreme of the rounded differences is the
+// rounded difference of the extreme, wmin - wc or wmax - wc, up to the sign
+// of a zero, which reaches neither numerator nor denominator.  A face with
+// d2 == 0 has num == den and never undercuts the minimum (which starts at
+// {1, 1} and keeps num <= den), so it needs no test.  On the device the
+// extremes of arithmetic results are single fmin/fmax instructions.
+FVK_HD void venkat_extend(double& dlo, double& dhi, double dphi) {
+    dlo = fmin(dlo, dphi);
+    dhi = fmax(dhi, dphi);
+}
+FVK_HD void venkat_pick_d(VenkatMin& b, double dlo, double dhi, double d2, double eps2) {
+    const double d1 = d2 > 0.0 ? dhi : dlo;
+    const double d11 = d1 * d1;
+    const double num = (d11 + eps2) + 2.0 * d1 * d2;
+    const double den = d11 + 2.0 * d2 * d2 + d1 * d2 + eps2;
+    if (num * b.den < b.num * den) b = VenkatMin{num, den};
+}
 
 }  // namespace fvk

@@ -46,9 +46,7 @@ __device__ __forceinline__ void tile_grad_with(const MeshView& m, double g, int 
     const State Uc{L.U[j], L.U[MG + j], L.U[2 * MG + j], L.U[3 * MG + j]};
     const double wc[4] = {L.U[j], L.W[j], L.W[MG + j], L.W[2 * MG + j]};
     double bx[4] = {0.0, 0.0, 0.0, 0.0}, by[4] = {0.0, 0.0, 0.0, 0.0};
-    double wmin[4], wmax[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) wmin[q] = wmax[q] = wc[q];
+    double dlo[4] = {0.0, 0.0, 0.0, 0.0}, dhi[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         if (en[k] == -1) continue;
@@ -86,8 +84,7 @@ __device__ __forceinline__ void tile_grad_with(const MeshView& m, double g, int 
             const double dphi = wn[q] - wc[q];
             bx[q] += egk.x * dphi;
             by[q] += egk.y * dphi;
-            wmin[q] = smin(wmin[q], wn[q]);
-            wmax[q] = smax(wmax[q], wn[q]);
+            venkat_extend(dlo[q], dhi[q], dphi);
         }
     }
     double gx[4], gy[4], lim[4];
@@ -109,7 +106,7 @@ __device__ __forceinline__ void tile_grad_with(const MeshView& m, double g, int 
             const double4 egk = EG(k);
             const double rx = egk.z, ry = egk.w;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) venkat_pick(vm[q], wmin[q], wmax[q], wc[q], gx[q] * rx + gy[q] * ry, eps2);
+            for (int q = 0; q < 4; ++q) venkat_pick_d(vm[q], dlo[q], dhi[q], gx[q] * rx + gy[q] * ry, eps2);
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) lim[q] = venkat_value(vm[q]);

@@ -11,8 +11,8 @@
 // include/HllcFlux.h:35-76, ghost states src/EulerFVMSolver.cpp:174-193), and
 // the second-order extension (LSQ gradients of rho, u, v, p with the
 // boundary neighbour at the face midpoint, the Venkatakrishnan limiter with
-// its minimum by cross-multiplication, MUSCL states with the cell-state
-// fallback).  Geometry is a function of the node coordinates X: face length,
+// its minimum by cross-multiplication, MUSCL primitive face states with the
+// cell-state fallback, and the fluxes on primitive states).  Geometry is a function of the node coordinates X: face length,
 // normal and midpoint from the face's nodes (include/UnstructuredMesh.h:
 // 89-105), cell area and centroid from the shoelace formulas over the cell's
 // polygon (src/MeshReader.cpp:32-76), evaluated relative to its first
@@ -63,8 +63,6 @@ template <class F> inline DualT<F> sqrt(DualT<F> a) {
     return {r, a.d / (F(2.0) * r)};
 }
 template <class F> inline DualT<F> fabs(DualT<F> a) { return a.v < F(0.0) ? -a : a; }
-// the value of b with the derivative of a
-template <class F> inline void set_value(DualT<F>& a, DualT<F> b) { a.v = b.v; }
 
 // std::min / std::max semantics (the first argument on a tie)
 template <class T> inline T smin(T a, T b) { return (b < a) ? b : a; }
@@ -117,6 +115,46 @@ template <class T> S4<T> hllc(const S4<T>& L, const S4<T>& R, T nx, T ny, double
             F.rv + SK * ((mn * ny + mt * nx) - UK.rv), F.E + SK * (Es - UK.E)};
 }
 
+// second order: primitive face states W = (rho, u, v, p) and the fluxes on
+// them (fvoracle.c's hllc_w / rusanov_w, the same operations)
+template <class T> struct P4 { T r, u, v, p; };
+template <class T> T specific_energy(const P4<T>& w, double g) {
+    return w.p / ((g - 1.0) * w.r) + 0.5 * (w.u * w.u + w.v * w.v);
+}
+template <class T> S4<T> phys_flux_w(const P4<T>& w, T E, T Vn, T nx, T ny) {
+    T m = w.r * Vn;
+    return {m, m * w.u + w.p * nx, m * w.v + w.p * ny, (E + w.p) * Vn};
+}
+template <class T> S4<T> rusanov_w(const P4<T>& L, const P4<T>& R, T nx, T ny, double g) {
+    T VnL = L.u * nx + L.v * ny, VnR = R.u * nx + R.v * ny;
+    T Smax = smax(fabs(VnL) + sqrt(g * L.p / L.r), fabs(VnR) + sqrt(g * R.p / R.r));
+    T EL = L.r * specific_energy(L, g), ER = R.r * specific_energy(R, g);
+    S4<T> FL = phys_flux_w(L, EL, VnL, nx, ny), FR = phys_flux_w(R, ER, VnR, nx, ny);
+    T h = 0.5 * Smax;
+    return {0.5 * (FL.r + FR.r) - h * (R.r - L.r), 0.5 * (FL.ru + FR.ru) - h * (R.r * R.u - L.r * L.u),
+            0.5 * (FL.rv + FR.rv) - h * (R.r * R.v - L.r * L.v), 0.5 * (FL.E + FR.E) - h * (ER - EL)};
+}
+template <class T> S4<T> hllc_w(const P4<T>& L, const P4<T>& R, T nx, T ny, double g) {
+    T VnL = L.u * nx + L.v * ny, VnR = R.u * nx + R.v * ny;
+    T cL = sqrt(g * L.p / L.r), cR = sqrt(g * R.p / R.r);
+    T SL = smin(VnL - cL, VnR - cR), SR = smax(VnL + cL, VnR + cR);
+    if (SL >= T(0.0)) return phys_flux_w(L, L.r * specific_energy(L, g), VnL, nx, ny);
+    if (SR <= T(0.0)) return phys_flux_w(R, R.r * specific_energy(R, g), VnR, nx, ny);
+    T mL = L.r * (SL - VnL), mR = R.r * (SR - VnR);
+    T Ss = (R.p - L.p + mL * VnL - mR * VnR) / (mL - mR);
+    const bool left = Ss >= T(0.0);
+    const P4<T>& K = left ? L : R;
+    T SK = left ? SL : SR, VnK = left ? VnL : VnR, mK = left ? mL : mR;
+    T VtK = K.v * nx - K.u * ny;
+    T eK = specific_energy(K, g), EK = K.r * eK;
+    T factor = mK / (SK - Ss);
+    T mn = factor * Ss, mt = factor * VtK;
+    T Es = factor * (eK + (Ss - VnK) * (Ss + K.p / mK));
+    S4<T> F = phys_flux_w(K, EK, VnK, nx, ny);
+    return {F.r + SK * (factor - K.r), F.ru + SK * ((mn * nx - mt * ny) - K.r * K.u),
+            F.rv + SK * ((mn * ny + mt * nx) - K.r * K.v), F.E + SK * (Es - EK)};
+}
+
 struct Mesh {
     int64_t nn, nc, nf;
     const int32_t *fnode, *fcell, *fpatch, *cptr, *cface, *nptr, *nodes;
@@ -140,6 +178,20 @@ template <class T> void prim(const S4<T>& s, double g, T* w) {
     w[1] = s.ru / s.r;
     w[2] = s.rv / s.r;
     w[3] = pressure(s, g);
+}
+
+template <class T> P4<T> ghost_w(const Mesh& m, int64_t f, const P4<T>& L, T nx, T ny) {
+    const fvk_bc& bc = m.bcs[m.fpatch[f]];
+    if (bc.type == FVK_BC_FARFIELD) {
+        const S4<T> ff{T(bc.farfield_state[0]), T(bc.farfield_state[1]), T(bc.farfield_state[2]),
+                       T(bc.farfield_state[3])};
+        T w[4];
+        prim(ff, m.p->gamma, w);
+        return {w[0], w[1], w[2], w[3]};
+    }
+    if (bc.type == FVK_BC_OUTFLOW) return L;
+    T Vn = L.u * nx + L.v * ny;
+    return {L.r, L.u - 2.0 * Vn * nx, L.v - 2.0 * Vn * ny, L.p};
 }
 
 template <class T> void residual(const Mesh& m, const T* U, const T* X, T* R) {
@@ -254,38 +306,28 @@ template <class T> void residual(const Mesh& m, const T* U, const T* X, T* R) {
         }
     }
     auto recon = [&](int64_t c, T rx, T ry) {
-        const S4<T> Uc = state(c);
         const T* gr = &G[8 * c];
         T d0 = gr[0] * rx + gr[1] * ry, d1 = gr[2] * rx + gr[3] * ry;
         T d2 = gr[4] * rx + gr[5] * ry, d3 = gr[6] * rx + gr[7] * ry;
         T w[4];
-        prim(Uc, g, w);
+        prim(state(c), g, w);
         T rho = w[0] + d0, pp = w[3] + d3;
-        if (!(rho > T(0.0)) || !(pp > T(0.0))) return Uc;
-        S4<T> Uf = from_prim(rho, w[1] + d1, w[2] + d2, pp, g);
-        // zero increments (a uniform neighbourhood) return the cell state
-        // itself; the derivative is still the MUSCL branch's, since any
-        // perturbation that makes the gradient nonzero takes that branch
-        if (d0 == T(0.0) && d1 == T(0.0) && d2 == T(0.0) && d3 == T(0.0)) {
-            set_value(Uf.r, Uc.r);
-            set_value(Uf.ru, Uc.ru);
-            set_value(Uf.rv, Uc.rv);
-            set_value(Uf.E, Uc.E);
-        }
-        return Uf;
+        if (!(rho > T(0.0)) || !(pp > T(0.0))) return P4<T>{w[0], w[1], w[2], w[3]};
+        return P4<T>{rho, w[1] + d1, w[2] + d2, pp};
     };
     std::vector<S4<T>> F(nf);
     for (int64_t f = 0; f < nf; ++f) {
         const int cl = m.fcell[2 * f], cr = m.fcell[2 * f + 1];
-        S4<T> L, Rs;
+        S4<T> Fs;
         if (P.order == 2) {
-            L = recon(cl, fmx[f] - cx[cl], fmy[f] - cy[cl]);
-            Rs = cr != -1 ? recon(cr, fmx[f] - cx[cr], fmy[f] - cy[cr]) : ghost(m, f, L, fnx[f], fny[f]);
+            const P4<T> L = recon(cl, fmx[f] - cx[cl], fmy[f] - cy[cl]);
+            const P4<T> Rs = cr != -1 ? recon(cr, fmx[f] - cx[cr], fmy[f] - cy[cr]) : ghost_w(m, f, L, fnx[f], fny[f]);
+            Fs = P.flux_scheme == FVK_FLUX_HLLC ? hllc_w(L, Rs, fnx[f], fny[f], g) : rusanov_w(L, Rs, fnx[f], fny[f], g);
         } else {
-            L = state(cl);
-            Rs = cr != -1 ? state(cr) : ghost(m, f, L, fnx[f], fny[f]);
+            const S4<T> L = state(cl);
+            const S4<T> Rs = cr != -1 ? state(cr) : ghost(m, f, L, fnx[f], fny[f]);
+            Fs = P.flux_scheme == FVK_FLUX_HLLC ? hllc(L, Rs, fnx[f], fny[f], g) : rusanov(L, Rs, fnx[f], fny[f], g);
         }
-        S4<T> Fs = P.flux_scheme == FVK_FLUX_HLLC ? hllc(L, Rs, fnx[f], fny[f], g) : rusanov(L, Rs, fnx[f], fny[f], g);
         const T A = fa[f];
         F[f] = {A * Fs.r, A * Fs.ru, A * Fs.rv, A * Fs.E};
     }

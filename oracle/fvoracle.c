@@ -777,41 +777,116 @@ static void limited_gradients(fvo_solver* s, const double* U) {
     }
 }
 
-/* Extension: MUSCL face state of cell c at offset (rx, ry): W + grad.r per
- * primitive; the cell's own conserved state when the increment is exactly
- * zero (so zero slopes reproduce the first-order path bit for bit) or when
- * the reconstructed density or pressure is not positive. */
-static st4 reconstruct(const fvo_solver* s, const double* U, int c, double rx, double ry) {
-    st4 Uc = ld4(U + 4 * c);
+/* ------------------------------------------------------------------------
+ * Extension: second-order faces on primitive face states.  A MUSCL face
+ * state is W = (rho, u, v, p); the fluxes take it as it is and form
+ * conserved quantities only where the flux uses them (equal, in exact
+ * arithmetic, to Rusanov/HLLC of from_primitive(W)).
+ * ---------------------------------------------------------------------- */
+
+/* MUSCL face state of cell c at offset (rx, ry): W + grad.r per primitive,
+ * or W itself where the reconstructed density or pressure is not positive. */
+static prim4 reconstruct_w(const fvo_solver* s, const double* U, int c, double rx, double ry) {
+    prim4 W = to_prim(ld4(U + 4 * c), s->p.gamma);
     const double* gr = s->grad8 + 8 * c;
     double d0 = gr[0] * rx + gr[1] * ry, d1 = gr[2] * rx + gr[3] * ry;
     double d2 = gr[4] * rx + gr[5] * ry, d3 = gr[6] * rx + gr[7] * ry;
-    if (d0 == 0.0 && d1 == 0.0 && d2 == 0.0 && d3 == 0.0) return Uc;
-    prim4 W = to_prim(Uc, s->p.gamma);
     double rho = W.r + d0, p = W.p + d3;
-    if (!(rho > 0.0) || !(p > 0.0)) return Uc;
-    return from_primitive(rho, W.u + d1, W.v + d2, p, s->p.gamma);
+    if (!(rho > 0.0) || !(p > 0.0)) return W;
+    prim4 o = {rho, W.u + d1, W.v + d2, p};
+    return o;
+}
+
+/* specific total energy E / rho of a primitive state */
+static inline double specific_energy(prim4 w, double g) {
+    return w.p / ((g - 1.0) * w.r) + 0.5 * (w.u * w.u + w.v * w.v);
+}
+
+/* physical flux of a primitive state with total energy E and normal velocity Vn */
+static inline st4 phys_flux_w(prim4 w, double E, double Vn, double nx, double ny) {
+    double m = w.r * Vn;
+    st4 f = {m, m * w.u + w.p * nx, m * w.v + w.p * ny, (E + w.p) * Vn};
+    return f;
+}
+
+static st4 rusanov_w(prim4 L, prim4 R, double nx, double ny, double g) {
+    double VnL = L.u * nx + L.v * ny, VnR = R.u * nx + R.v * ny;
+    double Smax = smax(fabs(VnL) + sqrt(g * L.p / L.r), fabs(VnR) + sqrt(g * R.p / R.r));
+    double EL = L.r * specific_energy(L, g), ER = R.r * specific_energy(R, g);
+    st4 FL = phys_flux_w(L, EL, VnL, nx, ny), FR = phys_flux_w(R, ER, VnR, nx, ny);
+    double h = 0.5 * Smax;
+    st4 o = {0.5 * (FL.r + FR.r) - h * (R.r - L.r), 0.5 * (FL.ru + FR.ru) - h * (R.r * R.u - L.r * L.u),
+             0.5 * (FL.rv + FR.rv) - h * (R.r * R.v - L.r * L.v), 0.5 * (FL.E + FR.E) - h * (ER - EL)};
+    return o;
+}
+
+/* HLLC on primitive states (Davis speeds, Toro 10.37-10.39); m_K =
+ * rho_K (S_K - Vn_K) is the mass flux through the outer wave */
+static st4 hllc_w(prim4 L, prim4 R, double nx, double ny, double g) {
+    double VnL = L.u * nx + L.v * ny, VnR = R.u * nx + R.v * ny;
+    double cL = sqrt(g * L.p / L.r), cR = sqrt(g * R.p / R.r);
+    double SL = smin(VnL - cL, VnR - cR);
+    double SR = smax(VnL + cL, VnR + cR);
+    if (SL >= 0.0) return phys_flux_w(L, L.r * specific_energy(L, g), VnL, nx, ny);
+    if (SR <= 0.0) return phys_flux_w(R, R.r * specific_energy(R, g), VnR, nx, ny);
+    double mL = L.r * (SL - VnL), mR = R.r * (SR - VnR);
+    double Ss = (R.p - L.p + mL * VnL - mR * VnR) / (mL - mR);
+    int left = Ss >= 0.0;
+    prim4 K = left ? L : R;
+    double SK = left ? SL : SR, VnK = left ? VnL : VnR, mK = left ? mL : mR;
+    double VtK = K.v * nx - K.u * ny;
+    double eK = specific_energy(K, g), EK = K.r * eK;
+    double factor = mK / (SK - Ss);
+    double mn = factor * Ss, mt = factor * VtK;
+    double Es = factor * (eK + (Ss - VnK) * (Ss + K.p / mK));
+    st4 F = phys_flux_w(K, EK, VnK, nx, ny);
+    st4 o = {F.r + SK * (factor - K.r), F.ru + SK * ((mn * nx - mt * ny) - K.r * K.u),
+             F.rv + SK * ((mn * ny + mt * nx) - K.r * K.v), F.E + SK * (Es - EK)};
+    return o;
+}
+
+static st4 num_flux_w(const fvk_params* p, prim4 L, prim4 R, double nx, double ny) {
+    double g = p->gamma;
+    switch (p->flux_scheme) {
+        case FVK_FLUX_HLLC: return hllc_w(L, R, nx, ny, g);
+        case FVK_FLUX_EXACT:
+            return num_flux(p, from_primitive(L.r, L.u, L.v, L.p, g), from_primitive(R.r, R.u, R.v, R.p, g), nx, ny);
+        default: return rusanov_w(L, R, nx, ny, g);
+    }
+}
+
+/* the Euler ghost state of a primitive face state: farfield -> the
+ * prescribed state, outflow -> copy, wall -> the normal velocity reflected */
+static prim4 ghost_w(const fvo_solver* s, int64_t f, prim4 L) {
+    const fvk_bc* bc = &s->bcs[s->fpatch[f]];
+    if (bc->type == FVK_BC_FARFIELD) return to_prim(ld4(bc->farfield_state), s->p.gamma);
+    if (bc->type == FVK_BC_OUTFLOW) return L;
+    double Vn = L.u * s->fnx[f] + L.v * s->fny[f];
+    prim4 o = {L.r, L.u - 2.0 * Vn * s->fnx[f], L.v - 2.0 * Vn * s->fny[f], L.p};
+    return o;
 }
 
 /* Face flux for the Euler path.  First order: the face pass of
  * EulerFVMSolver::step (src/EulerFVMSolver.cpp:64-82).  Second order: the
- * same with MUSCL face states. */
+ * same face loop over MUSCL primitive face states. */
 static st4 euler_face_flux(const fvo_solver* s, const double* U, int64_t f) {
     int cl = s->fcell[2 * f], cr = s->fcell[2 * f + 1];
-    st4 L, R;
+    st4 F;
     if (s->p.order == 2) {
-        L = reconstruct(s, U, cl, s->fmid[2 * f] - s->cxy[2 * cl], s->fmid[2 * f + 1] - s->cxy[2 * cl + 1]);
+        prim4 L = reconstruct_w(s, U, cl, s->fmid[2 * f] - s->cxy[2 * cl], s->fmid[2 * f + 1] - s->cxy[2 * cl + 1]);
+        prim4 R;
         if (cr != -1) {
             vec2 rc = right_centroid(s, f);
-            R = reconstruct(s, U, cr, s->fmid[2 * f] - rc.x, s->fmid[2 * f + 1] - rc.y);
+            R = reconstruct_w(s, U, cr, s->fmid[2 * f] - rc.x, s->fmid[2 * f + 1] - rc.y);
         } else {
-            R = ghost(s, f, L);
+            R = ghost_w(s, f, L);
         }
+        F = num_flux_w(&s->p, L, R, s->fnx[f], s->fny[f]);
     } else {
-        L = ld4(U + 4 * cl);
-        R = (cr != -1) ? ld4(U + 4 * cr) : ghost(s, f, L);
+        st4 L = ld4(U + 4 * cl);
+        st4 R = (cr != -1) ? ld4(U + 4 * cr) : ghost(s, f, L);
+        F = num_flux(&s->p, L, R, s->fnx[f], s->fny[f]);
     }
-    st4 F = num_flux(&s->p, L, R, s->fnx[f], s->fny[f]);
     double A = s->farea[f];
     st4 o = {A * F.r, A * F.ru, A * F.rv, A * F.E};
     return o;

@@ -24,12 +24,22 @@ static State add(const State& a, const State& d, double h) {
 }
 static double dot(const State& a, const State& b) { return a.r * b.r + a.ru * b.ru + a.rv * b.rv + a.E * b.E; }
 static State rand_dir() { return State{uni(-1, 1), uni(-1, 1), uni(-1, 1), uni(-1, 1)}; }
+static Prim prand() { return Prim{uni(-1, 1), uni(-1, 1), uni(-1, 1), uni(-1, 1)}; }
+static Prim padd(const Prim& a, const Prim& d, double h) {
+    return Prim{a.r + h * d.r, a.u + h * d.u, a.v + h * d.v, a.p + h * d.p};
+}
+static double pdot(const Prim& a, const Prim& b) { return a.r * b.r + a.u * b.u + a.v * b.v + a.p * b.p; }
+static double pdot(const Prim& a, const State& b) { return a.r * b.r + a.u * b.ru + a.v * b.rv + a.p * b.E; }
 
 struct Err {
     double max = 0.0;
     int n = 0;
     void add(double fd, double ad) {
         double e = std::fabs(fd - ad) / std::fmax(1e-8, std::fmax(std::fabs(fd), std::fabs(ad)));
+        if (e > max) max = e;
+        ++n;
+    }
+    void add_abs(double e) {
         if (e > max) max = e;
         ++n;
     }
@@ -40,7 +50,7 @@ int main() {
     const double g = 1.4, h = 1e-6;
     const int trials = 2000;
     // ---- fluxes, both schemes, all branches (Mach up to 2.5 reaches the supersonic early outs) ----
-    Err e_hllc, e_rus, e_wall, e_rec, e_prim, e_venk, e_edge;
+    Err e_hllc, e_rus, e_wall, e_rec, e_prim, e_venk, e_edge, e_hllc_w, e_rus_w, e_wall_w, e_same;
     int hits[4] = {0, 0, 0, 0};
     for (int t = 0; t < trials; ++t) {
         State L = rand_state(g, 2.5), R = rand_state(g, 2.5);
@@ -86,22 +96,23 @@ int main() {
             ghost_slip_wall_rev(L, nx, ny, g, w, Lb, nxb, nyb);
             e_wall.add(fd, dot(Lb, vL) + nxb * vnx + nyb * vny);
         }
-        // MUSCL reconstruction with small increments
+        // MUSCL face state with small increments (the primitive face state's adjoint)
         {
             double4 g0{uni(-.2, .2), uni(-.2, .2), uni(-.2, .2), uni(-.2, .2)};
             double4 g1{uni(-.2, .2), uni(-.2, .2), uni(-.2, .2), uni(-.2, .2)};
             double4 v0{uni(-1, 1), uni(-1, 1), uni(-1, 1), uni(-1, 1)}, v1{uni(-1, 1), uni(-1, 1), uni(-1, 1), uni(-1, 1)};
             double rx = uni(-.5, .5), ry = uni(-.5, .5), vrx = uni(-1, 1), vry = uni(-1, 1);
+            const Prim wfb = prand();
             auto f = [&](double s) {
                 double4 a0{g0.x + s * v0.x, g0.y + s * v0.y, g0.z + s * v0.z, g0.w + s * v0.w};
                 double4 a1{g1.x + s * v1.x, g1.y + s * v1.y, g1.z + s * v1.z, g1.w + s * v1.w};
                 double x = rx + s * vrx, y = ry + s * vry;
-                return dot(w, reconstruct(add(L, vL, s), a0, a1, x, y, g));
+                return pdot(wfb, face_state_w(add(L, vL, s), a0, a1, x, y, g));
             };
             double fd = (f(h) - f(-h)) / (2 * h);
             State Ub{0, 0, 0, 0};
             double db[4];
-            reconstruct_rev(L, g0, g1, rx, ry, g, w, Ub, db);
+            face_state_w_rev(L, g0, g1, rx, ry, g, wfb, Ub, db);
             // d_q = G_q . r: dG_q = db_q r, dr = sum_q db_q G_q
             double ad = dot(Ub, vL);
             ad += db[0] * (v0.x * rx + v0.y * ry) + db[1] * (v0.z * rx + v0.w * ry) + db[2] * (v1.x * rx + v1.y * ry) +
@@ -109,6 +120,41 @@ int main() {
             ad += (db[0] * g0.x + db[1] * g0.z + db[2] * g1.x + db[3] * g1.z) * vrx +
                   (db[0] * g0.y + db[1] * g0.w + db[2] * g1.y + db[3] * g1.w) * vry;
             e_rec.add(fd, ad);
+        }
+        // fluxes on primitive face states, both schemes, and the primitive slip-wall ghost
+        {
+            const Prim WL = to_prim(L, g), WR = to_prim(R, g);
+            const Prim vWL = prand(), vWR = prand();
+            for (int scheme = 0; scheme < 2; ++scheme) {
+                auto f = [&](double s) {
+                    Prim l = padd(WL, vWL, s), r = padd(WR, vWR, s);
+                    double x = nx + s * vnx, y = ny + s * vny;
+                    return dot(w, scheme ? hllc_w(l, r, x, y, g) : rusanov_w(l, r, x, y, g));
+                };
+                double fd = (f(h) - f(-h)) / (2 * h);
+                Prim Lb{0, 0, 0, 0}, Rb{0, 0, 0, 0};
+                double nxb = 0, nyb = 0;
+                if (scheme) hllc_w_rev(WL, WR, nx, ny, g, w, Lb, Rb, nxb, nyb);
+                else rusanov_w_rev(WL, WR, nx, ny, g, w, Lb, Rb, nxb, nyb);
+                (scheme ? e_hllc_w : e_rus_w).add(fd, pdot(Lb, vWL) + pdot(Rb, vWR) + nxb * vnx + nyb * vny);
+            }
+            const Prim gb = prand();
+            auto f = [&](double s) {
+                return pdot(gb, euler_ghost_w(BC_WALL, nullptr, padd(WL, vWL, s), nx + s * vnx, ny + s * vny, g));
+            };
+            double fd = (f(h) - f(-h)) / (2 * h);
+            Prim Lb{0, 0, 0, 0};
+            double nxb = 0, nyb = 0;
+            euler_ghost_w_rev(BC_WALL, WL, nx, ny, gb, Lb, nxb, nyb);
+            e_wall_w.add(fd, pdot(Lb, vWL) + nxb * vnx + nyb * vny);
+            // the primitive fluxes equal the conserved ones of from_primitive(W)
+            const State a = hllc_w(WL, WR, nx, ny, g), b = hllc(L, R, nx, ny, g);
+            const State c = rusanov_w(WL, WR, nx, ny, g), d = rusanov(L, R, nx, ny, g);
+            const double sc = 1.0 + std::fabs(b.r) + std::fabs(b.ru) + std::fabs(b.rv) + std::fabs(b.E);
+            e_same.add_abs((std::fabs(a.r - b.r) + std::fabs(a.ru - b.ru) + std::fabs(a.rv - b.rv) +
+                            std::fabs(a.E - b.E)) / sc);
+            e_same.add_abs((std::fabs(c.r - d.r) + std::fabs(c.ru - d.ru) + std::fabs(c.rv - d.rv) +
+                            std::fabs(c.E - d.E)) / sc);
         }
         // primitive map
         {
@@ -149,7 +195,11 @@ int main() {
     e_hllc.print("hllc");
     e_rus.print("rusanov");
     e_wall.print("slip_wall");
-    e_rec.print("reconstruct");
+    e_rec.print("face_state");
+    e_hllc_w.print("hllc_w");
+    e_rus_w.print("rusanov_w");
+    e_wall_w.print("slip_wall_w");
+    e_same.print("primitive_equals_conserved");
     e_prim.print("primitive");
     e_venk.print("venkatakrishnan");
     e_edge.print("edge");

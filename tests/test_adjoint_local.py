@@ -1,5 +1,6 @@
-"""The hand-written adjoints of the flux, ghost, reconstruction, limiter and
-geometry functions (finitevolume_amd/csrc/adjoint.h) against central finite
+"""The hand-written adjoints of the flux (on conserved and on primitive face
+states), ghost, face-state, limiter and geometry functions
+(finitevolume_amd/csrc/adjoint.h) against central finite
 differences of the forward functions (physics.h), compiled for the CPU with
 g++ from the same headers the kernels use.  A wrong sign, a dropped term or a
 swapped operand in any adjoint gives an O(1) error here."""
@@ -30,12 +31,21 @@ def results(tmp_path_factory):
     return res
 
 
-@pytest.mark.parametrize("name,tol", [("hllc", 1e-5), ("rusanov", 1e-5), ("slip_wall", 1e-5), ("reconstruct", 1e-4),
+@pytest.mark.parametrize("name,tol", [("hllc", 1e-5), ("rusanov", 1e-5), ("slip_wall", 1e-5), ("face_state", 1e-4),
+                                      ("hllc_w", 1e-5), ("rusanov_w", 1e-5), ("slip_wall_w", 1e-5),
                                       ("primitive", 1e-5), ("venkatakrishnan", 1e-5), ("edge", 1e-5)])
 def test_adjoint_matches_fd(results, name, tol):
     n, err = results[name]
     assert n >= 1000
     assert err < tol, f"{name}: max relative error {err:.3e}"
+
+
+def test_primitive_fluxes_equal_conserved(results):
+    # hllc_w / rusanov_w on W = to_prim(U) against hllc / rusanov on U: the
+    # same function, so they differ by rounding only (relative to the flux)
+    n, err = results["primitive_equals_conserved"]
+    assert n >= 1000
+    assert err < 1e-13, err
 
 
 def test_hllc_branches_covered(results):

@@ -309,17 +309,28 @@ def test_mass_conservation_torus():
 
 # ---- extensions: reductions to the reference path ----
 
-def test_second_order_with_zero_slopes_is_first_order():
-    """The first-order limiter (phi = 0) of the second-order path reproduces
-    the reference step bit for bit: zero increments keep the cell state."""
+@pytest.mark.parametrize("scheme", [abi.FLUX_HLLC, abi.FLUX_RUSANOV])
+def test_second_order_with_zero_slopes_is_first_order(scheme):
+    """The first-order limiter (phi = 0) of the second-order path is the
+    reference step: its face states are the cells' primitives, and the fluxes
+    on primitive states equal the reference fluxes on the conserved states up
+    to rounding.  Checked every step (state within 1e-13 relative, residual
+    within 1e-12), on a mesh with walls and far field, where a dropped term, a
+    wrong sign or a swapped side in the primitive fluxes or ghost states gives
+    an O(1) difference."""
     base = cases.case_naca(32, 16)
     m, bcs, p1, ic, _ = base
-    p2 = abi.default_params(equation=abi.EQ_EULER, flux_scheme=abi.FLUX_HLLC, cfl=0.8, order=2,
+    p1.flux_scheme = scheme
+    p2 = abi.default_params(equation=abi.EQ_EULER, flux_scheme=scheme, cfl=0.8, order=2,
                             limiter=abi.LIMITER_FIRST_ORDER)
     a = pyoracle.OracleSolver(m, bcs, p1, ic)
     b = pyoracle.OracleSolver(m, bcs, p2, ic)
-    ra, rb = a.step(20), b.step(20)
-    assert np.array_equal(a.field(), b.field()) and np.array_equal(ra, rb)
+    for _ in range(20):
+        ra, rb = a.step(1), b.step(1)
+        ua, ub = a.field(), b.field()
+        assert np.max(np.abs(ua - ub) / np.maximum(np.abs(ua), 1e-12)) <= 1e-13
+        assert np.all(np.abs(ra - rb) <= 1e-12 * np.abs(ra) + 1e-300)
+    assert np.all(ra > 0.0)  # the flow is moving
 
 
 def test_local_dt_minimum_is_reference_dt():
