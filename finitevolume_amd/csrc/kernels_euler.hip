@@ -82,6 +82,7 @@ struct EulerArgs {
     const int* tile_list;
     const int* n_tile_list;
     double* tile_partials;
+    int block;  // the workgroup size of this launch (a kernel argument, not a dispatch-packet read)
 };
 
 
@@ -584,6 +585,20 @@ __global__ __launch_bounds__(kMaxBlock) void k_e_gather(EulerArgs a) {
     }
 }
 
+// sum and minimum over a wavefront by a butterfly, in a fixed order (the
+// callers take lane 0's value; smin ignores a NaN from the partner lane, as
+// std::min ignores one in its second argument)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = smin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 // flux x length of tile face e (as k_e_face), from LDS; returns its dt candidate
 template <int SCHEME, int ORDER>
 __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L, int e, int fd,
@@ -654,7 +669,7 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
     constexpr bool PF = WPE < 4;
     const MeshView& m = a.m;
     const double g = a.gamma;
-    const int bs = blockDim.x, tid = threadIdx.x;
+    const int bs = a.block, tid = threadIdx.x;
     TileLds L;
     L.MG = m.tmax_g;
     L.M2 = m.tmax_2;
@@ -671,41 +686,56 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
         L.F = lds + 4 * MG;
     }
     L.Dt = L.F + 4 * MF;
+    // the tile's ranges: one record, one scalar load
     int c0 = 0, n = 0, hp = 0, h1 = 0, nh = 0, f0 = 0, nft = 0;
     if (t < m.ntiles) {
-        c0 = m.tstart[t];
-        n = m.tstart[t + 1] - c0;
-        hp = m.thptr[t];
-        h1 = m.th1[t];
-        nh = m.thptr[t + 1] - hp;
-        f0 = m.tfptr[t];
-        nft = m.tfptr[t + 1] - f0;
+        const int4 d0 = m.tdesc[2 * t], d1 = m.tdesc[2 * t + 1];
+        c0 = d0.x;
+        n = d0.y;
+        hp = d0.z;
+        h1 = d0.w;
+        nh = d1.x;
+        f0 = d1.y;
+        nft = d1.z;
     }
     const int ng = n + h1, nl = n + nh;
     L.ng = ng;
-    const int* te = m.tell + (int64_t)t * D * MG;
+    // the loads below are issued unconditionally, from addresses clamped into
+    // their arrays, and their values used only where they belong to a cell
+    // of the tile: with no branch around them the wait for the first ones
+    // leaves the later ones in flight (a spare workgroup past the last tile
+    // reads the last tile's rows)
+    const int ta = min(t, m.ntiles - 1), jt = min(tid, MG - 1);
+    const int* te = m.tell + (int64_t)ta * D * MG;
 
     // 0. loads issued before the staging barrier, so that their latency
-    // overlaps it: the local ELL of cell tid (its gradient, and its gather
-    // when owned), the gradient mesh data of cell tid and (PF) the mesh data
-    // of the first two faces of this thread
+    // overlaps it.  First the halo cells' device ids of this thread's first
+    // two staged cells (the states' addresses depend on them; thalo is
+    // non-empty whenever a tile has a halo); then the local ELL of cell tid
+    // (its gradient, and its gather when owned), its gradient mesh data and
+    // (PF) the mesh data of the first two faces of this thread
     // (LEAN: cells wider than triangles read their slots where they use them
     // instead, which keeps nine-slot polygons within half the register file)
+    int hc[2] = {0, 0};
+    if (m.nthalo > 0) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) hc[i] = m.thalo[min(max(hp + tid + i * bs - n, 0), m.nthalo - 1)];
+    }
     constexpr bool LEAN = D > 3;
     int pen[D];
     double4 peg[LEAN ? 1 : D], piv{0, 0, 0, 0};
-    if (!LEAN && tid < (ORDER == 2 ? ng : n)) {
+    if (!LEAN) {
 #pragma unroll
-        for (int k = 0; k < D; ++k) pen[k] = te[k * MG + tid];
+        for (int k = 0; k < D; ++k) pen[k] = te[k * MG + jt];
     }
     // tile-ordered geometry (second order only): no index load first
-    const double4* tg = ORDER == 2 ? m.tegeo + (int64_t)t * D * MG : nullptr;
-    if (ORDER == 2 && tid < ng) {
+    const double4* tg = ORDER == 2 ? m.tegeo + (int64_t)ta * D * MG : nullptr;
+    if (ORDER == 2) {
         if (!LEAN) {
 #pragma unroll
-            for (int k = 0; k < D; ++k) peg[k] = tg[k * MG + tid];
+            for (int k = 0; k < D; ++k) peg[k] = tg[k * MG + jt];
         }
-        piv = m.tclsq[(int64_t)t * MG + tid];
+        piv = m.tclsq[(int64_t)ta * MG + jt];
     }
     int pfd[2] = {0, 0};
     double4 pfg[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
@@ -719,16 +749,19 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
         }
     }
 
+    // (keep the scheduler from moving the use of the halo ids above the loads
+    // just issued, which would wait for the ids before issuing them)
+    __builtin_amdgcn_sched_barrier(0);
+
     // 1. states (and primitives) of the tile and its halo; the first two
-    // passes load before they store
+    // passes load (unconditionally, cell 0 for a thread without a cell)
+    // before they store
     auto stage_cell = [&](int j, const double4& u) { tile_stage(L, j, u, g, ORDER == 2); };
     {
         const int j0 = tid, j1 = tid + bs;
-        const int g0 = j0 < n ? c0 + j0 : (j0 < nl ? m.thalo[hp + j0 - n] : -1);
-        const int g1 = j1 < n ? c0 + j1 : (j1 < nl ? m.thalo[hp + j1 - n] : -1);
-        double4 u0{}, u1{};
-        if (g0 >= 0) u0 = a.U[g0];
-        if (g1 >= 0) u1 = a.U[g1];
+        const int g0 = j0 < n ? c0 + j0 : (j0 < nl ? hc[0] : -1);
+        const int g1 = j1 < n ? c0 + j1 : (j1 < nl ? hc[1] : -1);
+        const double4 u0 = a.U[max(g0, 0)], u1 = a.U[max(g1, 0)];
         if (g0 >= 0) stage_cell(j0, u0);
         if (g1 >= 0) stage_cell(j1, u1);
     }
@@ -808,49 +841,37 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
         update_cell_with(a, c0 + tid, upd, u, r0, r1, r2, r3, dmin);
     }
 
-    // 5. block reductions in the (now free) LDS: residual partials, dt minimum
-    int p2 = 64;  // the largest power of two not above the workgroup size
-    while (2 * p2 <= bs) p2 *= 2;
+    // 5. block reductions: a butterfly inside each wavefront, then the
+    // wavefronts' values in order (a fixed order, one barrier)
+    __shared__ double red[5][kMaxBlock / 64];
+    const int nw = bs >> 6, w = tid >> 6, lane = tid & 63;
     if (a.want_res) {
-        __syncthreads();
-        lds[tid] = r0 * r0;
-        lds[bs + tid] = r1 * r1;
-        lds[2 * bs + tid] = r2 * r2;
-        lds[3 * bs + tid] = r3 * r3;
-        __syncthreads();
-        if (tid < bs - p2) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) lds[q * bs + tid] += lds[q * bs + tid + p2];
-        }
-        __syncthreads();
-        for (int s2 = p2 / 2; s2 > 0; s2 >>= 1) {
-            if (tid < s2) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) lds[q * bs + tid] += lds[q * bs + tid + s2];
-            }
-            __syncthreads();
-        }
-        if (tid == 0) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (BATCH) a.tile_partials[(int64_t)q * m.ntiles + t] = lds[q * bs];
-                else a.partials[q * gridDim.x + blockIdx.x] = lds[q * bs];
-            }
+        const double s0 = wave_sum(r0 * r0), s1 = wave_sum(r1 * r1), s2 = wave_sum(r2 * r2), s3 = wave_sum(r3 * r3);
+        if (lane == 0) {
+            red[0][w] = s0;
+            red[1][w] = s1;
+            red[2][w] = s2;
+            red[3][w] = s3;
         }
     }
-    if (!BATCH && a.face_dt) {  // batched cases keep no step dt
-        __syncthreads();
-        lds[tid] = cand_min;
-        __syncthreads();
-        if (tid < bs - p2) lds[tid] = smin(lds[tid], lds[tid + p2]);
-        __syncthreads();
-        for (int s2 = p2 / 2; s2 > 0; s2 >>= 1) {
-            if (tid < s2) lds[tid] = smin(lds[tid], lds[tid + s2]);
-            __syncthreads();
-        }
-        // the step's dt (local dt: only its history) is reduced by the finalize
-        // from one entry per workgroup, not by device-wide atomics
-        if (tid == 0) a.partials[4 * gridDim.x + blockIdx.x] = lds[0];
+    const bool want_dt = !BATCH && a.face_dt;  // batched cases keep no step dt
+    if (want_dt) {
+        const double dmin = wave_min(cand_min);
+        if (lane == 0) red[4][w] = dmin;
+    }
+    __syncthreads();
+    if (a.want_res && tid < 4) {
+        double acc = red[tid][0];
+        for (int i = 1; i < nw; ++i) acc += red[tid][i];
+        if (BATCH) a.tile_partials[(int64_t)tid * m.ntiles + t] = acc;
+        else a.partials[tid * gridDim.x + blockIdx.x] = acc;
+    }
+    // the step's dt (local dt: only its history) is reduced by the finalize
+    // from one entry per workgroup, not by device-wide atomics
+    if (want_dt && tid == 4) {
+        double acc = red[4][0];
+        for (int i = 1; i < nw; ++i) acc = smin(acc, red[4][i]);
+        a.partials[4 * gridDim.x + blockIdx.x] = acc;
     }
 }
 
@@ -956,6 +977,7 @@ EulerArgs make_args(fvk_solver* s) {
     a.maxit = s->p.exact_riemann_max_iter;
     a.limiter = s->p.limiter;
     a.stages = s->p.rk_stages;
+    a.block = s->block;
     if (s->n_cases > 0) {
         a.case_start = s->d_case_start;
         a.active = s->d_active;

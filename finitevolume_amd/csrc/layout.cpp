@@ -352,6 +352,14 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     m.tmax_f = mf;
     m.tmax_2 = m2;
     m.twidth = width;
+    std::vector<int4> tdesc(2 * (size_t)nt);
+    for (int t = 0; t < nt; ++t) {
+        const int c0 = s->tile_start[t];
+        tdesc[2 * t] = int4{c0, s->tile_start[t + 1] - c0, thptr[t], th1[t]};
+        tdesc[2 * t + 1] = int4{thptr[t + 1] - thptr[t], tfptr[t], tfptr[t + 1] - tfptr[t], 0};
+    }
+    m.tdesc = upload(s, tdesc);
+    m.nthalo = (int)thalo.size();
     m.tstart = upload(s, s->tile_start);
     m.thptr = upload(s, thptr);
     m.thalo = thalo.empty() ? nullptr : upload(s, thalo);
@@ -368,7 +376,8 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     m.tegeo = second_order ? upload(s, tegeo) : nullptr;
     m.tclsq = second_order ? upload(s, tclsq) : nullptr;
     m.trec = second_order && periodic ? upload(s, trec) : nullptr;
-    if (!m.tstart || !m.thptr || (!thalo.empty() && !m.thalo) || !m.th1 || !m.tell || !m.tfptr || !m.tface || !m.tfc ||
+    if (!m.tdesc || !m.tstart || !m.thptr || (!thalo.empty() && !m.thalo) || !m.th1 || !m.tell || !m.tfptr ||
+        !m.tface || !m.tfc ||
         !m.tfgeo || (second_order && (!m.tegeo || !m.tclsq || (periodic && !m.trec))))
         return set_error(FVK_ERR_NOMEM, "device allocation for the tiles failed");
     // second order: U (4) and W (3: u, v, p) of the owned and ring-1 cells, W

@@ -26,7 +26,9 @@ struct TraceRange {
 
 constexpr int kMaxPatches = 1024;
 constexpr int kMaxEulerDegree = 16;  // most faces of one cell the per-cell Euler kernels take
-constexpr int kMaxTileLds = 163840;  // LDS bytes one tile may use: a gfx950 workgroup may take the CU's 160 KB
+// dynamic LDS bytes one tile may use: a gfx950 workgroup may take the CU's
+// 160 KB, less 1 KB for the tile kernels' static arrays (block reductions)
+constexpr int kMaxTileLds = 163840 - 1024;
 constexpr int kXcds = 8;             // accelerator complex dies of an MI355X, one L2 each
 
 // Workgroups are dispatched round-robin over the XCDs; the remap below gives
@@ -103,6 +105,9 @@ struct MeshView {
     int tmax_g, tmax_l, tmax_f; // most local cells with gradients (owned + ring 1), local cells, faces in a tile
     int tmax_2;                 // most ring-2 cells of a tile
     int twidth;                 // slot width of tell
+    const int4* tdesc;      // [ntiles][2] the ranges below in one record: {first cell, owned cells, first
+                            //   halo entry, ring-1 cells}, {halo cells, first tile face, tile faces, 0}
+    int nthalo;             // entries of thalo
     const int* tstart;      // [ntiles + 1]
     const int* thptr;       // [ntiles + 1] offsets into thalo
     const int* thalo;       // ring-1 then ring-2 cells of each tile (device ids)
