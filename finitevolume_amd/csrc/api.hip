@@ -64,24 +64,34 @@ void* host_mapped_alloc(fvk_solver* s, size_t bytes, void** dev) {
     return h;
 }
 
-void timer_begin(fvk_solver* s, const char* name, hipEvent_t* ev_stop) {
-    *ev_stop = nullptr;
-    if (!s->timing) return;
-    KernelTimer* t = nullptr;
+KernelTimer* timer_named(fvk_solver* s, const char* name) {
     for (auto& k : s->timers)
-        if (k.name == name) { t = &k; break; }
-    if (!t) {
-        s->timers.push_back(KernelTimer{});
-        t = &s->timers.back();
-        t->name = name;
-    }
+        if (k.name == name) return &k;
+    s->timers.push_back(KernelTimer{});
+    s->timers.back().name = name;
+    return &s->timers.back();
+}
+
+// records the start event of a pair covering `weight` launches or stages on
+// the solver's stream; the caller records *ev_stop after them
+void timer_start(fvk_solver* s, const char* name, int64_t weight, hipEvent_t* ev_stop) {
+    *ev_stop = nullptr;
+    KernelTimer* t = timer_named(s, name);
     hipEvent_t a, b;
     if (hipEventCreate(&a) != hipSuccess) return;
     if (hipEventCreate(&b) != hipSuccess) { hipEventDestroy(a); return; }
     hipEventRecord(a, s->stream);
     t->start.push_back(a);
     t->stop.push_back(b);
+    t->weight.push_back(weight);
     *ev_stop = b;
+}
+
+// timing mode 1: an event pair around one launch
+void timer_begin(fvk_solver* s, const char* name, hipEvent_t* ev_stop) {
+    *ev_stop = nullptr;
+    if (s->timing != 1) return;
+    timer_start(s, name, 1, ev_stop);
 }
 
 void timer_end(fvk_solver* s, hipEvent_t ev_stop) {
@@ -94,13 +104,14 @@ static void timers_collect(fvk_solver* s) {
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, t.start[i], t.stop[i]) == hipSuccess) {
                 t.total_ms += ms;
-                t.count += 1;
+                t.count += t.weight[i];
             }
             hipEventDestroy(t.start[i]);
             hipEventDestroy(t.stop[i]);
         }
         t.start.clear();
         t.stop.clear();
+        t.weight.clear();
     }
 }
 
@@ -592,6 +603,10 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
     while (done < nsteps) {
         int chunk = std::min(s->hist_cap, nsteps - done);
         bool early = false;
+        // timing mode 2: one event pair around the chunk's launches, counted
+        // as chunk x stages stages (no event between the kernels)
+        hipEvent_t region_stop = nullptr;
+        if (s->timing == 2) fvk::timer_start(s, "stages", (int64_t)chunk * stages, &region_stop);
         for (int i = 0; i < chunk; ++i) {
             int rc = FVK_OK;
             if (s->p.equation == FVK_EQ_DIFFUSION) {
@@ -621,6 +636,7 @@ int fvk_step(fvk_solver* s, int32_t nsteps, double* residual_hist, double* dt_hi
             int rc = flush_pending_finalize(s);
             if (rc != FVK_OK) return rc;
         }
+        if (region_stop) hipEventRecord(region_stop, s->stream);
         // the finalize kernels wrote the rows into mapped host memory: one sync per chunk
         hipError_t e = early ? hipEventSynchronize(s->row_ready) : hipStreamSynchronize(s->stream);
         if (e != hipSuccess) return hip_check(e, "step");
@@ -886,7 +902,8 @@ int fvk_set_kernel_timing(fvk_solver* s, int32_t enable) {
     if (int rc = fvk::settle(s); rc != FVK_OK) return rc;
     hipStreamSynchronize(s->stream);
     timers_collect(s);
-    s->timing = enable != 0;
+    if (enable < 0 || enable > 2) return set_error(FVK_ERR_INVALID, "timing mode %d (0, 1 or 2)", enable);
+    s->timing = enable;
     if (enable) {
         for (auto& t : s->timers) { t.total_ms = 0.0; t.count = 0; }
     }

@@ -703,9 +703,10 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
     // the loads below are issued unconditionally, from addresses clamped into
     // their arrays, and their values used only where they belong to a cell
     // of the tile: with no branch around them the wait for the first ones
-    // leaves the later ones in flight (a spare workgroup past the last tile
-    // reads the last tile's rows)
-    const int ta = min(t, m.ntiles - 1), jt = min(tid, MG - 1);
+    // leaves the later ones in flight (threads past the tile's staged cells
+    // read its last row again, from the cache; a spare workgroup past the
+    // last tile reads the last tile's first row)
+    const int ta = min(t, m.ntiles - 1), jt = min(tid, max(ng - 1, 0));
     const int* te = m.tell + (int64_t)ta * D * MG;
 
     // 0. loads issued before the staging barrier, so that their latency

@@ -157,6 +157,7 @@ struct AdjointWork;  // device buffers and index maps of the residual VJP (adjoi
 struct KernelTimer {
     std::string name;
     std::vector<hipEvent_t> start, stop;  // pending pairs
+    std::vector<int64_t> weight;          // launches (or stages) each pending pair covers
     double total_ms = 0.0;
     int64_t count = 0;
 };
@@ -252,7 +253,7 @@ struct fvk_solver {
     double* tile_partials = nullptr;             // [4][ntiles]
 
     // timing
-    bool timing = false;
+    int timing = 0;  // 1: an event pair around every launch; 2: one around each fvk_step call's stages
     std::vector<fvk::KernelTimer> timers;
 };
 

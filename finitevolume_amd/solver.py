@@ -264,8 +264,10 @@ class FVKSolver:
              "residual_final": rep.residual_final}
         return (dX, lam, r) if want_lambda else (dX, r)
 
-    def set_kernel_timing(self, on: bool):
-        _check(self.L.fvk_set_kernel_timing(self.h, 1 if on else 0))
+    def set_kernel_timing(self, mode):
+        """False/0: off; True/1: an event pair around every launch (per-kernel
+        times); 2: one pair around each step() call's launches ("stages")."""
+        _check(self.L.fvk_set_kernel_timing(self.h, int(mode)))
 
     def kernel_times(self) -> dict:
         cap = 32

@@ -447,9 +447,12 @@ int fvk_adjoint_solve_forces(fvk_solver* s, double rho_ref, double vx_ref, doubl
                              double w_cl, double w_cd, double cfl, int32_t max_iter, double tol, int32_t check_every,
                              double* dJ_dX, double* lambda_out, fvk_adjoint_report* report);
 
-/* Per-kernel device time accumulated while timing is on, measured with HIP
- * events on the solver's stream around each launch.  names is a
- * '\n'-separated list written into a caller buffer. */
+/* Device time accumulated while timing is on, measured with HIP events on
+ * the solver's stream: mode 1, an event pair around each launch (per-kernel
+ * times; the events themselves add a few microseconds to each launch);
+ * mode 2, one pair around all the launches of each fvk_step call, reported
+ * as "stages" with a count of steps x RK stages; 0 turns timing off.
+ * names is a '\n'-separated list written into a caller buffer. */
 int fvk_set_kernel_timing(fvk_solver* s, int32_t enable);
 int fvk_kernel_times(const fvk_solver* s, char* names, int64_t names_cap, double* total_ms, int64_t* count,
                      int32_t cap, int32_t* n_kernels);
