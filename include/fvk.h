@@ -89,7 +89,8 @@ enum fvk_ic_mode { FVK_IC_FREESTREAM = 0, FVK_IC_TWO_REGION = 1, FVK_IC_NONE = 2
 
 /* Extension switches (no reference code; defaults reproduce the reference). */
 /* FVK_LIMITER_FIRST_ORDER sets every slope to zero (phi = 0): the second-order
- * path then reproduces the first-order (reference) step bit for bit. */
+ * path then reproduces the first-order (reference) step up to rounding (its
+ * fluxes take primitive face states). */
 enum fvk_limiter { FVK_LIMITER_NONE = 0, FVK_LIMITER_VENKATAKRISHNAN = 1, FVK_LIMITER_FIRST_ORDER = 2 };
 
 /*
