@@ -273,7 +273,9 @@ def batch_leg(n_cases, max_steps, drop=1e-6):
             print(f"[bench] batch: step {done}, {active} cases active, {now:.1f}s", file=sys.stderr, flush=True)
 
     from finitevolume_amd import autotune
-    tuned = autotune.tune_batch(meshes, bcs, params, ics, use_cache=False)
+    tuned = autotune.tune_batch(meshes, bcs, params, ics, use_cache=False,
+                                progress=lambda name, ms: print(f"[bench] batch: autotune {name} {ms:.3f} ms/step",
+                                                                file=sys.stderr, flush=True))
     print(f"[bench] batch: autotune block {tuned['block']} ({tuned['table_ms_per_step']})", file=sys.stderr,
           flush=True)
     b = FVKBatch(meshes, bcs, params, ics)

@@ -120,12 +120,16 @@ BATCH_TILE_SHAPES = ((128, 112), (256, 192), (256, 216), (256, 240), (512, 448))
 
 
 def tune_batch(meshes: Sequence, bcs: Sequence, params: abi.ParamsC, ics: Sequence, steps: int = 10,
-               warmup: int = 2, blocks: Iterable[int] = (128, 256, 512), use_cache: bool = True) -> Dict:
-    """Fastest launch shape for a batch: the persistent fused tile kernel
-    (variant 3, SSP-RK stages only) at several tile shapes, and the face
+               warmup: int = 2, blocks: Iterable[int] = (128, 256, 512), use_cache: bool = True,
+               progress=None) -> Dict:
+    """Fastest launch shape for a batch: the fused tile kernel over the
+    active cases' tiles (variant 3, a workgroup per tile, SSP-RK stages
+    only) and its persistent form (variant 4, the workgroups that fit on the
+    device pulling tiles from a queue) at several tile shapes, and the face
     sweep + gather (variant 1) at several workgroup sizes; sets it on
     `params` too.  Variant-1 workgroups larger than the smallest case are
-    skipped."""
+    skipped.  progress(name, ms_per_step), if given, is called after each
+    shape is timed."""
     from .batch import FVKBatch
     n = sum(m.n_cells for m in meshes)
     nf = sum(m.n_faces for m in meshes)
@@ -144,16 +148,21 @@ def tune_batch(meshes: Sequence, bcs: Sequence, params: abi.ParamsC, ics: Sequen
             p.kernel_variant, p.block_size, p.tile_cells = 1, b, 0
             table[f"v1_b{b}"] = _time_solver(lambda: FVKBatch(meshes, bcs, p, ics), steps, warmup) * 1e3
             shape[f"v1_b{b}"] = (1, b, 0)
+            if progress:
+                progress(f"v1_b{b}", table[f"v1_b{b}"])
         if params.rk_stages >= 2:
-            for b, tc in BATCH_TILE_SHAPES:
-                p = abi.ParamsC.from_buffer_copy(params)
-                p.kernel_variant, p.block_size, p.tile_cells = 3, b, tc
-                try:
-                    table[f"v3_b{b}_t{tc}"] = _time_solver(lambda: FVKBatch(meshes, bcs, p, ics), steps,
-                                                           warmup) * 1e3
-                except Exception:
-                    continue  # not built for these meshes (LDS)
-                shape[f"v3_b{b}_t{tc}"] = (3, b, tc)
+            for v in (3, 4):
+                for b, tc in BATCH_TILE_SHAPES:
+                    p = abi.ParamsC.from_buffer_copy(params)
+                    p.kernel_variant, p.block_size, p.tile_cells = v, b, tc
+                    try:
+                        table[f"v{v}_b{b}_t{tc}"] = _time_solver(lambda: FVKBatch(meshes, bcs, p, ics), steps,
+                                                                 warmup) * 1e3
+                    except Exception:
+                        continue  # not built for these meshes (LDS)
+                    shape[f"v{v}_b{b}_t{tc}"] = (v, b, tc)
+                    if progress:
+                        progress(f"v{v}_b{b}_t{tc}", table[f"v{v}_b{b}_t{tc}"])
         name = min(table, key=table.get)
         v, b, tc = shape[name]
         best = {"variant": v, "block": b, "tile_cells": tc, "ms_per_step": table[name],

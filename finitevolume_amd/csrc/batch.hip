@@ -104,12 +104,18 @@ int fvk_batch_create(int32_t n_cases, const fvk_mesh_desc* meshes, const fvk_bc*
         return set_error(FVK_ERR_UNSUPPORTED, "batched cases are built for the Euler equations");
     if (!params->local_dt)
         return set_error(FVK_ERR_UNSUPPORTED, "batched cases step with local time steps (local_dt = 1)");
-    if (params->kernel_variant == 2)
+    if (params->kernel_variant == 2 || params->kernel_variant > 4)
         return set_error(FVK_ERR_UNSUPPORTED,
-                         "batched cases use the fused tile kernel (variant 3) or the face sweep + gather (variant 1)");
+                         "batched cases use the fused tile kernel (variant 3, or 4: persistent) or the face sweep + "
+                         "gather (variant 1)");
     // the fused tile kernel by default; it leaves a frozen case's state in
-    // place, which needs SSP-RK stages (forward Euler alternates buffers)
-    const int variant = params->kernel_variant != 0 ? params->kernel_variant : (params->rk_stages >= 2 ? 3 : 1);
+    // place, which needs SSP-RK stages (forward Euler alternates buffers).
+    // Variant 4 is the same kernel as one persistent launch per stage: a
+    // workgroup per CU slot pulls the active cases' tiles from a queue.
+    const bool persistent = params->kernel_variant == 4;
+    const int variant = persistent ? 3
+                                   : (params->kernel_variant != 0 ? params->kernel_variant
+                                                                  : (params->rk_stages >= 2 ? 3 : 1));
     if (variant == 3 && params->rk_stages < 2)
         return set_error(FVK_ERR_UNSUPPORTED, "batched cases on the tile kernel need rk_stages >= 2");
     // ---- union of the meshes ----
@@ -252,6 +258,15 @@ int fvk_batch_create(int32_t n_cases, const fvk_mesh_desc* meshes, const fvk_bc*
             s->tile_partials = static_cast<double*>(dev_alloc(s, sizeof(double) * 4 * std::max(nt, 1)));
             if (!s->d_tile_list || !s->d_n_tile_list || !s->d_case_tiles || !s->tile_partials)
                 rc = set_error(FVK_ERR_NOMEM, "batch: device allocation failed");
+        }
+        if (rc == FVK_OK && persistent) {
+            // the queue of the persistent launch: the next list entry and the
+            // workgroups done; zero here, and reset to zero by the last
+            // workgroup of every launch (no memset in a captured step)
+            s->tile_queue = static_cast<unsigned int*>(dev_alloc(s, 2 * sizeof(unsigned int)));
+            if (!s->tile_queue) rc = set_error(FVK_ERR_NOMEM, "batch: device allocation failed");
+            if (rc == FVK_OK) rc = hip_check(hipMemset(s->tile_queue, 0, 2 * sizeof(unsigned int)), "batch queue");
+            if (rc == FVK_OK) rc = set_persistent_grid(s);
         }
         if (rc == FVK_OK)
             rc = hip_check(hipMemcpy(s->d_case_tiles, s->case_tiles.data(), sizeof(int) * (n_cases + 1),

@@ -83,6 +83,7 @@ struct EulerArgs {
     const int* n_tile_list;
     double* tile_partials;
     int block;  // the workgroup size of this launch (a kernel argument, not a dispatch-packet read)
+    unsigned int* tile_queue;  // persistent batched launch: [0] next tile-list entry, [1] workgroups done
 };
 
 
@@ -891,6 +892,35 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
     tile_body<D, SCHEME, ORDER, WPE, true>(a, lds, __builtin_amdgcn_readfirstlane(a.tile_list[it]));
 }
 
+// batched cases, one persistent launch per stage (kernel variant 4): as many
+// workgroups as fit on the device at once, each pulling entries of the
+// active cases' tile list from a queue until it runs dry.  A workgroup
+// claims its next entry when it starts a tile, so the claim's latency
+// overlaps the tile; the claims alternate between two LDS words, so a slow
+// reader of one never sees the next claim.  Every workgroup leaves the loop
+// once a claim passes the list's end (claims only grow); the last one out
+// resets the queue to zero for the next launch.  The tiles' sums go to their
+// own rows, so the results do not depend on which workgroup ran a tile.
+template <int D, int SCHEME, int ORDER, int WPE>
+__global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_e_tile_persist(EulerArgs a) {
+    extern __shared__ double lds[];
+    __shared__ int claim[2];
+    const int n = *a.n_tile_list;
+    if (threadIdx.x == 0) claim[0] = (int)atomicAdd(&a.tile_queue[0], 1u);
+    __syncthreads();
+    int it = claim[0];
+    for (int i = 0; it < n; ++i) {
+        if (threadIdx.x == 0) claim[(i + 1) & 1] = (int)atomicAdd(&a.tile_queue[0], 1u);
+        tile_body<D, SCHEME, ORDER, WPE, true>(a, lds, __builtin_amdgcn_readfirstlane(a.tile_list[it]));
+        __syncthreads();
+        it = claim[(i + 1) & 1];
+    }
+    if (threadIdx.x == 0 && atomicAdd(&a.tile_queue[1], 1u) == gridDim.x - 1) {
+        atomicExch(&a.tile_queue[0], 0u);
+        atomicExch(&a.tile_queue[1], 0u);
+    }
+}
+
 template <int D, int SCHEME, int ORDER, int WPE>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_e_tile(EulerArgs a) {
     extern __shared__ double lds[];
@@ -1016,24 +1046,51 @@ struct Launch {
         // four wavefronts per SIMD when the CU's LDS holds enough workgroups for them
         const bool four = D == 3 && ORDER == 2 && s->p.flux_scheme != FLUX_EXACT &&
                           (int64_t)(kCuLds / std::max<size_t>(lds, 1)) * s->block >= 4 * 4 * 64;
-        // batched cases take the tile-list instance
-        auto go = [&](auto single, auto cases) {
-            const void* k = a.tile_list ? reinterpret_cast<const void*>(cases) : reinterpret_cast<const void*>(single);
+        // batched cases take the tile-list instance, or its persistent form
+        const bool persist = a.tile_list && s->tile_queue;
+        auto go = [&](auto single, auto cases, auto persistent) {
+            const void* k = persist ? reinterpret_cast<const void*>(persistent)
+                                    : a.tile_list ? reinterpret_cast<const void*>(cases)
+                                                  : reinterpret_cast<const void*>(single);
             allow_lds(k, lds);
-            if (a.tile_list) cases<<<grid, s->block, lds, s->stream>>>(a);
+            if (persist && s->persist_query) {
+                // the workgroups of this kernel that fit on the device at once
+                int per_cu = 0, dev = 0, cus = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent, s->block, lds) != hipSuccess ||
+                    hipGetDevice(&dev) != hipSuccess ||
+                    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+                    per_cu = cus = 0;
+                s->persist_grid = std::max(per_cu, 1) * std::max(cus, 1);
+                return;
+            }
+            if (persist) persistent<<<s->persist_grid, s->block, lds, s->stream>>>(a);
+            else if (a.tile_list) cases<<<grid, s->block, lds, s->stream>>>(a);
             else single<<<grid, s->block, lds, s->stream>>>(a);
         };
         if constexpr (D == 3) {
             if (four) {
-                if (s->p.flux_scheme == FLUX_HLLC) go(k_e_tile<3, FLUX_HLLC, 2, 4>, k_e_tile_cases<3, FLUX_HLLC, 2, 4>);
-                else go(k_e_tile<3, FLUX_RUSANOV, 2, 4>, k_e_tile_cases<3, FLUX_RUSANOV, 2, 4>);
+                if (s->p.flux_scheme == FLUX_HLLC)
+                    go(k_e_tile<3, FLUX_HLLC, 2, 4>, k_e_tile_cases<3, FLUX_HLLC, 2, 4>,
+                       k_e_tile_persist<3, FLUX_HLLC, 2, 4>);
+                else
+                    go(k_e_tile<3, FLUX_RUSANOV, 2, 4>, k_e_tile_cases<3, FLUX_RUSANOV, 2, 4>,
+                       k_e_tile_persist<3, FLUX_RUSANOV, 2, 4>);
                 return;
             }
         }
         switch (s->p.flux_scheme) {
-            case FLUX_HLLC: go(k_e_tile<D, FLUX_HLLC, ORDER, 1>, k_e_tile_cases<D, FLUX_HLLC, ORDER, 1>); break;
-            case FLUX_EXACT: go(k_e_tile<D, FLUX_EXACT, ORDER, 1>, k_e_tile_cases<D, FLUX_EXACT, ORDER, 1>); break;
-            default: go(k_e_tile<D, FLUX_RUSANOV, ORDER, 1>, k_e_tile_cases<D, FLUX_RUSANOV, ORDER, 1>); break;
+            case FLUX_HLLC:
+                go(k_e_tile<D, FLUX_HLLC, ORDER, 1>, k_e_tile_cases<D, FLUX_HLLC, ORDER, 1>,
+                   k_e_tile_persist<D, FLUX_HLLC, ORDER, 1>);
+                break;
+            case FLUX_EXACT:
+                go(k_e_tile<D, FLUX_EXACT, ORDER, 1>, k_e_tile_cases<D, FLUX_EXACT, ORDER, 1>,
+                   k_e_tile_persist<D, FLUX_EXACT, ORDER, 1>);
+                break;
+            default:
+                go(k_e_tile<D, FLUX_RUSANOV, ORDER, 1>, k_e_tile_cases<D, FLUX_RUSANOV, ORDER, 1>,
+                   k_e_tile_persist<D, FLUX_RUSANOV, ORDER, 1>);
+                break;
         }
     }
     static void tile(fvk_solver* s, const EulerArgs& a, int grid) {
@@ -1062,6 +1119,7 @@ void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
         a.tile_list = s->d_tile_list;
         a.n_tile_list = s->d_n_tile_list;
         a.tile_partials = s->tile_partials;
+        a.tile_queue = s->tile_queue;
         a.fin_nblocks = 0;
         timer_begin(s, "euler_tile", &ev);
         Launch<D>::tile(s, a, (s->mesh.ntiles + kXcds - 1) / kXcds * kXcds);
@@ -1153,6 +1211,19 @@ void run_stage_d(fvk_solver* s, EulerArgs& a, bool dt_pass) {
                                                          a.dt_hist, a.hist_idx, a.dtmin, a.slot);
         timer_end(s, ev);
     }
+}
+
+// sizes the persistent batched launch (variant 4) through the same dispatch
+// the stages take, without launching anything
+template <int D>
+void size_persistent(fvk_solver* s) {
+    EulerArgs a = make_args(s);
+    a.tile_list = s->d_tile_list;
+    a.n_tile_list = s->d_n_tile_list;
+    a.tile_queue = s->tile_queue;
+    s->persist_query = true;
+    Launch<D>::tile(s, a, 0);
+    s->persist_query = false;
 }
 
 void run_stage(fvk_solver* s, EulerArgs& a, bool dt_pass) {
@@ -1285,6 +1356,18 @@ __global__ void k_div_probe(const double* x, const double* y, double* q_fast, do
         q_fast[i] = fdiv(x[i], y[i]);
         q_ieee[i] = x[i] / y[i];
     }
+}
+
+int set_persistent_grid(fvk_solver* s) {
+    if (!s->tile_queue || !s->d_tile_list) return set_error(FVK_ERR_INVALID, "persistent launch without a tile queue");
+    const int deg = s->mesh.deg;
+    s->persist_grid = 0;
+    if (deg <= 3) size_persistent<3>(s);
+    else if (deg <= 4) size_persistent<4>(s);
+    else if (deg <= 9) size_persistent<9>(s);
+    else size_persistent<kMaxEulerDegree>(s);
+    if (s->persist_grid <= 0) return set_error(FVK_ERR_INVALID, "persistent launch: no grid");
+    return FVK_OK;
 }
 
 int launch_div_probe(const double* x, const double* y, double* q_fast, double* q_ieee, int64_t n, hipStream_t st) {

@@ -254,6 +254,11 @@ struct fvk_solver {
 
     // timing
     int timing = 0;  // 1: an event pair around every launch; 2: one around each fvk_step call's stages
+    // batched cases, kernel variant 4: one persistent tile launch per stage,
+    // persist_grid workgroups pulling the tile list from tile_queue
+    unsigned int* tile_queue = nullptr;
+    int persist_grid = 0;
+    bool persist_query = false;  // set_persistent_grid: the launcher sizes the grid instead of launching
     std::vector<fvk::KernelTimer> timers;
 };
 
@@ -273,6 +278,9 @@ void* dev_alloc(fvk_solver* s, size_t bytes);
 
 // ---- kernel launchers (kernels_*.hip) ----
 int launch_euler_stage(fvk_solver* s, int stage, int stages, int dt_mode_first, int want_res, int64_t step);
+// the persistent batched tile launch's grid (kernel variant 4): the
+// workgroups that fit on the device at once
+int set_persistent_grid(fvk_solver* s);
 int launch_visc_stage(fvk_solver* s, int stage, int stages, int want_res, int64_t step);
 int launch_diffusion_step(fvk_solver* s, int want_res, int64_t step);
 int launch_advdiff_step(fvk_solver* s, int want_res, int64_t step);

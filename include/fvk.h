@@ -163,7 +163,8 @@ typedef struct fvk_params {
     int32_t local_dt;              /* 0: one global CFL dt (reference); 1: per-cell local dt */
     /* ---- execution tuning (same results bit for bit; 0 = library default) ---- */
     int32_t kernel_variant;        /* Euler: 1 face sweep + per-cell gather, 2 per-cell fused flux and update,
-                                      3 one fused pass per cell tile with its halo staged in LDS (0: 1) */
+                                      3 one fused pass per cell tile with its halo staged in LDS (0: 1);
+                                      batches also 4: variant 3 as one persistent launch per stage */
     int32_t block_size;            /* threads per workgroup of the Euler kernels: 64, 128, 256 or 512,
                                       for variant 3 any multiple of 64 up to 512 (0: 512; variant 3: 256) */
     int32_t tile_cells;            /* variant 3: most cells of one tile, at most block_size (0: block_size) */
@@ -476,7 +477,10 @@ int fvk_divide_probe(int64_t n, const double* x, const double* y, double* q_fast
  * The cases' meshes are the disconnected parts of one device mesh, each a
  * contiguous range of cells; every kernel launch covers all cases, the
  * residual norms are reduced per case, and a case can be frozen (its work is
- * skipped, its state kept).  Euler only, local time steps, no periodic
+ * skipped, its state kept).  With SSP-RK stages a stage is one launch of the
+ * fused tile kernel over the active cases' tiles: a workgroup per tile
+ * (kernel_variant 3, the default) or, with kernel_variant 4, a persistent
+ * grid of the workgroups that fit on the device pulling tiles from a queue.  Euler only, local time steps, no periodic
  * faces, each case at least one workgroup of cells.  bcs holds the cases'
  * boundary conditions one after the other (n_bcs[k] for case k, whose patch
  * ids index its own); ics[k] is case k's initial condition; params are

@@ -44,18 +44,23 @@ def with_variant(params, variant, block=0, tile_cells=0):
     return p
 
 
-@pytest.mark.parametrize("variant,block,tc", [(1, 0, 0), (3, 0, 0), (3, 256, 200), (3, 512, 448)])
+@pytest.mark.parametrize("variant,block,tc", [(1, 0, 0), (3, 0, 0), (3, 256, 200), (3, 512, 448), (4, 0, 0),
+                                           (4, 256, 200), (4, 512, 448)])
 @pytest.mark.parametrize("graph", [False, True])
 def test_batch_matches_single_cases(three, graph, variant, block, tc):
-    """The persistent tile launch (variant 3, the default for SSP-RK) and the
-    face sweep + gather (variant 1) against each case alone."""
+    """The tile launch over the active cases' tiles (variant 3, the default
+    for SSP-RK), its persistent form pulling tiles from a queue (variant 4;
+    25 steps are 75 launches, so the queue must reset itself every launch,
+    under graph replay too) and the face sweep + gather (variant 1), against
+    each case alone (the single-case tile kernel for variants 3 and 4)."""
     meshes, bcs, params0, ics = three
     params = with_variant(params0, variant, block, tc)
+    single = with_variant(params0, 3 if variant == 4 else variant, block, tc)
     b = FVKBatch(meshes, bcs, params, ics)
     b.set_graph(graph)
     rb = b.step(25)
     for k in range(len(meshes)):
-        g = FVKSolver(meshes[k], bcs[k], params, ics[k])
+        g = FVKSolver(meshes[k], bcs[k], single, ics[k])
         rg = g.step(25)
         assert np.array_equal(b.field(k), g.field()), k
         assert np.allclose(rb[:, k, :], rg, rtol=1e-13, atol=1e-13 * np.abs(rg).max()), k
@@ -71,7 +76,7 @@ def test_batch_case_matches_oracle(three):
     assert cases.rel_state_diff(b.field(1), o.field(), floor=1e-12) <= 1e-12
 
 
-@pytest.mark.parametrize("variant", [1, 3])
+@pytest.mark.parametrize("variant", [1, 3, 4])
 def test_frozen_case_keeps_state(three, variant):
     meshes, bcs, params0, ics = three
     params = with_variant(params0, variant)
@@ -83,7 +88,7 @@ def test_frozen_case_keeps_state(three, variant):
     r = b.step(10)
     assert np.array_equal(b.field(1), U1)
     assert np.all(r[:, 1, :] == 0.0)
-    g0 = FVKSolver(meshes[0], bcs[0], params, ics[0])
+    g0 = FVKSolver(meshes[0], bcs[0], with_variant(params0, 3 if variant == 4 else variant), ics[0])
     g0.step(20)
     assert np.array_equal(b.field(0), g0.field())
 
