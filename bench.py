@@ -232,7 +232,7 @@ def _gen_batch_case(prm):
     return V.naca_voronoi(prm["n_target"], prm["seed"])
 
 
-def batch_leg(n_cases, max_steps, drop=1e-6):
+def batch_leg(n_cases, max_steps, drop=1e-6, shape=None):
     """Batched airfoil cases: case k a Voronoi mesh of about N_k cells around
     NACA 0012 at its own Mach number and angle of attack (default_rng(1000 +
     k)), config-2 numerics; every case runs until its density residual has
@@ -273,11 +273,14 @@ def batch_leg(n_cases, max_steps, drop=1e-6):
             print(f"[bench] batch: step {done}, {active} cases active, {now:.1f}s", file=sys.stderr, flush=True)
 
     from finitevolume_amd import autotune
-    tuned = autotune.tune_batch(meshes, bcs, params, ics, use_cache=False,
-                                progress=lambda name, ms: print(f"[bench] batch: autotune {name} {ms:.3f} ms/step",
-                                                                file=sys.stderr, flush=True))
-    print(f"[bench] batch: autotune block {tuned['block']} ({tuned['table_ms_per_step']})", file=sys.stderr,
-          flush=True)
+    if shape:  # a given launch shape (variant, block, tile cells) instead of the autotuner's
+        params.kernel_variant, params.block_size, params.tile_cells = shape
+        tuned = {"variant": shape[0], "block": shape[1], "tile_cells": shape[2], "given": True}
+    else:
+        tuned = autotune.tune_batch(meshes, bcs, params, ics, use_cache=False,
+                                    progress=lambda name, ms: print(f"[bench] batch: autotune {name} {ms:.3f} ms/step",
+                                                                    file=sys.stderr, flush=True))
+    print(f"[bench] batch: launch shape {tuned}", file=sys.stderr, flush=True)
     b = FVKBatch(meshes, bcs, params, ics)
     b.set_graph(True)
     rb = b.run_to_convergence(drop, max_steps, chunk=100, progress=progress)
@@ -511,7 +514,8 @@ def main():
         sys.exit(2)
 
     if args.batch:
-        line = batch_leg(args.batch_cases, args.batch_max_steps)
+        line = batch_leg(args.batch_cases, args.batch_max_steps,
+                         shape=(args.variant, args.block, args.tile_cells) if args.variant else None)
         line.update({"metric": "batched airfoil cases: wall time to convergence, batched vs sequential",
                      "n_gpus": 1, "dtype": "f64", "data": "synthetic (generated Voronoi meshes, free-stream start)"})
         print(json.dumps(line))

@@ -75,7 +75,7 @@ class LaunchShapeC(C.Structure):
     _fields_ = [
         ("kernel_variant", C.c_int32), ("block_size", C.c_int32), ("tile_cells", C.c_int32),
         ("n_tiles", C.c_int32), ("tile_max_staged", C.c_int32), ("tile_max_ring2", C.c_int32),
-        ("tile_max_faces", C.c_int32), ("reserved0", C.c_int32), ("tile_lds_bytes", C.c_int64),
+        ("tile_max_faces", C.c_int32), ("persistent_grid", C.c_int32), ("tile_lds_bytes", C.c_int64),
     ]
 
 

@@ -1,9 +1,10 @@
 """Launch-shape autotuner for the Euler stage kernels.
 
 For the mesh and numerics at hand it times every kernel variant (face sweep
-+ gather, fused per cell, fused per cell tile) and workgroup size (64 to
-512 threads; for tiles, pairs of workgroup size and tile capacity) for a
-few steps, and keeps the fastest.  Choices
++ gather, fused per cell, fused per cell tile; for batches also the tile
+kernel as one persistent launch per stage) and workgroup size (64 to 512
+threads; for tiles, pairs of workgroup size and tile capacity) for a few
+steps, and keeps the fastest.  Choices
 are cached in a JSON file (FVK_TUNE_CACHE, default
 ~/.cache/fvk/autotune.json) under a key made of the device name, the mesh
 size and largest cell degree, and the numerics that change the kernels, so

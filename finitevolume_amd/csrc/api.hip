@@ -248,7 +248,8 @@ int create_solver(const fvk_mesh_desc* mesh, const fvk_bc* bcs, int32_t n_bcs, c
     if (n_bcs > 0 && !bcs) return set_error(FVK_ERR_INVALID, "bcs is NULL");
 
     if (params->kernel_variant < 0 || params->kernel_variant > 3)
-        return set_error(FVK_ERR_INVALID, "kernel_variant must be 0, 1, 2 or 3 (got %d)", params->kernel_variant);
+        return set_error(FVK_ERR_INVALID, "kernel_variant must be 0, 1, 2 or 3 (got %d; the persistent tile launch, "
+                         "4, is for batches: fvk_batch_create)", params->kernel_variant);
     if (params->kernel_variant == 3 && (scalar_eq(params->equation) || (cell_case && params->equation != FVK_EQ_EULER)))
         return set_error(FVK_ERR_UNSUPPORTED,
                          "the fused tile kernels (variant 3) are built for Euler, Navier-Stokes and RANS cases "
@@ -678,7 +679,7 @@ double fvk_gas_gamma(const fvk_solver* s) { return s ? s->p.gamma : NAN; }
 int fvk_get_launch_shape(const fvk_solver* s, fvk_launch_shape* shape) {
     if (!s || !shape) return set_error(FVK_ERR_INVALID, "NULL argument");
     fvk_launch_shape o{};
-    o.kernel_variant = s->variant;
+    o.kernel_variant = s->variant == 3 && s->tile_queue ? 4 : s->variant;  // 4: a persistent batch
     o.block_size = s->block;
     if (s->variant == 3) {
         o.tile_cells = s->tile_cells;
@@ -687,6 +688,7 @@ int fvk_get_launch_shape(const fvk_solver* s, fvk_launch_shape* shape) {
         o.tile_max_ring2 = s->mesh.tmax_2;
         o.tile_max_faces = s->mesh.tmax_f;
         o.tile_lds_bytes = (int64_t)s->tile_lds;
+        o.persistent_grid = s->tile_queue ? s->persist_grid : 0;
     }
     *shape = o;
     return FVK_OK;

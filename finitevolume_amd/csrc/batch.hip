@@ -263,9 +263,9 @@ int fvk_batch_create(int32_t n_cases, const fvk_mesh_desc* meshes, const fvk_bc*
             // the queue of the persistent launch: the next list entry and the
             // workgroups done; zero here, and reset to zero by the last
             // workgroup of every launch (no memset in a captured step)
-            s->tile_queue = static_cast<unsigned int*>(dev_alloc(s, 2 * sizeof(unsigned int)));
+            s->tile_queue = static_cast<unsigned int*>(dev_alloc(s, kQueueWords * sizeof(unsigned int)));
             if (!s->tile_queue) rc = set_error(FVK_ERR_NOMEM, "batch: device allocation failed");
-            if (rc == FVK_OK) rc = hip_check(hipMemset(s->tile_queue, 0, 2 * sizeof(unsigned int)), "batch queue");
+            if (rc == FVK_OK) rc = hip_check(hipMemset(s->tile_queue, 0, kQueueWords * sizeof(unsigned int)), "batch queue");
             if (rc == FVK_OK) rc = set_persistent_grid(s);
         }
         if (rc == FVK_OK)

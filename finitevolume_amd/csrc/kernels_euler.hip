@@ -26,6 +26,7 @@
 // L2) sweeps one contiguous range of the RCM-ordered cells or faces.
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <mutex>
 #include <unordered_map>
 
@@ -83,7 +84,7 @@ struct EulerArgs {
     const int* n_tile_list;
     double* tile_partials;
     int block;  // the workgroup size of this launch (a kernel argument, not a dispatch-packet read)
-    unsigned int* tile_queue;  // persistent batched launch: [0] next tile-list entry, [1] workgroups done
+    unsigned int* tile_queue;  // persistent launches: per-XCD claim counters, then the workgroups done
 };
 
 
@@ -665,12 +666,14 @@ __device__ __forceinline__ double tile_face(const EulerArgs& a, const TileLds& L
 // small enough that four workgroups share a CU's LDS) it is loaded once the
 // thread's gradients are done, before the barrier that ends them; triangles
 // with the HLLC and Rusanov fluxes fit so without spills.
+// tid_: the thread index, when the caller has one (a persistent loop makes
+// it opaque per tile); else threadIdx.x
 template <int D, int SCHEME, int ORDER, int WPE, bool BATCH>
-__device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t) {
+__device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t, int tid_ = -1) {
     constexpr bool PF = WPE < 4;
     const MeshView& m = a.m;
     const double g = a.gamma;
-    const int bs = a.block, tid = threadIdx.x;
+    const int bs = a.block, tid = tid_ >= 0 ? tid_ : (int)threadIdx.x;
     TileLds L;
     L.MG = m.tmax_g;
     L.M2 = m.tmax_2;
@@ -892,33 +895,57 @@ __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))
     tile_body<D, SCHEME, ORDER, WPE, true>(a, lds, __builtin_amdgcn_readfirstlane(a.tile_list[it]));
 }
 
+// The queues of the persistent launches (kernel variant 4): the items
+// (tiles, or entries of a tile list) split into eight contiguous ranges, one
+// per XCD, each with its claim counter on a cache line of its own, then a
+// count of the workgroups done.  A workgroup takes the items of its XCD's
+// range in order (blockIdx.x mod 8, the XCD the dispatcher places it on), so
+// neighbouring tiles, which share halo cells, run in one L2.
+// the next item of this workgroup's range, or INT_MAX once it is drained
+__device__ __forceinline__ int queue_claim(unsigned int* q, int n) {
+    const int x = blockIdx.x % kXcds;
+    const int per = (n + kXcds - 1) / kXcds;
+    const int k = (int)atomicAdd(&q[x * kQueueStride], 1u);
+    const int t = x * per + k;
+    return (k < per && t < n) ? t : INT_MAX;
+}
+
+// thread 0 of every workgroup, after its last claim: the last one out resets
+// the queue to zero for the next launch (no memset in a captured step)
+__device__ __forceinline__ void queue_release(unsigned int* q) {
+    if (atomicAdd(&q[kXcds * kQueueStride], 1u) == gridDim.x - 1) {
+        for (int x = 0; x < kXcds; ++x) atomicExch(&q[x * kQueueStride], 0u);
+        atomicExch(&q[kXcds * kQueueStride], 0u);
+    }
+}
+
 // batched cases, one persistent launch per stage (kernel variant 4): as many
-// workgroups as fit on the device at once, each pulling entries of the
-// active cases' tile list from a queue until it runs dry.  A workgroup
-// claims its next entry when it starts a tile, so the claim's latency
-// overlaps the tile; the claims alternate between two LDS words, so a slow
-// reader of one never sees the next claim.  Every workgroup leaves the loop
-// once a claim passes the list's end (claims only grow); the last one out
-// resets the queue to zero for the next launch.  The tiles' sums go to their
-// own rows, so the results do not depend on which workgroup ran a tile.
+// workgroups as fit on the device at once, each taking entries of the active
+// cases' tile list from its XCD's queue until the range runs dry.  A
+// workgroup claims its next entry when it starts a tile, so the claim's
+// latency overlaps the tile; the claims alternate between two LDS words, so
+// a slow reader of one never sees the next claim.  Every workgroup leaves the
+// loop once its range is drained (claims only grow).  The tiles' sums go to
+// their own rows, so the results do not depend on which workgroup ran a tile.
 template <int D, int SCHEME, int ORDER, int WPE>
 __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(WPE))) void k_e_tile_persist(EulerArgs a) {
     extern __shared__ double lds[];
     __shared__ int claim[2];
     const int n = *a.n_tile_list;
-    if (threadIdx.x == 0) claim[0] = (int)atomicAdd(&a.tile_queue[0], 1u);
+    if (threadIdx.x == 0) claim[0] = queue_claim(a.tile_queue, n);
     __syncthreads();
     int it = claim[0];
     for (int i = 0; it < n; ++i) {
-        if (threadIdx.x == 0) claim[(i + 1) & 1] = (int)atomicAdd(&a.tile_queue[0], 1u);
-        tile_body<D, SCHEME, ORDER, WPE, true>(a, lds, __builtin_amdgcn_readfirstlane(a.tile_list[it]));
+        if (threadIdx.x == 0) claim[(i + 1) & 1] = queue_claim(a.tile_queue, n);
+        // the thread index made opaque per tile, so that nothing derived from
+        // it is kept live across the loop
+        int tid = threadIdx.x;
+        __asm__ volatile("" : "+v"(tid));
+        tile_body<D, SCHEME, ORDER, WPE, true>(a, lds, __builtin_amdgcn_readfirstlane(a.tile_list[it]), tid);
         __syncthreads();
         it = claim[(i + 1) & 1];
     }
-    if (threadIdx.x == 0 && atomicAdd(&a.tile_queue[1], 1u) == gridDim.x - 1) {
-        atomicExch(&a.tile_queue[0], 0u);
-        atomicExch(&a.tile_queue[1], 0u);
-    }
+    if (threadIdx.x == 0) queue_release(a.tile_queue);
 }
 
 template <int D, int SCHEME, int ORDER, int WPE>
@@ -1009,6 +1036,7 @@ EulerArgs make_args(fvk_solver* s) {
     a.limiter = s->p.limiter;
     a.stages = s->p.rk_stages;
     a.block = s->block;
+    a.tile_queue = s->tile_queue;
     if (s->n_cases > 0) {
         a.case_start = s->d_case_start;
         a.active = s->d_active;
@@ -1047,6 +1075,7 @@ struct Launch {
         const bool four = D == 3 && ORDER == 2 && s->p.flux_scheme != FLUX_EXACT &&
                           (int64_t)(kCuLds / std::max<size_t>(lds, 1)) * s->block >= 4 * 4 * 64;
         // batched cases take the tile-list instance, or its persistent form
+        // (variant 4) when the solver has a tile queue
         const bool persist = a.tile_list && s->tile_queue;
         auto go = [&](auto single, auto cases, auto persistent) {
             const void* k = persist ? reinterpret_cast<const void*>(persistent)
@@ -1054,13 +1083,22 @@ struct Launch {
                                                   : reinterpret_cast<const void*>(single);
             allow_lds(k, lds);
             if (persist && s->persist_query) {
-                // the workgroups of this kernel that fit on the device at once
-                int per_cu = 0, dev = 0, cus = 0;
-                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, persistent, s->block, lds) != hipSuccess ||
-                    hipGetDevice(&dev) != hipSuccess ||
-                    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-                    per_cu = cus = 0;
-                s->persist_grid = std::max(per_cu, 1) * std::max(cus, 1);
+                // the workgroups of this kernel that fit on the device at once:
+                // per CU, as many as its LDS (160 KB) holds and as many as its
+                // four SIMDs' registers hold (512 VGPRs per lane each, in
+                // granules of 8), times the CUs
+                hipFuncAttributes fa{};
+                int dev = 0, cus = 0;
+                if (hipFuncGetAttributes(&fa, k) != hipSuccess || hipGetDevice(&dev) != hipSuccess ||
+                    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+                    s->persist_grid = 0;
+                    return;
+                }
+                const int vgpr = std::max(8, (fa.numRegs + 7) / 8 * 8);
+                const int waves_simd = std::min(8, 512 / vgpr);
+                const int by_regs = 4 * waves_simd / std::max(1, s->block / 64);
+                const int by_lds = (int)(kCuLds / std::max<size_t>(1, lds + fa.sharedSizeBytes));
+                s->persist_grid = (std::max(1, std::min(by_regs, by_lds)) * cus + kXcds - 1) / kXcds * kXcds;
                 return;
             }
             if (persist) persistent<<<s->persist_grid, s->block, lds, s->stream>>>(a);
@@ -1359,7 +1397,7 @@ __global__ void k_div_probe(const double* x, const double* y, double* q_fast, do
 }
 
 int set_persistent_grid(fvk_solver* s) {
-    if (!s->tile_queue || !s->d_tile_list) return set_error(FVK_ERR_INVALID, "persistent launch without a tile queue");
+    if (!s->tile_queue) return set_error(FVK_ERR_INVALID, "persistent launch without a tile queue");
     const int deg = s->mesh.deg;
     s->persist_grid = 0;
     if (deg <= 3) size_persistent<3>(s);

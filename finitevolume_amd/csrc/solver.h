@@ -278,9 +278,13 @@ void* dev_alloc(fvk_solver* s, size_t bytes);
 
 // ---- kernel launchers (kernels_*.hip) ----
 int launch_euler_stage(fvk_solver* s, int stage, int stages, int dt_mode_first, int want_res, int64_t step);
-// the persistent batched tile launch's grid (kernel variant 4): the
-// workgroups that fit on the device at once
+// the persistent tile launches' grid (kernel variant 4): the workgroups
+// that fit on the device at once
 int set_persistent_grid(fvk_solver* s);
+// their queue: a claim counter per XCD, kQueueStride words apart (a cache
+// line each), then the count of workgroups done; kQueueWords in all
+constexpr int kQueueStride = 16;
+constexpr int kQueueWords = (kXcds + 1) * kQueueStride;
 int launch_visc_stage(fvk_solver* s, int stage, int stages, int want_res, int64_t step);
 int launch_diffusion_step(fvk_solver* s, int want_res, int64_t step);
 int launch_advdiff_step(fvk_solver* s, int want_res, int64_t step);

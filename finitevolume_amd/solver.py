@@ -187,7 +187,7 @@ class FVKSolver:
         """Kernel variant, workgroup size and (variant 3) tile layout in use."""
         sh = abi.LaunchShapeC()
         _check(self.L.fvk_get_launch_shape(self.h, C.byref(sh)))
-        return {f: getattr(sh, f) for f, _ in abi.LaunchShapeC._fields_ if f != "reserved0"}
+        return {f: getattr(sh, f) for f, _ in abi.LaunchShapeC._fields_}
 
     # ---- extensions ----
     def residual_eval(self) -> np.ndarray:

@@ -309,14 +309,14 @@ int fvk_run(fvk_solver* s, const fvk_run_config* cfg, fvk_run_result* result);
  * reference: it reports what the defaults or a tuner chose, and the tile
  * layout of variant 3). */
 typedef struct fvk_launch_shape {
-    int32_t kernel_variant;        /* 1, 2 or 3 */
+    int32_t kernel_variant;        /* 1, 2, 3 or 4 */
     int32_t block_size;            /* threads per workgroup */
     int32_t tile_cells;            /* variant 3: most owned cells of a tile (else 0) */
     int32_t n_tiles;               /* variant 3: number of tiles (else 0) */
     int32_t tile_max_staged;       /* variant 3: most owned + ring-1 cells of a tile */
     int32_t tile_max_ring2;        /* variant 3, order 2: most ring-2 cells of a tile */
     int32_t tile_max_faces;        /* variant 3: most faces of a tile */
-    int32_t reserved0;
+    int32_t persistent_grid;       /* variant 4: workgroups of the persistent launch (else 0) */
     int64_t tile_lds_bytes;        /* variant 3: dynamic LDS per workgroup */
 } fvk_launch_shape;
 int fvk_get_launch_shape(const fvk_solver* s, fvk_launch_shape* shape);
