@@ -55,8 +55,13 @@ def test_write_after_step():
     assert np.array_equal(r_a, r_b)
 
 
-def test_viscous_reads_between_steps():
+@pytest.mark.parametrize("variant", [1, 3])
+def test_viscous_reads_between_steps(variant):
+    """The SA solver, split passes (1) and the fused tile pass (3): reads,
+    a residual evaluation and a write between single steps give the bits of
+    an uninterrupted run."""
     m, bcs, p, ic, U0 = cases.case_rans_wavy(abi.EQ_RANS_SA, rk_stages=3, local_dt=1)
+    p.kernel_variant = variant
     ref = FVKSolver(m, bcs, p, ic)
     g = FVKSolver(m, bcs, p, ic)
     if U0 is not None:
@@ -64,7 +69,17 @@ def test_viscous_reads_between_steps():
         g.set_field(U0)
     r_ref = ref.step(10)
     U_ref = ref.field()
-    rows = [g.step(1)[0] for _ in range(10)]
+    R_ref = ref.residual_eval()
+    rows = []
+    for k in range(10):
+        rows.append(g.step(1)[0])
+        assert np.array_equal(g.residual(), rows[-1])
+        if k % 3 == 1:
+            g.field()  # a read while the step's last stages may still run
+        if k == 5:
+            U6, T6 = g.field()
+            g.set_field(U6, T6)  # a write of the same state: nothing in flight may overwrite it
     assert np.array_equal(np.array(rows), r_ref)
     for a, b in zip(g.field(), U_ref):
         assert np.array_equal(a, b)
+    assert np.array_equal(g.residual_eval(), R_ref)
