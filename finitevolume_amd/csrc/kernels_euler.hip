@@ -719,14 +719,15 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
     // non-empty whenever a tile has a halo); then the local ELL of cell tid
     // (its gradient, and its gather when owned), its gradient mesh data and
     // (PF) the mesh data of the first two faces of this thread
-    // (LEAN: cells wider than triangles read their slots where they use them
-    // instead, which keeps nine-slot polygons within half the register file)
+    // (up to nine slots per cell are held in registers, so their loads run
+    // ahead of their use; LEAN: cells wider than that read their slots
+    // where they use them, which keeps sixteen-slot cells from spilling)
     int hc[2] = {0, 0};
     if (m.nthalo > 0) {
 #pragma unroll
         for (int i = 0; i < 2; ++i) hc[i] = m.thalo[min(max(hp + tid + i * bs - n, 0), m.nthalo - 1)];
     }
-    constexpr bool LEAN = D > 3;
+    constexpr bool LEAN = D > 9;
     int pen[D];
     double4 peg[LEAN ? 1 : D], piv{0, 0, 0, 0};
     if (!LEAN) {
