@@ -8,6 +8,8 @@
 //                face gradients), turbulence advection/diffusion, dt candidate
 //   k_visc_cell  per cell: gather in Cell::faces order + volumetric sources +
 //                stage update + residual partials
+// or (kernel variant 3, local or no dt) one fused pass over cell tiles,
+// k_visc_tile, which does all three per tile in LDS.
 #include <cfloat>
 
 #include "norms.h"
@@ -43,7 +45,7 @@ struct ViscArgs {
     int face_dt, dt_src, first, want_res, slot;
     int64_t hist_idx;
     // the previous stage's norms, reduced by a spare workgroup of the
-    // gradient launch (fin_nblocks == 0: none)
+    // stage's first launch (gradient or tile; fin_nblocks == 0: none)
     int fin_nblocks, fin_slot, fin_nv;
     int fin_dt_row;    // the step's dt is the minimum of the partials' row fin_nv (local dt)
     int64_t fin_hist_idx;
@@ -178,10 +180,69 @@ __device__ __forceinline__ State visc_ghost(const DevBC& bc, const State& L, dou
 
 // ---------------------------------------------------------------------------
 // gradients (GradientCalculator::compute, src/GradientReconstruction.cpp:83-150)
-// of NF fields at once, plus per-cell turbulence sources.  PERIODIC selects
-// nothing any more (the per-entry LSQ data carry the periodic shift); the
-// launcher keeps the two instances
+// of NF fields at once, plus per-cell turbulence sources.  The per-cell pass
+// and the viscous tile kernel share these steps, so both give the same bits.
 // ---------------------------------------------------------------------------
+// One slot's term of the sums.  Least squares: the per-entry weight times
+// offset (geo = egeo: {w dx, w dy, ..}, periodic shift included) times the
+// difference to the neighbour (or boundary) value.  Green-Gauss: the face
+// value (distance-weighted, or the boundary value) times the normal pointing
+// away from the cell times the length (geo = {the cell's weight, n_x, n_y, A}).
+template <int NF, int GS>
+__device__ __forceinline__ void grad_slot(double (&sx)[NF], double (&sy)[NF], const double (&pc)[NF],
+                                          const double (&pn)[NF], bool bnd, const double4& geo) {
+    if (GS == FVK_GRAD_LEAST_SQUARES) {
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            const double dphi = pn[q] - pc[q];
+            sx[q] += geo.x * dphi;
+            sy[q] += geo.y * dphi;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NF; ++q) {
+            const double pf = bnd ? pn[q] : geo.x * pc[q] + (1.0 - geo.x) * pn[q];
+            sx[q] += pf * geo.y * geo.w;
+            sy[q] += pf * geo.z * geo.w;
+        }
+    }
+}
+
+// the gradients from the sums: iv = {LSQ inverse i0, i1, i2, .} (least
+// squares) or {volume, ...} (Green-Gauss)
+template <int NF, int GS>
+__device__ __forceinline__ void grad_finish(const double (&sx)[NF], const double (&sy)[NF], const double4& iv,
+                                            double (&gr)[2 * NF]) {
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+        if (GS == FVK_GRAD_LEAST_SQUARES) {
+            gr[2 * q] = iv.x * sx[q] + iv.y * sy[q];
+            gr[2 * q + 1] = iv.y * sx[q] + iv.z * sy[q];
+        } else {
+            gr[2 * q] = fdiv(sx[q], iv.x);
+            gr[2 * q + 1] = fdiv(sy[q], iv.x);
+        }
+    }
+}
+
+// a cell's turbulence sources from its density, fields and gradients:
+// SA's source in o[0]; SST's k and omega sources, F1 and nu_t in o[0..3]
+template <int EQ, int NF>
+__device__ __forceinline__ void visc_sources(const ViscArgs& a, double rho, double t0c, double t1c, double wd,
+                                             const double (&gr)[2 * NF], double (&o)[4]) {
+    if (EQ == EQ_SA) {
+        const double nul = fdiv(a.mu, rho);
+        const double om = fabs(gr[2] - gr[1]);
+        o[0] = sa_source(t0c, nul, om, wd, gr[6], gr[7], a.sa);
+    }
+    if (EQ == EQ_SST) {
+        const double nul = fdiv(a.mu, rho);
+        sst_source(rho, t0c, t1c, nul, wd, gr, gr + 2, gr + 6, gr + 2 * NF - 2, a.sst_variant, a.kato, a.sst, o);
+    }
+}
+
+// PERIODIC selects nothing any more (the per-entry LSQ data carry the
+// periodic shift); the launcher keeps the two instances
 template <int EQ, int GS, bool PERIODIC>
 __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
     constexpr int NF = EQ == EQ_SST ? 5 : (EQ == EQ_SA ? 4 : 3);
@@ -236,58 +297,31 @@ __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
             if (EQ == EQ_SA) pn[3] = a.U[4 * a.ld + o];
             if (EQ == EQ_SST) { pn[3] = a.U[4 * a.ld + o]; pn[NF - 1] = a.U[5 * a.ld + o]; }
         }
-        if (GS == FVK_GRAD_LEAST_SQUARES) {
-            const double4 eg = m.egeo[k * ldm + c];  // {w dx, w dy, ...}, w = 1/(dx^2 + dy^2)
-#pragma unroll
-            for (int q = 0; q < NF; ++q) {
-                double dphi = pn[q] - pc[q];
-                sx[q] += eg.x * dphi;
-                sy[q] += eg.y * dphi;
-            }
-        } else {
+        double4 geo;
+        if (GS == FVK_GRAD_LEAST_SQUARES) geo = m.egeo[k * ldm + c];  // {w dx, w dy, ...}, w = 1/(dx^2 + dy^2)
+        else {
             const bool owner = en.x & 1;
-            const double wc = owner ? m.fwl[f] : m.fwr[f];
-            const double nx = owner ? m.fnx[f] : -m.fnx[f];
-            const double ny = owner ? m.fny[f] : -m.fny[f];
-            const double A = m.fa[f];
-#pragma unroll
-            for (int q = 0; q < NF; ++q) {
-                double pf = (cr < 0) ? pn[q] : wc * pc[q] + (1.0 - wc) * pn[q];
-                sx[q] += pf * nx * A;
-                sy[q] += pf * ny * A;
-            }
+            geo = double4{owner ? m.fwl[f] : m.fwr[f], owner ? m.fnx[f] : -m.fnx[f], owner ? m.fny[f] : -m.fny[f],
+                          m.fa[f]};
         }
+        grad_slot<NF, GS>(sx, sy, pc, pn, cr < 0, geo);
     }
     double gr[2 * NF];
-    if (GS == FVK_GRAD_LEAST_SQUARES) {
-        const double i0 = m.lsq[c], i1 = m.lsq[m.nc + c], i2 = m.lsq[2 * m.nc + c];
-#pragma unroll
-        for (int q = 0; q < NF; ++q) {
-            gr[2 * q] = i0 * sx[q] + i1 * sy[q];
-            gr[2 * q + 1] = i1 * sx[q] + i2 * sy[q];
-        }
-    } else {
-        const double V = m.vol[c];
-#pragma unroll
-        for (int q = 0; q < NF; ++q) {
-            gr[2 * q] = fdiv(sx[q], V);
-            gr[2 * q + 1] = fdiv(sy[q], V);
-        }
-    }
+    if (GS == FVK_GRAD_LEAST_SQUARES)
+        grad_finish<NF, GS>(sx, sy, double4{m.lsq[c], m.lsq[m.nc + c], m.lsq[2 * m.nc + c], 0.0}, gr);
+    else
+        grad_finish<NF, GS>(sx, sy, double4{m.vol[c], 0.0, 0.0, 0.0}, gr);
     double2* gc = reinterpret_cast<double2*>(a.G + (int64_t)c * (2 * NF));
 #pragma unroll
     for (int q = 0; q < NF; ++q) gc[q] = double2{gr[2 * q], gr[2 * q + 1]};
-    if (EQ == EQ_SA) {
-        double nul = fdiv(a.mu, Uc.r);
-        double om = fabs(gr[2] - gr[1]);
-        a.aux[c] = sa_source(t0c, nul, om, m.wdist[c], gr[6], gr[7], a.sa);
-    }
-    if (EQ == EQ_SST) {
+    if (EQ != EQ_NS) {
         double o4[4];
-        double nul = fdiv(a.mu, Uc.r);
-        sst_source(Uc.r, t0c, t1c, nul, m.wdist[c], gr, gr + 2, gr + 6, gr + 8, a.sst_variant, a.kato, a.sst, o4);
+        visc_sources<EQ, NF>(a, Uc.r, t0c, t1c, m.wdist[c], gr, o4);
+        if (EQ == EQ_SA) a.aux[c] = o4[0];
+        if (EQ == EQ_SST) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) a.aux[q * a.ld + c] = o4[q];
+            for (int q = 0; q < 4; ++q) a.aux[q * a.ld + c] = o4[q];
+        }
     }
 }
 
@@ -552,20 +586,21 @@ __device__ __forceinline__ void vblock_partials(const double (&v)[NV], double* p
 // sources, then the stage update (or the residual output) of cell c from
 // its gathered face sum r and, for local dt, the minimum of its faces' dt
 // the update's inputs of cell c, loaded apart from the arithmetic so that a
-// kernel can issue them early: volume, source terms, dt (unless it is the
-// cell's own new local dt) and the RK base state
+// kernel can issue them early: volume, source terms (AUX: from the gradient
+// pass's output; else the caller sets them), dt (unless it is the cell's own
+// new local dt) and the RK base state
 template <int NV>
 struct ViscUpdateInputs {
     double V, s0, s1, dt;
     double u0[NV];
 };
-template <int EQ, int NV>
+template <int EQ, int NV, bool AUX = true>
 __device__ __forceinline__ ViscUpdateInputs<NV> load_visc_update_inputs(const ViscArgs& a, int c) {
     const int64_t ld = a.ld;
     ViscUpdateInputs<NV> x{};
     x.V = a.m.vol[c];
-    if (EQ == EQ_SA) x.s0 = a.aux[c];
-    if (EQ == EQ_SST) { x.s0 = a.aux[c]; x.s1 = a.aux[ld + c]; }
+    if (AUX && EQ == EQ_SA) x.s0 = a.aux[c];
+    if (AUX && EQ == EQ_SST) { x.s0 = a.aux[c]; x.s1 = a.aux[ld + c]; }
     if (!a.R) {
         if (a.dt_src == 1) x.dt = dt_read(a.dtmin, a.slot);
         else if (a.dt_src == 0) x.dt = a.dtc[c];
@@ -657,12 +692,14 @@ __global__ __launch_bounds__(kVB) void k_visc_cell(ViscArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// fused face + cell pass over the solver's cell tiles (kernel variant 3, local
-// or no dt): the states, gradients (and SST's F1 and nu_t) of the tile's
-// cells and its halo ring are staged in LDS, the fluxes of the tile's faces
-// go to LDS, then each owned cell gathers them in Cell::faces order and is
-// updated.  Same operations as k_visc_face + k_visc_cell, so the same bits;
-// a face on a tile's edge is evaluated by both tiles.
+// fused gradient + face + cell pass over the solver's cell tiles (kernel
+// variant 3, local or no dt).  A tile stages the states of its owned cells
+// and two halo rings; it computes the gradients (and the turbulence sources,
+// SST's F1 and nu_t) of its owned and ring-1 cells into LDS, the fluxes of
+// its faces into LDS, then each owned cell gathers them in Cell::faces order
+// and is updated.  Same operations as k_visc_grad + k_visc_face +
+// k_visc_cell, so the same bits; a face on a tile's edge is evaluated by both
+// tiles, a ring-1 cell's gradient by every tile it borders.
 // ---------------------------------------------------------------------------
 struct TileSides {
     const double *U, *G, *X;
@@ -673,57 +710,194 @@ struct TileSides {
     __device__ double aux(int s, int q) const { return X[(q - 2) * MG + c[s]]; }  // SST: F1 (2), nu_t (3)
 };
 
+// gradient and sources of local cell j of a viscous tile, from the
+// primitives P ([NF][ML]) and states U ([NV][MG]) in LDS: en and geo hold
+// its D slots' packed ELL entries and gradient geometry (-1: no slot), iv
+// its per-cell gradient data, wd its wall distance.  Writes G (and SST's F1,
+// nu_t to X); an owned cell's sources go to s0, s1.
+template <int EQ, int D, int GS>
+__device__ __forceinline__ void visc_tile_grad(const ViscArgs& a, const double* U, const double* P, double* G,
+                                               double* X, int MG, int ML, int f0, int j, bool owned,
+                                               const int (&en)[D], const double4 (&geo)[D], const double4& iv,
+                                               double wd, double& s0, double& s1) {
+    constexpr int NF = EQ == EQ_SST ? 5 : (EQ == EQ_SA ? 4 : 3);
+    const MeshView& m = a.m;
+    double pc[NF], sx[NF], sy[NF];
+#pragma unroll
+    for (int q = 0; q < NF; ++q) {
+        pc[q] = P[q * ML + j];
+        sx[q] = 0.0;
+        sy[q] = 0.0;
+    }
+    const double t0c = NF > 3 ? pc[3 % NF] : 0.0, t1c = NF > 4 ? pc[4 % NF] : 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        if (en[k] == -1) continue;  // ELL padding (after the cell's faces)
+        const bool bnd = tell_is_boundary(en[k]);
+        double pn[NF];
+        if (bnd) {
+            // the face and its patch from the tile's list (owned cell) or the mesh (ring 1)
+            const int x = tell_index(en[k]);
+            int f, code;
+            if (en[k] & 1) {
+                f = x;
+                code = m.fr[x];
+            } else {
+                const int4 fd = m.tface[f0 + x];
+                f = fd.x;
+                code = fd.z;
+            }
+            const State Uc{U[j], U[MG + j], U[2 * MG + j], U[3 * MG + j]};
+            double bv[7];
+            boundary_values<EQ>(a, m.bc[-1 - code], f, Uc, pc[0], pc[1], pc[2], t0c, t1c, bv);
+#pragma unroll
+            for (int q = 0; q < NF; ++q) pn[q] = bv[q];
+        } else {
+            const int l = tell_nb(en[k]);
+#pragma unroll
+            for (int q = 0; q < NF; ++q) pn[q] = P[q * ML + l];
+        }
+        grad_slot<NF, GS>(sx, sy, pc, pn, bnd, geo[k]);
+    }
+    double gr[2 * NF];
+    grad_finish<NF, GS>(sx, sy, iv, gr);
+#pragma unroll
+    for (int q = 0; q < 2 * NF; ++q) G[q * MG + j] = gr[q];
+    if (EQ == EQ_SA && owned) {
+        double o4[4];
+        visc_sources<EQ, NF>(a, U[j], t0c, t1c, wd, gr, o4);
+        s0 = o4[0];
+    }
+    if (EQ == EQ_SST) {
+        double o4[4];
+        visc_sources<EQ, NF>(a, U[j], t0c, t1c, wd, gr, o4);
+        X[j] = o4[2];
+        X[MG + j] = o4[3];
+        if (owned) {
+            s0 = o4[0];
+            s1 = o4[1];
+        }
+    }
+}
+
 constexpr int kVTileMax = 512;
-template <int EQ, int SCHEME>
+// D: slots per cell the gradient holds in registers (at least the mesh width)
+template <int EQ, int SCHEME, int D>
 __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
     constexpr int NV = EQ == EQ_SST ? 6 : (EQ == EQ_SA ? 5 : 4);
-    constexpr int NG = 2 * (NV - 1), NX = EQ == EQ_SST ? 2 : 0;
+    constexpr int NF = NV - 1, NG = 2 * NF, NX = EQ == EQ_SST ? 2 : 0;
     extern __shared__ double lds[];
     const MeshView& m = a.m;
-    const int t = logical_block();
-    const int bs = blockDim.x, tid = threadIdx.x;
-    const int MG = m.tmax_g, MF = m.tmax_f, W = m.twidth;
-    const int64_t ld = a.ld;
-    double* U = lds;
-    double* G = U + NV * MG;
-    double* X = G + NG * MG;
-    double* F = X + NX * MG;
-    double* Dt = F + NV * MF;
-    int c0 = 0, n = 0, hp = 0, nh = 0, f0 = 0, nft = 0;
-    if (t < m.ntiles) {
-        c0 = m.tstart[t];
-        n = m.tstart[t + 1] - c0;
-        hp = m.thptr[t];
-        nh = m.thptr[t + 1] - hp;
-        f0 = m.tfptr[t];
-        nft = m.tfptr[t + 1] - f0;
+    int t = logical_block();
+    // a pending finalize (the previous stage's norms) takes the first logical
+    // workgroup, dispatched first so that its one-workgroup reduction overlaps
+    // the tiles; the tiles follow it
+    if (a.fin_nblocks > 0) {
+        if (t == 0) {
+            finalize_norms(a.partials, a.fin_nblocks, a.fin_nv, a.res_hist, a.dt_hist, a.fin_hist_idx, a.dtmin,
+                           a.fin_slot, -1.0, lds, true, a.fin_dt_row != 0);
+            return;
+        }
+        --t;
     }
-    const int ng = n + nh;  // owned cells and ring 1 (the only halo ring here)
+    const int bs = blockDim.x, tid = threadIdx.x;
+    const int MG = m.tmax_g, ML = m.tmax_l, MF = m.tmax_f, W = m.twidth;
+    const int64_t ld = a.ld;
+    double* U = lds;            // [NV][MG] states of the owned and ring-1 cells
+    double* G = U + NV * MG;    // [NG][MG] their gradients
+    double* X = G + NG * MG;    // [NX][MG] SST: their F1 and nu_t
+    double* P = X + NX * MG;    // [NF][ML] u, v, T (and the turbulence fields) of every staged cell
+    double* F = P;              // [NV][MF] face fluxes and [MF] dt candidates, over P once the gradients are done
+    double* Dt = F + NV * MF;
+    int c0 = 0, n = 0, hp = 0, h1 = 0, nh = 0, f0 = 0, nft = 0;
+    if (t < m.ntiles) {
+        const int4 d0 = m.tdesc[2 * t], d1 = m.tdesc[2 * t + 1];
+        c0 = d0.x;
+        n = d0.y;
+        hp = d0.z;
+        h1 = d0.w;
+        nh = d1.x;
+        f0 = d1.y;
+        nft = d1.z;
+    }
+    const int ng = n + h1;  // owned and ring-1 cells (with gradients); ring 2 follows up to n + nh
+    const double g = a.gamma;
+    const bool lsq = a.gscheme == FVK_GRAD_LEAST_SQUARES;
+    const int* te = m.tell + (int64_t)t * W * MG;
+    const double4* tg = m.tegeo + (int64_t)t * W * MG;
+    const double4* tc = m.tclsq + (int64_t)t * MG;
 
-    // 1. states, gradients (and SST's F1, nu_t) of the tile and its halo
-    for (int j = tid; j < ng; j += bs) {
-        const int gc = j < n ? c0 + j : m.thalo[hp + j - n];
+    // local cell tid's device id, and its gradient inputs, issued before the
+    // staging so that their latency overlaps it (local cells from bs on,
+    // ring cells only, are read where they are used)
+    const int gc0 = tid < n ? c0 + tid : (tid < n + nh ? m.thalo[hp + tid - n] : 0);
+    int en0[D];
+    double4 geo0[D], iv0{};
+    double wd0 = 0.0;
+    if (tid < ng) {
 #pragma unroll
-        for (int q = 0; q < NV; ++q) U[q * MG + j] = a.U[q * ld + gc];
-        const double2* gsrc = reinterpret_cast<const double2*>(a.G + (int64_t)gc * NG);
+        for (int k = 0; k < D; ++k) en0[k] = k < W ? te[k * MG + tid] : -1;
 #pragma unroll
-        for (int q = 0; q < NG / 2; ++q) {
-            const double2 v = gsrc[q];
-            G[2 * q * MG + j] = v.x;
-            G[(2 * q + 1) * MG + j] = v.y;
+        for (int k = 0; k < D; ++k) geo0[k] = k < W ? tg[k * MG + tid] : double4{};
+        iv0 = tc[tid];
+        if (EQ != EQ_NS) wd0 = m.wdist[gc0];
+    }
+
+    // 1. states of the owned and ring-1 cells, primitives of every staged cell
+    for (int j = tid; j < n + nh; j += bs) {
+        const int gc = j == tid ? gc0 : m.thalo[hp + j - n];  // j > tid: j >= bs >= n, a ring cell
+        double u[NV];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) u[q] = a.U[q * ld + gc];
+        if (j < ng) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) U[q * MG + j] = u[q];
         }
-        if (EQ == EQ_SST) {
-            X[j] = a.aux[2 * ld + gc];
-            X[MG + j] = a.aux[3 * ld + gc];
-        }
+        const State sc{u[0], u[1], u[2], u[3]};
+        P[j] = fdiv(sc.ru, sc.r);
+        P[ML + j] = fdiv(sc.rv, sc.r);
+        P[2 * ML + j] = temperature(sc, g, a.Rgas);
+#pragma unroll
+        for (int q = 3; q < NF; ++q) P[q * ML + j] = u[q + 1];
+    }
+    __syncthreads();
+
+    // 2. gradients of the owned and ring-1 cells in Cell::faces order, and
+    // their sources (thread tid's owned cell tid keeps its own in registers)
+    double s0 = 0.0, s1 = 0.0;
+    if (tid < ng) {
+        if (lsq)
+            visc_tile_grad<EQ, D, FVK_GRAD_LEAST_SQUARES>(a, U, P, G, X, MG, ML, f0, tid, tid < n, en0, geo0, iv0,
+                                                          wd0, s0, s1);
+        else
+            visc_tile_grad<EQ, D, FVK_GRAD_GREEN_GAUSS>(a, U, P, G, X, MG, ML, f0, tid, tid < n, en0, geo0, iv0,
+                                                        wd0, s0, s1);
+    }
+    for (int j = tid + bs; j < ng; j += bs) {
+        int en[D];
+        double4 geo[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) en[k] = k < W ? te[k * MG + j] : -1;
+#pragma unroll
+        for (int k = 0; k < D; ++k) geo[k] = k < W ? tg[k * MG + j] : double4{};
+        const double wd = EQ == EQ_SST ? m.wdist[m.thalo[hp + j - n]] : 0.0;
+        double d0, d1;
+        if (lsq)
+            visc_tile_grad<EQ, D, FVK_GRAD_LEAST_SQUARES>(a, U, P, G, X, MG, ML, f0, j, false, en, geo, tc[j], wd,
+                                                          d0, d1);
+        else
+            visc_tile_grad<EQ, D, FVK_GRAD_GREEN_GAUSS>(a, U, P, G, X, MG, ML, f0, j, false, en, geo, tc[j], wd,
+                                                        d0, d1);
     }
     __syncthreads();
 
     // the update's inputs of owned cell tid, loaded while the faces are computed
     ViscUpdateInputs<NV> upd{};
-    if (tid < n) upd = load_visc_update_inputs<EQ, NV>(a, c0 + tid);
+    if (tid < n) upd = load_visc_update_inputs<EQ, NV, false>(a, c0 + tid);
+    upd.s0 = s0;
+    upd.s1 = s1;
 
-    // 2. fluxes (and dt candidates) of the tile's faces
+    // 3. fluxes (and dt candidates) of the tile's faces
     double cand_min = DBL_MAX;
     for (int e = tid; e < nft; e += bs) {
         const int4 fd = m.tface[f0 + e];
@@ -739,7 +913,7 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
     }
     __syncthreads();
 
-    // 3. gather in Cell::faces order (owner minus), sources and update of the owned cells
+    // 4. gather in Cell::faces order (owner minus), sources and update of the owned cells
     double r[NV];
 #pragma unroll
     for (int q = 0; q < NV; ++q) r[q] = 0.0;
@@ -765,7 +939,7 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
         visc_cell_finish_with<EQ, NV>(a, c0 + tid, upd, u, r, dmin);
     }
 
-    // 4. block reductions in the (now free) LDS: residual partials, dt minimum
+    // 5. block reductions in the (now free) LDS: residual partials, dt minimum
     int p2 = 64;  // the largest power of two not above the workgroup size
     while (2 * p2 <= bs) p2 *= 2;
     if (a.want_res) {
@@ -1036,18 +1210,23 @@ void launch_grad_eq(fvk_solver* s, int grid, const ViscArgs& a) {
     }
 }
 
-template <int EQ>
-void launch_tile_eq(fvk_solver* s, int grid, const ViscArgs& a0) {
-    ViscArgs a = a0;
-    a.fin_nblocks = 0;  // a pending finalize went to this stage's gradient launch
+template <int EQ, int D>
+void launch_tile_d(fvk_solver* s, int grid, const ViscArgs& a) {
     const size_t lds = s->tile_lds;
     auto go = [&](auto kernel) {
         allow_lds(reinterpret_cast<const void*>(kernel), lds);
         kernel<<<grid, s->block, lds, s->stream>>>(a);
     };
-    if (s->p.flux_scheme == FLUX_HLLC) go(k_visc_tile<EQ, FLUX_HLLC>);
-    else if (s->p.flux_scheme == FLUX_EXACT) go(k_visc_tile<EQ, FLUX_EXACT>);
-    else go(k_visc_tile<EQ, FLUX_RUSANOV>);
+    if (s->p.flux_scheme == FLUX_HLLC) go(k_visc_tile<EQ, FLUX_HLLC, D>);
+    else if (s->p.flux_scheme == FLUX_EXACT) go(k_visc_tile<EQ, FLUX_EXACT, D>);
+    else go(k_visc_tile<EQ, FLUX_RUSANOV, D>);
+}
+template <int EQ>
+void launch_tile_eq(fvk_solver* s, int grid, const ViscArgs& a) {
+    const int w = s->mesh.twidth;
+    if (w <= 4) launch_tile_d<EQ, 4>(s, grid, a);
+    else if (w <= 9) launch_tile_d<EQ, 9>(s, grid, a);
+    else launch_tile_d<EQ, kMaxEulerDegree>(s, grid, a);
 }
 
 template <int EQ>
@@ -1060,20 +1239,12 @@ void launch_face_eq(fvk_solver* s, int grid, const ViscArgs& a) {
 void run_visc(fvk_solver* s, ViscArgs& a) {
     const int gc = grid_for(s->nc, kVB);
     const int gf = grid_for(s->nf, kVB);
-    // a pending finalize takes the last logical workgroup of one more XCD round
-    const int gg = gc + (a.fin_nblocks > 0 ? kXcds : 0);
     hipEvent_t ev = nullptr;
-    timer_begin(s, "visc_grad", &ev);
-    switch (s->p.equation) {
-        case FVK_EQ_NAVIER_STOKES: launch_grad_eq<EQ_NS>(s, gg, a); break;
-        case FVK_EQ_RANS_SA: launch_grad_eq<EQ_SA>(s, gg, a); break;
-        default: launch_grad_eq<EQ_SST>(s, gg, a); break;
-    }
-    timer_end(s, ev);
     int nparts = gc;
     // the fused tile pass needs the step's dt before its updates: local or no dt only
     if (s->variant == 3 && (a.dt_src != 1 || a.R)) {
-        const int gt = (s->mesh.ntiles + kXcds - 1) / kXcds * kXcds;
+        // a pending finalize takes the first logical workgroup
+        const int gt = (s->mesh.ntiles + (a.fin_nblocks > 0 ? 1 : 0) + kXcds - 1) / kXcds * kXcds;
         timer_begin(s, "visc_tile", &ev);
         switch (s->p.equation) {
             case FVK_EQ_NAVIER_STOKES: launch_tile_eq<EQ_NS>(s, gt, a); break;
@@ -1083,6 +1254,15 @@ void run_visc(fvk_solver* s, ViscArgs& a) {
         timer_end(s, ev);
         nparts = gt;
     } else {
+        // a pending finalize takes the first logical workgroup of one more XCD round
+        const int gg = gc + (a.fin_nblocks > 0 ? kXcds : 0);
+        timer_begin(s, "visc_grad", &ev);
+        switch (s->p.equation) {
+            case FVK_EQ_NAVIER_STOKES: launch_grad_eq<EQ_NS>(s, gg, a); break;
+            case FVK_EQ_RANS_SA: launch_grad_eq<EQ_SA>(s, gg, a); break;
+            default: launch_grad_eq<EQ_SST>(s, gg, a); break;
+        }
+        timer_end(s, ev);
         timer_begin(s, "visc_face", &ev);
         switch (s->p.equation) {
             case FVK_EQ_NAVIER_STOKES: launch_face_eq<EQ_NS>(s, gf, a); break;
@@ -1100,7 +1280,7 @@ void run_visc(fvk_solver* s, ViscArgs& a) {
     }
     if (a.want_res) {
         if (s->p.rk_stages >= 2 && !a.R) {
-            // the next stage's gradient launch reduces this stage's partials
+            // the next stage's first launch reduces this stage's partials
             s->fin_nblocks = nparts;
             s->fin_slot = a.slot;
             s->fin_hist_idx = a.hist_idx;
@@ -1134,7 +1314,7 @@ int launch_visc_stage(fvk_solver* s, int stage, int stages, int want_res, int64_
     a.want_res = want_res && stage == 0;
     a.slot = (int)(step & 1);
     a.hist_idx = step % s->hist_cap;
-    if (s->fin_nblocks > 0) {  // the previous stage's norms, folded into this stage's gradient launch
+    if (s->fin_nblocks > 0) {  // the previous stage's norms, folded into this stage's first launch
         a.fin_nblocks = s->fin_nblocks;
         a.fin_slot = s->fin_slot;
         a.fin_hist_idx = s->fin_hist_idx;

@@ -214,12 +214,16 @@ void tile_order(fvk_solver* s, int n, const std::vector<int>& aptr, const std::v
 // Per tile: the halo rings, the tile-local copies of the ELL entries of the
 // cells that need gradients, the tile's face list and each owned cell's
 // slots as indices into it.  The static geometry the tile kernel reads
-// (LSQ and limiter offsets, LSQ inverse, face geometry and MUSCL offsets)
-// is also copied in tile order, so its loads are contiguous and need no
-// index loaded first.  Returns the LDS bytes a tile needs at most.
+// (per-slot gradient geometry sgeo, per-cell gradient data cdat, face
+// geometry and MUSCL offsets) is also copied in tile order, so its loads are
+// contiguous and need no index loaded first.  two_rings: the tile computes
+// the gradients of its owned and ring-1 cells (a second halo ring, ring-1
+// slots, sgeo and cdat); rec: also each slot's MUSCL offset (frec; the
+// periodic Euler tiles).  Returns the LDS bytes of a second-order Euler tile
+// (the viscous tiles size theirs from the maxima).
 int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<int>& cent, const std::vector<int>& fl,
-                const std::vector<int>& fr, int width, bool second_order, bool periodic,
-                const std::vector<double4>& egeo, const std::vector<double4>& clsq,
+                const std::vector<int>& fr, int width, bool two_rings, bool rec,
+                const std::vector<double4>& sgeo, const std::vector<double4>& cdat,
                 const std::vector<double4>& fgeo, const std::vector<double4>& frec, size_t* lds_bytes) {
     const int nt = (int)s->tile_start.size() - 1;
     const int T = s->block;  // the workgroup size (tiles hold at most tile_cells <= T cells)
@@ -250,7 +254,7 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
         };
         add_ring(0, n);
         const int h1 = (int)cells.size() - n;
-        if (second_order) add_ring(n, n + h1);
+        if (two_rings) add_ring(n, n + h1);
         const int h2 = (int)cells.size() - n - h1;
         info[t] = TileInfo{n, h1, h2};
         for (size_t j = n; j < cells.size(); ++j) thalo.push_back(cells[j]);
@@ -278,17 +282,17 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     }
     // the packed face cells hold local indices and patch codes in 16 bits each,
     // the packed slots local indices and tile face indices in 15
-    if (mg > 32767 || mf > 32766 || s->n_bcs > 32767)
-        return set_error(FVK_ERR_INVALID, "tiles of %d staged cells or %d patches exceed the packed face format", mg,
+    if (ml > 32767 || mf > 32766 || s->n_bcs > 32767)
+        return set_error(FVK_ERR_INVALID, "tiles of %d staged cells or %d patches exceed the packed face format", ml,
                          s->n_bcs);
     // pass 2: tile-local ELL and face slots (need the maxima for the strides)
     std::vector<int> tell((size_t)nt * width * mg, -1);
     std::vector<double4> tegeo, tclsq;
     std::vector<double2> trec;
-    if (second_order) {
+    if (two_rings) {
         tegeo.assign((size_t)nt * width * mg, double4{0.0, 0.0, 0.0, 0.0});
         tclsq.assign((size_t)nt * mg, double4{0.0, 0.0, 0.0, 0.0});
-        if (periodic) trec.assign((size_t)nt * width * mg, double2{0.0, 0.0});
+        if (rec) trec.assign((size_t)nt * width * mg, double2{0.0, 0.0});
     }
     const int64_t ld = s->mesh.ld;
     if (nf >= (1 << 30) - 1) return set_error(FVK_ERR_INVALID, "%lld faces exceed the packed tile format", (long long)nf);
@@ -301,14 +305,14 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
         for (size_t j = 0; j < cells.size(); ++j) loc[cells[j]] = (int)j;
         const int nft = tfptr[t + 1] - tfptr[t];
         for (int e = tfptr[t]; e < tfptr[t + 1]; ++e) fmark[tface[e].x] = e - tfptr[t];
-        // first order reads the local entries of the owned cells only (no ring-2 halo)
-        const int nell = second_order ? ti.n + ti.h1 : ti.n;
+        // without gradients the kernel reads the local entries of the owned cells only (no ring-2 halo)
+        const int nell = two_rings ? ti.n + ti.h1 : ti.n;
         for (int j = 0; j < nell; ++j) {
             const int c = cells[j];
-            if (second_order) {
-                tclsq[(size_t)t * mg + j] = clsq[c];
+            if (two_rings) {
+                tclsq[(size_t)t * mg + j] = cdat[c];
                 for (int k = 0; k < cptr[c + 1] - cptr[c]; ++k)
-                    tegeo[((size_t)t * width + k) * mg + j] = egeo[(size_t)k * ld + c];
+                    tegeo[((size_t)t * width + k) * mg + j] = sgeo[(size_t)k * ld + c];
             }
             for (int k = cptr[c]; k < cptr[c + 1]; ++k) {
                 const int slot = k - cptr[c];
@@ -330,7 +334,7 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
                     nb >= 0 ? tell_interior(nb, e & 1, ours ? lf : -1) : tell_boundary_slot(j < ti.n, lf, f);
                 // periodic: the MUSCL offset of the face's side (frec), which for
                 // cell_right differs from the limiter offset in rounding
-                if (second_order && periodic) {
+                if (two_rings && rec) {
                     const double4 fr4 = frec[f];
                     trec[((size_t)t * width + slot) * mg + j] =
                         (e & 1) ? double2{fr4.x, fr4.y} : double2{fr4.z, fr4.w};
@@ -373,19 +377,19 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     std::vector<double4> tfgeo(tface.size());
     for (size_t e = 0; e < tface.size(); ++e) tfgeo[e] = fgeo[tface[e].x];
     m.tfgeo = upload(s, tfgeo);
-    m.tegeo = second_order ? upload(s, tegeo) : nullptr;
-    m.tclsq = second_order ? upload(s, tclsq) : nullptr;
-    m.trec = second_order && periodic ? upload(s, trec) : nullptr;
+    m.tegeo = two_rings ? upload(s, tegeo) : nullptr;
+    m.tclsq = two_rings ? upload(s, tclsq) : nullptr;
+    m.trec = two_rings && rec ? upload(s, trec) : nullptr;
     if (!m.tdesc || !m.tstart || !m.thptr || (!thalo.empty() && !m.thalo) || !m.th1 || !m.tell || !m.tfptr ||
         !m.tface || !m.tfc ||
-        !m.tfgeo || (second_order && (!m.tegeo || !m.tclsq || (periodic && !m.trec))))
+        !m.tfgeo || (two_rings && (!m.tegeo || !m.tclsq || (rec && !m.trec))))
         return set_error(FVK_ERR_NOMEM, "device allocation for the tiles failed");
     // second order: U (4) and W (3: u, v, p) of the owned and ring-1 cells, W
     // (4) of ring 2, then the MUSCL increments (4 per side) of each tile face,
     // which each face's fluxes (4) and dt candidate (1) replace; first order:
     // U (4) of the owned and ring-1 cells, then the fluxes and dt candidates
-    const size_t words = second_order ? (size_t)7 * mg + (size_t)4 * m2 + (size_t)8 * mf
-                                      : (size_t)4 * mg + (size_t)5 * mf;
+    const size_t words = two_rings ? (size_t)7 * mg + (size_t)4 * m2 + (size_t)8 * mf
+                                   : (size_t)4 * mg + (size_t)5 * mf;
     *lds_bytes = sizeof(double) * std::max(words, (size_t)4 * T);  // at least the block reductions' 4 T doubles
     return FVK_OK;
 }
@@ -729,15 +733,42 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
     if (s->variant == 3) {
         size_t lds = 0;
         const bool euler = s->p.equation == FVK_EQ_EULER;
-        int rc = build_tiles(s, cptr, cent, fl, fr, width, euler && s->p.order == 2, sh != nullptr, egeo, clsq,
-                             fgeo, frec, &lds);
+        int rc;
+        if (!euler && width > kMaxEulerDegree)
+            return set_error(FVK_ERR_UNSUPPORTED, "viscous tiles take cells of at most %d faces (a cell has %d)",
+                             kMaxEulerDegree, deg);
+        if (euler) {
+            const bool second = s->p.order == 2;
+            rc = build_tiles(s, cptr, cent, fl, fr, width, second, second && sh != nullptr, egeo, clsq, fgeo, frec,
+                             &lds);
+        } else if (s->p.gradient_scheme == FVK_GRAD_LEAST_SQUARES) {
+            // viscous tiles compute their gradients: least squares reads the
+            // per-slot weight times offset and the LSQ inverse
+            rc = build_tiles(s, cptr, cent, fl, fr, width, true, false, egeo, clsq, fgeo, frec, &lds);
+        } else {
+            // Green-Gauss: per slot the cell's distance weight, the normal
+            // pointing away from the cell and the face length; per cell the volume
+            std::vector<double4> ggeo((size_t)width * ld, double4{0.0, 0.0, 0.0, 0.0}), gvol(ld, double4{1.0, 0.0, 0.0, 0.0});
+            for (int64_t i = 0; i < nc; ++i) {
+                gvol[i] = double4{vol[i], 0.0, 0.0, 0.0};
+                for (int k = cptr[i]; k < cptr[i + 1]; ++k) {
+                    const int f = cent[k] >> 1;
+                    const bool owner = cent[k] & 1;
+                    ggeo[(size_t)(k - cptr[i]) * ld + i] =
+                        double4{owner ? fwl[f] : fwr[f], owner ? fnx[f] : -fnx[f], owner ? fny[f] : -fny[f], fa[f]};
+                }
+            }
+            rc = build_tiles(s, cptr, cent, fl, fr, width, true, false, ggeo, gvol, fgeo, frec, &lds);
+        }
         if (rc != FVK_OK) return rc;
         if (!euler) {
             // viscous tiles: state, gradients (and SST's F1, nu_t) of the owned and
-            // ring-1 cells, then the face fluxes and dt candidates; at least the
-            // residual reduction's nv doubles per thread
-            const int nv = s->nv, nf2 = 2 * (nv - 1), nx = s->p.equation == FVK_EQ_RANS_SST ? 2 : 0;
-            const size_t words = (size_t)(nv + nf2 + nx) * m.tmax_g + (size_t)(nv + 1) * m.tmax_f;
+            // ring-1 cells, then the primitives of every staged cell, which the
+            // face fluxes and dt candidates replace once the gradients are
+            // done; at least the residual reduction's nv doubles per thread
+            const int nv = s->nv, nf = nv - 1, nx = s->p.equation == FVK_EQ_RANS_SST ? 2 : 0;
+            const size_t words = (size_t)(nv + 2 * nf + nx) * m.tmax_g +
+                                 std::max((size_t)nf * m.tmax_l, (size_t)(nv + 1) * m.tmax_f);
             lds = sizeof(double) * std::max(words, (size_t)nv * s->block);
         }
         if (lds > kMaxTileLds)

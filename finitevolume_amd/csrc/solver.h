@@ -118,11 +118,15 @@ struct MeshView {
     const int4* tface;      // per tile face: {device face, local cell_left, local cell_right or -1 - patch, 0}
     const int* tfc;         // per tile face, packed (tface_pack): local cell_left in the low 16 bits,
                             //   local cell_right or -1 - patch as a signed high half
-    // static geometry in tile order: egeo and clsq of each tile's owned and
-    // ring-1 cells ([ntiles][width][tmax_g], [ntiles][tmax_g]; second
-    // order), fgeo of each tile face (as tface); on a periodic mesh, the
-    // MUSCL offset (frec) of each slot's side of its face (trec, as tegeo:
-    // for cell_right it differs in rounding from egeo's midpoint offset)
+    // static geometry in tile order: the gradient data of each tile's owned
+    // and ring-1 cells ([ntiles][width][tmax_g] per slot, [ntiles][tmax_g]
+    // per cell; tiles that compute gradients): egeo and clsq for least
+    // squares, and for the viscous Green-Gauss gradient {the cell's distance
+    // weight, the normal pointing away from the cell, the face length} per
+    // slot and {volume, 0, 0, 0} per cell; fgeo of each tile face (as
+    // tface); on a periodic mesh, the MUSCL offset (frec) of each slot's side
+    // of its face (trec, as tegeo: for cell_right it differs in rounding from
+    // egeo's midpoint offset)
     const double4 *tegeo, *tclsq, *tfgeo;
     const double2* trec;
 };

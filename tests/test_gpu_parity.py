@@ -240,6 +240,50 @@ def test_viscous_tile_bit_identical(equation, gradient):
             assert np.array_equal(dt, dref), (equation, gradient, bs, local_dt)
 
 
+@pytest.mark.parametrize("gradient", [abi.GRAD_GREEN_GAUSS, abi.GRAD_LEAST_SQUARES])
+@pytest.mark.parametrize("kind", ["periodic", "triangles", "voronoi"])
+def test_viscous_tile_meshes(kind, gradient):
+    """The fused viscous tile (gradients of the owned and ring-1 cells in the
+    tile) on the other mesh kinds: a doubly periodic quad box (the LSQ offsets
+    carry the periodic shift), triangles (cells of three faces in four-slot
+    registers) and Voronoi polygons of up to nine faces, with walls the ring-1
+    cells touch.  Same state bits as the split passes after 30 SSP-RK3 steps
+    with local dt.  The periodic box has no wall, so no wall distance for the
+    turbulence models: Navier-Stokes only there."""
+    from finitevolume_amd import mesh as M, voronoi
+    ff = M.from_primitive(1.0, 0.3, 0.1, 1.0, 1.4)
+    if kind == "periodic":
+        m, bcs = M.periodic_box(20, 16), []
+    elif kind == "triangles":
+        m = M.tri_box(12, 10)
+        bcs = [abi.make_bc(abi.BC_WALL, is_isothermal_wall=1, wall_temperature=1.2), abi.make_bc(abi.BC_OUTFLOW),
+               abi.make_bc(abi.BC_FARFIELD, ff), abi.make_bc(abi.BC_FARFIELD, ff)]
+    else:
+        m = voronoi.naca_voronoi(1500, seed=3)
+        bcs = [abi.make_bc(abi.BC_WALL), abi.make_bc(abi.BC_FARFIELD, ff)]
+    for b in bcs:
+        b.farfield_nut, b.farfield_k, b.farfield_omega = 0.05, 2e-3, 3.0
+    equations = (abi.EQ_NAVIER_STOKES,) if kind == "periodic" else (
+        abi.EQ_NAVIER_STOKES, abi.EQ_RANS_SA, abi.EQ_RANS_SST)
+    for equation in equations:
+        out = []
+        for variant, bs in ((1, 256), (3, 64), (3, 256)):
+            p = abi.default_params(equation=equation, flux_scheme=abi.FLUX_HLLC, gradient_scheme=gradient, mu=0.01,
+                                   cfl=0.4, rk_stages=3, local_dt=1)
+            p.kernel_variant, p.block_size = variant, bs
+            ic = abi.initial_condition(abi.IC_FREESTREAM, rho=1.0, u=0.3, v=0.1, p=1.0, initial_nut=0.06,
+                                       initial_k=1e-3, initial_omega=5.0)
+            g = FVKSolver(m, bcs, p, ic)
+            r = g.step(30)
+            f = g.field()
+            out.append((np.concatenate([x.ravel() for x in (f if isinstance(f, tuple) else (f,))]), r))
+            del g
+        assert np.isfinite(out[0][0]).all(), (kind, gradient, equation)
+        for o, r in out[1:]:
+            assert np.array_equal(o, out[0][0]), (kind, gradient, equation)
+            assert np.allclose(r, out[0][1], rtol=1e-13, atol=0.0), (kind, gradient, equation)
+
+
 @pytest.mark.parametrize("equation", [abi.EQ_NAVIER_STOKES, abi.EQ_RANS_SA, abi.EQ_RANS_SST])
 def test_viscous_rk3_local_dt(equation):
     """SSP-RK3 with local time steps on the viscous paths (as the RANS
