@@ -144,10 +144,10 @@ __device__ __forceinline__ void adj_face_body(const AdjArgs& a, int f, int cr, c
     }
 }
 
-// Without the geometry adjoints the pass fits 168 VGPRs, three wavefronts per
-// SIMD, with no spills; with them it needs more and runs at two.
+// Without the geometry adjoints the pass fits 128 VGPRs, four wavefronts per
+// SIMD, with no spills; with them it needs more and runs at three.
 template <int SCHEME, int ORDER, bool WX>
-__global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(WX ? 1 : 3))) void k_adj_face(AdjArgs a) {
+__global__ __launch_bounds__(kAdjBlock) __attribute__((amdgpu_waves_per_eu(WX ? 1 : 4))) void k_adj_face(AdjArgs a) {
     const MeshView& m = a.m;
     const int f = blockIdx.x * kAdjBlock + threadIdx.x;
     if (f >= m.nf) return;
