@@ -721,7 +721,11 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
     // (PF) the mesh data of the first two faces of this thread
     // (up to nine slots per cell are held in registers, so their loads run
     // ahead of their use; LEAN: cells wider than that read their slots
-    // where they use them, which keeps sixteen-slot cells from spilling)
+    // where they use them, which keeps sixteen-slot cells from spilling).
+    // Where cells may have fewer faces than slots (D > 3), a slot's geometry
+    // is loaded only if the slot is a face: the padding is not fetched, and
+    // the wait for the slots coincides with the one for the halo ids
+    constexpr bool PAD = D > 3;
     int hc[2] = {0, 0};
     if (m.nthalo > 0) {
 #pragma unroll
@@ -739,7 +743,7 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
     if (ORDER == 2) {
         if (!LEAN) {
 #pragma unroll
-            for (int k = 0; k < D; ++k) peg[k] = tg[k * MG + jt];
+            for (int k = 0; k < D; ++k) peg[k] = !PAD || pen[k] != -1 ? tg[k * MG + jt] : double4{0, 0, 0, 0};
         }
         piv = m.tclsq[(int64_t)ta * MG + jt];
     }
@@ -791,7 +795,7 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
 #pragma unroll
             for (int k = 0; k < D; ++k) en[k] = te[k * MG + j];
 #pragma unroll
-            for (int k = 0; k < D; ++k) eg[k] = tg[k * MG + j];
+            for (int k = 0; k < D; ++k) eg[k] = !PAD || en[k] != -1 ? tg[k * MG + j] : double4{0, 0, 0, 0};
             tile_grad<D>(m, g, a.limiter, L, j, en, eg, m.tclsq[(int64_t)t * MG + j], f0, rec ? rec + j : nullptr);
         }
     }

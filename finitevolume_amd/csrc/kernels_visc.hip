@@ -829,7 +829,9 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
 
     // local cell tid's device id, and its gradient inputs, issued before the
     // staging so that their latency overlaps it (local cells from bs on,
-    // ring cells only, are read where they are used)
+    // ring cells only, are read where they are used); a slot's geometry only
+    // if the slot is a face, so the padding of cells with fewer faces than
+    // slots is not fetched
     const int gc0 = tid < n ? c0 + tid : (tid < n + nh ? m.thalo[hp + tid - n] : 0);
     int en0[D];
     double4 geo0[D], iv0{};
@@ -838,7 +840,7 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
 #pragma unroll
         for (int k = 0; k < D; ++k) en0[k] = k < W ? te[k * MG + tid] : -1;
 #pragma unroll
-        for (int k = 0; k < D; ++k) geo0[k] = k < W ? tg[k * MG + tid] : double4{};
+        for (int k = 0; k < D; ++k) geo0[k] = en0[k] != -1 ? tg[k * MG + tid] : double4{};
         iv0 = tc[tid];
         if (EQ != EQ_NS) wd0 = m.wdist[gc0];
     }
@@ -879,7 +881,7 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
 #pragma unroll
         for (int k = 0; k < D; ++k) en[k] = k < W ? te[k * MG + j] : -1;
 #pragma unroll
-        for (int k = 0; k < D; ++k) geo[k] = k < W ? tg[k * MG + j] : double4{};
+        for (int k = 0; k < D; ++k) geo[k] = en[k] != -1 ? tg[k * MG + j] : double4{};
         const double wd = EQ == EQ_SST ? m.wdist[m.thalo[hp + j - n]] : 0.0;
         double d0, d1;
         if (lsq)

@@ -111,8 +111,12 @@ T* upload(fvk_solver* s, const std::vector<T>& h) {
 // 7/8 of the capacity T; the next seed is the untiled cell with the most
 // tiled neighbours (the lowest RCM position first).  Tiles below T/2 are
 // then merged into the neighbouring tile they share most faces with, where
-// it has room.  The device order becomes tile by tile.
-void tile_order(fvk_solver* s, int n, const std::vector<int>& aptr, const std::vector<int>& adj, int T) {
+// it has room.  The device order becomes tile by tile, each tile's cells by
+// their face count (deg), then in RCM order: a wavefront then holds cells of
+// one or two face counts, so the kernels' slot loops skip the padding slots
+// for whole wavefronts instead of masking them lane by lane.
+void tile_order(fvk_solver* s, int n, const std::vector<int>& aptr, const std::vector<int>& adj, int T,
+                const int32_t* cfp) {
     const std::vector<int>& rcm = s->cell_new2old;
     std::vector<int> pos(n);
     for (int i = 0; i < n; ++i) pos[rcm[i]] = i;
@@ -191,7 +195,7 @@ void tile_order(fvk_solver* s, int n, const std::vector<int>& aptr, const std::v
         for (int u : cand) share[u] = 0;
     }
     // device order: tiles in the order of their first cell in RCM order,
-    // each tile's cells in RCM order
+    // each tile's cells by face count, then in RCM order
     std::vector<int> first(nt, n);
     for (int i = 0; i < n; ++i) first[tile[rcm[i]]] = std::min(first[tile[rcm[i]]], i);
     std::vector<int> tord;
@@ -204,7 +208,10 @@ void tile_order(fvk_solver* s, int n, const std::vector<int>& aptr, const std::v
     for (int t : tord) {
         s->tile_start.push_back((int)out.size());
         std::vector<int>& mm = members[t];
-        std::sort(mm.begin(), mm.end(), [&](int a, int b) { return pos[a] < pos[b]; });
+        std::sort(mm.begin(), mm.end(), [&](int a, int b) {
+            const int da = cfp[a + 1] - cfp[a], db = cfp[b + 1] - cfp[b];
+            return da != db ? da < db : pos[a] < pos[b];
+        });
         out.insert(out.end(), mm.begin(), mm.end());
     }
     s->tile_start.push_back(n);
@@ -254,6 +261,11 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
         };
         add_ring(0, n);
         const int h1 = (int)cells.size() - n;
+        // ring 1 by face count (its cells compute gradients, as the owned ones
+        // do), then in the order found
+        std::stable_sort(cells.begin() + n, cells.end(),
+                         [&](int a, int b) { return cptr[a + 1] - cptr[a] < cptr[b + 1] - cptr[b]; });
+        for (int j = n; j < n + h1; ++j) loc[cells[j]] = j;
         if (two_rings) add_ring(n, n + h1);
         const int h2 = (int)cells.size() - n - h1;
         info[t] = TileInfo{n, h1, h2};
@@ -470,7 +482,7 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
         for (int64_t c = 0; c < nc; ++c) s->case_cell_start[cc[c] + 1]++;
         for (int k = 0; k < s->n_cases; ++k) s->case_cell_start[k + 1] += s->case_cell_start[k];
     }
-    if (s->variant == 3) tile_order(s, (int)nc, aptr, adj, s->tile_cells);
+    if (s->variant == 3) tile_order(s, (int)nc, aptr, adj, s->tile_cells, d->cell_face_ptr);
     s->cell_old2new.assign(nc, 0);
     for (int64_t i = 0; i < nc; ++i) s->cell_old2new[s->cell_new2old[i]] = (int)i;
     const std::vector<int>& o2n = s->cell_old2new;
