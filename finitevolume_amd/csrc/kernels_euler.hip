@@ -743,7 +743,10 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
     if (ORDER == 2) {
         if (!LEAN) {
 #pragma unroll
-            for (int k = 0; k < D; ++k) peg[k] = !PAD || pen[k] != -1 ? tg[k * MG + jt] : double4{0, 0, 0, 0};
+            for (int k = 0; k < D; ++k) {
+                if constexpr (PAD) peg[k] = pen[k] != -1 ? tg[k * MG + jt] : double4{0, 0, 0, 0};
+                else peg[k] = tg[k * MG + jt];
+            }
         }
         piv = m.tclsq[(int64_t)ta * MG + jt];
     }
@@ -795,7 +798,10 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
 #pragma unroll
             for (int k = 0; k < D; ++k) en[k] = te[k * MG + j];
 #pragma unroll
-            for (int k = 0; k < D; ++k) eg[k] = !PAD || en[k] != -1 ? tg[k * MG + j] : double4{0, 0, 0, 0};
+            for (int k = 0; k < D; ++k) {
+                if constexpr (PAD) eg[k] = en[k] != -1 ? tg[k * MG + j] : double4{0, 0, 0, 0};
+                else eg[k] = tg[k * MG + j];
+            }
             tile_grad<D>(m, g, a.limiter, L, j, en, eg, m.tclsq[(int64_t)t * MG + j], f0, rec ? rec + j : nullptr);
         }
     }
