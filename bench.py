@@ -136,6 +136,20 @@ def adjoint_leg(s, m, bcs, params, ic, steps_done, reps=5, n_check=32):
     dev_p = sum(tot for tot, _ in tp.values()) / reps
     dev_a = sum(tot for tot, _ in ta.values()) / reps
     dev_u = sum(tot for tot, _ in tu.values()) / reps
+
+    # the adjoint solve's iteration (SSP-RK3 in pseudo time: three state-only
+    # VJPs and updates at the fixed state, whose forward gradients the solve
+    # forms once): the wall-time difference of a 40- and a 20-iteration solve
+    # of the lift objective, so set-up, the final node-coordinate pass and
+    # the copies cancel
+    def solve_wall(iters):
+        s.synchronize()
+        t = time.perf_counter()
+        s.adjoint_solve_forces(1.0, 0.5, 0.0, max_iter=iters, tol=0.0, check_every=iters)
+        return time.perf_counter() - t
+
+    solve_wall(20)
+    it_ms = (solve_wall(40) - solve_wall(20)) / 20 * 1e3
     out = {
         "workload": "config4: VJP of the config-2 residual (state + node coordinates) at the 1000-step state",
         "vjp_device_ms": round(dev_a, 4),
@@ -143,6 +157,10 @@ def adjoint_leg(s, m, bcs, params, ic, steps_done, reps=5, n_check=32):
         "adjoint_over_primal": round(dev_a / dev_p, 3),
         "vjp_state_only_device_ms": round(dev_u, 4),
         "adjoint_state_only_over_primal": round(dev_u / dev_p, 3),
+        "adjoint_solve_iteration_ms": round(it_ms, 4),
+        "adjoint_solve_vjp_over_primal": round(it_ms / 3 / dev_p, 3),
+        "adjoint_solve_note": "per VJP of the adjoint solve: a third of one pseudo-time RK3 iteration (state-only "
+                              "VJP and update, at the solve's fixed state and forward gradients), by wall time",
         "vjp_kernel_avg_ms": {k: round(tot / cnt, 5) for k, (tot, cnt) in sorted(ta.items()) if cnt},
         "vjp_wall_ms": round(wall_a * 1e3, 3),
         "primal_residual_wall_ms": round(wall_p * 1e3, 3),

@@ -450,14 +450,14 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_cell(AdjArgs a) {
 // of its faces (each face's edge adjoint is added by its owner, along the
 // owner's directed edge node1 -> node2)
 // ---------------------------------------------------------------------------
+// The state part of cell c's gather: its own face sides (ubp) plus the
+// neighbours' slot contributions, mapped from primitive to conservative
+// adjoints; WX: the neighbours' LSQ-point adjoints summed into cbx, cby.
 template <int D, bool WX>
-__global__ __launch_bounds__(kAdjBlock) void k_adj_gather(AdjArgs a) {
+__device__ __forceinline__ State adj_gather_state(const AdjArgs& a, int c, double& cbx, double& cby) {
     const MeshView& m = a.m;
-    const int c = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (c >= m.nc) return;
     const int64_t ld = m.ld;
     State Ub = st4(a.ubp[c]);
-    double cbx = 0.0, cby = 0.0;
     if (a.grad) {
         int2 en[D];
         int bs[D];
@@ -480,6 +480,17 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_gather(AdjArgs a) {
         }
         prim_rev(st4(a.U[c]), wb, Ub, a.gamma);
     }
+    return Ub;
+}
+
+template <int D, bool WX>
+__global__ __launch_bounds__(kAdjBlock) void k_adj_gather(AdjArgs a) {
+    const MeshView& m = a.m;
+    const int c = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (c >= m.nc) return;
+    const int64_t ld = m.ld;
+    double cbx = 0.0, cby = 0.0;
+    const State Ub = adj_gather_state<D, WX>(a, c, cbx, cby);
     a.ubar[c] = d4(Ub);
     if (!WX) return;
     int v[D], ps[D];
@@ -987,6 +998,73 @@ __global__ __launch_bounds__(kAdjBlock) void k_adj_rk(int nc, const double4* lam
     if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
+// k_adj_gather (state only) and k_adj_rk in one pass, for the adjoint
+// solve: the same operations in the same order, without the U_bar round
+// trip through memory.  copy (the first stage): lam0 takes lam first, as the
+// solve's copy before the stage did.
+template <int D>
+__global__ __launch_bounds__(kAdjBlock) void k_adj_gather_rk(AdjArgs a, double4* lam0, double4* lam, const double4* g,
+                                                             const double* P, double ca, double cb, int copy,
+                                                             double* partials) {
+    __shared__ double red[kAdjBlock];
+    const int c = blockIdx.x * kAdjBlock + threadIdx.x;
+    double sq = 0.0;
+    if (c < a.m.nc) {
+        double cbx = 0.0, cby = 0.0;
+        const State Ub = adj_gather_state<D, false>(a, c, cbx, cby);
+        const double4 u = d4(Ub), gg = g[c], l = lam[c];
+        const double4 l0 = copy ? l : lam0[c];
+        if (copy) lam0[c] = l;
+        const double4 r{u.x + gg.x, u.y + gg.y, u.z + gg.z, u.w + gg.w};
+        const double pc = P[c];
+        lam[c] = double4{ca * l0.x + cb * (l.x + pc * r.x), ca * l0.y + cb * (l.y + pc * r.y),
+                         ca * l0.z + cb * (l.z + pc * r.z), ca * l0.w + cb * (l.w + pc * r.w)};
+        sq = r.x * r.x + r.y * r.y + r.z * r.z + r.w * r.w;
+    }
+    if (!partials) return;
+    red[threadIdx.x] = sq;
+    __syncthreads();
+    for (int s2 = kAdjBlock / 2; s2 > 0; s2 >>= 1) {
+        if ((int)threadIdx.x < s2) red[threadIdx.x] += red[threadIdx.x + s2];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// one pseudo-time stage of the adjoint solve: the state-only face and cell
+// passes, then the gather fused with the update of lam
+int run_stage_rk(fvk_solver* s, double4* lam0, const double4* g, const double* P, double ca, double cb, bool copy,
+                 double* partials) {
+    AdjointWork* w = s->adj;
+    const AdjArgs a = make_args(s);
+    hipEvent_t ev = nullptr;
+    timer_begin(s, "adj_face", &ev);
+    if (a.order == 2) launch_faces<2>(s, a, false);
+    else launch_faces<1>(s, a, false);
+    timer_end(s, ev);
+    const int gc = grid_of(s->nc);
+    timer_begin(s, "adj_cell", &ev);
+    switch (w->D) {
+        case 3: k_adj_cell<3, false><<<gc, kAdjBlock, 0, s->stream>>>(a); break;
+        case 4: k_adj_cell<4, false><<<gc, kAdjBlock, 0, s->stream>>>(a); break;
+        case 9: k_adj_cell<9, false><<<gc, kAdjBlock, 0, s->stream>>>(a); break;
+        default: k_adj_cell<kMaxEulerDegree, false><<<gc, kAdjBlock, 0, s->stream>>>(a); break;
+    }
+    timer_end(s, ev);
+    timer_begin(s, "adj_gather_rk", &ev);
+    const int cp = copy ? 1 : 0;
+    switch (w->D) {
+        case 3: k_adj_gather_rk<3><<<gc, kAdjBlock, 0, s->stream>>>(a, lam0, w->lam, g, P, ca, cb, cp, partials); break;
+        case 4: k_adj_gather_rk<4><<<gc, kAdjBlock, 0, s->stream>>>(a, lam0, w->lam, g, P, ca, cb, cp, partials); break;
+        case 9: k_adj_gather_rk<9><<<gc, kAdjBlock, 0, s->stream>>>(a, lam0, w->lam, g, P, ca, cb, cp, partials); break;
+        default:
+            k_adj_gather_rk<kMaxEulerDegree><<<gc, kAdjBlock, 0, s->stream>>>(a, lam0, w->lam, g, P, ca, cb, cp,
+                                                                           partials);
+    }
+    timer_end(s, ev);
+    return hip_check(hipGetLastError(), "adjoint solve launch");
+}
+
 }  // namespace
 
 int adjoint_vjp(fvk_solver* s, const double* lambda, double* U_bar, double* X_bar) {
@@ -1050,15 +1128,8 @@ int adjoint_solve(fvk_solver* s, const double* g_host, double cfl, int max_iter,
     int it = 0;
     for (; it < max_iter; ++it) {
         const bool check = it % check_every == 0 || it + 1 == max_iter;
-        rc = hip_check(hipMemcpyAsync(lam0, w->lam, sizeof(double4) * ld, hipMemcpyDeviceToDevice, s->stream),
-                       "adjoint solve copy");
-        for (int st = 0; st < 3 && rc == FVK_OK; ++st) {
-            rc = run_passes(s, false);
-            if (rc != FVK_OK) break;
-            k_adj_rk<<<gc, kAdjBlock, 0, s->stream>>>((int)nc, lam0, w->lam, w->ubar, g, P, A[st], B[st],
-                                                     st == 0 && check ? part : nullptr);
-            rc = hip_check(hipGetLastError(), "adjoint solve launch");
-        }
+        for (int st = 0; st < 3 && rc == FVK_OK; ++st)
+            rc = run_stage_rk(s, lam0, g, P, A[st], B[st], st == 0, st == 0 && check ? part : nullptr);
         if (rc != FVK_OK) return rc;
         if (check) {
             rc = hip_check(hipMemcpyAsync(hp.data(), part, sizeof(double) * gc, hipMemcpyDeviceToHost, s->stream),
