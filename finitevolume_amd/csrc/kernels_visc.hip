@@ -325,22 +325,40 @@ __global__ __launch_bounds__(kVB) void k_visc_grad(ViscArgs a) {
     }
 }
 
+// The geometry of one face that the viscous face body reads: unit normal,
+// length, the Jasak distance d.n and correction vector k, the left cell's
+// distance weight (face_normal_geometry, src/GradientReconstruction.cpp:28-40)
+struct VFaceGeo {
+    double nx, ny, A, dc, wl, kx, ky;
+};
+__device__ __forceinline__ VFaceGeo face_geo(const MeshView& m, int f) {
+    return VFaceGeo{m.fnx[f], m.fny[f], m.fa[f], m.fdc[f], m.fwl[f], m.fkx[f], m.fky[f]};
+}
+// from a viscous tile's face record: r0 = {n_x, n_y, A, w_left}, r1 = {w_right, d.n, k_x, k_y}
+__device__ __forceinline__ VFaceGeo face_geo(const double4& r0, const double4& r1) {
+    return VFaceGeo{r0.x, r0.y, r0.z, r1.y, r0.w, r1.z, r1.w};
+}
+
 // Jasak corrected face gradient, src/GradientReconstruction.cpp:153-185, 195-200
 struct FG {
     double dn, gx, gy;
 };
-__device__ __forceinline__ FG face_grad(const MeshView& m, int f, bool b, double phiL, double phiRv, double gLx,
+__device__ __forceinline__ FG face_grad(const VFaceGeo& fg, bool b, double phiL, double phiRv, double gLx,
                                         double gLy, double gRx, double gRy) {
-    double direct = (phiRv - phiL) / m.fdc[f];
+    double direct = (phiRv - phiL) / fg.dc;
     FG o;
     if (b) { o.gx = gLx; o.gy = gLy; }
     else {
-        double wL = m.fwl[f];
+        double wL = fg.wl;
         o.gx = wL * gLx + (1.0 - wL) * gRx;
         o.gy = wL * gLy + (1.0 - wL) * gRy;
     }
-    o.dn = direct + o.gx * m.fkx[f] + o.gy * m.fky[f];
+    o.dn = direct + o.gx * fg.kx + o.gy * fg.ky;
     return o;
+}
+__device__ __forceinline__ FG face_grad(const MeshView& m, int f, bool b, double phiL, double phiRv, double gLx,
+                                        double gLy, double gRx, double gRy) {
+    return face_grad(face_geo(m, f), b, phiL, phiRv, gLx, gLy, gRx, gRy);
 }
 __device__ __forceinline__ void corrected(const FG& fg, double nx, double ny, double& gx, double& gy) {
     double avg = fg.gx * nx + fg.gy * ny;
@@ -360,14 +378,15 @@ __device__ __forceinline__ void corrected(const FG& fg, double nx, double ny, do
 // gradient component q, aux(side, q) per-cell auxiliary field q.  The face
 // and tile kernels read them from HBM or LDS.
 template <int EQ, int SCHEME, class Acc>
-__device__ __forceinline__ double visc_face_body(const ViscArgs& a, int f, int cr, const Acc& acc, double (&out)[6]) {
+__device__ __forceinline__ double visc_face_body(const ViscArgs& a, int f, int cr, const VFaceGeo& fgeo,
+                                                 const Acc& acc, double (&out)[6]) {
     const MeshView& m = a.m;
     double cand = DBL_MAX;
     {
         const double g = a.gamma;
         bool b = cr < 0;
         const DevBC* bc = b ? &m.bc[-1 - cr] : nullptr;
-        double nx = m.fnx[f], ny = m.fny[f], A = m.fa[f];
+        double nx = fgeo.nx, ny = fgeo.ny, A = fgeo.A;
         State L{acc.u(0, 0), acc.u(0, 1), acc.u(0, 2), acc.u(0, 3)};
         State R = b ? L : State{acc.u(1, 0), acc.u(1, 1), acc.u(1, 2), acc.u(1, 3)};
         State Rs = b ? visc_ghost(*bc, L, g) : R;
@@ -405,9 +424,9 @@ __device__ __forceinline__ double visc_face_body(const ViscArgs& a, int f, int c
                 gRu0 = acc.g(1, 0); gRu1 = acc.g(1, 1); gRv0 = acc.g(1, 2); gRv1 = acc.g(1, 3);
                 gRT0 = acc.g(1, 4); gRT1 = acc.g(1, 5);
             }
-            FG fu = face_grad(m, f, b, uL, b ? bv[0] : uR, gLu0, gLu1, gRu0, gRu1);
-            FG fv = face_grad(m, f, b, vL, b ? bv[1] : vR, gLv0, gLv1, gRv0, gRv1);
-            FG fT = face_grad(m, f, b, TL, b ? bv[2] : TR, gLT0, gLT1, gRT0, gRT1);
+            FG fu = face_grad(fgeo, b, uL, b ? bv[0] : uR, gLu0, gLu1, gRu0, gRu1);
+            FG fv = face_grad(fgeo, b, vL, b ? bv[1] : vR, gLv0, gLv1, gRv0, gRv1);
+            FG fT = face_grad(fgeo, b, TL, b ? bv[2] : TR, gLT0, gLT1, gRT0, gRT1);
             double gux, guy, gvx, gvy, gTx, gTy;
             corrected(fu, nx, ny, gux, guy);
             corrected(fv, nx, ny, gvx, gvy);
@@ -447,7 +466,7 @@ __device__ __forceinline__ double visc_face_body(const ViscArgs& a, int f, int c
             if (EQ == EQ_SA) {
                 double nlf = 0.5 * (nulL + nulR);
                 double ntf = 0.5 * (ntL + ntR);
-                FG fn = face_grad(m, f, b, t0L, b ? bv[3] : t0R, acc.g(0, 6), acc.g(0, 7),
+                FG fn = face_grad(fgeo, b, t0L, b ? bv[3] : t0R, acc.g(0, 6), acc.g(0, 7),
                                   b ? 0.0 : acc.g(1, 6), b ? 0.0 : acc.g(1, 7));
                 double dif = (nlf + ntf) / a.sa.sigma;
                 dif0 = -dif * fn.dn * A;
@@ -459,9 +478,9 @@ __device__ __forceinline__ double visc_face_body(const ViscArgs& a, int f, int c
                 double sk = F1f * a.sst.sigma_k1 + (1.0 - F1f) * a.sst.sigma_k2;
                 double sw = F1f * a.sst.sigma_omega1 + (1.0 - F1f) * a.sst.sigma_omega2;
                 double mek = a.mu + sk * tdvf, mew = a.mu + sw * tdvf;
-                FG fk = face_grad(m, f, b, t0L, b ? bv[3] : t0R, acc.g(0, 6), acc.g(0, 7),
+                FG fk = face_grad(fgeo, b, t0L, b ? bv[3] : t0R, acc.g(0, 6), acc.g(0, 7),
                                   b ? 0.0 : acc.g(1, 6), b ? 0.0 : acc.g(1, 7));
-                FG fw = face_grad(m, f, b, t1L, b ? bv[4] : t1R, acc.g(0, 8), acc.g(0, 9),
+                FG fw = face_grad(fgeo, b, t1L, b ? bv[4] : t1R, acc.g(0, 8), acc.g(0, 9),
                                   b ? 0.0 : acc.g(1, 8), b ? 0.0 : acc.g(1, 9));
                 dif0 = -mek * fk.dn * A;
                 dif1 = -mew * fw.dn * A;
@@ -490,7 +509,7 @@ __device__ __forceinline__ double visc_face_body(const ViscArgs& a, int f, int c
                     wmax = smax(wmax, t1R);
                 }
             }
-            double len = m.fdc[f];
+            double len = fgeo.dc;
             double nu = (a.mu > 0.0) ? fdiv(a.mu, rmin) : 0.0;
             if (EQ == EQ_NS) {
                 double St = Smax + 2.0 * nu / len;
@@ -534,7 +553,7 @@ __global__ __launch_bounds__(kVB) void k_visc_face(ViscArgs a) {
         const int cl = m.fl[f], cr = m.fr[f];
         const GlobalSides<2 * (NV - 1)> acc{a.U, a.G, a.aux, a.ld, {cl, cr < 0 ? cl : cr}};
         double out[6];
-        cand = visc_face_body<EQ, SCHEME>(a, f, cr, acc, out);
+        cand = visc_face_body<EQ, SCHEME>(a, f, cr, face_geo(m, f), acc, out);
 #pragma unroll
         for (int q = 0; q < NV; ++q) a.flux[q * a.ldf + f] = out[q];
         if (a.face_dt && a.dt_src == 2) a.dtf[f] = cand;
@@ -780,6 +799,21 @@ __device__ __forceinline__ void visc_tile_grad(const ViscArgs& a, const double* 
     }
 }
 
+// The gradient geometry of a viscous tile's slot entry en (tgk: the slot's
+// per-slot copy): least squares, or a slot whose face is not a tile face,
+// reads the copy; a Green-Gauss slot with a tile face forms it from the face's
+// record, {the cell's distance weight, the normal pointing away from the
+// cell, the length}, the values the copy holds.  No slot (-1): zeros.
+__device__ __forceinline__ double4 visc_slot_geo(const MeshView& m, bool lsq, int en, const double4* tgk, int f0) {
+    if (en == -1) return double4{0.0, 0.0, 0.0, 0.0};
+    const int lf = tell_lf(en);
+    if (lsq || lf < 0) return *tgk;
+    const double4* r = m.tvf + 2 * (int64_t)(f0 + lf);
+    const double4 r0 = r[0];
+    if (tell_owner(en)) return double4{r0.w, r0.x, r0.y, r0.z};
+    return double4{r[1].x, -r0.x, -r0.y, r0.z};
+}
+
 constexpr int kVTileMax = 512;
 // D: slots per cell the gradient holds in registers (at least the mesh width)
 template <int EQ, int SCHEME, int D>
@@ -831,7 +865,8 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
     // staging so that their latency overlaps it (local cells from bs on,
     // ring cells only, are read where they are used); a slot's geometry only
     // if the slot is a face, so the padding of cells with fewer faces than
-    // slots is not fetched
+    // slots is not fetched, and for Green-Gauss from its tile face's record
+    // where it has one (the face pass reads the same records)
     const int gc0 = tid < n ? c0 + tid : (tid < n + nh ? m.thalo[hp + tid - n] : 0);
     int en0[D];
     double4 geo0[D], iv0{};
@@ -840,7 +875,7 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
 #pragma unroll
         for (int k = 0; k < D; ++k) en0[k] = k < W ? te[k * MG + tid] : -1;
 #pragma unroll
-        for (int k = 0; k < D; ++k) geo0[k] = en0[k] != -1 ? tg[k * MG + tid] : double4{};
+        for (int k = 0; k < D; ++k) geo0[k] = visc_slot_geo(m, lsq, en0[k], tg + k * MG + tid, f0);
         iv0 = tc[tid];
         if (EQ != EQ_NS) wd0 = m.wdist[gc0];
     }
@@ -881,7 +916,7 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
 #pragma unroll
         for (int k = 0; k < D; ++k) en[k] = k < W ? te[k * MG + j] : -1;
 #pragma unroll
-        for (int k = 0; k < D; ++k) geo[k] = en[k] != -1 ? tg[k * MG + j] : double4{};
+        for (int k = 0; k < D; ++k) geo[k] = visc_slot_geo(m, lsq, en[k], tg + k * MG + j, f0);
         const double wd = EQ == EQ_SST ? m.wdist[m.thalo[hp + j - n]] : 0.0;
         double d0, d1;
         if (lsq)
@@ -903,9 +938,10 @@ __global__ __launch_bounds__(kVTileMax) void k_visc_tile(ViscArgs a) {
     double cand_min = DBL_MAX;
     for (int e = tid; e < nft; e += bs) {
         const int4 fd = m.tface[f0 + e];
+        const double4* r = m.tvf + 2 * (int64_t)(f0 + e);
         const TileSides acc{U, G, X, MG, {fd.y, fd.z < 0 ? fd.y : fd.z}};
         double out[6];
-        const double cand = visc_face_body<EQ, SCHEME>(a, fd.x, fd.z, acc, out);
+        const double cand = visc_face_body<EQ, SCHEME>(a, fd.x, fd.z, face_geo(r[0], r[1]), acc, out);
 #pragma unroll
         for (int q = 0; q < NV; ++q) F[q * MF + e] = out[q];
         if (a.face_dt) {

@@ -231,7 +231,8 @@ void tile_order(fvk_solver* s, int n, const std::vector<int>& aptr, const std::v
 int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<int>& cent, const std::vector<int>& fl,
                 const std::vector<int>& fr, int width, bool two_rings, bool rec,
                 const std::vector<double4>& sgeo, const std::vector<double4>& cdat,
-                const std::vector<double4>& fgeo, const std::vector<double4>& frec, size_t* lds_bytes) {
+                const std::vector<double4>& fgeo, const std::vector<double4>& frec, size_t* lds_bytes,
+                const std::vector<double4>* vrec = nullptr) {
     const int nt = (int)s->tile_start.size() - 1;
     const int T = s->block;  // the workgroup size (tiles hold at most tile_cells <= T cells)
     const int64_t nc = s->nc, nf = s->nf;
@@ -389,6 +390,16 @@ int build_tiles(fvk_solver* s, const std::vector<int>& cptr, const std::vector<i
     std::vector<double4> tfgeo(tface.size());
     for (size_t e = 0; e < tface.size(); ++e) tfgeo[e] = fgeo[tface[e].x];
     m.tfgeo = upload(s, tfgeo);
+    m.tvf = nullptr;
+    if (vrec) {  // the viscous face records (two per device face) in tile face order
+        std::vector<double4> tvf(2 * tface.size());
+        for (size_t e = 0; e < tface.size(); ++e) {
+            tvf[2 * e] = (*vrec)[2 * (size_t)tface[e].x];
+            tvf[2 * e + 1] = (*vrec)[2 * (size_t)tface[e].x + 1];
+        }
+        m.tvf = upload(s, tvf);
+        if (!m.tvf) return set_error(FVK_ERR_NOMEM, "device allocation for the tiles failed");
+    }
     m.tegeo = two_rings ? upload(s, tegeo) : nullptr;
     m.tclsq = two_rings ? upload(s, tclsq) : nullptr;
     m.trec = two_rings && rec ? upload(s, trec) : nullptr;
@@ -746,6 +757,15 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
         size_t lds = 0;
         const bool euler = s->p.equation == FVK_EQ_EULER;
         int rc;
+        // viscous face records: {n_x, n_y, A, w_left}, {w_right, d.n, k_x, k_y} per device face
+        std::vector<double4> vrec;
+        if (!euler) {
+            vrec.resize(2 * (size_t)nf);
+            for (int64_t i = 0; i < nf; ++i) {
+                vrec[2 * i] = double4{fnx[i], fny[i], fa[i], fwl[i]};
+                vrec[2 * i + 1] = double4{fwr[i], fdc[i], fkx[i], fky[i]};
+            }
+        }
         if (!euler && width > kMaxEulerDegree)
             return set_error(FVK_ERR_UNSUPPORTED, "viscous tiles take cells of at most %d faces (a cell has %d)",
                              kMaxEulerDegree, deg);
@@ -756,7 +776,7 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
         } else if (s->p.gradient_scheme == FVK_GRAD_LEAST_SQUARES) {
             // viscous tiles compute their gradients: least squares reads the
             // per-slot weight times offset and the LSQ inverse
-            rc = build_tiles(s, cptr, cent, fl, fr, width, true, false, egeo, clsq, fgeo, frec, &lds);
+            rc = build_tiles(s, cptr, cent, fl, fr, width, true, false, egeo, clsq, fgeo, frec, &lds, &vrec);
         } else {
             // Green-Gauss: per slot the cell's distance weight, the normal
             // pointing away from the cell and the face length; per cell the volume
@@ -770,7 +790,7 @@ int build_layout(fvk_solver* s, const fvk_mesh_desc* d) {
                         double4{owner ? fwl[f] : fwr[f], owner ? fnx[f] : -fnx[f], owner ? fny[f] : -fny[f], fa[f]};
                 }
             }
-            rc = build_tiles(s, cptr, cent, fl, fr, width, true, false, ggeo, gvol, fgeo, frec, &lds);
+            rc = build_tiles(s, cptr, cent, fl, fr, width, true, false, ggeo, gvol, fgeo, frec, &lds, &vrec);
         }
         if (rc != FVK_OK) return rc;
         if (!euler) {

@@ -129,6 +129,11 @@ struct MeshView {
     // egeo's midpoint offset)
     const double4 *tegeo, *tclsq, *tfgeo;
     const double2* trec;
+    // viscous tiles: per tile face (as tface) two records, {n_x, n_y, A,
+    // left distance weight} and {right distance weight, d.n, k_x, k_y}: the
+    // face pass's geometry, and the Green-Gauss geometry of every slot whose
+    // face is a tile face
+    const double4* tvf;
 };
 
 // Packed tile ELL slot (MeshView::tell).  -1 pads.  An interior face: bit 31
