@@ -34,6 +34,46 @@ def case_1r(n=64, flux=abi.FLUX_RUSANOV, **ext):
     return m, [], p, None, U0
 
 
+def hub_mesh(n=12, lo=4, hi=8):
+    """n x n quads of side 1/4 on [0, n/4]^2 with the centre block [lo, hi)^2
+    merged into one cell whose 4 (hi - lo) edges each face a small quad (16
+    faces at the defaults: the widest cells the kernels take).  Patches:
+    0 bottom, 1 right, 2 top, 3 left."""
+    h = 0.25
+
+    def nid(i, j):
+        return j * (n + 1) + i
+    I, J = np.meshgrid(np.arange(n + 1), np.arange(n + 1))
+    node_xy = np.stack([I.ravel() * h, J.ravel() * h], axis=1).astype(np.float64)
+    cells = [[nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)]
+             for j in range(n) for i in range(n) if not (lo <= i < hi and lo <= j < hi)]
+    hub = ([nid(i, lo) for i in range(lo, hi)] + [nid(hi, j) for j in range(lo, hi)] +
+           [nid(i, hi) for i in range(hi, lo, -1)] + [nid(lo, j) for j in range(hi, lo, -1)])
+    cells.insert(len(cells) // 2, hub)
+    ptr = np.cumsum([0] + [len(c) for c in cells])
+    i = np.arange(n)
+    edges = [np.stack([nid(i, 0), nid(i + 1, 0)], 1), np.stack([nid(n, i), nid(n, i + 1)], 1),
+             np.stack([nid(i + 1, n), nid(i, n)], 1), np.stack([nid(0, i + 1), nid(0, i)], 1)]
+    patches = np.concatenate([np.full(e.shape[0], k) for k, e in enumerate(edges)])
+    return M.from_polygons(node_xy, ptr, np.concatenate(cells), np.concatenate(edges), patches,
+                           ["bottom", "right", "top", "left"])
+
+
+def case_euler_hub(flux=abi.FLUX_HLLC, **ext):
+    """Second-order Euler (LSQ + Venkatakrishnan, SSP-RK3, local dt) on the
+    hub mesh: slip wall below, farfield elsewhere, an isentropic vortex on a
+    Mach 0.5 stream passing over the sixteen-face cell."""
+    m = hub_mesh()
+    U0 = M.isentropic_vortex(m.cell_xy, radius=0.4, center=(1.0, 1.2))
+    ff = M.from_primitive(1.0, 0.5 * math.sqrt(1.4) * math.cos(math.radians(45.0)),
+                          0.5 * math.sqrt(1.4) * math.sin(math.radians(45.0)), 1.0, GAMMA)
+    bcs = [abi.make_bc(abi.BC_WALL)] + [abi.make_bc(abi.BC_FARFIELD, ff) for _ in range(3)]
+    kw = dict(order=2, limiter=abi.LIMITER_VENKATAKRISHNAN, venkat_k=5.0, rk_stages=3, local_dt=1)
+    kw.update(ext)
+    p = abi.default_params(equation=abi.EQ_EULER, flux_scheme=flux, cfl=0.6, **kw)
+    return m, bcs, p, None, U0
+
+
 def case_sod(flux=abi.FLUX_HLLC, nx=64, ny=8):
     """Sod tube between slip walls (euler_init tworegion), x0 = 0.5."""
     m = M.cartesian(nx, ny)

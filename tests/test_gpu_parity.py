@@ -240,22 +240,41 @@ def test_viscous_tile_bit_identical(equation, gradient):
             assert np.array_equal(dt, dref), (equation, gradient, bs, local_dt)
 
 
+def test_sixteen_face_cells_match_oracle():
+    """Cells wider than nine faces take the sixteen-slot kernels (the per-cell
+    passes and the tile kernel reading its slots where it uses them): the
+    hub mesh's centre cell has sixteen faces.  Second-order Euler with local
+    dt against the oracle bit for bit, for the split and the fused passes
+    at two tile sizes."""
+    for variant, bs in ((1, 256), (3, 64), (3, 256)):
+        m, bcs, p, ic, U0 = cases.case_euler_hub()
+        p.kernel_variant, p.block_size = variant, bs
+        g, o = pair((m, bcs, p, ic, U0))
+        assert g.launch_shape()["kernel_variant"] == variant
+        for _ in range(3):
+            rg = g.step(10)
+            ro = o.step(10)
+            assert cases.residual_close(rg, ro, 1e-12), (variant, bs)
+            assert_bitwise(g, o)
+
+
 @pytest.mark.parametrize("gradient", [abi.GRAD_GREEN_GAUSS, abi.GRAD_LEAST_SQUARES])
-@pytest.mark.parametrize("kind", ["periodic", "triangles", "voronoi"])
+@pytest.mark.parametrize("kind", ["periodic", "triangles", "voronoi", "hub"])
 def test_viscous_tile_meshes(kind, gradient):
     """The fused viscous tile (gradients of the owned and ring-1 cells in the
     tile) on the other mesh kinds: a doubly periodic quad box (the LSQ offsets
     carry the periodic shift), triangles (cells of three faces in four-slot
     registers) and Voronoi polygons of up to nine faces, with walls the ring-1
-    cells touch.  Same state bits as the split passes after 30 SSP-RK3 steps
+    cells touch, and the hub mesh's sixteen-face cell (the widest slot
+    instance).  Same state bits as the split passes after 30 SSP-RK3 steps
     with local dt.  The periodic box has no wall, so no wall distance for the
     turbulence models: Navier-Stokes only there."""
     from finitevolume_amd import mesh as M, voronoi
     ff = M.from_primitive(1.0, 0.3, 0.1, 1.0, 1.4)
     if kind == "periodic":
         m, bcs = M.periodic_box(20, 16), []
-    elif kind == "triangles":
-        m = M.tri_box(12, 10)
+    elif kind in ("triangles", "hub"):
+        m = M.tri_box(12, 10) if kind == "triangles" else cases.hub_mesh()
         bcs = [abi.make_bc(abi.BC_WALL, is_isothermal_wall=1, wall_temperature=1.2), abi.make_bc(abi.BC_OUTFLOW),
                abi.make_bc(abi.BC_FARFIELD, ff), abi.make_bc(abi.BC_FARFIELD, ff)]
     else:
