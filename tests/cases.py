@@ -200,6 +200,9 @@ ADJOINT_CASES = {
     # the free stream with a density bump at the leading edge: away from it
     # the state is exactly uniform (zero gradients, zero MUSCL increments)
     "naca_hllc_o2_bump": lambda: adjoint_naca_bump(),
+    # the hub mesh: a sixteen-face cell (the widest slot instances), hanging
+    # nodes on its edges
+    "hub_hllc_o2": lambda: case_euler_hub(),
 }
 # steps of the oracle from the case's start to the state the adjoint is taken at
 ADJOINT_STEPS = {"naca_hllc_o2_bump": 0}
