@@ -61,30 +61,41 @@ __device__ __forceinline__ State ldS(const double* U, int64_t ld, int c) {
 __device__ __forceinline__ double sa_fv1(double nut, double nu, const fvk_sa_constants& c) {
     if (nut <= 0.0) return 0.0;
     double chi = fdiv(nut, nu), chi3 = chi * chi * chi;
-    return chi3 / (chi3 + c.cv1 * c.cv1 * c.cv1);
+    return fdiv(chi3, chi3 + c.cv1 * c.cv1 * c.cv1);
 }
 __device__ __forceinline__ double sa_eddy(double nut, double nu, const fvk_sa_constants& c) {
     return nut * sa_fv1(nut, nu, c);
 }
+// The SA source's powers, the largest single cost of the SA stage as pow
+// calls: x^6 in three multiplications for the reference's pow(x, 6.0), and
+// y^(1/6) as cbrt(sqrt(y)) for its pow(y, 1.0 / 6.0); each within a few ulp
+// (the device's pow is not the host's to the last bit either, so the SA
+// path is held to the oracle within a tolerance, not bit for bit)
+__device__ __forceinline__ double pow6(double x) {
+    const double x2 = x * x;
+    return x2 * x2 * x2;
+}
+__device__ __forceinline__ double root6(double y) { return cbrt(sqrt(y)); }
 __device__ double sa_source(double nut, double nu, double omega, double d, double gx, double gy,
                             const fvk_sa_constants& c) {
     double chi = (nut > 0.0) ? fdiv(nut, nu) : 0.0;
     double fv1 = sa_fv1(nut, nu, c);
-    double fv2 = 1.0 - chi / (1.0 + chi * fv1);
+    double fv2 = 1.0 - fdiv(chi, 1.0 + chi * fv1);
     double d2 = d * d;
-    double sbar = (nut / (c.kappa * c.kappa * d2)) * fv2;
+    double sbar = fdiv(nut, c.kappa * c.kappa * d2) * fv2;
     double st;
     if (sbar >= -c.cv2 * omega) st = omega + sbar;
-    else st = omega + omega * (c.cv2 * c.cv2 * omega + c.cv3 * sbar) / ((c.cv3 - 2.0 * c.cv2) * omega - sbar);
+    else st = omega + (omega * (c.cv2 * c.cv2 * omega + c.cv3 * sbar)) / ((c.cv3 - 2.0 * c.cv2) * omega - sbar);
     double P = c.cb1 * st * nut;
     double D = 0.0;
     if (nut > 0.0) {
         double sts = smax(st, 1e-10);
-        double r = smin(nut / (sts * c.kappa * c.kappa * d2), 10.0);
-        double gg = r + c.cw2 * (pow(r, 6.0) - r);
-        double fw = gg * pow((1.0 + pow(c.cw3, 6.0)) / (pow(gg, 6.0) + pow(c.cw3, 6.0)), 1.0 / 6.0);
+        double r = smin(fdiv(nut, sts * c.kappa * c.kappa * d2), 10.0);
+        double gg = r + c.cw2 * (pow6(r) - r);
+        const double cw36 = pow6(c.cw3);
+        double fw = gg * root6(fdiv(1.0 + cw36, pow6(gg) + cw36));
         double nod = fdiv(nut, d);
-        double cw1 = c.cb1 / (c.kappa * c.kappa) + (1.0 + c.cb2) / c.sigma;
+        double cw1 = fdiv(c.cb1, c.kappa * c.kappa) + fdiv(1.0 + c.cb2, c.sigma);
         D = cw1 * fw * nod * nod;
     }
     double CD = (fdiv(c.cb2, c.sigma)) * (gx * gx + gy * gy);
