@@ -356,7 +356,7 @@ struct FG {
 };
 __device__ __forceinline__ FG face_grad(const VFaceGeo& fg, bool b, double phiL, double phiRv, double gLx,
                                         double gLy, double gRx, double gRy) {
-    double direct = (phiRv - phiL) / fg.dc;
+    double direct = fdiv(phiRv - phiL, fg.dc);
     FG o;
     if (b) { o.gx = gLx; o.gy = gLy; }
     else {
@@ -523,15 +523,15 @@ __device__ __forceinline__ double visc_face_body(const ViscArgs& a, int f, int c
             double len = fgeo.dc;
             double nu = (a.mu > 0.0) ? fdiv(a.mu, rmin) : 0.0;
             if (EQ == EQ_NS) {
-                double St = Smax + 2.0 * nu / len;
-                cand = a.cfl * len / St;
+                double St = Smax + fdiv(2.0 * nu, len);
+                cand = fdiv(a.cfl * len, St);
             } else {
                 double nue = nu + ntmax;
-                double St = Smax + 2.0 * nue / len;
-                cand = a.cfl * len / St;
+                double St = Smax + fdiv(2.0 * nue, len);
+                cand = fdiv(a.cfl * len, St);
                 if (EQ == EQ_SST) {
                     double coef = smax(a.sst.beta_star, smax(a.sst.beta1, a.sst.beta2));
-                    double dts = a.cfl / (coef * smax(wmax, 1e-30));
+                    double dts = fdiv(a.cfl, coef * smax(wmax, 1e-30));
                     cand = smin(cand, dts);
                 }
             }
