@@ -378,6 +378,50 @@ __device__ __forceinline__ void corrected(const FG& fg, double nx, double ny, do
     gy = fg.gy + delta * ny;
 }
 
+// The compute_dt candidate of one face (NS src/NavierStokesFVMSolver.cpp:231-258,
+// SA src/RANSTurbulenceSASolver.cpp:313-346, SST src/RANSTurbulenceSSTSolver.cpp:380-429)
+// from its two states (R unused on a boundary face, b), their velocities and
+// turbulence fields, the normal and the face-normal distance len
+template <int EQ>
+__device__ __forceinline__ double visc_dt_candidate(const ViscArgs& a, bool b, const State& L, const State& R,
+                                                    double uL, double vL, double uR, double vR, double t0L,
+                                                    double t1L, double t0R, double t1R, double nx, double ny,
+                                                    double len) {
+    const double g = a.gamma;
+    double cand;
+    double VnL = uL * nx + vL * ny;
+    double Smax = fabs(VnL) + sound_speed(L, g);
+    double rmin = L.r;
+    double ntmax = 0.0, wmax = 0.0;
+    if (EQ == EQ_SA) ntmax = sa_eddy(t0L, fdiv(a.mu, L.r), a.sa);
+    if (EQ == EQ_SST) { ntmax = (t0L > 0.0 && t1L > 0.0) ? fdiv(t0L, t1L) : 0.0; wmax = t1L; }
+    if (!b) {
+        double VnR = uR * nx + vR * ny;
+        Smax = smax(Smax, fabs(VnR) + sound_speed(R, g));
+        rmin = smin(rmin, R.r);
+        if (EQ == EQ_SA) ntmax = smax(ntmax, sa_eddy(t0R, fdiv(a.mu, R.r), a.sa));
+        if (EQ == EQ_SST) {
+            ntmax = smax(ntmax, (t0R > 0.0 && t1R > 0.0) ? fdiv(t0R, t1R) : 0.0);
+            wmax = smax(wmax, t1R);
+        }
+    }
+    double nu = (a.mu > 0.0) ? fdiv(a.mu, rmin) : 0.0;
+    if (EQ == EQ_NS) {
+        double St = Smax + fdiv(2.0 * nu, len);
+        cand = fdiv(a.cfl * len, St);
+    } else {
+        double nue = nu + ntmax;
+        double St = Smax + fdiv(2.0 * nue, len);
+        cand = fdiv(a.cfl * len, St);
+        if (EQ == EQ_SST) {
+            double coef = smax(a.sst.beta_star, smax(a.sst.beta1, a.sst.beta2));
+            double dts = fdiv(a.cfl, coef * smax(wmax, 1e-30));
+            cand = smin(cand, dts);
+        }
+    }
+    return cand;
+}
+
 // ---------------------------------------------------------------------------
 // face pass: NavierStokesFVMSolver::step pass 1 (src/NavierStokesFVMSolver.cpp:119-189),
 // RANSTurbulenceSASolver (src/RANSTurbulenceSASolver.cpp:157-259),
@@ -503,41 +547,56 @@ __device__ __forceinline__ double visc_face_body(const ViscArgs& a, int f, int c
         out[3] = A * (inv.E - vE);
         if (EQ == EQ_SA) out[4] = adv0 + dif0;
         if (EQ == EQ_SST) { out[4] = adv0 + dif0; out[5] = adv1 + dif1; }
-        if (a.face_dt) {
-            double VnL = uL * nx + vL * ny;
-            double Smax = fabs(VnL) + sound_speed(L, g);
-            double rmin = L.r;
-            double ntmax = 0.0, wmax = 0.0;
-            if (EQ == EQ_SA) ntmax = sa_eddy(t0L, fdiv(a.mu, L.r), a.sa);
-            if (EQ == EQ_SST) { ntmax = (t0L > 0.0 && t1L > 0.0) ? fdiv(t0L, t1L) : 0.0; wmax = t1L; }
-            if (!b) {
-                double VnR = uR * nx + vR * ny;
-                Smax = smax(Smax, fabs(VnR) + sound_speed(R, g));
-                rmin = smin(rmin, R.r);
-                if (EQ == EQ_SA) ntmax = smax(ntmax, sa_eddy(t0R, fdiv(a.mu, R.r), a.sa));
-                if (EQ == EQ_SST) {
-                    ntmax = smax(ntmax, (t0R > 0.0 && t1R > 0.0) ? fdiv(t0R, t1R) : 0.0);
-                    wmax = smax(wmax, t1R);
-                }
-            }
-            double len = fgeo.dc;
-            double nu = (a.mu > 0.0) ? fdiv(a.mu, rmin) : 0.0;
-            if (EQ == EQ_NS) {
-                double St = Smax + fdiv(2.0 * nu, len);
-                cand = fdiv(a.cfl * len, St);
-            } else {
-                double nue = nu + ntmax;
-                double St = Smax + fdiv(2.0 * nue, len);
-                cand = fdiv(a.cfl * len, St);
-                if (EQ == EQ_SST) {
-                    double coef = smax(a.sst.beta_star, smax(a.sst.beta1, a.sst.beta2));
-                    double dts = fdiv(a.cfl, coef * smax(wmax, 1e-30));
-                    cand = smin(cand, dts);
-                }
-            }
-        }
+        if (a.face_dt) cand = visc_dt_candidate<EQ>(a, b, L, R, uL, vL, uR, vR, t0L, t1L, t0R, t1R, nx, ny, fgeo.dc);
     }
     return cand;
+}
+
+// global dt beside the fused tile pass (its first stage): the faces'
+// compute_dt candidates as the face pass forms them.  Each workgroup takes a
+// contiguous range of faces and leaves one atomic minimum; the grid is
+// bounded, since device-wide atomics on one address are serialised across
+// the XCDs.
+constexpr int kVDtGridMax = 4 * 256;
+template <int EQ>
+__global__ __launch_bounds__(kVB) void k_visc_dt(ViscArgs a) {
+    const MeshView& m = a.m;
+    const int64_t ld = a.ld;
+    const int64_t per = ((int64_t)m.nf + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)logical_block() * per;
+    const int64_t hi = lo + per < m.nf ? lo + per : m.nf;
+    double cand = DBL_MAX;
+    for (int64_t f = lo + threadIdx.x; f < hi; f += kVB) {
+        const int cl = m.fl[f], cr = m.fr[f];
+        const bool b = cr < 0;
+        const State L = ldS(a.U, ld, cl);
+        const State R = b ? L : ldS(a.U, ld, cr);
+        const double uL = fdiv(L.ru, L.r), vL = fdiv(L.rv, L.r);
+        double uR = 0.0, vR = 0.0;
+        if (!b) {
+            uR = fdiv(R.ru, R.r);
+            vR = fdiv(R.rv, R.r);
+        }
+        double t0L = 0.0, t1L = 0.0, t0R = 0.0, t1R = 0.0;
+        if (EQ == EQ_SA || EQ == EQ_SST) {
+            t0L = a.U[4 * ld + cl];
+            if (!b) t0R = a.U[4 * ld + cr];
+        }
+        if (EQ == EQ_SST) {
+            t1L = a.U[5 * ld + cl];
+            if (!b) t1R = a.U[5 * ld + cr];
+        }
+        cand = smin(cand, visc_dt_candidate<EQ>(a, b, L, R, uL, vL, uR, vR, t0L, t1L, t0R, t1R, m.fnx[f], m.fny[f],
+                                                m.fdc[f]));
+    }
+    __shared__ double sm[kVB];
+    sm[threadIdx.x] = cand;
+    __syncthreads();
+    for (int s2 = kVB / 2; s2 > 0; s2 >>= 1) {
+        if ((int)threadIdx.x < s2) sm[threadIdx.x] = smin(sm[threadIdx.x], sm[threadIdx.x + s2]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dt_atomic_min(a.dtmin, a.slot, sm[0]);
 }
 
 // global field access for the face sweep (NG gradients per cell, cell-major)
@@ -1290,15 +1349,29 @@ void run_visc(fvk_solver* s, ViscArgs& a) {
     const int gf = grid_for(s->nf, kVB);
     hipEvent_t ev = nullptr;
     int nparts = gc;
-    // the fused tile pass needs the step's dt before its updates: local or no dt only
-    if (s->variant == 3 && (a.dt_src != 1 || a.R)) {
+    if (s->variant == 3) {
+        // the fused tile pass updates its cells in the launch that evaluates
+        // their faces: a global dt is formed first, by its own pass over the
+        // faces (first stage; later stages read the step's slot)
+        ViscArgs at = a;
+        if (a.dt_src == 1 && a.face_dt && !a.R) {
+            const int gd = std::min(gf, kVDtGridMax);
+            timer_begin(s, "visc_dt", &ev);
+            switch (s->p.equation) {
+                case FVK_EQ_NAVIER_STOKES: k_visc_dt<EQ_NS><<<gd, kVB, 0, s->stream>>>(a); break;
+                case FVK_EQ_RANS_SA: k_visc_dt<EQ_SA><<<gd, kVB, 0, s->stream>>>(a); break;
+                default: k_visc_dt<EQ_SST><<<gd, kVB, 0, s->stream>>>(a); break;
+            }
+            timer_end(s, ev);
+            at.face_dt = 0;
+        }
         // a pending finalize takes the first logical workgroup
         const int gt = (s->mesh.ntiles + (a.fin_nblocks > 0 ? 1 : 0) + kXcds - 1) / kXcds * kXcds;
         timer_begin(s, "visc_tile", &ev);
         switch (s->p.equation) {
-            case FVK_EQ_NAVIER_STOKES: launch_tile_eq<EQ_NS>(s, gt, a); break;
-            case FVK_EQ_RANS_SA: launch_tile_eq<EQ_SA>(s, gt, a); break;
-            default: launch_tile_eq<EQ_SST>(s, gt, a); break;
+            case FVK_EQ_NAVIER_STOKES: launch_tile_eq<EQ_NS>(s, gt, at); break;
+            case FVK_EQ_RANS_SA: launch_tile_eq<EQ_SA>(s, gt, at); break;
+            default: launch_tile_eq<EQ_SST>(s, gt, at); break;
         }
         timer_end(s, ev);
         nparts = gt;

@@ -213,10 +213,11 @@ def test_global_dt_variants_match_oracle():
 @pytest.mark.parametrize("gradient", [abi.GRAD_GREEN_GAUSS, abi.GRAD_LEAST_SQUARES])
 @pytest.mark.parametrize("equation", [abi.EQ_NAVIER_STOKES, abi.EQ_RANS_SA, abi.EQ_RANS_SST])
 def test_viscous_tile_bit_identical(equation, gradient):
-    """The fused viscous tile pass (faces and cell updates per tile, local dt)
-    against the face sweep + cell pass: the same state bits at several tile
-    sizes; residual norms to round-off (their block grouping differs).  The
-    global-dt mode keeps the split passes, and must equal them too."""
+    """The fused viscous tile pass (gradients, faces and cell updates per
+    tile) against the gradient, face and cell passes: the same state bits at
+    several tile sizes; residual norms to round-off (their block grouping
+    differs).  With a global dt the tile pass follows a dt pass over the
+    faces (first stage), and the step's dt must be the same bits too."""
     def run(variant, bs, local_dt):
         if equation == abi.EQ_NAVIER_STOKES:
             m, bcs, p, ic, _ = cases.case_ns_wavy(gradient=gradient, n=24, rk_stages=3, local_dt=local_dt)
@@ -234,7 +235,8 @@ def test_viscous_tile_bit_identical(equation, gradient):
         ref, rref, _, dref = run(1, 256, local_dt)
         for bs in (64, 128, 256):
             out, r, names, dt = run(3, bs, local_dt)
-            assert ("visc_tile" in names) == (local_dt == 1), (bs, sorted(names))
+            assert "visc_tile" in names and "visc_grad" not in names, (bs, sorted(names))
+            assert ("visc_dt" in names) == (local_dt == 0), (bs, sorted(names))
             assert np.array_equal(out, ref), (equation, gradient, bs, local_dt)
             assert np.allclose(r, rref, rtol=1e-13, atol=0.0), (equation, gradient, bs, local_dt)
             assert np.array_equal(dt, dref), (equation, gradient, bs, local_dt)
