@@ -164,6 +164,9 @@ typedef struct fvk_params {
     /* ---- execution tuning (same results bit for bit; 0 = library default) ---- */
     int32_t kernel_variant;        /* Euler: 1 face sweep + per-cell gather, 2 per-cell fused flux and update,
                                       3 one fused pass per cell tile with its halo staged in LDS (0: 1);
+                                      NS/RANS: 1 gradient, face and cell passes, 3 one fused pass per tile
+                                      computing its gradients (cells of at most 16 faces); a global dt adds
+                                      a dt pass before the first stage's tile pass;
                                       batches also 4: variant 3 as one persistent launch per stage */
     int32_t block_size;            /* threads per workgroup of the Euler kernels: 64, 128, 256 or 512,
                                       for variant 3 any multiple of 64 up to 512 (0: 512; variant 3: 256) */
