@@ -1,6 +1,6 @@
 """Benchmark of the explicit FP64 residual path on one MI355X.
 
-Workload: second-order Euler, HLLC, weighted least-squares
+Workload (config 2): second-order Euler, HLLC, weighted least-squares
 primitive gradients with the Venkatakrishnan limiter (K = 5), SSP-RK3 with
 local time stepping at CFL 0.8, on a 1024 x 512 NACA 0012 O-grid split into
 1,048,576 triangles (far field 50 chords, wall-normal ratio 1.05), slip wall
