@@ -765,6 +765,9 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
     // (keep the scheduler from moving the use of the halo ids above the loads
     // just issued, which would wait for the ids before issuing them)
     __builtin_amdgcn_sched_barrier(0);
+    // the staging's loads at a raised wave priority: a workgroup starting a
+    // tile issues them ahead of the arithmetic of the CU's other workgroups
+    __builtin_amdgcn_s_setprio(2);
 
     // 1. states (and primitives) of the tile and its halo; the first two
     // passes load (unconditionally, cell 0 for a thread without a cell)
@@ -779,6 +782,7 @@ __device__ __forceinline__ void tile_body(const EulerArgs& a, double* lds, int t
         if (g1 >= 0) stage_cell(j1, u1);
     }
     for (int j = tid + 2 * bs; j < nl; j += bs) stage_cell(j, a.U[j < n ? c0 + j : m.thalo[hp + j - n]]);
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
 
     // 2. limited LSQ gradients of the owned cells and ring 1, and the MUSCL
