@@ -575,14 +575,15 @@ def main():
     params.block_size = args.block
     params.tile_cells = args.tile_cells
     nc, nf = m.n_cells, m.n_faces
-    sweep = {}
+    sweep, refined = {}, {}
     if not args.no_tune and not (args.variant or args.block):
         # the project's launch-shape autotuner picks the kernel variant and workgroup size
         from finitevolume_amd import autotune
         best = autotune.tune(m, bcs, params, ic, steps=30, use_cache=False)
         sweep = best["table_ms_per_step"]
-        print(f"[bench] autotune: variant {best['variant']} block {best['block']} ({sweep})", file=sys.stderr,
-              flush=True)
+        refined = best.get("refined_device_ms_per_step", {})
+        print(f"[bench] autotune: variant {best['variant']} block {best['block']} ({sweep}; refined {refined})",
+              file=sys.stderr, flush=True)
     fpc = nf / nc
     stages = params.rk_stages
     print(f"[bench] mesh {nc} cells {nf} faces; building solver", file=sys.stderr, flush=True)
@@ -667,6 +668,7 @@ def main():
     }
     if sweep:
         out["autotune_ms_per_step"] = sweep
+        out["autotune_refined_device_ms_per_step"] = refined
     if probe:
         out["copy_probe"] = probe
     if not args.no_adjoint:
