@@ -475,6 +475,15 @@ FVK_HD void face_state_w_rev(const State& Uc, const double4& g0, const double4& 
     prim_rev(Uc, Wfb, Ucb, gam);
 }
 
+// whether face_state_w reconstructs (true) or returns W(U) itself (false)
+FVK_HD bool face_state_w_reconstructs(const State& Uc, const double4& g0, const double4& g1, double rx, double ry,
+                                      double gam) {
+    const double d0 = g0.x * rx + g0.y * ry;
+    const double d3 = g1.z * rx + g1.w * ry;
+    const Prim w = to_prim(Uc, gam);
+    return (w.r + d0 > 0.0) && (w.p + d3 > 0.0);
+}
+
 // partial derivatives of venkatakrishnan(d1, d2, eps2)
 FVK_HD void venkatakrishnan_grad(double d1, double d2, double eps2, double& p1, double& p2, double& pe) {
     const double N = d1 * d1 + eps2 + 2.0 * d1 * d2;

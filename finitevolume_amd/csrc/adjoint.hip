@@ -40,7 +40,11 @@ namespace fvk {
 struct AdjointWork {
     int D = 0;
     bool geo = false;
-    double4 *lam = nullptr, *ful = nullptr, *fdl = nullptr, *fur = nullptr, *fdr = nullptr;
+    // per face side: the adjoint of the side's face state (second order: the
+    // primitive face state; first order: the conserved state), and fok, the
+    // sides whose face state was reconstructed (bit 0 left, bit 1 right)
+    double4 *lam = nullptr, *ful = nullptr, *fur = nullptr;
+    unsigned char* fok = nullptr;
     double4 *ubp = nullptr, *S = nullptr, *ubar = nullptr;
     double2* fe = nullptr;
     int* bslot = nullptr;
@@ -66,7 +70,8 @@ struct AdjArgs {
     const double4* U;
     const double4* G;
     const double4* lam;
-    double4 *ful, *fdl, *fur, *fdr, *ubp, *S, *ubar, *cb;
+    double4 *ful, *fur, *ubp, *S, *ubar, *cb;
+    unsigned char* fok;
     double2 *fe, *mb, *pb, *xcv, *xbar;
     const int* bslot;
     const int *poly, *pslot;
@@ -115,18 +120,16 @@ __device__ __forceinline__ void adj_face_body(const AdjArgs& a, int f, int cr, c
         Prim Lfb{0, 0, 0, 0}, Rfb{0, 0, 0, 0};
         num_flux_w_rev<SCHEME>(Lf, Rf, nx, ny, g, Hb, Lfb, Rfb, nxb, nyb);
         if (!interior) euler_ghost_w_rev(bc->type, Lf, nx, ny, Rfb, Lfb, nxb, nyb);
-        State ULb{0, 0, 0, 0};
-        double dl[4];
-        face_state_w_rev(L, gl0, gl1, ro.x, ro.y, g, Lfb, ULb, dl);
-        a.ful[f] = d4(ULb);
-        a.fdl[f] = double4{dl[0], dl[1], dl[2], dl[3]};
+        // each side's primitive face-state adjoint: the cell pass forms from
+        // it both the state adjoint (through W(U), one conversion per cell)
+        // and, where the side was reconstructed, the increments' adjoint
+        a.ful[f] = double4{Lfb.r, Lfb.u, Lfb.v, Lfb.p};
+        int ok = face_state_w_reconstructs(L, gl0, gl1, ro.x, ro.y, g) ? 1 : 0;
         if (interior) {
-            State URb{0, 0, 0, 0};
-            double dr[4];
-            face_state_w_rev(R, gr0, gr1, ro.z, ro.w, g, Rfb, URb, dr);
-            a.fur[f] = d4(URb);
-            a.fdr[f] = double4{dr[0], dr[1], dr[2], dr[3]};
+            a.fur[f] = double4{Rfb.r, Rfb.u, Rfb.v, Rfb.p};
+            if (face_state_w_reconstructs(R, gr0, gr1, ro.z, ro.w, g)) ok |= 2;
         }
+        a.fok[f] = (unsigned char)ok;
     } else {
         const State Rf = interior ? R : euler_ghost_state(bc->type, bc->ff, L, nx, ny, g);
         const State H = num_flux<SCHEME>(L, Rf, nx, ny, g, 0.0, 0);
@@ -187,21 +190,25 @@ __device__ __forceinline__ void adj_cell_body(const AdjArgs& a, int c, const int
     const MeshView& m = a.m;
     const double g = a.gamma;
     const int64_t ld = m.ld;
-    State Ub{0, 0, 0, 0};
     double Gb[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     double rb[D][2];
 #pragma unroll
     for (int k = 0; k < D; ++k) rb[k][0] = rb[k][1] = 0.0;
-    // the face sides of this cell
+    // the face sides of this cell, summed in ws (first order: conserved-state
+    // adjoints; second order: primitive face-state adjoints, converted once
+    // below; an increment's adjoint is the side's where the face state was
+    // reconstructed, else zero)
+    double ws[4] = {0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < D; ++k) {
         if (en[k].x < 0) continue;
         const int f = en[k].x >> 1;
         const bool owner = en[k].x & 1;
         const double4 u = owner ? a.ful[f] : a.fur[f];
-        Ub.r += u.x; Ub.ru += u.y; Ub.rv += u.z; Ub.E += u.w;
+        ws[0] += u.x; ws[1] += u.y; ws[2] += u.z; ws[3] += u.w;
         if (a.order == 2) {
-            const double4 d = owner ? a.fdl[f] : a.fdr[f];
+            const bool rec = (a.fok[f] >> (owner ? 0 : 1)) & 1;
+            const double4 d = rec ? u : double4{0.0, 0.0, 0.0, 0.0};
             const double rx = eg[k].z, ry = eg[k].w;
             Gb[0][0] += d.x * rx; Gb[0][1] += d.x * ry;
             Gb[1][0] += d.y * rx; Gb[1][1] += d.y * ry;
@@ -217,6 +224,7 @@ __device__ __forceinline__ void adj_cell_body(const AdjArgs& a, int c, const int
     double nbx[D], nby[D], Pbx[D], Pby[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) nbx[k] = nby[k] = Pbx[k] = Pby[k] = 0.0;
+    State Ub{0, 0, 0, 0};
     if (a.grad) {
         // ---- forward recompute ----
         const Prim wc = to_prim(Uc, g);
@@ -281,7 +289,7 @@ __device__ __forceinline__ void adj_cell_body(const AdjArgs& a, int c, const int
             for (int q = 0; q < 4; ++q) lim[q] = venkat_value(vm[q]);
         }
         // ---- reverse: G = lim g ----
-        double gbx[4], gby[4], wcb[4] = {0, 0, 0, 0}, wnb[D][4], eps2b = 0.0;
+        double gbx[4], gby[4], wcb[4] = {ws[0], ws[1], ws[2], ws[3]}, wnb[D][4], eps2b = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) wnb[k][0] = wnb[k][1] = wnb[k][2] = wnb[k][3] = 0.0;
 #pragma unroll
@@ -393,6 +401,10 @@ __device__ __forceinline__ void adj_cell_body(const AdjArgs& a, int c, const int
             const double V = m.cgeo[c].z, K = a.venk_k;
             Vb = eps2b * 1.5 * K * K * K * sqrt(V);
         }
+    } else if (a.order == 2) {
+        prim_rev(Uc, Prim{ws[0], ws[1], ws[2], ws[3]}, Ub, g);
+    } else {
+        Ub = State{ws[0], ws[1], ws[2], ws[3]};
     }
     a.ubp[c] = d4(Ub);
     if (WX) {
@@ -778,14 +790,13 @@ int ensure_work(fvk_solver* s, bool want_x) {
         w->lam = alloc_n<double4>(s, ld);
         w->ful = alloc_n<double4>(s, nf);
         w->fur = alloc_n<double4>(s, nf);
-        w->fdl = alloc_n<double4>(s, nf);
-        w->fdr = alloc_n<double4>(s, nf);
+        w->fok = alloc_n<unsigned char>(s, nf);
         w->ubp = alloc_n<double4>(s, ld);
         w->S = alloc_n<double4>(s, (int64_t)w->D * ld);
         w->ubar = alloc_n<double4>(s, ld);
         w->fe = alloc_n<double2>(s, nf);
         w->bslot = alloc_n<int>(s, (int64_t)w->D * ld);
-        if (!w->lam || !w->ful || !w->fur || !w->fdl || !w->fdr || !w->ubp || !w->S || !w->ubar || !w->fe ||
+        if (!w->lam || !w->ful || !w->fur || !w->fok || !w->ubp || !w->S || !w->ubar || !w->fe ||
             !w->bslot) {
             adjoint_release(s);
             return set_error(FVK_ERR_NOMEM, "adjoint: device allocation failed");
@@ -862,7 +873,7 @@ AdjArgs make_args(fvk_solver* s) {
     a.U = reinterpret_cast<const double4*>(s->U[s->cur]);
     a.G = reinterpret_cast<const double4*>(s->grad);
     a.lam = w->lam;
-    a.ful = w->ful; a.fdl = w->fdl; a.fur = w->fur; a.fdr = w->fdr;
+    a.ful = w->ful; a.fur = w->fur; a.fok = w->fok;
     a.ubp = w->ubp; a.S = w->S; a.ubar = w->ubar; a.fe = w->fe; a.bslot = w->bslot;
     a.mb = w->mb; a.pb = w->pb; a.cb = w->cb; a.xcv = w->xcv; a.xbar = w->xbar;
     a.poly = w->poly; a.pslot = w->pslot; a.cvptr = w->cvptr;
